@@ -1,0 +1,22 @@
+"""aeolus_hip — MI355X (gfx950) dynamical-core step of badulion/Aeolus.jl.
+
+Host mirror, over the C ABI of libaeolus_hip.so (include/aeolus_hip.h), of
+the reference's operator surface for the fractional-step step: advection
+schemes (AdvectionSchemes), pressure solvers (PressureSolvers) and the
+SSP-RK3 projection step (Simulations). All arithmetic runs in the library's
+HIP kernels; a missing library raises AeolusError.
+"""
+from . import _lib
+from ._lib import AeolusError, init, sync
+from .advection import advection_scheme, advect_weno_z, momentum_advect_weno_z, required_halo
+from .fields import DeviceArray, Grid, faces_geometric, faces_uniform, fill_halos
+from .pressure import (ColumnTridiagonalSolver, FFTPoissonSolver, solve, solve_interior, solve_source_buffer,
+                       solver_for, solver_stats, source_buffer)
+from .simulation import BoussinesqSimulation
+
+__all__ = [
+    "AeolusError", "init", "sync", "advection_scheme", "advect_weno_z", "momentum_advect_weno_z", "required_halo",
+    "DeviceArray", "Grid", "faces_geometric", "faces_uniform", "fill_halos", "ColumnTridiagonalSolver",
+    "FFTPoissonSolver", "solve", "solve_interior", "solve_source_buffer", "solver_for", "solver_stats",
+    "source_buffer", "BoussinesqSimulation",
+]

@@ -1,0 +1,123 @@
+"""ctypes binding of libaeolus_hip.so (include/aeolus_hip.h).
+
+The library is the product: every operator of this package runs in its HIP
+kernels. There is no CPU fallback; when the library is missing, every call
+raises `AeolusError`.
+"""
+import ctypes
+import os
+import threading
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libaeolus_hip.so")
+
+F64, F32 = 0, 1
+CELL, FACE_X, FACE_Y, FACE_Z = 0, 1, 2, 3
+PERIODIC, NOFLUX, WALL = 0, 1, 2
+CENTERED, UPWIND, WENO5, WENO_JS5, WENO_Z5, WENO_Z7 = 0, 1, 2, 3, 4, 5
+CONVERGED, MAXIT, BREAKDOWN, ZERO_RHS, DIRECT = 0, 1, 2, 3, 4
+STATUS_NAMES = {CONVERGED: "converged", MAXIT: "maxit", BREAKDOWN: "breakdown", ZERO_RHS: "zero_rhs",
+                DIRECT: "direct"}
+
+
+class AeolusError(RuntimeError):
+    """A failed call of libaeolus_hip (the text of aeo_last_error)."""
+
+
+class StepParams(ctypes.Structure):
+    _fields_ = [
+        ("mom_scheme", ctypes.c_int), ("scal_scheme", ctypes.c_int), ("weno_p", ctypes.c_int),
+        ("nu", ctypes.c_double), ("theta0", ctypes.c_double), ("g_accel", ctypes.c_double),
+        ("flux_bot", ctypes.c_double), ("flux_top", ctypes.c_double),
+        ("bc_x", ctypes.c_int), ("bc_y", ctypes.c_int), ("bc_z", ctypes.c_int),
+        ("tbc_x", ctypes.c_int), ("tbc_y", ctypes.c_int), ("tbc_z", ctypes.c_int),
+        ("precision", ctypes.c_int),
+    ]
+
+
+_vp = ctypes.c_void_p
+_pvp = ctypes.POINTER(ctypes.c_void_p)
+_i = ctypes.c_int
+_d = ctypes.c_double
+_pd = ctypes.POINTER(ctypes.c_double)
+_pi = ctypes.POINTER(ctypes.c_int)
+_sz = ctypes.c_size_t
+
+# name -> argtypes (every entry point declared in include/aeolus_hip.h)
+SIGNATURES = {
+    "aeo_init": [_i],
+    "aeo_shutdown": [],
+    "aeo_sync": [],
+    "aeo_last_error": [],
+    "aeo_malloc": [_pvp, _sz],
+    "aeo_free": [_vp],
+    "aeo_memcpy_h2d": [_vp, _vp, _sz],
+    "aeo_memcpy_d2h": [_vp, _vp, _sz],
+    "aeo_memset": [_vp, _i, _sz],
+    "aeo_grid_create": [_i, _i, _i, _i, _d, _d, _d, _pd, _pvp],
+    "aeo_grid_destroy": [_vp],
+    "aeo_grid_metrics": [_vp, _pd, _pd],
+    "aeo_fill_halos": [_vp, _i, _vp, _i, _i, _i, _i],
+    "aeo_advect_scalar": [_vp, _i, _i, _i, _vp, _vp, _vp, _vp, _vp],
+    "aeo_advect_momentum": [_vp, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp],
+    "aeo_poisson_fft_create": [_i, _i, _i, _d, _d, _d, _pvp],
+    "aeo_poisson_column_create": [_i, _i, _i, _d, _d, _i, _i, _pd, _pd, _pd, _pd, _pd, _pd, _pvp],
+    "aeo_poisson_source_buffer": [_vp, _pvp],
+    "aeo_poisson_solve_source_buffer": [_vp, _vp],
+    "aeo_poisson_solve": [_vp, _i, _vp, _vp, _i],
+    "aeo_poisson_stats": [_vp, _pi, _pd, _pi],
+    "aeo_poisson_destroy": [_vp],
+    "aeo_stepper_create": [_vp, ctypes.POINTER(StepParams), _vp, _pvp],
+    "aeo_stepper_destroy": [_vp],
+    "aeo_step_rk3": [_vp, _vp, _vp, _vp, _vp, _vp, _d],
+    "aeo_step_euler": [_vp, _vp, _vp, _vp, _vp, _vp, _d],
+    "aeo_max_divergence": [_vp, _i, _vp, _vp, _vp, _pd],
+    "aeo_max_advective_rate": [_vp, _i, _vp, _vp, _vp, _pd],
+    "aeo_profile_enable": [_i],
+    "aeo_profile_reset": [],
+    "aeo_profile_get": [ctypes.c_char_p, _pi, _pd],
+}
+
+_lib = None
+_lock = threading.Lock()
+_initialised = False
+
+
+def load():
+    """The loaded library (raises AeolusError when it is not built)."""
+    global _lib
+    with _lock:
+        if _lib is None:
+            if not os.path.exists(LIB_PATH):
+                raise AeolusError(f"libaeolus_hip.so is not built ({LIB_PATH}); run aeolus.jl_amd/build.py")
+            lib = ctypes.CDLL(LIB_PATH)
+            for name, argt in SIGNATURES.items():
+                f = getattr(lib, name)
+                f.argtypes = argt
+                f.restype = ctypes.c_char_p if name == "aeo_last_error" else ctypes.c_int
+            _lib = lib
+    return _lib
+
+
+def check(rc):
+    if rc != 0:
+        msg = load().aeo_last_error()
+        raise AeolusError(msg.decode() if msg else f"libaeolus_hip error {rc}")
+
+
+def init(device=0):
+    """aeo_init once per process (the device's stream)."""
+    global _initialised
+    if not _initialised:
+        check(load().aeo_init(device))
+        _initialised = True
+    return load()
+
+
+def sync():
+    check(init().aeo_sync())
+
+
+def dptr(a):
+    """The device pointer of a DeviceArray (or None)."""
+    return None if a is None else ctypes.c_void_p(a.ptr)

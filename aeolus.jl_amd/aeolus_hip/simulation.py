@@ -1,0 +1,103 @@
+"""The Boussinesq dynamical core on the device: BoussinesqSimulation with
+step_rk3! and step! (src/Simulations.jl:168-288, 474-487, 843-904).
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib as L
+from .advection import scheme_id
+from .fields import BC_NAMES, Grid
+from .pressure import FFTPoissonSolver, solver_for
+
+
+def _bc(b):
+    return BC_NAMES[b] if isinstance(b, str) else b
+
+
+class BoussinesqSimulation:
+    """State u, v, w, theta, p on the device and one stepper.
+
+    bcs / theta_bcs are (x, y, z) kinds: "periodic", "noflux" or "wall" (z
+    only). The pressure solver follows the velocity BCs (_default_solver,
+    src/Simulations.jl:1137-1144): FFT when every axis is periodic, else the
+    column solver with the grid's z metrics.
+    """
+
+    def __init__(self, grid: Grid, scheme="centered", weno_p=2, nu=0.0, theta0=0.0, g_accel=9.81,
+                 flux_bot=0.0, flux_top=0.0, bcs=("periodic",) * 3, theta_bcs=None, precision=L.F64,
+                 solver=None):
+        self.grid = grid
+        self.precision = precision
+        self.bcs = tuple(bcs)
+        self.theta_bcs = tuple(theta_bcs) if theta_bcs is not None else self.bcs
+        self.u = grid.field(precision)
+        self.v = grid.field(precision)
+        self.w = grid.field(precision)
+        self.theta = grid.field(precision)
+        self.p = grid.field(precision)
+        if solver is None:
+            if all(_bc(b) == L.PERIODIC for b in self.bcs):
+                solver = FFTPoissonSolver(grid)
+            else:
+                solver = solver_for(grid)
+        self.solver = solver
+        bx, by, bz = (_bc(b) for b in self.bcs)
+        tx, ty, tz = (_bc(b) for b in self.theta_bcs)
+        sid = scheme_id(scheme)
+        self.params = L.StepParams(sid, sid, int(weno_p), float(nu), float(theta0), float(g_accel),
+                                   float(flux_bot), float(flux_top), bx, by, bz, tx, ty, tz, precision)
+        h = ctypes.c_void_p()
+        L.check(L.init().aeo_stepper_create(grid.handle, ctypes.byref(self.params), solver.handle, ctypes.byref(h)))
+        self._h = h.value
+        self.t = 0.0
+
+    # ---- state ----
+    def set(self, u=None, v=None, w=None, theta=None):
+        for dev, a in ((self.u, u), (self.v, v), (self.w, w), (self.theta, theta)):
+            if a is not None:
+                dev.upload(a)
+
+    def state(self):
+        return {n: getattr(self, n).download() for n in ("u", "v", "w", "theta", "p")}
+
+    # ---- steps ----
+    def step_rk3(self, dt):
+        L.check(L.init().aeo_step_rk3(ctypes.c_void_p(self._h), L.dptr(self.u), L.dptr(self.v), L.dptr(self.w),
+                                      L.dptr(self.theta), L.dptr(self.p), float(dt)))
+        self.t += dt
+
+    def step(self, dt):
+        L.check(L.init().aeo_step_euler(ctypes.c_void_p(self._h), L.dptr(self.u), L.dptr(self.v),
+                                        L.dptr(self.w), L.dptr(self.theta), L.dptr(self.p), float(dt)))
+        self.t += dt
+
+    # ---- diagnostics ----
+    def fill_velocity_halos(self):
+        from .fields import fill_halos
+        fill_halos(self.grid, self.u, L.FACE_X, self.bcs)
+        fill_halos(self.grid, self.v, L.FACE_Y, self.bcs)
+        fill_halos(self.grid, self.w, L.FACE_Z, self.bcs)
+
+    def max_divergence(self):
+        """max_divergence(sim) (src/Simulations.jl:1045-1050)."""
+        self.fill_velocity_halos()
+        out = ctypes.c_double()
+        L.check(L.init().aeo_max_divergence(self.grid.handle, self.precision, L.dptr(self.u), L.dptr(self.v),
+                                            L.dptr(self.w), ctypes.byref(out)))
+        return out.value
+
+    def advective_dt(self, C=0.5):
+        """dt = C / max Σ|uᵢ|/Δxᵢ (CFLConstraints C_advective, src/TimeIntegration.jl:127-182)."""
+        self.fill_velocity_halos()
+        out = ctypes.c_double()
+        L.check(L.init().aeo_max_advective_rate(self.grid.handle, self.precision, L.dptr(self.u), L.dptr(self.v),
+                                                L.dptr(self.w), ctypes.byref(out)))
+        return C / out.value if out.value > 0 else np.inf
+
+    def __del__(self):
+        try:
+            if self._h and L._lib is not None and L._initialised:
+                L._lib.aeo_stepper_destroy(ctypes.c_void_p(self._h))
+        except Exception:
+            pass

@@ -1,0 +1,709 @@
+// The C ABI of libaeolus_hip (declared in include/aeolus_hip.h).
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/aeolus_hip.h"
+#include "kernels.h"
+
+using namespace aeo;
+
+namespace {
+
+thread_local char g_err[1024] = "";
+
+int fail(int code, const char *fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof g_err, fmt, ap);
+    va_end(ap);
+    return code;
+}
+
+#define HIPCHK(x)                                                                              \
+    do {                                                                                       \
+        hipError_t e_ = (x);                                                                   \
+        if (e_ != hipSuccess) return fail(AEO_ERR_HIP, "%s: %s", #x, hipGetErrorString(e_));   \
+    } while (0)
+
+struct Context {
+    int device = -1;
+    hipStream_t stream = nullptr;
+    bool profile = false;
+    struct Rec {
+        std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;
+        int count = 0;
+        double ms = 0;
+    };
+    std::map<std::string, Rec> prof;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> pool;
+};
+Context *g_ctx = nullptr;
+
+// timing of one launch site: events recorded on the library stream
+struct Timed {
+    Context::Rec *rec = nullptr;
+    hipEvent_t b{}, e{};
+    Timed(const char *name) {
+        if (!g_ctx || !g_ctx->profile) return;
+        rec = &g_ctx->prof[name];
+        if (!g_ctx->pool.empty()) {
+            b = g_ctx->pool.back().first;
+            e = g_ctx->pool.back().second;
+            g_ctx->pool.pop_back();
+        } else {
+            hipEventCreate(&b);
+            hipEventCreate(&e);
+        }
+        hipEventRecord(b, g_ctx->stream);
+    }
+    ~Timed() {
+        if (!rec) return;
+        hipEventRecord(e, g_ctx->stream);
+        rec->ev.push_back({b, e});
+    }
+};
+
+void profile_collect() {
+    if (!g_ctx) return;
+    bool any = false;
+    for (auto &kv : g_ctx->prof) any |= !kv.second.ev.empty();
+    if (!any) return;
+    hipStreamSynchronize(g_ctx->stream);
+    for (auto &kv : g_ctx->prof) {
+        for (auto &p : kv.second.ev) {
+            float ms = 0;
+            hipEventElapsedTime(&ms, p.first, p.second);
+            kv.second.ms += ms;
+            kv.second.count++;
+            g_ctx->pool.push_back(p);
+        }
+        kv.second.ev.clear();
+    }
+}
+
+#define NEED_CTX() \
+    if (!g_ctx) return fail(AEO_ERR_STATE, "aeo_init has not been called")
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// grid
+// ---------------------------------------------------------------------------
+struct aeo_grid {
+    int Nx, Ny, Nz, H;
+    double dx, dy, dz;
+    int stretched;
+    std::vector<double> idzc, idzf, wlo, whi, alo, ahi;  // host, storage levels
+    double *d64[6] = {};                                 // device copies
+    float *d32[6] = {};
+    Metrics<double> m64;
+    Metrics<float> m32;
+};
+
+template <typename T> static const Metrics<T> &metrics(const aeo_grid *g);
+template <> const Metrics<double> &metrics<double>(const aeo_grid *g) { return g->m64; }
+template <> const Metrics<float> &metrics<float>(const aeo_grid *g) { return g->m32; }
+
+template <typename T>
+static void fill_metrics(Metrics<T> &m, const aeo_grid *g, T *const *dv) {
+    m.Nx = g->Nx; m.Ny = g->Ny; m.Nz = g->Nz; m.H = g->H;
+    m.sx = g->Nx + 2 * g->H; m.sy = g->Ny + 2 * g->H; m.sz = g->Nz + 2 * g->H;
+    m.idx = T(1 / g->dx); m.idy = T(1 / g->dy);
+    m.idx2 = T(1 / (g->dx * g->dx)); m.idy2 = T(1 / (g->dy * g->dy));
+    m.stretched = g->stretched;
+    m.idz = T(1 / g->dz); m.idz2 = T(1 / (g->dz * g->dz));
+    m.idzc = dv[0]; m.idzf = dv[1]; m.wlo = dv[2]; m.whi = dv[3]; m.alo = dv[4]; m.ahi = dv[5];
+}
+
+extern "C" int aeo_grid_create(int Nx, int Ny, int Nz, int H, double dx, double dy, double dz,
+                               const double *zf_interior, aeo_grid **out) {
+    NEED_CTX();
+    if (!out) return fail(AEO_ERR_ARG, "aeo_grid_create: null output handle");
+    if (Nx < 1 || Ny < 1 || Nz < 1) return fail(AEO_ERR_ARG, "size must be positive, got (%d,%d,%d)", Nx, Ny, Nz);
+    if (H < 1) return fail(AEO_ERR_ARG, "halo must be >= 1, got %d", H);
+    if (Nx < H || Ny < H || Nz <= H) return fail(AEO_ERR_ARG, "interior (%d,%d,%d) must exceed the halo %d", Nx, Ny, Nz, H);
+    if (!(dx > 0) || !(dy > 0)) return fail(AEO_ERR_ARG, "dx, dy must be positive");
+    auto g = std::make_unique<aeo_grid>();
+    g->Nx = Nx; g->Ny = Ny; g->Nz = Nz; g->H = H; g->dx = dx; g->dy = dy; g->dz = dz;
+    const int n = Nz + 2 * H;
+    g->idzc.assign(n, 0); g->idzf.assign(n + 1, 0);
+    g->wlo.assign(n, 0); g->whi.assign(n, 0); g->alo.assign(n, 0); g->ahi.assign(n, 0);
+    if (!zf_interior) {
+        if (!(dz > 0)) return fail(AEO_ERR_ARG, "dz must be positive");
+        g->stretched = 0;
+        for (int k = 0; k < n; k++) g->idzc[k] = 1 / dz;
+        for (int k = 0; k <= n; k++) g->idzf[k] = 1 / dz;
+    } else {
+        g->stretched = 1;
+        for (int k = 0; k < Nz; k++)
+            if (!(zf_interior[k + 1] > zf_interior[k]))
+                return fail(AEO_ERR_ARG, "custom face vector must be strictly increasing");
+        // _extend_faces, src/StretchedGrids.jl:319-337 (1-based k -> 0-based k-1)
+        std::vector<double> zf(n + 1), zc(n), dzc(n), dzf(n + 1);
+        for (int k = 1; k <= Nz + 1; k++) zf[H + k - 1] = zf_interior[k - 1];
+        for (int k = H; k >= 1; k--) zf[k - 1] = 2 * zf[k] - zf[k + 1];
+        for (int k = H + Nz + 2; k <= n + 1; k++) zf[k - 1] = 2 * zf[k - 2] - zf[k - 3];
+        // StretchedGrid constructor, :375-388
+        for (int k = 0; k < n; k++) dzc[k] = zf[k + 1] - zf[k];
+        for (int k = 0; k < n; k++) zc[k] = (zf[k] + zf[k + 1]) / 2;
+        dzf[0] = dzc[0];
+        for (int k = 1; k < n; k++) dzf[k] = zc[k] - zc[k - 1];
+        dzf[n] = dzc[n - 1];
+        // level_metrics, :570-600
+        for (int k = 0; k < n; k++) g->idzc[k] = 1 / dzc[k];
+        for (int k = 0; k <= n; k++) g->idzf[k] = 1 / dzf[k];
+        for (int k = 1; k < n; k++) {
+            g->wlo[k] = dzc[k] / (dzc[k - 1] + dzc[k]);
+            g->whi[k] = dzc[k - 1] / (dzc[k - 1] + dzc[k]);
+            double s = dzc[k - 1] + dzc[k];
+            g->alo[k] = dzc[k - 1] / s;
+            g->ahi[k] = dzc[k] / s;
+        }
+    }
+    const std::vector<double> *vs[6] = {&g->idzc, &g->idzf, &g->wlo, &g->whi, &g->alo, &g->ahi};
+    for (int q = 0; q < 6; q++) {
+        size_t len = vs[q]->size();
+        HIPCHK(hipMalloc(&g->d64[q], sizeof(double) * len));
+        HIPCHK(hipMemcpy(g->d64[q], vs[q]->data(), sizeof(double) * len, hipMemcpyHostToDevice));
+        std::vector<float> f(len);
+        for (size_t c = 0; c < len; c++) f[c] = (float)(*vs[q])[c];
+        HIPCHK(hipMalloc(&g->d32[q], sizeof(float) * len));
+        HIPCHK(hipMemcpy(g->d32[q], f.data(), sizeof(float) * len, hipMemcpyHostToDevice));
+    }
+    fill_metrics<double>(g->m64, g.get(), g->d64);
+    fill_metrics<float>(g->m32, g.get(), g->d32);
+    *out = g.release();
+    return AEO_OK;
+}
+
+extern "C" int aeo_grid_destroy(aeo_grid *g) {
+    if (!g) return AEO_OK;
+    for (int q = 0; q < 6; q++) {
+        hipFree(g->d64[q]);
+        hipFree(g->d32[q]);
+    }
+    delete g;
+    return AEO_OK;
+}
+
+extern "C" int aeo_grid_metrics(const aeo_grid *g, double *idzc, double *idzf) {
+    if (!g) return fail(AEO_ERR_ARG, "null grid");
+    if (idzc) memcpy(idzc, g->idzc.data(), sizeof(double) * g->idzc.size());
+    if (idzf) memcpy(idzf, g->idzf.data(), sizeof(double) * g->idzf.size());
+    return AEO_OK;
+}
+
+// ---------------------------------------------------------------------------
+// context and memory
+// ---------------------------------------------------------------------------
+extern "C" const char *aeo_last_error(void) { return g_err; }
+
+extern "C" int aeo_init(int device) {
+    if (g_ctx) {
+        if (g_ctx->device == device) return AEO_OK;
+        return fail(AEO_ERR_STATE, "already initialised on device %d", g_ctx->device);
+    }
+    int n = 0;
+    HIPCHK(hipGetDeviceCount(&n));
+    if (device < 0 || device >= n) return fail(AEO_ERR_ARG, "device %d out of range (%d devices)", device, n);
+    HIPCHK(hipSetDevice(device));
+    auto c = std::make_unique<Context>();
+    c->device = device;
+    HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    g_ctx = c.release();
+    return AEO_OK;
+}
+
+extern "C" int aeo_shutdown(void) {
+    if (!g_ctx) return AEO_OK;
+    hipStreamSynchronize(g_ctx->stream);
+    for (auto &kv : g_ctx->prof)
+        for (auto &p : kv.second.ev) g_ctx->pool.push_back(p);
+    for (auto &p : g_ctx->pool) {
+        hipEventDestroy(p.first);
+        hipEventDestroy(p.second);
+    }
+    hipStreamDestroy(g_ctx->stream);
+    delete g_ctx;
+    g_ctx = nullptr;
+    return AEO_OK;
+}
+
+extern "C" int aeo_sync(void) {
+    NEED_CTX();
+    HIPCHK(hipStreamSynchronize(g_ctx->stream));
+    HIPCHK(hipGetLastError());
+    return AEO_OK;
+}
+
+extern "C" int aeo_malloc(void **dptr, size_t bytes) {
+    NEED_CTX();
+    if (!dptr) return fail(AEO_ERR_ARG, "null pointer");
+    HIPCHK(hipMalloc(dptr, bytes ? bytes : 1));
+    return AEO_OK;
+}
+extern "C" int aeo_free(void *dptr) {
+    NEED_CTX();
+    HIPCHK(hipStreamSynchronize(g_ctx->stream));
+    HIPCHK(hipFree(dptr));
+    return AEO_OK;
+}
+extern "C" int aeo_memcpy_h2d(void *dst, const void *src, size_t bytes) {
+    NEED_CTX();
+    HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, g_ctx->stream));
+    HIPCHK(hipStreamSynchronize(g_ctx->stream));
+    return AEO_OK;
+}
+extern "C" int aeo_memcpy_d2h(void *dst, const void *src, size_t bytes) {
+    NEED_CTX();
+    HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, g_ctx->stream));
+    HIPCHK(hipStreamSynchronize(g_ctx->stream));
+    return AEO_OK;
+}
+extern "C" int aeo_memset(void *dptr, int value, size_t bytes) {
+    NEED_CTX();
+    HIPCHK(hipMemsetAsync(dptr, value, bytes, g_ctx->stream));
+    return AEO_OK;
+}
+
+// ---------------------------------------------------------------------------
+// halos and advection
+// ---------------------------------------------------------------------------
+static int check_bc(int bc) { return bc == AEO_PERIODIC || bc == AEO_NOFLUX || bc == AEO_WALL; }
+
+extern "C" int aeo_fill_halos(aeo_grid *g, int precision, void *field, int loc, int bc_x, int bc_y, int bc_z) {
+    NEED_CTX();
+    if (!g || !field) return fail(AEO_ERR_ARG, "null argument");
+    if (!check_bc(bc_x) || !check_bc(bc_y) || !check_bc(bc_z) || loc < 0 || loc > 3)
+        return fail(AEO_ERR_ARG, "bad location or boundary kind");
+    if (bc_x == AEO_WALL || bc_y == AEO_WALL)
+        return fail(AEO_ERR_UNSUPPORTED, "ImpermeableWall is a z boundary (the reference fills walls in z only)");
+    Timed t("halo");
+    if (precision == AEO_F64) launch_halo<double>(g_ctx->stream, (double *)field, g->Nx, g->Ny, g->Nz, g->H, loc, bc_x, bc_y, bc_z);
+    else if (precision == AEO_F32) launch_halo<float>(g_ctx->stream, (float *)field, g->Nx, g->Ny, g->Nz, g->H, loc, bc_x, bc_y, bc_z);
+    else return fail(AEO_ERR_ARG, "bad precision %d", precision);
+    HIPCHK(hipGetLastError());
+    return AEO_OK;
+}
+
+static int required_halo(int scheme) {
+    switch (scheme) {
+    case AEO_CENTERED: case AEO_UPWIND: return 1;
+    case AEO_WENO5: case AEO_WENO_JS5: case AEO_WENO_Z5: return 3;
+    case AEO_WENO_Z7: return 4;
+    }
+    return -1;
+}
+
+static int check_scheme(const aeo_grid *g, int scheme, int p) {
+    int h = required_halo(scheme);
+    if (h < 0) return fail(AEO_ERR_ARG, "unknown advection scheme %d", scheme);
+    if ((scheme == AEO_WENO_Z5 || scheme == AEO_WENO_Z7) && p != 1 && p != 2)
+        return fail(AEO_ERR_ARG, "WENO-Z τ-exponent p must be 1 or 2 (got %d)", p);
+    // _check_halo, src/AdvectionSchemes.jl:668-674; the momentum stencils read
+    // one more cell across for the transporting velocity
+    if (g->H < h) return fail(AEO_ERR_ARG, "scheme %d advection requires halo >= %d (got %d)", scheme, h, g->H);
+    if (g->H < 2) return fail(AEO_ERR_ARG, "this build needs halo >= 2");
+    return AEO_OK;
+}
+
+extern "C" int aeo_advect_scalar(aeo_grid *g, int precision, int scheme, int p, void *tend, const void *u,
+                                 const void *v, const void *w, const void *phi) {
+    NEED_CTX();
+    if (!g || !tend || !u || !v || !w || !phi) return fail(AEO_ERR_ARG, "null argument");
+    int rc = check_scheme(g, scheme, p);
+    if (rc) return rc;
+    Timed t("advect");
+    if (precision == AEO_F64)
+        rc = launch_advect<double>(g_ctx->stream, g->m64, 0, scheme, p, (const double *)u, (const double *)v,
+                                   (const double *)w, (const double *)phi, nullptr, nullptr, nullptr, (double *)tend);
+    else if (precision == AEO_F32)
+        rc = launch_advect<float>(g_ctx->stream, g->m32, 0, scheme, p, (const float *)u, (const float *)v,
+                                  (const float *)w, (const float *)phi, nullptr, nullptr, nullptr, (float *)tend);
+    else return fail(AEO_ERR_ARG, "bad precision %d", precision);
+    if (rc) return fail(AEO_ERR_ARG, "advection launch failed");
+    HIPCHK(hipGetLastError());
+    return AEO_OK;
+}
+
+extern "C" int aeo_advect_momentum(aeo_grid *g, int precision, int scheme, int p, void *tu, void *tv, void *tw,
+                                   const void *u, const void *v, const void *w) {
+    NEED_CTX();
+    if (!g || !tu || !tv || !tw || !u || !v || !w) return fail(AEO_ERR_ARG, "null argument");
+    int rc = check_scheme(g, scheme, p);
+    if (rc) return rc;
+    Timed t("advect");
+    if (precision == AEO_F64)
+        rc = launch_advect<double>(g_ctx->stream, g->m64, 1, scheme, p, (const double *)u, (const double *)v,
+                                   (const double *)w, nullptr, (double *)tu, (double *)tv, (double *)tw, nullptr);
+    else if (precision == AEO_F32)
+        rc = launch_advect<float>(g_ctx->stream, g->m32, 1, scheme, p, (const float *)u, (const float *)v,
+                                  (const float *)w, nullptr, (float *)tu, (float *)tv, (float *)tw, nullptr);
+    else return fail(AEO_ERR_ARG, "bad precision %d", precision);
+    if (rc) return fail(AEO_ERR_ARG, "advection launch failed");
+    HIPCHK(hipGetLastError());
+    return AEO_OK;
+}
+
+// ---------------------------------------------------------------------------
+// pressure solvers
+// ---------------------------------------------------------------------------
+struct aeo_poisson {
+    int kind = 0;  // 0 fft, 1 column
+    int Nx, Ny, Nz;
+    bool periodic_x, periodic_y, periodic_z;
+    SpectralPlan *sp = nullptr;
+    double *src = nullptr, *phi = nullptr;  // interior-sized FP64
+    int iters = 0, status = AEO_DIRECT;
+    double resid = NAN;
+};
+
+static int poisson_alloc(aeo_poisson *h) {
+    size_t n = (size_t)h->Nx * h->Ny * h->Nz;
+    HIPCHK(hipMalloc(&h->src, sizeof(double) * n));
+    HIPCHK(hipMalloc(&h->phi, sizeof(double) * n));
+    HIPCHK(hipMemsetAsync(h->src, 0, sizeof(double) * n, g_ctx->stream));
+    HIPCHK(hipMemsetAsync(h->phi, 0, sizeof(double) * n, g_ctx->stream));
+    return AEO_OK;
+}
+
+extern "C" int aeo_poisson_fft_create(int Nx, int Ny, int Nz, double dx, double dy, double dz, aeo_poisson **out) {
+    NEED_CTX();
+    if (!out) return fail(AEO_ERR_ARG, "null output handle");
+    auto h = std::make_unique<aeo_poisson>();
+    h->kind = 0; h->Nx = Nx; h->Ny = Ny; h->Nz = Nz;
+    h->periodic_x = h->periodic_y = h->periodic_z = true;
+    char err[512] = "";
+    h->sp = spectral_create(Nx, Ny, Nz, g_ctx->stream, err, sizeof err);
+    if (!h->sp) return fail(AEO_ERR_UNSUPPORTED, "%s", err);
+    int rc = spectral_fft_setup(h->sp, dx, dy, dz);
+    if (rc == -2) { spectral_destroy(h->sp); return fail(AEO_ERR_UNSUPPORTED, "FFT solver: Nz (%d) must be a power of two <= 512", Nz); }
+    if (rc) { spectral_destroy(h->sp); return fail(AEO_ERR_HIP, "FFT solver setup failed"); }
+    rc = poisson_alloc(h.get());
+    if (rc) return rc;
+    *out = h.release();
+    return AEO_OK;
+}
+
+extern "C" int aeo_poisson_column_create(int Nx, int Ny, int Nz, double dx, double dy, int bc_x, int bc_y,
+                                         const double *lower, const double *upper, const double *hweight,
+                                         const double *measure, const double *inv_coupling, const double *gauge,
+                                         aeo_poisson **out) {
+    NEED_CTX();
+    if (!out || !lower || !upper || !hweight || !measure || !inv_coupling || !gauge)
+        return fail(AEO_ERR_ARG, "null argument");
+    // src/PressureSolvers.jl:631-636
+    if (!(lower[0] == 0 && upper[Nz - 1] == 0))
+        return fail(AEO_ERR_ARG, "the bottom and top faces carry no flux: lower[1] = upper[Nz] = 0");
+    if (bc_x != AEO_PERIODIC || bc_y != AEO_PERIODIC)
+        return fail(AEO_ERR_UNSUPPORTED, "column solver: only periodic lateral boundaries in this build");
+    auto h = std::make_unique<aeo_poisson>();
+    h->kind = 1; h->Nx = Nx; h->Ny = Ny; h->Nz = Nz;
+    h->periodic_x = h->periodic_y = true;
+    h->periodic_z = false;
+    char err[512] = "";
+    h->sp = spectral_create(Nx, Ny, Nz, g_ctx->stream, err, sizeof err);
+    if (!h->sp) return fail(AEO_ERR_UNSUPPORTED, "%s", err);
+    if (spectral_column_setup(h->sp, dx, dy, lower, upper, hweight, measure, inv_coupling, gauge)) {
+        spectral_destroy(h->sp);
+        return fail(AEO_ERR_HIP, "column solver setup failed");
+    }
+    int rc = poisson_alloc(h.get());
+    if (rc) return rc;
+    *out = h.release();
+    return AEO_OK;
+}
+
+extern "C" int aeo_poisson_source_buffer(aeo_poisson *h, double **dptr) {
+    if (!h || !dptr) return fail(AEO_ERR_ARG, "null argument");
+    *dptr = h->src;
+    return AEO_OK;
+}
+
+static int poisson_run(aeo_poisson *h, double *phi) {
+    Timed t("poisson");
+    int rc = h->kind == 0 ? spectral_fft_solve(h->sp, g_ctx->stream, phi, h->src)
+                          : spectral_column_solve(h->sp, g_ctx->stream, phi, h->src);
+    if (rc) return fail(AEO_ERR_STATE, "solver not set up");
+    HIPCHK(hipGetLastError());
+    h->iters = 0;
+    h->resid = NAN;
+    h->status = AEO_DIRECT;
+    return AEO_OK;
+}
+
+extern "C" int aeo_poisson_solve_source_buffer(aeo_poisson *h, double *phi) {
+    NEED_CTX();
+    if (!h || !phi) return fail(AEO_ERR_ARG, "null argument");
+    return poisson_run(h, phi);
+}
+
+template <typename T>
+__global__ void copy_in_kernel(double *__restrict__ buf, const T *__restrict__ f, int Nx, int Ny, int Nz, int H) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y, k = blockIdx.z;
+    if (i >= Nx) return;
+    buf[(size_t)i + (size_t)Nx * (j + (size_t)Ny * k)] = (double)f[lin(i + H, j + H, k + H, Nx + 2 * H, Ny + 2 * H)];
+}
+template <typename T>
+__global__ void copy_out_kernel(T *__restrict__ f, const double *__restrict__ buf, int Nx, int Ny, int Nz, int H) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y, k = blockIdx.z;
+    if (i >= Nx) return;
+    f[lin(i + H, j + H, k + H, Nx + 2 * H, Ny + 2 * H)] = (T)buf[(size_t)i + (size_t)Nx * (j + (size_t)Ny * k)];
+}
+
+extern "C" int aeo_poisson_solve(aeo_poisson *h, int precision, void *p, const void *source, int H) {
+    NEED_CTX();
+    if (!h || !p || !source || H < 0) return fail(AEO_ERR_ARG, "null argument");
+    dim3 grid((h->Nx + 255) / 256, h->Ny, h->Nz);
+    if (precision == AEO_F64) hipLaunchKernelGGL(copy_in_kernel<double>, grid, dim3(256), 0, g_ctx->stream, h->src, (const double *)source, h->Nx, h->Ny, h->Nz, H);
+    else if (precision == AEO_F32) hipLaunchKernelGGL(copy_in_kernel<float>, grid, dim3(256), 0, g_ctx->stream, h->src, (const float *)source, h->Nx, h->Ny, h->Nz, H);
+    else return fail(AEO_ERR_ARG, "bad precision %d", precision);
+    int rc = poisson_run(h, h->phi);
+    if (rc) return rc;
+    if (precision == AEO_F64) hipLaunchKernelGGL(copy_out_kernel<double>, grid, dim3(256), 0, g_ctx->stream, (double *)p, h->phi, h->Nx, h->Ny, h->Nz, H);
+    else hipLaunchKernelGGL(copy_out_kernel<float>, grid, dim3(256), 0, g_ctx->stream, (float *)p, h->phi, h->Nx, h->Ny, h->Nz, H);
+    HIPCHK(hipGetLastError());
+    return AEO_OK;
+}
+
+extern "C" int aeo_poisson_stats(aeo_poisson *h, int *iterations, double *residual, int *status) {
+    if (!h) return fail(AEO_ERR_ARG, "null argument");
+    if (iterations) *iterations = h->iters;
+    if (residual) *residual = h->resid;
+    if (status) *status = h->status;
+    return AEO_OK;
+}
+
+extern "C" int aeo_poisson_destroy(aeo_poisson *h) {
+    if (!h) return AEO_OK;
+    if (g_ctx) hipStreamSynchronize(g_ctx->stream);
+    spectral_destroy(h->sp);
+    hipFree(h->src);
+    hipFree(h->phi);
+    delete h;
+    return AEO_OK;
+}
+
+// ---------------------------------------------------------------------------
+// the stepper
+// ---------------------------------------------------------------------------
+struct aeo_stepper {
+    aeo_grid *g;
+    aeo_step_params sp;
+    aeo_poisson *solver;
+    size_t n;     // storage points
+    size_t esz;   // element size
+    void *us, *vs, *ws;  // predictor u*
+    void *ua, *va, *wa;  // the stage state
+    void *ta, *tb;       // θ stage buffers
+};
+
+extern "C" int aeo_stepper_create(aeo_grid *g, const aeo_step_params *sp, aeo_poisson *solver, aeo_stepper **out) {
+    NEED_CTX();
+    if (!g || !sp || !solver || !out) return fail(AEO_ERR_ARG, "null argument");
+    if (sp->precision != AEO_F64 && sp->precision != AEO_F32) return fail(AEO_ERR_ARG, "bad precision");
+    int rc = check_scheme(g, sp->mom_scheme, sp->weno_p);
+    if (rc) return rc;
+    rc = check_scheme(g, sp->scal_scheme, sp->weno_p);
+    if (rc) return rc;
+    if (sp->mom_scheme != sp->scal_scheme)
+        return fail(AEO_ERR_UNSUPPORTED, "this build advects momentum and θ with one scheme");
+    if (solver->Nx != g->Nx || solver->Ny != g->Ny || solver->Nz != g->Nz)
+        return fail(AEO_ERR_ARG, "solver size (%d,%d,%d) != grid interior (%d,%d,%d)", solver->Nx, solver->Ny,
+                    solver->Nz, g->Nx, g->Ny, g->Nz);
+    int bcs[6] = {sp->bc_x, sp->bc_y, sp->bc_z, sp->tbc_x, sp->tbc_y, sp->tbc_z};
+    for (int q = 0; q < 6; q++)
+        if (!check_bc(bcs[q])) return fail(AEO_ERR_ARG, "bad boundary kind");
+    if (sp->bc_x == AEO_WALL || sp->bc_y == AEO_WALL || sp->tbc_x == AEO_WALL || sp->tbc_y == AEO_WALL)
+        return fail(AEO_ERR_UNSUPPORTED, "walls are z boundaries");
+    // the solver's axes must be the velocity BCs' Poisson kinds (poisson_axis_bcs, :347-354)
+    if ((sp->bc_x == AEO_PERIODIC) != solver->periodic_x || (sp->bc_y == AEO_PERIODIC) != solver->periodic_y ||
+        (sp->bc_z == AEO_PERIODIC) != solver->periodic_z)
+        return fail(AEO_ERR_ARG, "the pressure solver's axis kinds do not match the velocity boundary conditions");
+    auto h = std::make_unique<aeo_stepper>();
+    h->g = g; h->sp = *sp; h->solver = solver;
+    h->n = (size_t)(g->Nx + 2 * g->H) * (g->Ny + 2 * g->H) * (g->Nz + 2 * g->H);
+    h->esz = sp->precision == AEO_F64 ? 8 : 4;
+    void **bufs[8] = {&h->us, &h->vs, &h->ws, &h->ua, &h->va, &h->wa, &h->ta, &h->tb};
+    for (int q = 0; q < 8; q++) {
+        HIPCHK(hipMalloc(bufs[q], h->n * h->esz));
+        HIPCHK(hipMemsetAsync(*bufs[q], 0, h->n * h->esz, g_ctx->stream));
+    }
+    *out = h.release();
+    return AEO_OK;
+}
+
+extern "C" int aeo_stepper_destroy(aeo_stepper *h) {
+    if (!h) return AEO_OK;
+    if (g_ctx) hipStreamSynchronize(g_ctx->stream);
+    void *bufs[8] = {h->us, h->vs, h->ws, h->ua, h->va, h->wa, h->ta, h->tb};
+    for (auto b : bufs) hipFree(b);
+    delete h;
+    return AEO_OK;
+}
+
+template <typename T>
+__global__ void copy_interior_kernel(T *__restrict__ dst, const T *__restrict__ src, int Nx, int Ny, int Nz, int H) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y, k = blockIdx.z;
+    if (i >= Nx) return;
+    size_t c = lin(i + H, j + H, k + H, Nx + 2 * H, Ny + 2 * H);
+    dst[c] = src[c];
+}
+
+// One projected forward-Euler substage (_euler_projection!, src/Simulations.jl:
+// 531-650) in the buffers of the RK3 driver; stage 0 is step!'s plain Euler.
+template <typename T>
+static int substage(aeo_stepper *h, int stage, T *u, T *v, T *w, T *th, T *p, double dt) {
+    aeo_grid *g = h->g;
+    const aeo_step_params &sp = h->sp;
+    hipStream_t st = g_ctx->stream;
+    const int Nx = g->Nx, Ny = g->Ny, Nz = g->Nz, H = g->H;
+    T *us = (T *)h->us, *vs = (T *)h->vs, *ws = (T *)h->ws;
+    T *ua = (T *)h->ua, *va = (T *)h->va, *wa = (T *)h->wa;
+    T *ta = (T *)h->ta, *tb = (T *)h->tb;
+    // inputs and outputs of this substage (the step-start state is the caller's)
+    T *ui = stage <= 1 ? u : ua, *vi = stage <= 1 ? v : va, *wi = stage <= 1 ? w : wa;
+    T *ti = stage <= 1 ? th : (stage == 2 ? ta : tb);
+    T *to = stage == 0 ? ta : (stage == 1 ? ta : (stage == 2 ? tb : th));
+    T *uo = (stage == 0 || stage == 3) ? u : ua, *vo = (stage == 0 || stage == 3) ? v : va,
+      *wo = (stage == 0 || stage == 3) ? w : wa;
+    {
+        Timed t("halo");
+        launch_halo<T>(st, ui, Nx, Ny, Nz, H, LOC_FX, sp.bc_x, sp.bc_y, sp.bc_z);
+        launch_halo<T>(st, vi, Nx, Ny, Nz, H, LOC_FY, sp.bc_x, sp.bc_y, sp.bc_z);
+        launch_halo<T>(st, wi, Nx, Ny, Nz, H, LOC_FZ, sp.bc_x, sp.bc_y, sp.bc_z);
+        launch_halo<T>(st, ti, Nx, Ny, Nz, H, LOC_CC, sp.tbc_x, sp.tbc_y, sp.tbc_z);
+    }
+    StepPhys ph{sp.nu, dt, sp.theta0, sp.g_accel, sp.flux_bot, sp.flux_top, g->idzc[H], g->idzc[H + Nz - 1]};
+    {
+        Timed t("tendency");
+        if (launch_tendency_step<T>(st, metrics<T>(g), sp.mom_scheme, sp.weno_p, ui, vi, wi, ti, us, vs, ws, to, th,
+                                    ph, stage))
+            return fail(AEO_ERR_ARG, "tendency launch failed");
+    }
+    {
+        Timed t("halo");
+        launch_halo<T>(st, us, Nx, Ny, Nz, H, LOC_FX, sp.bc_x, sp.bc_y, sp.bc_z);
+        launch_halo<T>(st, vs, Nx, Ny, Nz, H, LOC_FY, sp.bc_x, sp.bc_y, sp.bc_z);
+        launch_halo<T>(st, ws, Nx, Ny, Nz, H, LOC_FZ, sp.bc_x, sp.bc_y, sp.bc_z);
+    }
+    {
+        Timed t("divergence");
+        launch_div_into<T>(st, h->solver->src, us, vs, ws, g->m64);
+    }
+    int rc = poisson_run(h->solver, h->solver->phi);
+    if (rc) return rc;
+    double ca, cb;
+    int combine;
+    stage_coeffs<double>(stage, &ph, &ca, &cb, &combine);
+    {
+        Timed t("correct");
+        // φ's halo: the velocity BCs at a cell centre (a wall mirrors)
+        int bx = sp.bc_x == AEO_PERIODIC ? BC_PERIODIC : BC_NOFLUX;
+        int by = sp.bc_y == AEO_PERIODIC ? BC_PERIODIC : BC_NOFLUX;
+        int bz = sp.bc_z == AEO_PERIODIC ? BC_PERIODIC : BC_NOFLUX;
+        launch_correct<T>(st, uo, vo, wo, us, vs, ws, u, v, w, h->solver->phi, p, g->m64, bx, by, bz, ca, cb, combine, dt);
+    }
+    if (stage == 3) {
+        // the halo shell of the combined state (_shell_map!, src/Simulations.jl:741, 693)
+        Timed t("shell");
+        launch_shell<T>(st, u, us, u, Nx, Ny, Nz, H, ca, cb);
+        launch_shell<T>(st, v, vs, v, Nx, Ny, Nz, H, ca, cb);
+        launch_shell<T>(st, w, ws, w, Nx, Ny, Nz, H, ca, cb);
+        launch_shell<T>(st, th, ti, th, Nx, Ny, Nz, H, ca, cb);
+    }
+    if (stage == 0) {
+        // step!: u's halo holds u*'s fill; θ advanced in place over the interior
+        Timed t("shell");
+        launch_shell<T>(st, u, us, u, Nx, Ny, Nz, H, 0.0, 1.0);
+        launch_shell<T>(st, v, vs, v, Nx, Ny, Nz, H, 0.0, 1.0);
+        launch_shell<T>(st, w, ws, w, Nx, Ny, Nz, H, 0.0, 1.0);
+        dim3 grid((Nx + 255) / 256, Ny, Nz);
+        hipLaunchKernelGGL(copy_interior_kernel<T>, grid, dim3(256), 0, st, th, ta, Nx, Ny, Nz, H);
+    }
+    HIPCHK(hipGetLastError());
+    return AEO_OK;
+}
+
+template <typename T>
+static int run_step(aeo_stepper *h, void *u, void *v, void *w, void *th, void *p, double dt, bool rk3) {
+    if (!rk3) return substage<T>(h, 0, (T *)u, (T *)v, (T *)w, (T *)th, (T *)p, dt);
+    for (int s = 1; s <= 3; s++) {
+        int rc = substage<T>(h, s, (T *)u, (T *)v, (T *)w, (T *)th, (T *)p, dt);
+        if (rc) return rc;
+    }
+    return AEO_OK;
+}
+
+extern "C" int aeo_step_rk3(aeo_stepper *h, void *u, void *v, void *w, void *theta, void *p, double dt) {
+    NEED_CTX();
+    if (!h || !u || !v || !w || !theta) return fail(AEO_ERR_ARG, "null argument");
+    if (!(dt > 0) || !std::isfinite(dt)) return fail(AEO_ERR_ARG, "dt must be positive and finite");
+    return h->sp.precision == AEO_F64 ? run_step<double>(h, u, v, w, theta, p, dt, true)
+                                      : run_step<float>(h, u, v, w, theta, p, dt, true);
+}
+
+extern "C" int aeo_step_euler(aeo_stepper *h, void *u, void *v, void *w, void *theta, void *p, double dt) {
+    NEED_CTX();
+    if (!h || !u || !v || !w || !theta) return fail(AEO_ERR_ARG, "null argument");
+    if (!(dt > 0) || !std::isfinite(dt)) return fail(AEO_ERR_ARG, "dt must be positive and finite");
+    return h->sp.precision == AEO_F64 ? run_step<double>(h, u, v, w, theta, p, dt, false)
+                                      : run_step<float>(h, u, v, w, theta, p, dt, false);
+}
+
+// ---------------------------------------------------------------------------
+// diagnostics and profiling
+// ---------------------------------------------------------------------------
+static int run_max(aeo_grid *g, int precision, const void *u, const void *v, const void *w, int what, double *out) {
+    NEED_CTX();
+    if (!g || !u || !v || !w || !out) return fail(AEO_ERR_ARG, "null argument");
+    unsigned long long *d = nullptr;
+    HIPCHK(hipMalloc(&d, sizeof(unsigned long long)));
+    HIPCHK(hipMemsetAsync(d, 0, sizeof(unsigned long long), g_ctx->stream));
+    if (precision == AEO_F64) launch_max<double>(g_ctx->stream, d, (const double *)u, (const double *)v, (const double *)w, g->m64, what);
+    else launch_max<float>(g_ctx->stream, d, (const float *)u, (const float *)v, (const float *)w, g->m64, what);
+    unsigned long long bits = 0;
+    HIPCHK(hipMemcpyAsync(&bits, d, sizeof bits, hipMemcpyDeviceToHost, g_ctx->stream));
+    HIPCHK(hipStreamSynchronize(g_ctx->stream));
+    HIPCHK(hipFree(d));
+    memcpy(out, &bits, sizeof(double));
+    return AEO_OK;
+}
+
+extern "C" int aeo_max_divergence(aeo_grid *g, int precision, const void *u, const void *v, const void *w, double *out) {
+    return run_max(g, precision, u, v, w, 0, out);
+}
+extern "C" int aeo_max_advective_rate(aeo_grid *g, int precision, const void *u, const void *v, const void *w,
+                                      double *out) {
+    return run_max(g, precision, u, v, w, 1, out);
+}
+
+extern "C" int aeo_profile_enable(int on) {
+    NEED_CTX();
+    g_ctx->profile = on != 0;
+    return AEO_OK;
+}
+extern "C" int aeo_profile_reset(void) {
+    NEED_CTX();
+    profile_collect();
+    for (auto &kv : g_ctx->prof) {
+        kv.second.count = 0;
+        kv.second.ms = 0;
+    }
+    return AEO_OK;
+}
+extern "C" int aeo_profile_get(const char *kernel, int *count, double *total_ms) {
+    NEED_CTX();
+    if (!kernel) return fail(AEO_ERR_ARG, "null kernel name");
+    profile_collect();
+    auto it = g_ctx->prof.find(kernel);
+    if (count) *count = it == g_ctx->prof.end() ? 0 : it->second.count;
+    if (total_ms) *total_ms = it == g_ctx->prof.end() ? 0 : it->second.ms;
+    return AEO_OK;
+}

@@ -1,0 +1,193 @@
+// Shared definitions of the gfx950 kernels of libaeolus_hip.
+//
+// Storage layout: the reference's (src/Grids.jl:98-140), a field is a
+// column-major (Nx+2H) x (Ny+2H) x (Nz+2H) array, x fastest. Device code
+// uses 0-based storage indices; the interior is H .. H+N-1 on every axis.
+// Metric vectors are indexed by the 0-based storage level (the reference's
+// 1-based level k is entry k-1 here).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cfloat>
+#include <cstdint>
+#include <cstddef>
+
+namespace aeo {
+
+enum Loc { LOC_CC = 0, LOC_FX = 1, LOC_FY = 2, LOC_FZ = 3 };
+enum Bc { BC_PERIODIC = 0, BC_NOFLUX = 1, BC_WALL = 2 };
+enum Scheme {
+    SCH_CENTERED = 0,  // Centered2 (advect_centered! / momentum_advect!)
+    SCH_UPWIND1 = 1,   // _UpwindFirst
+    SCH_WENO5 = 2,     // FiniteVolume._weno5_left (classic WENO-5, :weno5)
+    SCH_WENO_JS5 = 3,  // weno_js5_left
+    SCH_WENO_Z5 = 4,   // _weno_z5_left
+    SCH_WENO_Z7 = 5    // _weno_z7_left
+};
+
+template <typename T> struct Eps;
+template <> struct Eps<double> { static constexpr double v = DBL_EPSILON; };
+template <> struct Eps<float> { static constexpr float v = FLT_EPSILON; };
+
+// Grid metrics of one precision (src/FiniteVolume.jl:36-186).
+template <typename T>
+struct Metrics {
+    int Nx, Ny, Nz, H;
+    int sx, sy, sz;  // storage extents
+    T idx, idy, idx2, idy2;
+    int stretched;
+    T idz, idz2;  // UniformZ
+    const T *idzc, *idzf, *wlo, *whi, *alo, *ahi;  // LevelZ (device)
+    __device__ __forceinline__ T m_idz(int k) const { return stretched ? idzc[k] : idz; }
+    __device__ __forceinline__ T m_idzf(int k) const { return stretched ? idzf[k] : idz; }
+    __device__ __forceinline__ T m_zface(int k, T lo, T hi) const {
+        return stretched ? wlo[k] * lo + whi[k] * hi : T(0.5) * (lo + hi);
+    }
+    __device__ __forceinline__ T m_zflux(int k, T lo, T hi) const {
+        return stretched ? alo[k] * lo + ahi[k] * hi : T(0.5) * (lo + hi);
+    }
+    // the vertical term of the conservative Laplacian, _lapz (src/FiniteVolume.jl:166,177-181)
+    __device__ __forceinline__ T m_lapz(int loc, int k, T fm, T c, T fp) const {
+        if (!stretched) return ((fp - T(2) * c) + fm) * idz2;
+        if (loc == LOC_FZ) return ((fp - c) * idzc[k] - (c - fm) * idzc[k - 1]) * idzf[k];
+        return ((fp - c) * idzf[k + 1] - (c - fm) * idzf[k]) * idzc[k];
+    }
+};
+
+__device__ __forceinline__ size_t lin(int i, int j, int k, int sx, int sy) {
+    return (size_t)i + (size_t)sx * ((size_t)j + (size_t)sy * (size_t)k);
+}
+
+// ---------------------------------------------------------------------------
+// Reconstructions, in the reference's evaluation order (no FMA contraction:
+// the library is built with -ffp-contract=off).
+// ---------------------------------------------------------------------------
+template <typename T> __device__ __forceinline__ T beta3(T a1, T a2) { return a1 * a1 + T(13.0 / 3.0) * (a2 * a2); }
+template <typename T> __device__ __forceinline__ T beta4(T a1, T a2, T a3) {
+    return ((a1 * a1 + T(1.0 / 2.0) * (a1 * a3)) + T(13.0 / 3.0) * (a2 * a2)) + T(3129.0 / 80.0) * (a3 * a3);
+}
+template <typename T> __device__ __forceinline__ T b3_m2(T v1, T v2, T v3) {
+    return beta3<T>(((v1 - T(4) * v2) + T(3) * v3) / T(2), ((v1 - T(2) * v2) + v3) / T(2));
+}
+template <typename T> __device__ __forceinline__ T b3_m1(T v1, T v2, T v3) {
+    return beta3<T>((v3 - v1) / T(2), ((v1 - T(2) * v2) + v3) / T(2));
+}
+template <typename T> __device__ __forceinline__ T b3_p0(T v1, T v2, T v3) {
+    return beta3<T>(((T(-3) * v1 + T(4) * v2) - v3) / T(2), ((v1 - T(2) * v2) + v3) / T(2));
+}
+template <typename T> __device__ __forceinline__ T b4_m3(T v1, T v2, T v3, T v4) {
+    return beta4<T>((((T(-7) * v1 + T(33) * v2) - T(69) * v3) + T(43) * v4) / T(24),
+                    (((-v1 + T(4) * v2) - T(5) * v3) + T(2) * v4) / T(2), (((-v1 + T(3) * v2) - T(3) * v3) + v4) / T(6));
+}
+template <typename T> __device__ __forceinline__ T b4_p0(T v1, T v2, T v3, T v4) {
+    return beta4<T>((((T(-43) * v1 + T(69) * v2) - T(33) * v3) + T(7) * v4) / T(24),
+                    (((T(2) * v1 - T(5) * v2) + T(4) * v3) - v4) / T(2), (((-v1 + T(3) * v2) - T(3) * v3) + v4) / T(6));
+}
+template <typename T> __device__ __forceinline__ T b4_m2(T v1, T v2, T v3, T v4) {
+    return beta4<T>((((T(5) * v1 - T(27) * v2) + T(15) * v3) + T(7) * v4) / T(24), ((v2 - T(2) * v3) + v4) / T(2),
+                    (((-v1 + T(3) * v2) - T(3) * v3) + v4) / T(6));
+}
+template <typename T> __device__ __forceinline__ T b4_m1(T v1, T v2, T v3, T v4) {
+    return beta4<T>((((T(-7) * v1 - T(15) * v2) + T(27) * v3) - T(5) * v4) / T(24), ((v1 - T(2) * v2) + v3) / T(2),
+                    (((-v1 + T(3) * v2) - T(3) * v3) + v4) / T(6));
+}
+template <int P, typename T> __device__ __forceinline__ T ipow(T x) {
+    if constexpr (P == 2) return x * x;
+    else return x;
+}
+
+// _weno_z5_left, src/AdvectionSchemes.jl:240-255
+template <int P, typename T>
+__device__ __forceinline__ T weno_z5(T v1, T v2, T v3, T v4, T v5) {
+    T q0 = ((T(2) * v1 - T(7) * v2) + T(11) * v3) / T(6);
+    T q1 = ((-v2 + T(5) * v3) + T(2) * v4) / T(6);
+    T q2 = ((T(2) * v3 + T(5) * v4) - v5) / T(6);
+    T b0 = b3_m2<T>(v1, v2, v3), b1 = b3_m1<T>(v2, v3, v4), b2 = b3_p0<T>(v3, v4, v5);
+    const T eps = Eps<T>::v * Eps<T>::v;
+    T tau = fabs(b0 - b2);
+    T a0 = T(1.0 / 10.0) * (T(1) + ipow<P>(tau / (b0 + eps)));
+    T a1 = T(6.0 / 10.0) * (T(1) + ipow<P>(tau / (b1 + eps)));
+    T a2 = T(3.0 / 10.0) * (T(1) + ipow<P>(tau / (b2 + eps)));
+    return ((a0 * q0 + a1 * q1) + a2 * q2) / ((a0 + a1) + a2);
+}
+// _weno_z7_left, src/AdvectionSchemes.jl:284-303
+template <int P, typename T>
+__device__ __forceinline__ T weno_z7(T v1, T v2, T v3, T v4, T v5, T v6, T v7) {
+    T q0 = (((T(-3) * v1 + T(13) * v2) - T(23) * v3) + T(25) * v4) / T(12);
+    T q1 = (((v2 - T(5) * v3) + T(13) * v4) + T(3) * v5) / T(12);
+    T q2 = (((-v3 + T(7) * v4) + T(7) * v5) - v6) / T(12);
+    T q3 = (((T(3) * v4 + T(13) * v5) - T(5) * v6) + v7) / T(12);
+    T b0 = b4_m3<T>(v1, v2, v3, v4), b1 = b4_m2<T>(v2, v3, v4, v5);
+    T b2 = b4_m1<T>(v3, v4, v5, v6), b3 = b4_p0<T>(v4, v5, v6, v7);
+    const T eps = Eps<T>::v * Eps<T>::v;
+    T tau = fabs(((b0 - b1) - b2) + b3);
+    T a0 = T(1.0 / 35.0) * (T(1) + ipow<P>(tau / (b0 + eps)));
+    T a1 = T(12.0 / 35.0) * (T(1) + ipow<P>(tau / (b1 + eps)));
+    T a2 = T(18.0 / 35.0) * (T(1) + ipow<P>(tau / (b2 + eps)));
+    T a3 = T(4.0 / 35.0) * (T(1) + ipow<P>(tau / (b3 + eps)));
+    return (((a0 * q0 + a1 * q1) + a2 * q2) + a3 * q3) / (((a0 + a1) + a2) + a3);
+}
+// weno_js5_left, src/AdvectionSchemes.jl:319-342 (ε = 1e-6)
+template <typename T>
+__device__ __forceinline__ T weno_js5(T v1, T v2, T v3, T v4, T v5) {
+    T q0 = ((T(2) * v1 - T(7) * v2) + T(11) * v3) / T(6);
+    T q1 = ((-v2 + T(5) * v3) + T(2) * v4) / T(6);
+    T q2 = ((T(2) * v3 + T(5) * v4) - v5) / T(6);
+    T b0 = b3_m2<T>(v1, v2, v3), b1 = b3_m1<T>(v2, v3, v4), b2 = b3_p0<T>(v3, v4, v5);
+    const T e = T(1e-6);
+    T d0 = b0 + e, d1 = b1 + e, d2 = b2 + e;
+    T a0 = T(1.0 / 10.0) / (d0 * d0), a1 = T(6.0 / 10.0) / (d1 * d1), a2 = T(3.0 / 10.0) / (d2 * d2);
+    return ((a0 * q0 + a1 * q1) + a2 * q2) / ((a0 + a1) + a2);
+}
+// FiniteVolume._weno5_left, src/FiniteVolume.jl:1582-1596
+template <typename T>
+__device__ __forceinline__ T weno5_classic(T fm2, T fm1, T fc, T fp1, T fp2) {
+    T q0 = (T(1.0 / 3.0) * fm2 - T(7.0 / 6.0) * fm1) + T(11.0 / 6.0) * fc;
+    T q1 = (T(-1.0 / 6.0) * fm1 + T(5.0 / 6.0) * fc) + T(1.0 / 3.0) * fp1;
+    T q2 = (T(1.0 / 3.0) * fc + T(5.0 / 6.0) * fp1) - T(1.0 / 6.0) * fp2;
+    T t0 = (fm2 - T(2) * fm1) + fc, s0 = (fm2 - T(4) * fm1) + T(3) * fc;
+    T t1 = (fm1 - T(2) * fc) + fp1, s1 = fm1 - fp1;
+    T t2 = (fc - T(2) * fp1) + fp2, s2 = (T(3) * fc - T(4) * fp1) + fp2;
+    T b0 = T(13.0 / 12.0) * (t0 * t0) + T(1.0 / 4.0) * (s0 * s0);
+    T b1 = T(13.0 / 12.0) * (t1 * t1) + T(1.0 / 4.0) * (s1 * s1);
+    T b2 = T(13.0 / 12.0) * (t2 * t2) + T(1.0 / 4.0) * (s2 * s2);
+    const T e = T(1e-6);
+    T d0 = b0 + e, d1 = b1 + e, d2 = b2 + e;
+    T a0 = T(1.0 / 10.0) / (d0 * d0), a1 = T(6.0 / 10.0) / (d1 * d1), a2 = T(3.0 / 10.0) / (d2 * d2);
+    T s = (a0 + a1) + a2;
+    return ((a0 * q0 + a1 * q1) + a2 * q2) / s;
+}
+
+// Stencil half-width of a scheme: the face between cells c and c+1 reads
+// c-R+1 .. c+R (the union of the left- and right-biased stencils).
+template <int S> struct SchemeInfo { static constexpr int R = 3; };
+template <> struct SchemeInfo<SCH_CENTERED> { static constexpr int R = 1; };
+template <> struct SchemeInfo<SCH_UPWIND1> { static constexpr int R = 1; };
+template <> struct SchemeInfo<SCH_WENO_Z7> { static constexpr int R = 4; };
+
+// The upwinded face value at the interface between c and c+1 (index along
+// the axis), read from d[base + m*stride]: the left-biased stencil for
+// vel >= 0, else the mirrored right-biased one, into one left
+// reconstruction (_upwind_value, src/AdvectionSchemes.jl:659-666).
+template <int S, int P, typename T>
+__device__ __forceinline__ T face_value(const T *__restrict__ d, long base, long stride, int c, T vel) {
+    if constexpr (S == SCH_CENTERED) {
+        return (d[base + (long)c * stride] + d[base + (long)(c + 1) * stride]) / T(2);
+    } else if constexpr (S == SCH_UPWIND1) {
+        return vel >= T(0) ? d[base + (long)c * stride] : d[base + (long)(c + 1) * stride];
+    } else {
+        constexpr int R = SchemeInfo<S>::R;
+        T a[2 * R];
+#pragma unroll
+        for (int m = 0; m < 2 * R; m++) a[m] = d[base + (long)(c - R + 1 + m) * stride];
+        const bool up = vel >= T(0);
+        T t[2 * R - 1];
+#pragma unroll
+        for (int m = 0; m < 2 * R - 1; m++) t[m] = up ? a[m] : a[2 * R - 1 - m];
+        if constexpr (S == SCH_WENO5) return weno5_classic<T>(t[0], t[1], t[2], t[3], t[4]);
+        else if constexpr (S == SCH_WENO_JS5) return weno_js5<T>(t[0], t[1], t[2], t[3], t[4]);
+        else if constexpr (S == SCH_WENO_Z5) return weno_z5<P, T>(t[0], t[1], t[2], t[3], t[4]);
+        else return weno_z7<P, T>(t[0], t[1], t[2], t[3], t[4], t[5], t[6]);
+    }
+}
+
+}  // namespace aeo

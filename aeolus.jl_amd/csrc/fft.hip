@@ -1,0 +1,433 @@
+// Direct Poisson solvers on gfx950: hand-written FFTs in LDS, FP64 Thomas.
+//
+// FFTPoissonSolver (src/PressureSolvers.jl:103-174): r2c along x, complex
+// DFT along y, then along z fused with the division by the exact 7-point
+// eigenvalues and the inverse z DFT (one LDS round trip), inverse y, c2r x.
+//
+// ColumnTridiagonalSolver (src/PressureSolvers.jl:544-805), periodic x-y:
+// r2c along x and a complex DFT along y per level, the Thomas algorithm per
+// horizontal wavenumber in FP64 (one work item per column, coalesced across
+// kx), the λ = 0 column integrated as a flux with the gauge mean
+// (_mean_mode!), and the inverse transforms with the 1/(Nx Ny) of the
+// unnormalised pair. The reference's half-complex R2HC basis and this
+// complex basis diagonalise the same operator with the same eigenvalue per
+// wavenumber, so the solution is the same discrete one.
+//
+// Spectrum layout: hat[kx + LD*(ky + Ny*kz)], kx = 0 .. Nx/2, LD = Nx/2+1
+// rounded up to a multiple of 16 (aligned column groups for the y/z passes).
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "kernels.h"
+
+namespace aeo {
+
+struct SpectralPlan {
+    int Nx, Ny, Nz, LD;
+    double2 *hat = nullptr;
+    double *cp = nullptr;  // Thomas scratch c'_k per column
+    double2 *tw[12] = {};  // tw[l][m] = exp(-2 pi i m / 2^l), m < 2^l
+    // FFT solver: squared modified wavenumbers and 1/(Nx Ny Nz)
+    double *kx2 = nullptr, *ky2 = nullptr, *kz2 = nullptr;
+    double nrm = 0;
+    // column solver
+    double *lx = nullptr, *ly = nullptr;  // eigenvalue magnitudes per wavenumber
+    double *lower = nullptr, *upper = nullptr, *hweight = nullptr, *measure = nullptr, *icoup = nullptr,
+           *gauge = nullptr;
+    double measure_sum = 0, gauge_sum = 0, norm = 0;
+    int mode = -1;  // 0 fft, 1 column
+};
+
+static int ilog2(int n) {
+    int l = 0;
+    while ((1 << l) < n) l++;
+    return (1 << l) == n ? l : -1;
+}
+
+__device__ __forceinline__ int bitrev(int m, int lg) { return (int)(__brev((unsigned)m) >> (32 - lg)); }
+
+__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
+    return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+// In-LDS radix-2 DIT FFT of `nb` interleaved sequences of length L stored
+// s[m * ld + b] in bit-reversed order; sign -1 forward, +1 inverse.
+__device__ void lds_fft(double2 *s, int L, int lg, int nb, int ld, const double2 *__restrict__ tw, int sign) {
+    const int nbf = (L >> 1) * nb;
+    for (int len = 2, st = L >> 1; len <= L; len <<= 1, st >>= 1) {
+        const int half = len >> 1;
+        for (int t = threadIdx.x; t < nbf; t += blockDim.x) {
+            const int b = t % nb, bf = t / nb;
+            const int q = bf & (half - 1), g = bf >> (__ffs(half) - 1);
+            const int i0 = g * len + q, i1 = i0 + half;
+            double2 w = tw[q * st];
+            if (sign > 0) w.y = -w.y;
+            double2 x0 = s[i0 * ld + b], x1 = s[i1 * ld + b];
+            double2 y = cmul(x1, w);
+            s[i0 * ld + b] = make_double2(x0.x + y.x, x0.y + y.y);
+            s[i1 * ld + b] = make_double2(x0.x - y.x, x0.y - y.y);
+        }
+        __syncthreads();
+    }
+    (void)lg;
+}
+
+// ---- r2c along x: rows of N = 2L reals -> N/2+1 complex ---------------------
+// z[m] = x[2m] + i x[2m+1]; Z = DFT_L(z); X[k] = E[k] + W^k O[k] with
+// E = (Z[k] + conj Z[L-k]) / 2, O = (Z[k] - conj Z[L-k]) / 2i, W = e^{-2 pi i/N}.
+__global__ void r2c_rows_kernel(double2 *__restrict__ hat, const double *__restrict__ src, int L, int lg, int rpb,
+                                int nrows, int LD, const double2 *__restrict__ twL, const double2 *__restrict__ twN) {
+    extern __shared__ double2 sm[];
+    const int row0 = blockIdx.x * rpb;
+    const int N = 2 * L;
+    for (int t = threadIdx.x; t < rpb * L; t += blockDim.x) {
+        int r = t / L, m = t - r * L;
+        int row = row0 + r;
+        double2 z = make_double2(0, 0);
+        if (row < nrows) z = reinterpret_cast<const double2 *>(src + (size_t)row * N)[m];
+        sm[bitrev(m, lg) * rpb + r] = z;
+    }
+    __syncthreads();
+    lds_fft(sm, L, lg, rpb, rpb, twL, -1);
+    for (int t = threadIdx.x; t < rpb * (L + 1); t += blockDim.x) {
+        int r = t / (L + 1), k = t - r * (L + 1);
+        int row = row0 + r;
+        if (row >= nrows) continue;
+        double2 a = sm[(k == L ? 0 : k) * rpb + r];
+        double2 b = sm[(k == 0 ? 0 : L - k) * rpb + r];
+        b.y = -b.y;  // conj Z[L-k]
+        double2 e = make_double2(0.5 * (a.x + b.x), 0.5 * (a.y + b.y));
+        double2 d = make_double2(a.x - b.x, a.y - b.y);
+        double2 o = make_double2(0.5 * d.y, -0.5 * d.x);
+        double2 w = twN[k];
+        double2 wo = cmul(w, o);
+        hat[(size_t)row * LD + k] = make_double2(e.x + wo.x, e.y + wo.y);
+    }
+}
+
+// ---- c2r along x: N/2+1 Hermitian-half complex -> N reals, times scale ------
+__global__ void c2r_rows_kernel(double *__restrict__ dst, const double2 *__restrict__ hat, int L, int lg, int rpb,
+                                int nrows, int LD, const double2 *__restrict__ twL, const double2 *__restrict__ twN,
+                                double scale) {
+    extern __shared__ double2 sm[];
+    const int row0 = blockIdx.x * rpb;
+    const int N = 2 * L;
+    for (int t = threadIdx.x; t < rpb * L; t += blockDim.x) {
+        int r = t / L, k = t - r * L;
+        int row = row0 + r;
+        double2 z = make_double2(0, 0);
+        if (row < nrows) {
+            double2 a = hat[(size_t)row * LD + k];
+            double2 b = hat[(size_t)row * LD + (L - k)];
+            if (k == 0) { a.y = 0; b.y = 0; }  // DC and Nyquist are real
+            b.y = -b.y;
+            double2 s = make_double2(a.x + b.x, a.y + b.y);
+            double2 d = make_double2(a.x - b.x, a.y - b.y);
+            double2 w = twN[k];
+            w.y = -w.y;  // W^{-k}
+            double2 wd = cmul(w, d);
+            z = make_double2(s.x - wd.y, s.y + wd.x);  // s + i·wd
+        }
+        sm[bitrev(k, lg) * rpb + r] = z;
+    }
+    __syncthreads();
+    lds_fft(sm, L, lg, rpb, rpb, twL, +1);
+    for (int t = threadIdx.x; t < rpb * L; t += blockDim.x) {
+        int r = t / L, m = t - r * L;
+        int row = row0 + r;
+        if (row >= nrows) continue;
+        double2 z = sm[m * rpb + r];
+        reinterpret_cast<double2 *>(dst + (size_t)row * N)[m] = make_double2(z.x * scale, z.y * scale);
+    }
+}
+
+// ---- complex DFT along a strided axis -------------------------------------
+// Sequence (g, o): element m at base + m*es + b, b < TB, base = o*os + g*TB.
+// mode 0: one transform of `sign`; mode 1: forward, divide by the 7-point
+// eigenvalue (FFTPoissonSolver, the zero mode annihilated), inverse.
+struct EigArgs {
+    const double *kx2, *ky2, *kz2;
+    double nrm;
+    int nkx;
+};
+
+template <int TB>
+__global__ void c2c_cols_kernel(double2 *__restrict__ d, int L, int lg, long es, long os, int ncols, int sign, int mode,
+                                const double2 *__restrict__ tw, EigArgs ea) {
+    extern __shared__ double2 sm[];
+    const int g = blockIdx.x, o = blockIdx.y;
+    const long base = (long)o * os + (long)g * TB;
+    for (int t = threadIdx.x; t < L * TB; t += blockDim.x) {
+        int m = t / TB, b = t - m * TB;
+        double2 z = make_double2(0, 0);
+        if (g * TB + b < ncols) z = d[base + (long)m * es + b];
+        sm[bitrev(m, lg) * TB + b] = z;
+    }
+    __syncthreads();
+    lds_fft(sm, L, lg, TB, TB, tw, sign);
+    if (mode == 1) {
+        // here the axis is z (m = kz), o = ky, column g*TB + b = kx
+        for (int t = threadIdx.x; t < L * TB; t += blockDim.x) {
+            int m = t / TB, b = t - m * TB;
+            int kx = g * TB + b;
+            double2 z = sm[m * TB + b];
+            double l = kx < ea.nkx ? -((ea.kx2[kx] + ea.ky2[o]) + ea.kz2[m]) : 1.0;
+            double inv = l == 0 ? 0.0 : ea.nrm / l;
+            z.x *= inv;
+            z.y *= inv;
+            sm[m * TB + b] = z;
+        }
+        __syncthreads();
+        // back to bit-reversed order for the inverse: each pair swapped once
+        for (int t = threadIdx.x; t < L * TB; t += blockDim.x) {
+            int m = t / TB, b = t - m * TB;
+            int r = bitrev(m, lg);
+            if (m < r) {
+                double2 x = sm[m * TB + b];
+                sm[m * TB + b] = sm[r * TB + b];
+                sm[r * TB + b] = x;
+            }
+        }
+        __syncthreads();
+        lds_fft(sm, L, lg, TB, TB, tw, +1);
+    }
+    for (int t = threadIdx.x; t < L * TB; t += blockDim.x) {
+        int m = t / TB, b = t - m * TB;
+        if (g * TB + b < ncols) d[base + (long)m * es + b] = sm[m * TB + b];
+    }
+}
+
+// ---- the Thomas sweeps (_thomas_slab!, src/PressureSolvers.jl:754-772) ------
+// One work item per wavenumber column (kx, ky); the factorisation of the
+// reference's constructor (:648-660) is evaluated on the fly, the same
+// operations in the same order, with c'_k kept in `cp` for the back
+// substitution. The (0, 0) column is the mean mode (_mean_mode!, :780-805).
+__global__ void thomas_kernel(double2 *__restrict__ hat, double *__restrict__ cp, int nkx, int Ny, int Nz, int LD,
+                              const double *__restrict__ lx, const double *__restrict__ ly,
+                              const double *__restrict__ lower, const double *__restrict__ upper,
+                              const double *__restrict__ hweight, const double *__restrict__ measure,
+                              const double *__restrict__ icoup, const double *__restrict__ gauge, double measure_sum,
+                              double gauge_sum) {
+    const int kx = blockIdx.x * blockDim.x + threadIdx.x;
+    const int ky = blockIdx.y;
+    if (kx >= nkx) return;
+    const size_t plane = (size_t)LD * Ny;
+    const size_t c0 = (size_t)kx + (size_t)LD * ky;
+    if (kx == 0 && ky == 0) {
+        // the mean mode: remove the m-weighted mean, integrate the flux up, gauge
+        double mb = 0.0;
+        for (int k = 0; k < Nz; k++) mb += hat[c0 + plane * k].x * measure[k];
+        mb /= measure_sum;
+        double F = 0.0, ph = 0.0;
+        double prev = hat[c0].x - mb;
+        cp[c0] = 0.0;  // the column, in the scratch
+        for (int k = 1; k < Nz; k++) {
+            F += prev * measure[k - 1];
+            ph += F * icoup[k];
+            prev = hat[c0 + plane * k].x - mb;
+            cp[c0 + plane * k] = ph;
+        }
+        double mphi = 0.0;
+        for (int k = 0; k < Nz; k++) mphi += cp[c0 + plane * k] * gauge[k];
+        double sh = mphi / gauge_sum;
+        for (int k = 0; k < Nz; k++) hat[c0 + plane * k] = make_double2(cp[c0 + plane * k] - sh, 0.0);
+        return;
+    }
+    const double lam = lx[kx] + ly[ky];
+    double cprev = 0.0;
+    double2 dprev = make_double2(0, 0);
+    for (int k = 0; k < Nz; k++) {
+        const double b = -(lower[k] + upper[k]) - lam * hweight[k];
+        const double den = b - lower[k] * cprev;
+        const double inv_den = 1 / den;
+        cprev = upper[k] / den;
+        cp[c0 + plane * k] = cprev;
+        double2 v = hat[c0 + plane * k];
+        if (k == 0) {
+            v.x = v.x * inv_den;
+            v.y = v.y * inv_den;
+        } else {
+            const double a = lower[k];
+            v.x = (v.x - a * dprev.x) * inv_den;
+            v.y = (v.y - a * dprev.y) * inv_den;
+        }
+        hat[c0 + plane * k] = v;
+        dprev = v;
+    }
+    double2 nxt = dprev;
+    for (int k = Nz - 2; k >= 0; k--) {
+        double2 v = hat[c0 + plane * k];
+        const double c = cp[c0 + plane * k];
+        v.x -= c * nxt.x;
+        v.y -= c * nxt.y;
+        hat[c0 + plane * k] = v;
+        nxt = v;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+#define HCHK(x)                                                                                   \
+    do {                                                                                          \
+        hipError_t e_ = (x);                                                                      \
+        if (e_ != hipSuccess) {                                                                   \
+            snprintf(err, errlen, "%s: %s", #x, hipGetErrorString(e_));                           \
+            return nullptr;                                                                       \
+        }                                                                                         \
+    } while (0)
+
+static hipError_t upload_table(double2 **dst, int n) {
+    std::vector<double2> h(n);
+    for (int m = 0; m < n; m++) h[m] = make_double2(cos(2.0 * M_PI * m / n), -sin(2.0 * M_PI * m / n));
+    hipError_t e = hipMalloc(dst, sizeof(double2) * n);
+    if (e != hipSuccess) return e;
+    return hipMemcpy(*dst, h.data(), sizeof(double2) * n, hipMemcpyHostToDevice);
+}
+
+SpectralPlan *spectral_create(int Nx, int Ny, int Nz, hipStream_t st, char *err, size_t errlen) {
+    if (ilog2(Nx) < 2 || ilog2(Ny) < 1 || Nx > 2048 || Ny > 512) {
+        snprintf(err, errlen, "spectral solver: Nx (%d) and Ny (%d) must be powers of two (Nx in 4..2048, Ny in 2..512)",
+                 Nx, Ny);
+        return nullptr;
+    }
+    SpectralPlan *p = new SpectralPlan();
+    p->Nx = Nx; p->Ny = Ny; p->Nz = Nz;
+    p->LD = ((Nx / 2 + 1) + 15) / 16 * 16;
+    size_t nh = (size_t)p->LD * Ny * Nz;
+    HCHK(hipMalloc(&p->hat, sizeof(double2) * nh));
+    HCHK(hipMemsetAsync(p->hat, 0, sizeof(double2) * nh, st));
+    HCHK(hipMalloc(&p->cp, sizeof(double) * nh));
+    int need[4] = {Nx, Nx / 2, Ny, ilog2(Nz) >= 1 ? Nz : 0};
+    for (int q = 0; q < 4; q++) {
+        int l = need[q] > 0 ? ilog2(need[q]) : -1;
+        if (l >= 0 && !p->tw[l]) HCHK(upload_table(&p->tw[l], need[q]));
+    }
+    HCHK(hipStreamSynchronize(st));
+    return p;
+}
+
+void spectral_destroy(SpectralPlan *p) {
+    if (!p) return;
+    hipFree(p->hat); hipFree(p->cp);
+    for (auto &t : p->tw) if (t) hipFree(t);
+    hipFree(p->kx2); hipFree(p->ky2); hipFree(p->kz2);
+    hipFree(p->lx); hipFree(p->ly);
+    hipFree(p->lower); hipFree(p->upper); hipFree(p->hweight); hipFree(p->measure); hipFree(p->icoup); hipFree(p->gauge);
+    delete p;
+}
+
+static int upload(double **dst, const std::vector<double> &h) {
+    if (hipMalloc(dst, sizeof(double) * h.size()) != hipSuccess) return -1;
+    return hipMemcpy(*dst, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice) == hipSuccess ? 0 : -1;
+}
+
+// FFTPoissonSolver constructor, src/PressureSolvers.jl:121-138
+int spectral_fft_setup(SpectralPlan *p, double dx, double dy, double dz) {
+    if (ilog2(p->Nz) < 1 || p->Nz > 512) return -2;
+    int Nx = p->Nx, Ny = p->Ny, Nz = p->Nz;
+    std::vector<double> kx(Nx / 2 + 1), ky(Ny), kz(Nz);
+    for (int m = 0; m <= Nx / 2; m++) { double s = sin(M_PI * m / Nx); kx[m] = 4 / (dx * dx) * (s * s); }
+    for (int m = 0; m < Ny; m++) { double s = sin(M_PI * m / Ny); ky[m] = 4 / (dy * dy) * (s * s); }
+    for (int m = 0; m < Nz; m++) { double s = sin(M_PI * m / Nz); kz[m] = 4 / (dz * dz) * (s * s); }
+    p->nrm = 1 / (double)((long long)Nx * Ny * Nz);
+    if (upload(&p->kx2, kx) || upload(&p->ky2, ky) || upload(&p->kz2, kz)) return -1;
+    p->mode = 0;
+    return 0;
+}
+
+// ColumnTridiagonalSolver constructor (periodic x-y), src/PressureSolvers.jl:626-666
+int spectral_column_setup(SpectralPlan *p, double dx, double dy, const double *lower, const double *upper,
+                          const double *hweight, const double *measure, const double *inv_coupling,
+                          const double *gauge) {
+    int Nx = p->Nx, Ny = p->Ny, Nz = p->Nz;
+    std::vector<double> lx(Nx / 2 + 1), ly(Ny);
+    // _r2r_eigen: half-complex index p <-> frequency min(p, N-p)
+    for (int q = 0; q <= Nx / 2; q++) { double s = sin(M_PI * (double)q / Nx); lx[q] = 4 / (dx * dx) * (s * s); }
+    for (int q = 0; q < Ny; q++) {
+        int f = q < Ny - q ? q : Ny - q;
+        double s = sin(M_PI * (double)f / Ny);
+        ly[q] = 4 / (dy * dy) * (s * s);
+    }
+    std::vector<double> lo(lower, lower + Nz), up(upper, upper + Nz), hw(hweight, hweight + Nz),
+        ms(measure, measure + Nz), ic(inv_coupling, inv_coupling + Nz + 1), ga(gauge, gauge + Nz);
+    double msum = 0, gsum = 0;
+    for (int k = 0; k < Nz; k++) msum += ms[k];
+    for (int k = 0; k < Nz; k++) gsum += ga[k];
+    p->measure_sum = msum;
+    p->gauge_sum = gsum;
+    p->norm = 1.0 / (double)((long long)Nx * Ny);
+    if (upload(&p->lx, lx) || upload(&p->ly, ly) || upload(&p->lower, lo) || upload(&p->upper, up) ||
+        upload(&p->hweight, hw) || upload(&p->measure, ms) || upload(&p->icoup, ic) || upload(&p->gauge, ga))
+        return -1;
+    p->mode = 1;
+    return 0;
+}
+
+static void xfwd(SpectralPlan *p, hipStream_t st, const double *src) {
+    int L = p->Nx / 2, lg = ilog2(L);
+    int tpr = L / 2 > 0 ? L / 2 : 1;
+    int rpb = tpr >= 256 ? 1 : 256 / tpr;
+    int nrows = p->Ny * p->Nz;
+    int bt = tpr >= 256 ? tpr : 256;
+    size_t lds = sizeof(double2) * (size_t)rpb * L;
+    hipLaunchKernelGGL(r2c_rows_kernel, dim3((nrows + rpb - 1) / rpb), dim3(bt), lds, st, p->hat, src, L, lg, rpb,
+                       nrows, p->LD, p->tw[lg], p->tw[lg + 1]);
+}
+static void xinv(SpectralPlan *p, hipStream_t st, double *dst, double scale) {
+    int L = p->Nx / 2, lg = ilog2(L);
+    int tpr = L / 2 > 0 ? L / 2 : 1;
+    int rpb = tpr >= 256 ? 1 : 256 / tpr;
+    int nrows = p->Ny * p->Nz;
+    int bt = tpr >= 256 ? tpr : 256;
+    size_t lds = sizeof(double2) * (size_t)rpb * L;
+    hipLaunchKernelGGL(c2r_rows_kernel, dim3((nrows + rpb - 1) / rpb), dim3(bt), lds, st, dst, p->hat, L, lg, rpb,
+                       nrows, p->LD, p->tw[lg], p->tw[lg + 1], scale);
+}
+// a complex DFT along y (axis = 1) or z (axis = 2) of every kx column group
+static void cols(SpectralPlan *p, hipStream_t st, int axis, int sign, int mode) {
+    int L = axis == 1 ? p->Ny : p->Nz, lg = ilog2(L);
+    long es = axis == 1 ? (long)p->LD : (long)p->LD * p->Ny;
+    long os = axis == 1 ? (long)p->LD * p->Ny : (long)p->LD;
+    int nouter = axis == 1 ? p->Nz : p->Ny;
+    int ncols = p->Nx / 2 + 1;
+    EigArgs ea{p->kx2, p->ky2, p->kz2, p->nrm, ncols};
+    if (L <= 256) {
+        constexpr int TB = 16;
+        size_t lds = sizeof(double2) * (size_t)L * TB;
+        hipLaunchKernelGGL(c2c_cols_kernel<TB>, dim3(p->LD / TB, nouter), dim3(256), lds, st, p->hat, L, lg, es, os,
+                           ncols, sign, mode, p->tw[lg], ea);
+    } else {
+        constexpr int TB = 8;
+        size_t lds = sizeof(double2) * (size_t)L * TB;
+        hipLaunchKernelGGL(c2c_cols_kernel<TB>, dim3(p->LD / TB, nouter), dim3(256), lds, st, p->hat, L, lg, es, os,
+                           ncols, sign, mode, p->tw[lg], ea);
+    }
+}
+
+int spectral_fft_solve(SpectralPlan *p, hipStream_t st, double *phi, const double *src) {
+    if (p->mode != 0) return -1;
+    xfwd(p, st, src);
+    cols(p, st, 1, -1, 0);
+    cols(p, st, 2, -1, 1);  // forward z, 1/λ, inverse z
+    cols(p, st, 1, +1, 0);
+    xinv(p, st, phi, 1.0);
+    return 0;
+}
+
+int spectral_column_solve(SpectralPlan *p, hipStream_t st, double *phi, const double *src) {
+    if (p->mode != 1) return -1;
+    xfwd(p, st, src);
+    cols(p, st, 1, -1, 0);
+    int nkx = p->Nx / 2 + 1;
+    hipLaunchKernelGGL(thomas_kernel, dim3((nkx + 63) / 64, p->Ny), dim3(64), 0, st, p->hat, p->cp, nkx, p->Ny, p->Nz,
+                       p->LD, p->lx, p->ly, p->lower, p->upper, p->hweight, p->measure, p->icoup, p->gauge,
+                       p->measure_sum, p->gauge_sum);
+    cols(p, st, 1, +1, 0);
+    xinv(p, st, phi, p->norm);
+    return 0;
+}
+
+}  // namespace aeo

@@ -1,0 +1,56 @@
+// Launchers of the gfx950 kernels (internal to libaeolus_hip).
+#pragma once
+#include "common.h"
+
+namespace aeo {
+
+// Physics of one substage of the step (src/Simulations.jl:168-288 fields).
+struct StepPhys {
+    double nu, dt, theta0, g_accel, flux_bot, flux_top;
+    double idz_bot, idz_top;  // 1/Δz_c of the bottom and top interior levels
+};
+
+// The RK3 stage combination c(x, snap) = a·snap + b·x of step_rk3!
+// (src/Simulations.jl:884-896): stage 2 (3/4, 1/4), stage 3 (1/3, 2/3).
+template <typename T>
+inline void stage_coeffs(int stage, const StepPhys *, T *a, T *b, int *combine) {
+    if (stage == 2) { *a = T(0.75); *b = T(0.25); *combine = 1; }
+    else if (stage == 3) { double inv3 = 1.0 / 3; *a = T(inv3); *b = T(2 * inv3); *combine = 1; }
+    else { *a = T(0); *b = T(1); *combine = 0; }
+}
+
+template <typename T>
+void launch_halo(hipStream_t st, T *d, int Nx, int Ny, int Nz, int H, int loc, int bcx, int bcy, int bcz);
+template <typename T>
+int launch_advect(hipStream_t st, const Metrics<T> &m, int mode, int scheme, int p, const T *u, const T *v,
+                  const T *w, const T *s, T *tu, T *tv, T *tw, T *ts);
+template <typename T>
+int launch_tendency_step(hipStream_t st, const Metrics<T> &m, int scheme, int p, const T *u, const T *v, const T *w,
+                         const T *s, T *uo, T *vo, T *wo, T *so, const T *s0, const StepPhys &ph, int stage);
+template <typename T>
+void launch_div_into(hipStream_t st, double *buf, const T *u, const T *v, const T *w, const Metrics<double> &md);
+template <typename T>
+void launch_correct(hipStream_t st, T *uo, T *vo, T *wo, const T *us, const T *vs, const T *ws, const T *u0,
+                    const T *v0, const T *w0, const double *phi, T *p, const Metrics<double> &md, int bcx, int bcy,
+                    int bcz, double ca, double cb, int combine, double dt);
+template <typename T>
+void launch_shell(hipStream_t st, T *out, const T *x, const T *snap, int Nx, int Ny, int Nz, int H, double ca,
+                  double cb);
+template <typename T>
+void launch_max(hipStream_t st, unsigned long long *out, const T *u, const T *v, const T *w, const Metrics<double> &md,
+                int what);
+
+// ---- spectral / column solvers (fft.hip) ----
+struct SpectralPlan;  // opaque: twiddle tables and work arrays on the device
+SpectralPlan *spectral_create(int Nx, int Ny, int Nz, hipStream_t st, char *err, size_t errlen);
+void spectral_destroy(SpectralPlan *p);
+// triply periodic: φ = L⁻¹(src − mean) with the exact 7-point eigenvalues
+int spectral_fft_setup(SpectralPlan *p, double dx, double dy, double dz);
+int spectral_fft_solve(SpectralPlan *p, hipStream_t st, double *phi, const double *src);
+// column solver: periodic x-y transform + FP64 Thomas per wavenumber + mean mode
+int spectral_column_setup(SpectralPlan *p, double dx, double dy, const double *lower, const double *upper,
+                          const double *hweight, const double *measure, const double *inv_coupling,
+                          const double *gauge);
+int spectral_column_solve(SpectralPlan *p, hipStream_t st, double *phi, const double *src);
+
+}  // namespace aeo

@@ -1,0 +1,577 @@
+// Stencil kernels of the dynamical-core step (gfx950).
+//
+//   halo_gather      fill_halos! of one field in one launch
+//   adv_march        advection tendencies (WENO-5/WENO-JS5/WENO-Z5/WENO-Z7,
+//                    centred, upwind), fused with the viscous term, the
+//                    buoyancy, the surface flux and the predictor
+//   div_into         the Arakawa-C divergence into the solver's buffer
+//   correct          the projection's correction + RK3 combination + p
+//   shell_combine    the RK3 combination of the halo shell
+//   reductions       max |div|, max advective rate
+#include "kernels.h"
+
+namespace aeo {
+
+// ---------------------------------------------------------------------------
+// Halo fill (src/BoundaryConditions.jl:182-259; halo_kernels.jl:100-147).
+// The reference fills x, then y, then z, each over the whole face slab; the
+// result at any storage point is the interior value at (mx(i), my(j), mz(k))
+// (periodic wrap or mirror per axis), negated about a wall for w in z, with
+// the wall faces themselves zero. Every source is interior, so one gather
+// launch over the halo points (and the wall plane) gives the same values.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int map_axis(int idx, int H, int N, int bc) {
+    if (idx < H) return bc == BC_PERIODIC ? idx + N : 2 * H - 1 - idx;
+    if (idx >= H + N) return bc == BC_PERIODIC ? idx - N : 2 * (H + N) - 1 - idx;
+    return idx;
+}
+
+template <typename T>
+__global__ void halo_gather_kernel(T *__restrict__ d, int Nx, int Ny, int Nz, int H, int bcx, int bcy, int bcz,
+                                   int wall_w) {
+    const int sx = Nx + 2 * H, sy = Ny + 2 * H, sz = Nz + 2 * H;
+    // enumeration: z-slabs (2H planes), then y-slabs, then x-slabs, then the
+    // w wall plane k = H (interior) when wall_w
+    const long nz_slab = (long)2 * H * sx * sy;
+    const long ny_slab = (long)Nz * 2 * H * sx;
+    const long nx_slab = (long)Nz * Ny * 2 * H;
+    const long nwall = wall_w ? (long)sx * sy : 0;
+    const long total = nz_slab + ny_slab + nx_slab + nwall;
+    for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < total; t += (long)gridDim.x * blockDim.x) {
+        int i, j, k;
+        if (t < nz_slab) {
+            long q = t;
+            i = (int)(q % sx); q /= sx;
+            j = (int)(q % sy); q /= sy;
+            int l = (int)q;
+            k = l < H ? l : H + Nz + (l - H);
+        } else if (t < nz_slab + ny_slab) {
+            long q = t - nz_slab;
+            i = (int)(q % sx); q /= sx;
+            int l = (int)(q % (2 * H)); q /= (2 * H);
+            j = l < H ? l : H + Ny + (l - H);
+            k = H + (int)q;
+        } else if (t < nz_slab + ny_slab + nx_slab) {
+            long q = t - nz_slab - ny_slab;
+            int l = (int)(q % (2 * H)); q /= (2 * H);
+            i = l < H ? l : H + Nx + (l - H);
+            j = H + (int)(q % Ny);
+            k = H + (int)(q / Ny);
+        } else {
+            long q = t - nz_slab - ny_slab - nx_slab;
+            i = (int)(q % sx);
+            j = (int)(q / sx);
+            k = H;
+        }
+        T val;
+        if (wall_w && bcz == BC_WALL) {
+            if (k == H || k == H + Nz) {
+                val = T(0);
+            } else {
+                int kk = k;
+                T sgn = T(1);
+                if (k < H) { kk = 2 * H - k; sgn = T(-1); }
+                else if (k > H + Nz) { kk = 2 * (H + Nz) - k; sgn = T(-1); }
+                int ii = map_axis(i, H, Nx, bcx), jj = map_axis(j, H, Ny, bcy);
+                val = sgn < T(0) ? -d[lin(ii, jj, kk, sx, sy)] : d[lin(ii, jj, kk, sx, sy)];
+            }
+        } else {
+            int ii = map_axis(i, H, Nx, bcx), jj = map_axis(j, H, Ny, bcy);
+            int kk = map_axis(k, H, Nz, bcz == BC_PERIODIC ? BC_PERIODIC : BC_NOFLUX);
+            val = d[lin(ii, jj, kk, sx, sy)];
+        }
+        d[lin(i, j, k, sx, sy)] = val;
+        (void)sz;
+    }
+}
+
+template <typename T>
+void launch_halo(hipStream_t st, T *d, int Nx, int Ny, int Nz, int H, int loc, int bcx, int bcy, int bcz) {
+    int wall_w = (loc == LOC_FZ && bcz == BC_WALL) ? 1 : 0;
+    const int sx = Nx + 2 * H, sy = Ny + 2 * H;
+    long total = (long)2 * H * sx * sy + (long)Nz * 2 * H * sx + (long)Nz * Ny * 2 * H + (wall_w ? (long)sx * sy : 0);
+    int blocks = (int)((total + 255) / 256);
+    if (blocks > 65535) blocks = 65535;
+    hipLaunchKernelGGL(halo_gather_kernel<T>, dim3(blocks), dim3(256), 0, st, d, Nx, Ny, Nz, H, bcx, bcy, bcz, wall_w);
+}
+template void launch_halo<double>(hipStream_t, double *, int, int, int, int, int, int, int, int);
+template void launch_halo<float>(hipStream_t, float *, int, int, int, int, int, int, int, int);
+
+// ---------------------------------------------------------------------------
+// Face fluxes vel * face_value of each transported field, per direction, at
+// the interface between c and c+1 along the axis. Transporting velocities
+// as in _b_recon! (src/AdvectionSchemes.jl:685-697) and _b_mrecon_u!/_v!/_w!
+// (:1203-1256); on a stretched grid the horizontal faces of a w volume take
+// the volume-flux mean (_zflux), the transporting velocity of the centred
+// operator (src/FiniteVolume.jl:775-797).
+// ---------------------------------------------------------------------------
+enum Field { F_U = 0, F_V = 1, F_W = 2, F_S = 3 };
+
+template <typename T>
+struct Fields {
+    const T *__restrict__ u;
+    const T *__restrict__ v;
+    const T *__restrict__ w;
+    const T *__restrict__ s;
+};
+
+template <int F, typename T>
+__device__ __forceinline__ const T *field_ptr(const Fields<T> &f) {
+    if constexpr (F == F_U) return f.u;
+    else if constexpr (F == F_V) return f.v;
+    else if constexpr (F == F_W) return f.w;
+    else return f.s;
+}
+
+template <int F, int S, int P, typename T>
+__device__ __forceinline__ T flux_x(const Fields<T> &f, const Metrics<T> &m, int c, int j, int k) {
+    const int sx = m.sx, sy = m.sy;
+    T vel;
+    if constexpr (F == F_S) vel = f.u[lin(c + 1, j, k, sx, sy)];
+    else if constexpr (F == F_U) vel = (f.u[lin(c, j, k, sx, sy)] + f.u[lin(c + 1, j, k, sx, sy)]) / T(2);
+    else if constexpr (F == F_V) vel = (f.u[lin(c + 1, j - 1, k, sx, sy)] + f.u[lin(c + 1, j, k, sx, sy)]) / T(2);
+    else {
+        T lo = f.u[lin(c + 1, j, k - 1, sx, sy)], hi = f.u[lin(c + 1, j, k, sx, sy)];
+        vel = m.stretched ? m.m_zflux(k, lo, hi) : (lo + hi) / T(2);
+    }
+    const T *d = field_ptr<F>(f);
+    return vel * face_value<S, P, T>(d, (long)lin(0, j, k, sx, sy), 1, c, vel);
+}
+
+template <int F, int S, int P, typename T>
+__device__ __forceinline__ T flux_y(const Fields<T> &f, const Metrics<T> &m, int i, int c, int k) {
+    const int sx = m.sx, sy = m.sy;
+    T vel;
+    if constexpr (F == F_S) vel = f.v[lin(i, c + 1, k, sx, sy)];
+    else if constexpr (F == F_U) vel = (f.v[lin(i - 1, c + 1, k, sx, sy)] + f.v[lin(i, c + 1, k, sx, sy)]) / T(2);
+    else if constexpr (F == F_V) vel = (f.v[lin(i, c, k, sx, sy)] + f.v[lin(i, c + 1, k, sx, sy)]) / T(2);
+    else {
+        T lo = f.v[lin(i, c + 1, k - 1, sx, sy)], hi = f.v[lin(i, c + 1, k, sx, sy)];
+        vel = m.stretched ? m.m_zflux(k, lo, hi) : (lo + hi) / T(2);
+    }
+    const T *d = field_ptr<F>(f);
+    return vel * face_value<S, P, T>(d, (long)lin(i, 0, k, sx, sy), (long)sx, c, vel);
+}
+
+template <int F, int S, int P, typename T>
+__device__ __forceinline__ T flux_z(const Fields<T> &f, const Metrics<T> &m, int i, int j, int c) {
+    const int sx = m.sx, sy = m.sy;
+    T vel;
+    if constexpr (F == F_S) vel = f.w[lin(i, j, c + 1, sx, sy)];
+    else if constexpr (F == F_U) vel = (f.w[lin(i - 1, j, c + 1, sx, sy)] + f.w[lin(i, j, c + 1, sx, sy)]) / T(2);
+    else if constexpr (F == F_V) vel = (f.w[lin(i, j - 1, c + 1, sx, sy)] + f.w[lin(i, j, c + 1, sx, sy)]) / T(2);
+    else vel = (f.w[lin(i, j, c, sx, sy)] + f.w[lin(i, j, c + 1, sx, sy)]) / T(2);
+    const T *d = field_ptr<F>(f);
+    return vel * face_value<S, P, T>(d, (long)lin(i, j, 0, sx, sy), (long)sx * sy, c, vel);
+}
+
+// _lap_at, src/FiniteVolume.jl:271-283
+template <typename T>
+__device__ __forceinline__ T lap_at(const T *__restrict__ f, const Metrics<T> &m, int i, int j, int k, int loc) {
+    const int sx = m.sx, sy = m.sy;
+    T c = f[lin(i, j, k, sx, sy)];
+    return (((f[lin(i + 1, j, k, sx, sy)] - T(2) * c) + f[lin(i - 1, j, k, sx, sy)]) * m.idx2 +
+            ((f[lin(i, j + 1, k, sx, sy)] - T(2) * c) + f[lin(i, j - 1, k, sx, sy)]) * m.idy2) +
+           m.m_lapz(loc, k, f[lin(i, j, k - 1, sx, sy)], c, f[lin(i, j, k + 1, sx, sy)]);
+}
+
+// ---------------------------------------------------------------------------
+// The marching advection kernel.
+//
+// A block is 64 x BY threads: each wave is one j-row of 64 cells along x,
+// and every thread marches up a chunk of KC levels. Each face flux is
+// computed once (the reference evaluates every interior face from both of
+// its cells; both evaluations are the same expression on the same values, so
+// sharing one evaluation gives the same bits):
+//   z: the top-face flux of level k is the bottom-face flux of level k+1
+//      (carried in a register);
+//   x: each lane computes its west face, the east face comes from the next
+//      lane; the east face of a wave's last cell is computed for all KC
+//      levels of the chunk in one batched pass before the march (lane l does
+//      level k0 + l) and kept in LDS;
+//   y: each wave computes its south faces, the north face comes from the
+//      next wave through LDS; the block's top row computes its north face.
+// ---------------------------------------------------------------------------
+enum Mode { MODE_SCALAR = 0, MODE_MOMENTUM = 1, MODE_STEP = 2 };
+
+template <typename T>
+struct MarchArgs {
+    Metrics<T> m;
+    Fields<T> in;
+    // MODE_SCALAR: ts -= adv(s); MODE_MOMENTUM: tu, tv, tw -= adv
+    T *tu, *tv, *tw, *ts;
+    // MODE_STEP: predictor u* (uo, vo, wo) and the advanced scalar so
+    T *uo, *vo, *wo, *so;
+    const T *s0;  // step-start scalar for the stage combination
+    T nu, dt, theta0, coef, fb, ft;
+    T ca, cb;
+    int combine, has_visc, has_buoy, has_sflux;
+    int kc;  // levels per chunk (<= 64)
+};
+
+constexpr int MW = 64;  // lanes per wave (gfx950)
+
+template <int MODE> struct ModeFields { static constexpr int lo = F_U, hi = F_S; };
+template <> struct ModeFields<MODE_SCALAR> { static constexpr int lo = F_S, hi = F_S; };
+template <> struct ModeFields<MODE_MOMENTUM> { static constexpr int lo = F_U, hi = F_W; };
+
+template <int MODE, int S, int P, typename T, int BY>
+__global__ void __launch_bounds__(MW *BY) adv_march_kernel(MarchArgs<T> a) {
+    constexpr int F0 = ModeFields<MODE>::lo, F1 = ModeFields<MODE>::hi, NF = F1 - F0 + 1;
+    const Metrics<T> &m = a.m;
+    const int H = m.H;
+    const int lane = threadIdx.x, ty = threadIdx.y;
+    const int i0 = H + blockIdx.x * MW;
+    const int i = i0 + lane, j = H + blockIdx.y * BY + ty;
+    const int k0 = H + blockIdx.z * a.kc;
+    const int k1 = min(k0 + a.kc, H + m.Nz);
+    const int ilast = min(i0 + MW - 1, H + m.Nx - 1);  // the last cell of this wave's row
+    const int jlast = min(H + (int)(blockIdx.y + 1) * BY - 1, H + m.Ny - 1);
+    const bool row_ok = j < H + m.Ny;
+    const bool ok = row_ok && i < H + m.Nx;
+
+    __shared__ T xb[NF][BY][MW];      // east boundary x-flux of each row, per level of the chunk
+    __shared__ T ys[2][NF][BY][MW];   // south y-flux, double buffered over k
+
+    // batched east-boundary x-fluxes: lane l does level k0 + l
+    if (row_ok) {
+        const int kb = k0 + lane;
+        if (kb < k1) {
+#pragma unroll
+            for (int F = F0; F <= F1; F++) {
+                T v;
+                if (F == F_U) v = flux_x<F_U, S, P, T>(a.in, m, ilast, j, kb);
+                else if (F == F_V) v = flux_x<F_V, S, P, T>(a.in, m, ilast, j, kb);
+                else if (F == F_W) v = flux_x<F_W, S, P, T>(a.in, m, ilast, j, kb);
+                else v = flux_x<F_S, S, P, T>(a.in, m, ilast, j, kb);
+                xb[F - F0][ty][lane] = v;
+            }
+        }
+    }
+
+    // bottom-face z-flux of the first level
+    T fzb[NF];
+    if (ok) {
+#pragma unroll
+        for (int F = F0; F <= F1; F++) {
+            if (F == F_U) fzb[F - F0] = flux_z<F_U, S, P, T>(a.in, m, i, j, k0 - 1);
+            else if (F == F_V) fzb[F - F0] = flux_z<F_V, S, P, T>(a.in, m, i, j, k0 - 1);
+            else if (F == F_W) fzb[F - F0] = flux_z<F_W, S, P, T>(a.in, m, i, j, k0 - 1);
+            else fzb[F - F0] = flux_z<F_S, S, P, T>(a.in, m, i, j, k0 - 1);
+        }
+    }
+    __syncthreads();
+
+    int buf = 0;
+    for (int k = k0; k < k1; k++) {
+        T fxw[NF], fxe[NF], fys[NF], fyn[NF], fzt[NF];
+        if (ok) {
+#pragma unroll
+            for (int F = F0; F <= F1; F++) {
+                const int q = F - F0;
+                if (F == F_U) { fxw[q] = flux_x<F_U, S, P, T>(a.in, m, i - 1, j, k); fys[q] = flux_y<F_U, S, P, T>(a.in, m, i, j - 1, k); fzt[q] = flux_z<F_U, S, P, T>(a.in, m, i, j, k); }
+                else if (F == F_V) { fxw[q] = flux_x<F_V, S, P, T>(a.in, m, i - 1, j, k); fys[q] = flux_y<F_V, S, P, T>(a.in, m, i, j - 1, k); fzt[q] = flux_z<F_V, S, P, T>(a.in, m, i, j, k); }
+                else if (F == F_W) { fxw[q] = flux_x<F_W, S, P, T>(a.in, m, i - 1, j, k); fys[q] = flux_y<F_W, S, P, T>(a.in, m, i, j - 1, k); fzt[q] = flux_z<F_W, S, P, T>(a.in, m, i, j, k); }
+                else { fxw[q] = flux_x<F_S, S, P, T>(a.in, m, i - 1, j, k); fys[q] = flux_y<F_S, S, P, T>(a.in, m, i, j - 1, k); fzt[q] = flux_z<F_S, S, P, T>(a.in, m, i, j, k); }
+                ys[buf][q][ty][lane] = fys[q];
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < NF; q++) {
+            T nb = __shfl_down(ok ? fxw[q] : T(0), 1);
+            fxe[q] = (i == ilast) ? xb[q][ty][k - k0] : nb;
+        }
+        __syncthreads();
+        if (ok) {
+            if (j < jlast) {
+#pragma unroll
+                for (int q = 0; q < NF; q++) fyn[q] = ys[buf][q][ty + 1][lane];
+            } else {
+#pragma unroll
+                for (int F = F0; F <= F1; F++) {
+                    if (F == F_U) fyn[F - F0] = flux_y<F_U, S, P, T>(a.in, m, i, j, k);
+                    else if (F == F_V) fyn[F - F0] = flux_y<F_V, S, P, T>(a.in, m, i, j, k);
+                    else if (F == F_W) fyn[F - F0] = flux_y<F_W, S, P, T>(a.in, m, i, j, k);
+                    else fyn[F - F0] = flux_y<F_S, S, P, T>(a.in, m, i, j, k);
+                }
+            }
+            const size_t c = lin(i, j, k, m.sx, m.sy);
+#pragma unroll
+            for (int F = F0; F <= F1; F++) {
+                const int q = F - F0;
+                const T idzF = (F == F_W) ? m.m_idzf(k) : m.m_idz(k);
+                const T adv = ((fxe[q] - fxw[q]) * m.idx + (fyn[q] - fys[q]) * m.idy) + (fzt[q] - fzb[q]) * idzF;
+                if constexpr (MODE == MODE_SCALAR) {
+                    a.ts[c] = a.ts[c] - adv;
+                } else if constexpr (MODE == MODE_MOMENTUM) {
+                    if (F == F_U) a.tu[c] = a.tu[c] - adv;
+                    else if (F == F_V) a.tv[c] = a.tv[c] - adv;
+                    else a.tw[c] = a.tw[c] - adv;
+                } else {
+                    // the tendency from zero (+ ν∇²f), then buoyancy / surface flux
+                    T t = T(0) - adv;
+                    const T *fp = F == F_U ? a.in.u : F == F_V ? a.in.v : F == F_W ? a.in.w : a.in.s;
+                    const int loc = F == F_U ? LOC_FX : F == F_V ? LOC_FY : F == F_W ? LOC_FZ : LOC_CC;
+                    if (a.has_visc) t = t + a.nu * lap_at<T>(fp, m, i, j, k, loc);
+                    if (F == F_W && a.has_buoy) {
+                        T tf = m.m_zface(k, a.in.s[lin(i, j, k - 1, m.sx, m.sy)], a.in.s[c]);
+                        t += a.coef * (tf - a.theta0);
+                    }
+                    if (F == F_S && a.has_sflux) {
+                        if (k == H) t += a.fb;
+                        if (k == H + m.Nz - 1) t -= a.ft;
+                    }
+                    const T x = fp[c];
+                    if (F == F_U) a.uo[c] = x + a.dt * t;
+                    else if (F == F_V) a.vo[c] = x + a.dt * t;
+                    else if (F == F_W) a.wo[c] = x + a.dt * t;
+                    else {
+                        T y = x + a.dt * t;
+                        a.so[c] = a.combine ? a.ca * a.s0[c] + a.cb * y : y;
+                    }
+                }
+                fzb[q] = fzt[q];
+            }
+        }
+        buf ^= 1;
+    }
+}
+
+template <int MODE, int S, int P, typename T>
+static void launch_march_sp(hipStream_t st, const MarchArgs<T> &a) {
+    constexpr int BY = 4;
+    const Metrics<T> &m = a.m;
+    dim3 grid((m.Nx + MW - 1) / MW, (m.Ny + BY - 1) / BY, (m.Nz + a.kc - 1) / a.kc);
+    hipLaunchKernelGGL((adv_march_kernel<MODE, S, P, T, BY>), grid, dim3(MW, BY), 0, st, a);
+}
+
+template <int MODE, typename T>
+static int launch_march_mode(hipStream_t st, const MarchArgs<T> &a, int scheme, int p) {
+    switch (scheme) {
+    case SCH_CENTERED: launch_march_sp<MODE, SCH_CENTERED, 0, T>(st, a); return 0;
+    case SCH_UPWIND1: launch_march_sp<MODE, SCH_UPWIND1, 0, T>(st, a); return 0;
+    case SCH_WENO5: launch_march_sp<MODE, SCH_WENO5, 0, T>(st, a); return 0;
+    case SCH_WENO_JS5: launch_march_sp<MODE, SCH_WENO_JS5, 0, T>(st, a); return 0;
+    case SCH_WENO_Z5:
+        if (p == 1) launch_march_sp<MODE, SCH_WENO_Z5, 1, T>(st, a);
+        else launch_march_sp<MODE, SCH_WENO_Z5, 2, T>(st, a);
+        return 0;
+    case SCH_WENO_Z7:
+        if (p == 1) launch_march_sp<MODE, SCH_WENO_Z7, 1, T>(st, a);
+        else launch_march_sp<MODE, SCH_WENO_Z7, 2, T>(st, a);
+        return 0;
+    }
+    return -1;
+}
+
+template <typename T>
+int launch_advect(hipStream_t st, const Metrics<T> &m, int mode, int scheme, int p, const T *u, const T *v,
+                  const T *w, const T *s, T *tu, T *tv, T *tw, T *ts) {
+    MarchArgs<T> a{};
+    a.m = m;
+    a.in = Fields<T>{u, v, w, s};
+    a.tu = tu; a.tv = tv; a.tw = tw; a.ts = ts;
+    a.kc = m.Nz < 32 ? m.Nz : 32;
+    if (mode == MODE_SCALAR) return launch_march_mode<MODE_SCALAR, T>(st, a, scheme, p);
+    return launch_march_mode<MODE_MOMENTUM, T>(st, a, scheme, p);
+}
+template int launch_advect<double>(hipStream_t, const Metrics<double> &, int, int, int, const double *, const double *,
+                                   const double *, const double *, double *, double *, double *, double *);
+template int launch_advect<float>(hipStream_t, const Metrics<float> &, int, int, int, const float *, const float *,
+                                  const float *, const float *, float *, float *, float *, float *);
+
+template <typename T>
+int launch_tendency_step(hipStream_t st, const Metrics<T> &m, int scheme, int p, const T *u, const T *v, const T *w,
+                         const T *s, T *uo, T *vo, T *wo, T *so, const T *s0, const StepPhys &ph, int stage) {
+    MarchArgs<T> a{};
+    a.m = m;
+    a.in = Fields<T>{u, v, w, s};
+    a.uo = uo; a.vo = vo; a.wo = wo; a.so = so; a.s0 = s0;
+    a.nu = T(ph.nu); a.dt = T(ph.dt); a.theta0 = T(ph.theta0);
+    a.coef = T(ph.g_accel / ph.theta0);
+    a.has_visc = ph.nu != 0;
+    a.has_buoy = ph.theta0 != 0;
+    a.has_sflux = ph.flux_bot != 0 || ph.flux_top != 0;
+    // F_bot · 1/Δz_c at the bottom level, F_top · 1/Δz_c at the top (src/FiniteVolume.jl:431-439)
+    a.fb = T(ph.flux_bot * ph.idz_bot);
+    a.ft = T(ph.flux_top * ph.idz_top);
+    stage_coeffs(stage, &ph, &a.ca, &a.cb, &a.combine);
+    a.kc = m.Nz < 32 ? m.Nz : 32;
+    return launch_march_mode<MODE_STEP, T>(st, a, scheme, p);
+}
+template int launch_tendency_step<double>(hipStream_t, const Metrics<double> &, int, int, const double *,
+                                          const double *, const double *, const double *, double *, double *,
+                                          double *, double *, const double *, const StepPhys &, int);
+template int launch_tendency_step<float>(hipStream_t, const Metrics<float> &, int, int, const float *, const float *,
+                                         const float *, const float *, float *, float *, float *, float *,
+                                         const float *, const StepPhys &, int);
+
+// ---------------------------------------------------------------------------
+// _div_at into the interior-sized solver buffer (src/FiniteVolume.jl:312-315,
+// 1869-1888), accumulated in FP64.
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void div_into_kernel(double *__restrict__ buf, const T *__restrict__ u, const T *__restrict__ v,
+                                const T *__restrict__ w, Metrics<double> md) {
+    const int H = md.H, sx = md.sx, sy = md.sy;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y, k = blockIdx.z;
+    if (i >= md.Nx) return;
+    const int I = i + H, J = j + H, K = k + H;
+    const size_t c = lin(I, J, K, sx, sy);
+    double du = (double)u[c + 1] - (double)u[c];
+    double dv = (double)v[c + sx] - (double)v[c];
+    double dw = (double)w[c + (size_t)sx * sy] - (double)w[c];
+    buf[(size_t)i + (size_t)md.Nx * ((size_t)j + (size_t)md.Ny * k)] = (du * md.idx + dv * md.idy) + dw * md.m_idz(K);
+}
+
+template <typename T>
+void launch_div_into(hipStream_t st, double *buf, const T *u, const T *v, const T *w, const Metrics<double> &md) {
+    dim3 grid((md.Nx + 255) / 256, md.Ny, md.Nz);
+    hipLaunchKernelGGL(div_into_kernel<T>, grid, dim3(256), 0, st, buf, u, v, w, md);
+}
+template void launch_div_into<double>(hipStream_t, double *, const double *, const double *, const double *,
+                                      const Metrics<double> &);
+template void launch_div_into<float>(hipStream_t, double *, const float *, const float *, const float *,
+                                     const Metrics<double> &);
+
+// ---------------------------------------------------------------------------
+// _b_correct! (src/FiniteVolume.jl:1823-1833) with the stage combination
+// c(x, snap) = a·snap + b·x (src/Simulations.jl:1192-1194) and p = φ/Δt. φ
+// is the solver's interior-sized FP64 solution; its halo (the velocity BCs
+// at a cell centre: periodic wrap or mirror) is read through the index map.
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void correct_kernel(T *uo, T *vo, T *wo,
+                               const T *__restrict__ us, const T *__restrict__ vs, const T *__restrict__ ws,
+                               const T *u0, const T *v0, const T *w0,
+                               const double *__restrict__ phi, T *__restrict__ p, Metrics<double> md, int bcx,
+                               int bcy, int bcz, double ca, double cb, int combine, double dt) {
+    const int H = md.H, sx = md.sx, sy = md.sy, Nx = md.Nx, Ny = md.Ny, Nz = md.Nz;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y, k = blockIdx.z;
+    if (i >= Nx) return;
+    auto P = [&](int ii, int jj, int kk) {
+        // interior-local index map of φ's unit halo
+        if (ii < 0) ii = bcx == BC_PERIODIC ? ii + Nx : -1 - ii;
+        if (jj < 0) jj = bcy == BC_PERIODIC ? jj + Ny : -1 - jj;
+        if (kk < 0) kk = bcz == BC_PERIODIC ? kk + Nz : -1 - kk;
+        return phi[(size_t)ii + (size_t)Nx * ((size_t)jj + (size_t)Ny * kk)];
+    };
+    const double pc = P(i, j, k);
+    const int K = k + H;
+    const size_t c = lin(i + H, j + H, K, sx, sy);
+    double un = (double)us[c] - (pc - P(i - 1, j, k)) * md.idx;
+    double vn = (double)vs[c] - (pc - P(i, j - 1, k)) * md.idy;
+    double wn = (double)ws[c] - (pc - P(i, j, k - 1)) * md.m_idzf(K);
+    if (combine) {
+        un = ca * (double)u0[c] + cb * un;
+        vn = ca * (double)v0[c] + cb * vn;
+        wn = ca * (double)w0[c] + cb * wn;
+    }
+    uo[c] = (T)un;
+    vo[c] = (T)vn;
+    wo[c] = (T)wn;
+    if (p) p[c] = (T)(pc / dt);
+}
+
+template <typename T>
+void launch_correct(hipStream_t st, T *uo, T *vo, T *wo, const T *us, const T *vs, const T *ws, const T *u0,
+                    const T *v0, const T *w0, const double *phi, T *p, const Metrics<double> &md, int bcx, int bcy,
+                    int bcz, double ca, double cb, int combine, double dt) {
+    dim3 grid((md.Nx + 255) / 256, md.Ny, md.Nz);
+    hipLaunchKernelGGL(correct_kernel<T>, grid, dim3(256), 0, st, uo, vo, wo, us, vs, ws, u0, v0, w0, phi, p, md, bcx,
+                       bcy, bcz, ca, cb, combine, dt);
+}
+template void launch_correct<double>(hipStream_t, double *, double *, double *, const double *, const double *,
+                                     const double *, const double *, const double *, const double *, const double *,
+                                     double *, const Metrics<double> &, int, int, int, double, double, int, double);
+template void launch_correct<float>(hipStream_t, float *, float *, float *, const float *, const float *,
+                                    const float *, const float *, const float *, const float *, const double *,
+                                    float *, const Metrics<double> &, int, int, int, double, double, int, double);
+
+// ---------------------------------------------------------------------------
+// The RK3 combination of the halo shell, out = a·snap + b·x at every storage
+// point outside the interior (_shell_map!, src/Simulations.jl:1249-1286).
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void shell_kernel(T *__restrict__ out, const T *__restrict__ x, const T *__restrict__ snap, int Nx, int Ny,
+                             int Nz, int H, double ca, double cb) {
+    const int sx = Nx + 2 * H, sy = Ny + 2 * H, sz = Nz + 2 * H;
+    const long total = (long)sx * sy * sz;
+    for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < total; t += (long)gridDim.x * blockDim.x) {
+        int i = (int)(t % sx);
+        long q = t / sx;
+        int j = (int)(q % sy), k = (int)(q / sy);
+        bool in = i >= H && i < H + Nx && j >= H && j < H + Ny && k >= H && k < H + Nz;
+        if (in) {
+            // skip the interior rows quickly
+            continue;
+        }
+        // ca = 0, cb = 1 is a copy (step!'s halo of u*), written as one
+        out[t] = (ca == 0.0 && cb == 1.0) ? x[t] : (T)(ca * (double)snap[t] + cb * (double)x[t]);
+    }
+}
+
+template <typename T>
+void launch_shell(hipStream_t st, T *out, const T *x, const T *snap, int Nx, int Ny, int Nz, int H, double ca,
+                  double cb) {
+    long total = (long)(Nx + 2 * H) * (Ny + 2 * H) * (Nz + 2 * H);
+    int blocks = (int)((total + 255) / 256);
+    if (blocks > 8192) blocks = 8192;
+    hipLaunchKernelGGL(shell_kernel<T>, dim3(blocks), dim3(256), 0, st, out, x, snap, Nx, Ny, Nz, H, ca, cb);
+}
+template void launch_shell<double>(hipStream_t, double *, const double *, const double *, int, int, int, int, double,
+                                   double);
+template void launch_shell<float>(hipStream_t, float *, const float *, const float *, int, int, int, int, double,
+                                  double);
+
+// ---------------------------------------------------------------------------
+// Maxima (exact in any order): max |∇·u| (max_divergence, src/Simulations.jl:
+// 1045-1050) and the advective CFL rate (_adv_rate_at, src/TimeIntegration.jl:
+// 172-182). Non-negative doubles order as their bit patterns, so the maximum
+// is an atomicMax on the bits (NaN-propagating, as the reference's max).
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void max_kernel(unsigned long long *__restrict__ out, const T *__restrict__ u, const T *__restrict__ v,
+                           const T *__restrict__ w, Metrics<double> md, int what) {
+    const int H = md.H, sx = md.sx, sy = md.sy;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y, k = blockIdx.z;
+    double r = 0.0;
+    if (i < md.Nx) {
+        const int K = k + H;
+        const size_t c = lin(i + H, j + H, K, sx, sy);
+        if (what == 0) {
+            double du = (double)u[c + 1] - (double)u[c];
+            double dv = (double)v[c + sx] - (double)v[c];
+            double dw = (double)w[c + (size_t)sx * sy] - (double)w[c];
+            r = fabs((du * md.idx + dv * md.idy) + dw * md.m_idz(K));
+        } else {
+            double au = fmax(fabs((double)u[c]), fabs((double)u[c + 1]));
+            double av = fmax(fabs((double)v[c]), fabs((double)v[c + sx]));
+            double aw = fmax(fabs((double)w[c]), fabs((double)w[c + (size_t)sx * sy]));
+            r = (au * md.idx + av * md.idy) + aw * md.m_idz(K);
+        }
+    }
+    unsigned long long bits = __double_as_longlong(r);
+    // wave-level max, then one atomic per wave
+    for (int off = 32; off > 0; off >>= 1) {
+        unsigned long long o = __shfl_down(bits, off);
+        bits = o > bits ? o : bits;
+    }
+    if ((threadIdx.x & 63) == 0) atomicMax(out, bits);
+}
+
+template <typename T>
+void launch_max(hipStream_t st, unsigned long long *out, const T *u, const T *v, const T *w, const Metrics<double> &md,
+                int what) {
+    dim3 grid((md.Nx + 255) / 256, md.Ny, md.Nz);
+    hipLaunchKernelGGL(max_kernel<T>, grid, dim3(256), 0, st, out, u, v, w, md, what);
+}
+template void launch_max<double>(hipStream_t, unsigned long long *, const double *, const double *, const double *,
+                                 const Metrics<double> &, int);
+template void launch_max<float>(hipStream_t, unsigned long long *, const float *, const float *, const float *,
+                                const Metrics<double> &, int);
+
+}  // namespace aeo

@@ -1,0 +1,162 @@
+/*
+ * libaeolus_hip — the C ABI of the MI355X (gfx950) dynamical-core step.
+ *
+ * Drop-in replacement, behind a C ABI, for the CPU path of badulion/Aeolus.jl
+ * (Julia) for one fractional-step dynamical-core step: the WENO advection
+ * tendencies (AdvectionSchemes), the SSP-RK3 stage updates (Simulations) and
+ * the pressure projection (PressureSolvers). Every entry point cites the
+ * reference interface it replaces (paths relative to the reference root).
+ *
+ * Conventions
+ *  - All functions return an int status: AEO_OK (0) or a negative AEO_ERR_*;
+ *    aeo_last_error() gives the text of the last failure (thread-local).
+ *    Nothing throws across the ABI.
+ *  - Field arrays are device pointers in the reference's storage layout
+ *    (src/Grids.jl:98-140): (Nx+2H) x (Ny+2H) x (Nz+2H), column-major, x
+ *    fastest, interior H .. H+N-1 (0-based) on every axis; FaceX/FaceY/FaceZ
+ *    fields share this shape, face i being the west/south/bottom face of cell
+ *    i. `precision` selects the element type: AEO_F64 (double) or AEO_F32
+ *    (float). Solver buffers are FP64.
+ *  - All work of the library is ordered on one HIP stream of the current
+ *    context; calls return when the work is queued (aeo_sync waits).
+ *    Use one context per host thread.
+ */
+#ifndef AEOLUS_HIP_H
+#define AEOLUS_HIP_H
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    AEO_OK = 0,
+    AEO_ERR_ARG = -1,          /* bad argument (reference: ArgumentError / DimensionMismatch) */
+    AEO_ERR_HIP = -2,          /* a HIP runtime call failed                                  */
+    AEO_ERR_UNSUPPORTED = -3,  /* a configuration this build does not implement              */
+    AEO_ERR_STATE = -4         /* no context (aeo_init not called)                           */
+};
+enum { AEO_F64 = 0, AEO_F32 = 1 };
+/* field locations, src/Grids.jl:60-83 */
+enum { AEO_CELL = 0, AEO_FACE_X = 1, AEO_FACE_Y = 2, AEO_FACE_Z = 3 };
+/* per-axis boundary kinds: Periodic, NoFlux (mirror), ImpermeableWall
+ * (src/BoundaryConditions.jl:182-259) */
+enum { AEO_PERIODIC = 0, AEO_NOFLUX = 1, AEO_WALL = 2 };
+/* face reconstructions, advection_scheme names (src/AdvectionSchemes.jl:1704-1775) */
+enum {
+    AEO_CENTERED = 0, /* :centered  (advect_centered!, momentum_advect!)       */
+    AEO_UPWIND = 1,   /* :upwind                                              */
+    AEO_WENO5 = 2,    /* :weno5     (FiniteVolume advect_weno5!)              */
+    AEO_WENO_JS5 = 3, /* :weno_js5                                            */
+    AEO_WENO_Z5 = 4,  /* :weno_z5                                             */
+    AEO_WENO_Z7 = 5   /* :weno_z7                                             */
+};
+/* pressure-solver status, pcg! (src/Backends.jl:1482-1487) */
+enum { AEO_CONVERGED = 0, AEO_MAXIT = 1, AEO_BREAKDOWN = 2, AEO_ZERO_RHS = 3, AEO_DIRECT = 4 };
+
+/* ---- context ------------------------------------------------------------ */
+/* No reference counterpart: selects the device and creates the stream. */
+int aeo_init(int device);
+int aeo_shutdown(void);
+int aeo_sync(void);
+const char *aeo_last_error(void);
+
+/* ---- device memory (replaces the reference's Array storage of Field.data,
+ *      src/Fields.jl:21-39; the caller owns what it allocates here) -------- */
+int aeo_malloc(void **dptr, size_t bytes);
+int aeo_free(void *dptr);
+int aeo_memcpy_h2d(void *dst, const void *src, size_t bytes);
+int aeo_memcpy_d2h(void *dst, const void *src, size_t bytes);
+int aeo_memset(void *dptr, int value, size_t bytes);
+
+/* ---- grids ---------------------------------------------------------------
+ * RegularGrid (src/Grids.jl:98-140) when zf_interior is NULL (uniform dz);
+ * StretchedGrid with (:custom, zf) stretching (src/StretchedGrids.jl:117-402)
+ * when zf_interior holds the Nz+1 interior face heights: the faces are
+ * extended into the halo (_extend_faces, :319-337) and the per-level metrics
+ * built as level_metrics does (:570-600). */
+typedef struct aeo_grid aeo_grid;
+int aeo_grid_create(int Nx, int Ny, int Nz, int H, double dx, double dy, double dz, const double *zf_interior,
+                    aeo_grid **g);
+int aeo_grid_destroy(aeo_grid *g);
+/* the storage-level metric vectors (host copies): idzc (Nz+2H), idzf (Nz+2H+1) */
+int aeo_grid_metrics(const aeo_grid *g, double *idzc, double *idzf);
+
+/* fill_halos!(field, bcs), src/BoundaryConditions.jl:182-259 */
+int aeo_fill_halos(aeo_grid *g, int precision, void *field, int loc, int bc_x, int bc_y, int bc_z);
+
+/* ---- advection operators: the AdvectionScheme callable
+ * scheme(tend, u, v, w, ϕ) (src/AdvectionSchemes.jl:1712-1718), which ADDS
+ * −∂(uⱼϕ)/∂xⱼ to tend; `p` is the WENO-Z τ exponent (1 or 2). Halos of the
+ * inputs must be filled and H >= required_halo(scheme) (3; 4 for WENO-Z7). */
+int aeo_advect_scalar(aeo_grid *g, int precision, int scheme, int p, void *tend, const void *u, const void *v,
+                      const void *w, const void *phi);
+/* momentum_advect_weno_z!(tu, tv, tw, u, v, w) (src/AdvectionSchemes.jl:1195-1270)
+ * and the momentum_advection keyword of step! (src/Simulations.jl:418-425):
+ * ADDS −∂(uⱼuᵢ)/∂xⱼ to the face-centred tendencies. */
+int aeo_advect_momentum(aeo_grid *g, int precision, int scheme, int p, void *tu, void *tv, void *tw, const void *u,
+                        const void *v, const void *w);
+
+/* ---- pressure solvers: AbstractPressureSolver (src/PressureSolvers.jl:19-80) */
+typedef struct aeo_poisson aeo_poisson;
+/* FFTPoissonSolver(mesh), src/PressureSolvers.jl:103-174 (triply periodic) */
+int aeo_poisson_fft_create(int Nx, int Ny, int Nz, double dx, double dy, double dz, aeo_poisson **h);
+/* ColumnTridiagonalSolver(T, (Nx,Ny,Nz), (dx,dy); lower, upper, hweight,
+ * measure, inv_coupling, gauge, bc_x, bc_y), src/PressureSolvers.jl:626-666.
+ * Vectors are host arrays of length Nz (inv_coupling Nz+1). Lateral BCs:
+ * AEO_PERIODIC only in this build. */
+int aeo_poisson_column_create(int Nx, int Ny, int Nz, double dx, double dy, int bc_x, int bc_y,
+                              const double *lower, const double *upper, const double *hweight,
+                              const double *measure, const double *inv_coupling, const double *gauge,
+                              aeo_poisson **h);
+/* source_buffer(solver) (src/PressureSolvers.jl:57-69): the device FP64
+ * interior-sized (Nx, Ny, Nz) array the caller may write the source into */
+int aeo_poisson_source_buffer(aeo_poisson *h, double **dptr);
+/* solve_source_buffer!(p, solver) (:71-80): φ into the device FP64
+ * interior-sized array phi (the buffer is overwritten) */
+int aeo_poisson_solve_source_buffer(aeo_poisson *h, double *phi);
+/* solve!(p, source, solver) (:19-41): storage-layout fields of halo H; p's
+ * interior is written, its halo untouched */
+int aeo_poisson_solve(aeo_poisson *h, int precision, void *p, const void *source, int H);
+/* solver_stats(solver) (:43-55): (iterations, residual); direct solvers (0, NaN) */
+int aeo_poisson_stats(aeo_poisson *h, int *iterations, double *residual, int *status);
+int aeo_poisson_destroy(aeo_poisson *h);
+
+/* ---- the step: BoussinesqSimulation + step_rk3! / step! -------------------
+ * (src/Simulations.jl:168-288, 531-650, 711-744, 843-904) with the momentum
+ * advection and the scalar (θ) advection of the given schemes. */
+typedef struct {
+    int mom_scheme, scal_scheme, weno_p;
+    double nu, theta0, g_accel, flux_bot, flux_top;
+    int bc_x, bc_y, bc_z;    /* velocity BCs (sim.bcs)  */
+    int tbc_x, tbc_y, tbc_z; /* θ BCs (sim.θ_bcs)        */
+    int precision;           /* of the state arrays and the tendency arithmetic */
+} aeo_step_params;
+typedef struct aeo_stepper aeo_stepper;
+int aeo_stepper_create(aeo_grid *g, const aeo_step_params *sp, aeo_poisson *solver, aeo_stepper **h);
+int aeo_stepper_destroy(aeo_stepper *h);
+/* step_rk3!(sim, dt): u, v, w, theta, p are the state arrays (device, storage
+ * layout), updated in place */
+int aeo_step_rk3(aeo_stepper *h, void *u, void *v, void *w, void *theta, void *p, double dt);
+/* step!(sim, dt) (src/Simulations.jl:474-487): one projected forward-Euler step */
+int aeo_step_euler(aeo_stepper *h, void *u, void *v, void *w, void *theta, void *p, double dt);
+
+/* ---- diagnostics (halos of the inputs filled) ------------------------------ */
+/* max_divergence (src/Simulations.jl:1045-1050) */
+int aeo_max_divergence(aeo_grid *g, int precision, const void *u, const void *v, const void *w, double *out);
+/* _max_advective_rate (src/TimeIntegration.jl:164-182): dt = C / rate */
+int aeo_max_advective_rate(aeo_grid *g, int precision, const void *u, const void *v, const void *w, double *out);
+
+/* ---- kernel timing (no reference counterpart; the timing the benchmark and
+ * the roofline read): HIP events around every launch of the named kernels on
+ * the library's stream */
+int aeo_profile_enable(int on);
+int aeo_profile_reset(void);
+/* kernel names: "tendency", "advect", "halo", "divergence", "correct",
+ * "poisson", "shell" */
+int aeo_profile_get(const char *kernel, int *count, double *total_ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,1738 @@
+/*
+ * CPU oracle for the fractional-step dynamical-core step (see aeolus_oracle.h).
+ *
+ * TEST INFRASTRUCTURE ONLY: the checker of the parity tests and the timed
+ * CPU baseline. Never linked into the product library.
+ *
+ * Every function restates a reference operator (badulion/Aeolus.jl, Julia)
+ * in the reference's own evaluation order; Julia evaluates `a + b + c` as
+ * `(a + b) + c`, `a * b * c` as `(a * b) * c`, `x^2` as `x * x`, and does
+ * not contract to FMA, so this file is compiled with -ffp-contract=off.
+ * Indices below are 1-based, as in the reference, through the ID macro.
+ */
+#include "aeolus_oracle.h"
+
+#include <float.h>
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define ID(i, j, k) \
+    ((size_t)((i) - 1) + (size_t)sx * ((size_t)((j) - 1) + (size_t)sy * (size_t)((k) - 1)))
+
+#define GRID_DIMS(g)            \
+    const int sx = (g)->Nx + 2 * (g)->H; \
+    const int sy = (g)->Ny + 2 * (g)->H; \
+    const int sz = (g)->Nz + 2 * (g)->H; \
+    (void)sz
+
+/* ------------------------------------------------------------------------
+ * Grid metrics accessors: src/FiniteVolume.jl:130-186
+ * ------------------------------------------------------------------------ */
+static inline double m_idz(const orc_grid *g, int k) { return g->stretched ? g->idzc[k - 1] : 1.0 / g->dz; }
+static inline double m_idzf(const orc_grid *g, int k) { return g->stretched ? g->idzf[k - 1] : 1.0 / g->dz; }
+static inline double m_zface(const orc_grid *g, int k, double lo, double hi) {
+    return g->stretched ? g->wlo[k - 1] * lo + g->whi[k - 1] * hi : 0.5 * (lo + hi);
+}
+static inline double m_zflux(const orc_grid *g, int k, double lo, double hi) {
+    return g->stretched ? g->alo[k - 1] * lo + g->ahi[k - 1] * hi : 0.5 * (lo + hi);
+}
+/* _lapz, src/FiniteVolume.jl:166,177-181 */
+static inline double m_lapz(const orc_grid *g, int loc, int k, double fm, double c, double fp) {
+    if (!g->stretched) {
+        double idz2 = 1.0 / (g->dz * g->dz);
+        return ((fp - 2 * c) + fm) * idz2;
+    }
+    if (loc == ORC_FZ)
+        return ((fp - c) * g->idzc[k - 1] - (c - fm) * g->idzc[k - 2]) * g->idzf[k - 1];
+    return ((fp - c) * g->idzf[k] - (c - fm) * g->idzf[k - 1]) * g->idzc[k - 1];
+}
+
+/* ------------------------------------------------------------------------
+ * Reconstructions
+ * ------------------------------------------------------------------------ */
+/* _beta3, src/AdvectionSchemes.jl:105 */
+static inline double beta3(double a1, double a2) { return a1 * a1 + (13.0 / 3.0) * (a2 * a2); }
+/* _beta4, src/AdvectionSchemes.jl:107-109 */
+static inline double beta4(double a1, double a2, double a3) {
+    return ((a1 * a1 + (1.0 / 2.0) * (a1 * a3)) + (13.0 / 3.0) * (a2 * a2)) + (3129.0 / 80.0) * (a3 * a3);
+}
+/* _b3_m2/_m1/_p0, src/AdvectionSchemes.jl:117-119 */
+static inline double b3_m2(double v1, double v2, double v3) {
+    return beta3(((v1 - 4 * v2) + 3 * v3) / 2, ((v1 - 2 * v2) + v3) / 2);
+}
+static inline double b3_m1(double v1, double v2, double v3) {
+    return beta3((v3 - v1) / 2, ((v1 - 2 * v2) + v3) / 2);
+}
+static inline double b3_p0(double v1, double v2, double v3) {
+    return beta3(((-3 * v1 + 4 * v2) - v3) / 2, ((v1 - 2 * v2) + v3) / 2);
+}
+/* _b4_*, src/AdvectionSchemes.jl:122-140 */
+static inline double b4_m3(double v1, double v2, double v3, double v4) {
+    return beta4((((-7 * v1 + 33 * v2) - 69 * v3) + 43 * v4) / 24, (((-v1 + 4 * v2) - 5 * v3) + 2 * v4) / 2,
+                 (((-v1 + 3 * v2) - 3 * v3) + v4) / 6);
+}
+static inline double b4_p0(double v1, double v2, double v3, double v4) {
+    return beta4((((-43 * v1 + 69 * v2) - 33 * v3) + 7 * v4) / 24, (((2 * v1 - 5 * v2) + 4 * v3) - v4) / 2,
+                 (((-v1 + 3 * v2) - 3 * v3) + v4) / 6);
+}
+static inline double b4_m2(double v1, double v2, double v3, double v4) {
+    return beta4((((5 * v1 - 27 * v2) + 15 * v3) + 7 * v4) / 24, ((v2 - 2 * v3) + v4) / 2,
+                 (((-v1 + 3 * v2) - 3 * v3) + v4) / 6);
+}
+static inline double b4_m1(double v1, double v2, double v3, double v4) {
+    return beta4((((-7 * v1 - 15 * v2) + 27 * v3) - 5 * v4) / 24, ((v1 - 2 * v2) + v3) / 2,
+                 (((-v1 + 3 * v2) - 3 * v3) + v4) / 6);
+}
+/* _ipow, src/AdvectionSchemes.jl:173-180 (p = 1, 2) */
+static inline double ipow(double x, int p) { return p == 2 ? x * x : x; }
+
+/* _weno_z5_left, src/AdvectionSchemes.jl:240-255; _zeps = eps(T)^2 (:152) */
+double orc_weno_z5_left(double v1, double v2, double v3, double v4, double v5, int p) {
+    double q0 = ((2 * v1 - 7 * v2) + 11 * v3) / 6;
+    double q1 = ((-v2 + 5 * v3) + 2 * v4) / 6;
+    double q2 = ((2 * v3 + 5 * v4) - v5) / 6;
+    double b0 = b3_m2(v1, v2, v3);
+    double b1 = b3_m1(v2, v3, v4);
+    double b2 = b3_p0(v3, v4, v5);
+    double eps = DBL_EPSILON * DBL_EPSILON;
+    double tau = fabs(b0 - b2);
+    double a0 = (1.0 / 10.0) * (1.0 + ipow(tau / (b0 + eps), p));
+    double a1 = (6.0 / 10.0) * (1.0 + ipow(tau / (b1 + eps), p));
+    double a2 = (3.0 / 10.0) * (1.0 + ipow(tau / (b2 + eps), p));
+    return ((a0 * q0 + a1 * q1) + a2 * q2) / ((a0 + a1) + a2);
+}
+
+/* _weno_z7_left, src/AdvectionSchemes.jl:284-303 */
+double orc_weno_z7_left(double v1, double v2, double v3, double v4, double v5, double v6, double v7, int p) {
+    double q0 = (((-3 * v1 + 13 * v2) - 23 * v3) + 25 * v4) / 12;
+    double q1 = (((v2 - 5 * v3) + 13 * v4) + 3 * v5) / 12;
+    double q2 = (((-v3 + 7 * v4) + 7 * v5) - v6) / 12;
+    double q3 = (((3 * v4 + 13 * v5) - 5 * v6) + v7) / 12;
+    double b0 = b4_m3(v1, v2, v3, v4);
+    double b1 = b4_m2(v2, v3, v4, v5);
+    double b2 = b4_m1(v3, v4, v5, v6);
+    double b3 = b4_p0(v4, v5, v6, v7);
+    double eps = DBL_EPSILON * DBL_EPSILON;
+    double tau = fabs(((b0 - b1) - b2) + b3);
+    double a0 = (1.0 / 35.0) * (1.0 + ipow(tau / (b0 + eps), p));
+    double a1 = (12.0 / 35.0) * (1.0 + ipow(tau / (b1 + eps), p));
+    double a2 = (18.0 / 35.0) * (1.0 + ipow(tau / (b2 + eps), p));
+    double a3 = (4.0 / 35.0) * (1.0 + ipow(tau / (b3 + eps), p));
+    return (((a0 * q0 + a1 * q1) + a2 * q2) + a3 * q3) / (((a0 + a1) + a2) + a3);
+}
+
+/* weno_js5_left, src/AdvectionSchemes.jl:319-342 */
+double orc_weno_js5_left(double v1, double v2, double v3, double v4, double v5, double e) {
+    double q0 = ((2 * v1 - 7 * v2) + 11 * v3) / 6;
+    double q1 = ((-v2 + 5 * v3) + 2 * v4) / 6;
+    double q2 = ((2 * v3 + 5 * v4) - v5) / 6;
+    double b0 = b3_m2(v1, v2, v3);
+    double b1 = b3_m1(v2, v3, v4);
+    double b2 = b3_p0(v3, v4, v5);
+    double d0 = b0 + e, d1 = b1 + e, d2 = b2 + e;
+    double a0 = (1.0 / 10.0) / (d0 * d0);
+    double a1 = (6.0 / 10.0) / (d1 * d1);
+    double a2 = (3.0 / 10.0) / (d2 * d2);
+    return ((a0 * q0 + a1 * q1) + a2 * q2) / ((a0 + a1) + a2);
+}
+
+/* FiniteVolume._weno5_left, src/FiniteVolume.jl:1582-1596 */
+double orc_weno5_classic_left(double fm2, double fm1, double fc, double fp1, double fp2) {
+    double q0 = ((1.0 / 3.0) * fm2 - (7.0 / 6.0) * fm1) + (11.0 / 6.0) * fc;
+    double q1 = ((-1.0 / 6.0) * fm1 + (5.0 / 6.0) * fc) + (1.0 / 3.0) * fp1;
+    double q2 = ((1.0 / 3.0) * fc + (5.0 / 6.0) * fp1) - (1.0 / 6.0) * fp2;
+    double t0 = (fm2 - 2 * fm1) + fc, s0 = (fm2 - 4 * fm1) + 3 * fc;
+    double t1 = (fm1 - 2 * fc) + fp1, s1 = fm1 - fp1;
+    double t2 = (fc - 2 * fp1) + fp2, s2 = (3 * fc - 4 * fp1) + fp2;
+    double b0 = (13.0 / 12.0) * (t0 * t0) + (1.0 / 4.0) * (s0 * s0);
+    double b1 = (13.0 / 12.0) * (t1 * t1) + (1.0 / 4.0) * (s1 * s1);
+    double b2 = (13.0 / 12.0) * (t2 * t2) + (1.0 / 4.0) * (s2 * s2);
+    double e = 1e-6;
+    double d0 = b0 + e, d1 = b1 + e, d2 = b2 + e;
+    double a0 = (1.0 / 10.0) / (d0 * d0);
+    double a1 = (6.0 / 10.0) / (d1 * d1);
+    double a2 = (3.0 / 10.0) / (d2 * d2);
+    double s = (a0 + a1) + a2;
+    return ((a0 * q0 + a1 * q1) + a2 * q2) / s;
+}
+
+/* reconstruct_left of a descriptor on the tuple t (src/AdvectionSchemes.jl:572-596) */
+static inline double recon_left(int scheme, int p, const double *t) {
+    switch (scheme) {
+    case ORC_CENTERED: return (t[0] + t[1]) / 2;
+    case ORC_UPWIND1: return t[0];
+    case ORC_WENO5: return orc_weno5_classic_left(t[0], t[1], t[2], t[3], t[4]);
+    case ORC_WENO_JS5: return orc_weno_js5_left(t[0], t[1], t[2], t[3], t[4], 1e-6);
+    case ORC_WENO_Z5: return orc_weno_z5_left(t[0], t[1], t[2], t[3], t[4], p);
+    case ORC_WENO_Z7: return orc_weno_z7_left(t[0], t[1], t[2], t[3], t[4], t[5], t[6], p);
+    }
+    return NAN;
+}
+/* stencil_left/right geometry (src/AdvectionSchemes.jl:553-565, 1781-1784):
+ * (offset of first cell relative to the left cell c, width) */
+static void stencil_geom(int scheme, int *lo_l, int *lo_r, int *n) {
+    switch (scheme) {
+    case ORC_CENTERED: *lo_l = 0; *lo_r = 0; *n = 2; break;
+    case ORC_UPWIND1: *lo_l = 0; *lo_r = 1; *n = 1; break;
+    case ORC_WENO_Z7: *lo_l = -3; *lo_r = -2; *n = 7; break;
+    default: *lo_l = -2; *lo_r = -1; *n = 5; break;
+    }
+}
+
+/* The upwinded face value (_face_x/_face_y/_face_z + _upwind_value,
+ * src/AdvectionSchemes.jl:623-666): the left stencil for vel >= 0, else the
+ * reversed right stencil, into one reconstruct_left. `d` is read at
+ * base + (c + m) * stride. Centered2 takes the left tuple whatever vel. */
+static inline double face_value(int scheme, int p, const double *d, size_t base, ptrdiff_t stride,
+                                int c, double vel) {
+    int lo_l, lo_r, n;
+    double t[8];
+    stencil_geom(scheme, &lo_l, &lo_r, &n);
+    if (scheme == ORC_CENTERED || vel >= 0) {
+        for (int m = 0; m < n; m++) t[m] = d[base + (ptrdiff_t)(c + lo_l + m) * stride];
+    } else {
+        for (int m = 0; m < n; m++) t[m] = d[base + (ptrdiff_t)(c + lo_r + n - 1 - m) * stride];
+    }
+    return recon_left(scheme, p, t);
+}
+
+/* ------------------------------------------------------------------------
+ * Halo fills: src/BoundaryConditions.jl:182-259, halo_kernels.jl:100-147
+ * ------------------------------------------------------------------------ */
+static void halo_axis(double *d, int sx, int sy, int sz, int ax, int H, int N, int kind, int loc) {
+    int n1 = sx, n2 = sy, n3 = sz;
+    int len = ax == 1 ? n1 : (ax == 2 ? n2 : n3);
+    int t1 = ax == 1 ? n2 : n1, t2 = ax == 3 ? n2 : n3;
+#define AXID(l, a, b)                                                                     \
+    (ax == 1 ? ((size_t)((l) - 1) + (size_t)n1 * ((size_t)((a) - 1) + (size_t)n2 * ((b) - 1))) \
+     : ax == 2 ? ((size_t)((a) - 1) + (size_t)n1 * ((size_t)((l) - 1) + (size_t)n2 * ((b) - 1))) \
+               : ((size_t)((a) - 1) + (size_t)n1 * ((size_t)((b) - 1) + (size_t)n2 * ((l) - 1))))
+    if (kind == ORC_WALL && ax == 3 && loc == ORC_FZ) {
+        /* zero the wall faces H+1, H+N+1; antisymmetric about them (:252-259) */
+        for (int b = 1; b <= t2; b++)
+            for (int a = 1; a <= t1; a++) {
+                d[AXID(H + 1, a, b)] = 0.0;
+                if (H + N + 1 <= len) d[AXID(H + N + 1, a, b)] = 0.0;
+            }
+        for (int b = 1; b <= t2; b++)
+            for (int a = 1; a <= t1; a++)
+                for (int h = 1; h <= H; h++) {
+                    d[AXID(H + 1 - h, a, b)] = -d[AXID(H + 1 + h, a, b)];
+                    int dst = H + N + 1 + h;
+                    if (dst <= len) d[AXID(dst, a, b)] = -d[AXID(H + N + 1 - h, a, b)];
+                }
+        return;
+    }
+    for (int b = 1; b <= t2; b++)
+        for (int a = 1; a <= t1; a++)
+            for (int h = 1; h <= H; h++) {
+                if (kind == ORC_PERIODIC) {
+                    d[AXID(h, a, b)] = d[AXID(N + h, a, b)];
+                    d[AXID(H + N + h, a, b)] = d[AXID(H + h, a, b)];
+                } else { /* NoFlux, or a wall for a non-FaceZ field: mirror */
+                    d[AXID(H + 1 - h, a, b)] = d[AXID(H + h, a, b)];
+                    d[AXID(H + N + h, a, b)] = d[AXID(H + N + 1 - h, a, b)];
+                }
+            }
+#undef AXID
+}
+
+void orc_fill_halos(double *a, const orc_grid *g, int loc, int bcx, int bcy, int bcz) {
+    GRID_DIMS(g);
+    halo_axis(a, sx, sy, sz, 1, g->H, g->Nx, bcx, loc);
+    halo_axis(a, sx, sy, sz, 2, g->H, g->Ny, bcy, loc);
+    halo_axis(a, sx, sy, sz, 3, g->H, g->Nz, bcz, loc);
+}
+
+/* ------------------------------------------------------------------------
+ * Scalar advection
+ * ------------------------------------------------------------------------ */
+/* _adv_centered_at, src/FiniteVolume.jl:1682-1693 */
+static inline double adv_centered_at(const orc_grid *g, int sx, int sy, int i, int j, int k, const double *u,
+                                     const double *v, const double *w, const double *d) {
+    double idx = 1.0 / g->dx, idy = 1.0 / g->dy;
+    double ue = u[ID(i + 1, j, k)] * 0.5 * (d[ID(i, j, k)] + d[ID(i + 1, j, k)]);
+    double uw = u[ID(i, j, k)] * 0.5 * (d[ID(i - 1, j, k)] + d[ID(i, j, k)]);
+    double vn = v[ID(i, j + 1, k)] * 0.5 * (d[ID(i, j, k)] + d[ID(i, j + 1, k)]);
+    double vs = v[ID(i, j, k)] * 0.5 * (d[ID(i, j - 1, k)] + d[ID(i, j, k)]);
+    double wt = w[ID(i, j, k + 1)] * 0.5 * (d[ID(i, j, k)] + d[ID(i, j, k + 1)]);
+    double wb = w[ID(i, j, k)] * 0.5 * (d[ID(i, j, k - 1)] + d[ID(i, j, k)]);
+    return ((ue - uw) * idx + (vn - vs) * idy) + (wt - wb) * m_idz(g, k);
+}
+
+/* The flux divergence of _b_recon! (src/AdvectionSchemes.jl:685-697) and
+ * of _b_weno5! (src/FiniteVolume.jl:1601-1635, same values), with the
+ * vertical metric 1/dz_c[k] on a stretched grid. */
+static inline double adv_recon_at(const orc_grid *g, int sx, int sy, int scheme, int p, int i, int j, int k,
+                                  const double *u, const double *v, const double *w, const double *d) {
+    double idx = 1.0 / g->dx, idy = 1.0 / g->dy, idz = m_idz(g, k);
+    size_t bx = ID(1, j, k) - 0, by = ID(i, 1, k), bz = ID(i, j, 1);
+    ptrdiff_t stx = 1, sty = sx, stz = (ptrdiff_t)sx * sy;
+    /* base indices address element index c via base + (c-1)*stride */
+    bx -= stx; by -= sty; bz -= stz;
+    double uw = u[ID(i, j, k)], ue = u[ID(i + 1, j, k)];
+    double Fx = (ue * face_value(scheme, p, d, bx, stx, i, ue) - uw * face_value(scheme, p, d, bx, stx, i - 1, uw)) * idx;
+    double vs = v[ID(i, j, k)], vn = v[ID(i, j + 1, k)];
+    double Fy = (vn * face_value(scheme, p, d, by, sty, j, vn) - vs * face_value(scheme, p, d, by, sty, j - 1, vs)) * idy;
+    double wb = w[ID(i, j, k)], wt = w[ID(i, j, k + 1)];
+    double Fz = (wt * face_value(scheme, p, d, bz, stz, k, wt) - wb * face_value(scheme, p, d, bz, stz, k - 1, wb)) * idz;
+    return (Fx + Fy) + Fz;
+}
+
+static inline double scalar_adv_at(const orc_grid *g, int sx, int sy, int scheme, int p, int i, int j, int k,
+                                   const double *u, const double *v, const double *w, const double *d) {
+    if (scheme == ORC_CENTERED) return adv_centered_at(g, sx, sy, i, j, k, u, v, w, d);
+    return adv_recon_at(g, sx, sy, scheme, p, i, j, k, u, v, w, d);
+}
+
+void orc_advect_scalar(double *tend, const double *u, const double *v, const double *w, const double *phi,
+                       const orc_grid *g, int scheme, int p) {
+    GRID_DIMS(g);
+    int H = g->H;
+#pragma omp parallel for schedule(static)
+    for (int k = H + 1; k <= H + g->Nz; k++)
+        for (int j = H + 1; j <= H + g->Ny; j++)
+            for (int i = H + 1; i <= H + g->Nx; i++)
+                tend[ID(i, j, k)] -= scalar_adv_at(g, sx, sy, scheme, p, i, j, k, u, v, w, phi);
+}
+
+/* ------------------------------------------------------------------------
+ * Momentum advection
+ * ------------------------------------------------------------------------ */
+/* _mom_u_at/_mom_v_at/_mom_w_at, src/FiniteVolume.jl:730-797 */
+static inline double mom_u_at(const orc_grid *g, int sx, int sy, int i, int j, int k, const double *u,
+                              const double *v, const double *w) {
+    double idx = 1.0 / g->dx, idy = 1.0 / g->dy;
+    double uce = 0.5 * (u[ID(i, j, k)] + u[ID(i + 1, j, k)]);
+    double ucw = 0.5 * (u[ID(i - 1, j, k)] + u[ID(i, j, k)]);
+    double d_uu_dx = (uce * uce - ucw * ucw) * idx;
+    double vun = 0.5 * (v[ID(i - 1, j + 1, k)] + v[ID(i, j + 1, k)]);
+    double uvn = 0.5 * (u[ID(i, j, k)] + u[ID(i, j + 1, k)]);
+    double vus = 0.5 * (v[ID(i - 1, j, k)] + v[ID(i, j, k)]);
+    double uvs = 0.5 * (u[ID(i, j - 1, k)] + u[ID(i, j, k)]);
+    double d_vu_dy = (vun * uvn - vus * uvs) * idy;
+    double wut = 0.5 * (w[ID(i - 1, j, k + 1)] + w[ID(i, j, k + 1)]);
+    double uwt = 0.5 * (u[ID(i, j, k)] + u[ID(i, j, k + 1)]);
+    double wub = 0.5 * (w[ID(i - 1, j, k)] + w[ID(i, j, k)]);
+    double uwb = 0.5 * (u[ID(i, j, k - 1)] + u[ID(i, j, k)]);
+    double d_wu_dz = (wut * uwt - wub * uwb) * m_idz(g, k);
+    return (d_uu_dx + d_vu_dy) + d_wu_dz;
+}
+static inline double mom_v_at(const orc_grid *g, int sx, int sy, int i, int j, int k, const double *u,
+                              const double *v, const double *w) {
+    double idx = 1.0 / g->dx, idy = 1.0 / g->dy;
+    double uve = 0.5 * (u[ID(i + 1, j - 1, k)] + u[ID(i + 1, j, k)]);
+    double vue = 0.5 * (v[ID(i, j, k)] + v[ID(i + 1, j, k)]);
+    double uvw = 0.5 * (u[ID(i, j - 1, k)] + u[ID(i, j, k)]);
+    double vuw = 0.5 * (v[ID(i - 1, j, k)] + v[ID(i, j, k)]);
+    double d_uv_dx = (uve * vue - uvw * vuw) * idx;
+    double vcn = 0.5 * (v[ID(i, j, k)] + v[ID(i, j + 1, k)]);
+    double vcs = 0.5 * (v[ID(i, j - 1, k)] + v[ID(i, j, k)]);
+    double d_vv_dy = (vcn * vcn - vcs * vcs) * idy;
+    double wvt = 0.5 * (w[ID(i, j - 1, k + 1)] + w[ID(i, j, k + 1)]);
+    double vwt = 0.5 * (v[ID(i, j, k)] + v[ID(i, j, k + 1)]);
+    double wvb = 0.5 * (w[ID(i, j - 1, k)] + w[ID(i, j, k)]);
+    double vwb = 0.5 * (v[ID(i, j, k - 1)] + v[ID(i, j, k)]);
+    double d_wv_dz = (wvt * vwt - wvb * vwb) * m_idz(g, k);
+    return (d_uv_dx + d_vv_dy) + d_wv_dz;
+}
+static inline double mom_w_at(const orc_grid *g, int sx, int sy, int i, int j, int k, const double *u,
+                              const double *v, const double *w) {
+    double idx = 1.0 / g->dx, idy = 1.0 / g->dy;
+    double uwe = m_zflux(g, k, u[ID(i + 1, j, k - 1)], u[ID(i + 1, j, k)]);
+    double wue = 0.5 * (w[ID(i, j, k)] + w[ID(i + 1, j, k)]);
+    double uww = m_zflux(g, k, u[ID(i, j, k - 1)], u[ID(i, j, k)]);
+    double wuw = 0.5 * (w[ID(i - 1, j, k)] + w[ID(i, j, k)]);
+    double d_uw_dx = (uwe * wue - uww * wuw) * idx;
+    double vwn = m_zflux(g, k, v[ID(i, j + 1, k - 1)], v[ID(i, j + 1, k)]);
+    double wvn = 0.5 * (w[ID(i, j, k)] + w[ID(i, j + 1, k)]);
+    double vws = m_zflux(g, k, v[ID(i, j, k - 1)], v[ID(i, j, k)]);
+    double wvs = 0.5 * (w[ID(i, j - 1, k)] + w[ID(i, j, k)]);
+    double d_vw_dy = (vwn * wvn - vws * wvs) * idy;
+    double wct = 0.5 * (w[ID(i, j, k)] + w[ID(i, j, k + 1)]);
+    double wcb = 0.5 * (w[ID(i, j, k - 1)] + w[ID(i, j, k)]);
+    double d_ww_dz = (wct * wct - wcb * wcb) * m_idzf(g, k);
+    return (d_uw_dx + d_vw_dy) + d_ww_dz;
+}
+
+/* _b_mrecon_u!/_v!/_w!, src/AdvectionSchemes.jl:1203-1256. On a stretched
+ * grid the vertical flux difference is divided by the control-volume height
+ * (dz_c for u, v; dz_f for w) and the transporting velocity on the
+ * horizontal faces of a w volume is the volume-flux mean (_zflux), the
+ * metrics of the centred operator above. */
+static inline double mrecon_u_at(const orc_grid *g, int sx, int sy, int sc, int p, int i, int j, int k,
+                                 const double *u, const double *v, const double *w) {
+    double idx = 1.0 / g->dx, idy = 1.0 / g->dy, idz = m_idz(g, k);
+    ptrdiff_t stx = 1, sty = sx, stz = (ptrdiff_t)sx * sy;
+    size_t bx = ID(1, j, k) - stx, by = ID(i, 1, k) - sty, bz = ID(i, j, 1) - stz;
+    double Ue = (u[ID(i, j, k)] + u[ID(i + 1, j, k)]) / 2;
+    double Uw = (u[ID(i - 1, j, k)] + u[ID(i, j, k)]) / 2;
+    double d_uu_dx = (Ue * face_value(sc, p, u, bx, stx, i, Ue) - Uw * face_value(sc, p, u, bx, stx, i - 1, Uw)) * idx;
+    double Vn = (v[ID(i - 1, j + 1, k)] + v[ID(i, j + 1, k)]) / 2;
+    double Vs = (v[ID(i - 1, j, k)] + v[ID(i, j, k)]) / 2;
+    double d_vu_dy = (Vn * face_value(sc, p, u, by, sty, j, Vn) - Vs * face_value(sc, p, u, by, sty, j - 1, Vs)) * idy;
+    double Wt = (w[ID(i - 1, j, k + 1)] + w[ID(i, j, k + 1)]) / 2;
+    double Wb = (w[ID(i - 1, j, k)] + w[ID(i, j, k)]) / 2;
+    double d_wu_dz = (Wt * face_value(sc, p, u, bz, stz, k, Wt) - Wb * face_value(sc, p, u, bz, stz, k - 1, Wb)) * idz;
+    return (d_uu_dx + d_vu_dy) + d_wu_dz;
+}
+static inline double mrecon_v_at(const orc_grid *g, int sx, int sy, int sc, int p, int i, int j, int k,
+                                 const double *u, const double *v, const double *w) {
+    double idx = 1.0 / g->dx, idy = 1.0 / g->dy, idz = m_idz(g, k);
+    ptrdiff_t stx = 1, sty = sx, stz = (ptrdiff_t)sx * sy;
+    size_t bx = ID(1, j, k) - stx, by = ID(i, 1, k) - sty, bz = ID(i, j, 1) - stz;
+    double Ue = (u[ID(i + 1, j - 1, k)] + u[ID(i + 1, j, k)]) / 2;
+    double Uw = (u[ID(i, j - 1, k)] + u[ID(i, j, k)]) / 2;
+    double d_uv_dx = (Ue * face_value(sc, p, v, bx, stx, i, Ue) - Uw * face_value(sc, p, v, bx, stx, i - 1, Uw)) * idx;
+    double Vn = (v[ID(i, j, k)] + v[ID(i, j + 1, k)]) / 2;
+    double Vs = (v[ID(i, j - 1, k)] + v[ID(i, j, k)]) / 2;
+    double d_vv_dy = (Vn * face_value(sc, p, v, by, sty, j, Vn) - Vs * face_value(sc, p, v, by, sty, j - 1, Vs)) * idy;
+    double Wt = (w[ID(i, j - 1, k + 1)] + w[ID(i, j, k + 1)]) / 2;
+    double Wb = (w[ID(i, j - 1, k)] + w[ID(i, j, k)]) / 2;
+    double d_wv_dz = (Wt * face_value(sc, p, v, bz, stz, k, Wt) - Wb * face_value(sc, p, v, bz, stz, k - 1, Wb)) * idz;
+    return (d_uv_dx + d_vv_dy) + d_wv_dz;
+}
+static inline double zmean(const orc_grid *g, int k, double lo, double hi) {
+    return g->stretched ? m_zflux(g, k, lo, hi) : (lo + hi) / 2;
+}
+static inline double mrecon_w_at(const orc_grid *g, int sx, int sy, int sc, int p, int i, int j, int k,
+                                 const double *u, const double *v, const double *w) {
+    double idx = 1.0 / g->dx, idy = 1.0 / g->dy, idz = m_idzf(g, k);
+    ptrdiff_t stx = 1, sty = sx, stz = (ptrdiff_t)sx * sy;
+    size_t bx = ID(1, j, k) - stx, by = ID(i, 1, k) - sty, bz = ID(i, j, 1) - stz;
+    double Ue = zmean(g, k, u[ID(i + 1, j, k - 1)], u[ID(i + 1, j, k)]);
+    double Uw = zmean(g, k, u[ID(i, j, k - 1)], u[ID(i, j, k)]);
+    double d_uw_dx = (Ue * face_value(sc, p, w, bx, stx, i, Ue) - Uw * face_value(sc, p, w, bx, stx, i - 1, Uw)) * idx;
+    double Vn = zmean(g, k, v[ID(i, j + 1, k - 1)], v[ID(i, j + 1, k)]);
+    double Vs = zmean(g, k, v[ID(i, j, k - 1)], v[ID(i, j, k)]);
+    double d_vw_dy = (Vn * face_value(sc, p, w, by, sty, j, Vn) - Vs * face_value(sc, p, w, by, sty, j - 1, Vs)) * idy;
+    double Wt = (w[ID(i, j, k)] + w[ID(i, j, k + 1)]) / 2;
+    double Wb = (w[ID(i, j, k - 1)] + w[ID(i, j, k)]) / 2;
+    double d_ww_dz = (Wt * face_value(sc, p, w, bz, stz, k, Wt) - Wb * face_value(sc, p, w, bz, stz, k - 1, Wb)) * idz;
+    return (d_uw_dx + d_vw_dy) + d_ww_dz;
+}
+
+void orc_advect_momentum(double *tu, double *tv, double *tw, const double *u, const double *v, const double *w,
+                         const orc_grid *g, int scheme, int p) {
+    GRID_DIMS(g);
+    int H = g->H;
+#pragma omp parallel for schedule(static)
+    for (int k = H + 1; k <= H + g->Nz; k++)
+        for (int j = H + 1; j <= H + g->Ny; j++)
+            for (int i = H + 1; i <= H + g->Nx; i++) {
+                size_t c = ID(i, j, k);
+                if (scheme == ORC_CENTERED) {
+                    tu[c] -= mom_u_at(g, sx, sy, i, j, k, u, v, w);
+                    tv[c] -= mom_v_at(g, sx, sy, i, j, k, u, v, w);
+                    tw[c] -= mom_w_at(g, sx, sy, i, j, k, u, v, w);
+                } else {
+                    int sc = scheme == ORC_WENO5 ? ORC_WENO5 : scheme;
+                    tu[c] -= mrecon_u_at(g, sx, sy, sc, p, i, j, k, u, v, w);
+                    tv[c] -= mrecon_v_at(g, sx, sy, sc, p, i, j, k, u, v, w);
+                    tw[c] -= mrecon_w_at(g, sx, sy, sc, p, i, j, k, u, v, w);
+                }
+            }
+}
+
+/* _lap_at, src/FiniteVolume.jl:271-283 */
+static inline double lap_at(const orc_grid *g, int sx, int sy, int i, int j, int k, const double *f, int loc) {
+    double idx2 = 1.0 / (g->dx * g->dx), idy2 = 1.0 / (g->dy * g->dy);
+    double c = f[ID(i, j, k)];
+    return (((f[ID(i + 1, j, k)] - 2 * c) + f[ID(i - 1, j, k)]) * idx2 +
+            ((f[ID(i, j + 1, k)] - 2 * c) + f[ID(i, j - 1, k)]) * idy2) +
+           m_lapz(g, loc, k, f[ID(i, j, k - 1)], c, f[ID(i, j, k + 1)]);
+}
+
+void orc_laplacian(double *out, const double *f, const orc_grid *g, int loc) {
+    GRID_DIMS(g);
+    int H = g->H;
+#pragma omp parallel for schedule(static)
+    for (int k = H + 1; k <= H + g->Nz; k++)
+        for (int j = H + 1; j <= H + g->Ny; j++)
+            for (int i = H + 1; i <= H + g->Nx; i++) out[ID(i, j, k)] = lap_at(g, sx, sy, i, j, k, f, loc);
+}
+
+/* _div_at / _divergence_into!, src/FiniteVolume.jl:312-315, 1869-1888 */
+static inline double div_at(const orc_grid *g, int sx, int sy, int i, int j, int k, const double *u,
+                            const double *v, const double *w) {
+    return ((u[ID(i + 1, j, k)] - u[ID(i, j, k)]) * (1.0 / g->dx) + (v[ID(i, j + 1, k)] - v[ID(i, j, k)]) * (1.0 / g->dy)) +
+           (w[ID(i, j, k + 1)] - w[ID(i, j, k)]) * m_idz(g, k);
+}
+
+void orc_divergence_into(double *buf, const double *u, const double *v, const double *w, const orc_grid *g) {
+    GRID_DIMS(g);
+    int H = g->H, Nx = g->Nx, Ny = g->Ny;
+#pragma omp parallel for schedule(static)
+    for (int k = 1; k <= g->Nz; k++)
+        for (int j = 1; j <= Ny; j++)
+            for (int i = 1; i <= Nx; i++)
+                buf[(size_t)(i - 1) + (size_t)Nx * ((size_t)(j - 1) + (size_t)Ny * (k - 1))] =
+                    div_at(g, sx, sy, i + H, j + H, k + H, u, v, w);
+}
+
+/* max_divergence, src/Simulations.jl:1045-1050 (after the caller's halo fill) */
+double orc_max_divergence(const double *u, const double *v, const double *w, const orc_grid *g) {
+    GRID_DIMS(g);
+    int H = g->H;
+    double m = 0.0;
+    for (int k = H + 1; k <= H + g->Nz; k++)
+        for (int j = H + 1; j <= H + g->Ny; j++)
+            for (int i = H + 1; i <= H + g->Nx; i++) {
+                double d = fabs(div_at(g, sx, sy, i, j, k, u, v, w));
+                if (d > m || d != d) m = d;
+            }
+    return m;
+}
+
+/* _adv_rate_at, src/TimeIntegration.jl:172-182 */
+double orc_max_adv_rate(const double *u, const double *v, const double *w, const orc_grid *g) {
+    GRID_DIMS(g);
+    int H = g->H;
+    double m = 0.0;
+    for (int k = H + 1; k <= H + g->Nz; k++)
+        for (int j = H + 1; j <= H + g->Ny; j++)
+            for (int i = H + 1; i <= H + g->Nx; i++) {
+                double au = fmax(fabs(u[ID(i, j, k)]), fabs(u[ID(i + 1, j, k)]));
+                double av = fmax(fabs(v[ID(i, j, k)]), fabs(v[ID(i, j + 1, k)]));
+                double aw = fmax(fabs(w[ID(i, j, k)]), fabs(w[ID(i, j, k + 1)]));
+                double r = (au * (1.0 / g->dx) + av * (1.0 / g->dy)) + aw * m_idz(g, k);
+                if (r > m) m = r;
+            }
+    return m;
+}
+
+/* ------------------------------------------------------------------------
+ * Canonical reductions, src/Backends.jl:703-750 and ordered_sum (:944)
+ * ------------------------------------------------------------------------ */
+static double plane_sum_lanes(const double *a, int sx, int sy, int i1, int n_i, int j1, int n_j, int k) {
+    double l[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    int nb = n_i >> 3;
+    for (int j = j1; j < j1 + n_j; j++) {
+        int i = i1;
+        for (int b = 0; b < nb; b++) {
+            for (int r = 0; r < 8; r++) l[r] += a[ID(i + r, j, k)];
+            i += 8;
+        }
+        for (int ii = i; ii < i1 + n_i; ii++) l[0] += a[ID(ii, j, k)];
+    }
+    return ((l[0] + l[1]) + (l[2] + l[3])) + ((l[4] + l[5]) + (l[6] + l[7]));
+}
+
+double orc_interior_sum(const double *a, int sx, int sy, int i1, int n_i, int j1, int n_j, int k1, int n_k) {
+    double *part = (double *)malloc(sizeof(double) * (size_t)n_k);
+#pragma omp parallel for schedule(static)
+    for (int c = 0; c < n_k; c++) part[c] = plane_sum_lanes(a, sx, sy, i1, n_i, j1, n_j, k1 + c);
+    double s = 0.0;
+    for (int c = 0; c < n_k; c++) s += part[c];
+    free(part);
+    return s;
+}
+
+/* ------------------------------------------------------------------------
+ * Transforms. The reference calls FFTW (R2HC/HC2R, REDFT10/REDFT01, r2c/c2r,
+ * complex DFT); these are the same unnormalised transforms by their
+ * definitions, evaluated through a radix-2 FFT (power-of-two lengths) or a
+ * direct DFT (other lengths).
+ * ------------------------------------------------------------------------ */
+typedef struct {
+    int n;
+    int pow2;
+    double *cw, *sw; /* cos/sin(2 pi m / n), m = 0..n-1 */
+} fft_plan;
+
+static fft_plan *fft_plan_new(int n) {
+    fft_plan *p = (fft_plan *)calloc(1, sizeof(fft_plan));
+    p->n = n;
+    p->pow2 = n > 0 && (n & (n - 1)) == 0;
+    p->cw = (double *)malloc(sizeof(double) * (size_t)n);
+    p->sw = (double *)malloc(sizeof(double) * (size_t)n);
+    for (int m = 0; m < n; m++) {
+        p->cw[m] = cos(2.0 * M_PI * (double)m / (double)n);
+        p->sw[m] = sin(2.0 * M_PI * (double)m / (double)n);
+    }
+    return p;
+}
+static void fft_plan_free(fft_plan *p) {
+    if (!p) return;
+    free(p->cw);
+    free(p->sw);
+    free(p);
+}
+
+/* In-place complex DFT, sign = -1 forward (e^{-i...}), +1 backward, unnormalised. */
+static void cfft(const fft_plan *p, double *re, double *im, int sign, double *tr, double *ti) {
+    int n = p->n;
+    if (!p->pow2) {
+        for (int k = 0; k < n; k++) {
+            double sr = 0, si = 0;
+            for (int m = 0; m < n; m++) {
+                int q = (int)(((long long)k * m) % n);
+                double c = p->cw[q], s = sign * p->sw[q];
+                sr += re[m] * c - im[m] * s;
+                si += re[m] * s + im[m] * c;
+            }
+            tr[k] = sr;
+            ti[k] = si;
+        }
+        memcpy(re, tr, sizeof(double) * (size_t)n);
+        memcpy(im, ti, sizeof(double) * (size_t)n);
+        return;
+    }
+    /* bit reversal */
+    for (int i = 1, j = 0; i < n; i++) {
+        int bit = n >> 1;
+        for (; j & bit; bit >>= 1) j ^= bit;
+        j ^= bit;
+        if (i < j) {
+            double t = re[i]; re[i] = re[j]; re[j] = t;
+            t = im[i]; im[i] = im[j]; im[j] = t;
+        }
+    }
+    for (int len = 2; len <= n; len <<= 1) {
+        int half = len >> 1, step = n / len;
+        for (int s0 = 0; s0 < n; s0 += len)
+            for (int q = 0; q < half; q++) {
+                double c = p->cw[q * step], s = sign * p->sw[q * step];
+                double xr = re[s0 + q + half], xi = im[s0 + q + half];
+                double yr = xr * c - xi * s, yi = xr * s + xi * c;
+                re[s0 + q + half] = re[s0 + q] - yr;
+                im[s0 + q + half] = im[s0 + q] - yi;
+                re[s0 + q] += yr;
+                im[s0 + q] += yi;
+            }
+    }
+}
+
+typedef struct {
+    int n, kind; /* kind 0: R2HC/HC2R (periodic), 1: REDFT10/REDFT01 (Neumann) */
+    fft_plan *p; /* length n (periodic) or 2n (DCT) */
+    double *c4, *s4; /* cos/sin(pi m / (2n)) for the DCT twiddles */
+} r2r_plan;
+
+static r2r_plan *r2r_new(int n, int kind) {
+    r2r_plan *r = (r2r_plan *)calloc(1, sizeof(r2r_plan));
+    r->n = n;
+    r->kind = kind;
+    r->p = fft_plan_new(kind == 0 ? n : 2 * n);
+    if (kind == 1) {
+        r->c4 = (double *)malloc(sizeof(double) * (size_t)n);
+        r->s4 = (double *)malloc(sizeof(double) * (size_t)n);
+        for (int m = 0; m < n; m++) {
+            r->c4[m] = cos(M_PI * (double)m / (2.0 * n));
+            r->s4[m] = sin(M_PI * (double)m / (2.0 * n));
+        }
+    }
+    return r;
+}
+static void r2r_free(r2r_plan *r) {
+    if (!r) return;
+    fft_plan_free(r->p);
+    free(r->c4);
+    free(r->s4);
+    free(r);
+}
+
+/* Forward: R2HC or REDFT10 of x (stride st) in place. w: 4*(2n) scratch. */
+static void r2r_fwd(const r2r_plan *r, double *x, ptrdiff_t st, double *w) {
+    int n = r->n;
+    if (r->kind == 0) {
+        double *re = w, *im = w + n, *tr = w + 2 * n, *ti = w + 3 * n;
+        for (int m = 0; m < n; m++) { re[m] = x[m * st]; im[m] = 0.0; }
+        cfft(r->p, re, im, -1, tr, ti);
+        for (int q = 0; q <= n / 2; q++) x[q * st] = re[q];
+        for (int q = n / 2 + 1; q < n; q++) x[q * st] = im[n - q];
+    } else {
+        int N2 = 2 * n;
+        double *re = w, *im = w + N2, *tr = w + 2 * N2, *ti = w + 3 * N2;
+        for (int m = 0; m < n; m++) {
+            re[m] = x[m * st]; re[N2 - 1 - m] = x[m * st];
+            im[m] = 0.0; im[N2 - 1 - m] = 0.0;
+        }
+        cfft(r->p, re, im, -1, tr, ti);
+        /* REDFT10_k = Re(e^{-i pi k / 2n} Y_k) */
+        for (int q = 0; q < n; q++) x[q * st] = re[q] * r->c4[q] + im[q] * r->s4[q];
+    }
+}
+/* Backward: HC2R or REDFT01 of x (stride st) in place, unnormalised. */
+static void r2r_inv(const r2r_plan *r, double *x, ptrdiff_t st, double *w) {
+    int n = r->n;
+    if (r->kind == 0) {
+        double *re = w, *im = w + n, *tr = w + 2 * n, *ti = w + 3 * n;
+        re[0] = x[0]; im[0] = 0.0;
+        for (int q = 1; q < (n + 1) / 2; q++) {
+            re[q] = x[q * st]; im[q] = x[(n - q) * st];
+            re[n - q] = re[q]; im[n - q] = -im[q];
+        }
+        if (n % 2 == 0) { re[n / 2] = x[(n / 2) * st]; im[n / 2] = 0.0; }
+        cfft(r->p, re, im, +1, tr, ti);
+        for (int m = 0; m < n; m++) x[m * st] = re[m];
+    } else {
+        int N2 = 2 * n;
+        double *re = w, *im = w + N2, *tr = w + 2 * N2, *ti = w + 3 * N2;
+        /* Y_k = x_0 + 2 sum_{m>=1} x_m cos(pi m (2k+1)/(2n)) = Re sum_m z_m e^{+2 pi i m k/(2n)},
+         * z_m = w_m x_m e^{i pi m/(2n)}, w_0 = 1, w_m = 2 */
+        for (int m = 0; m < N2; m++) { re[m] = 0.0; im[m] = 0.0; }
+        for (int m = 0; m < n; m++) {
+            double wm = m == 0 ? 1.0 : 2.0;
+            re[m] = wm * x[m * st] * r->c4[m];
+            im[m] = wm * x[m * st] * r->s4[m];
+        }
+        cfft(r->p, re, im, +1, tr, ti);
+        for (int q = 0; q < n; q++) x[q * st] = re[q];
+    }
+}
+
+/* ------------------------------------------------------------------------
+ * FFTPoissonSolver, src/PressureSolvers.jl:103-174 (triply periodic): the
+ * discrete eigenvalues and the zero mode annihilated, by a complex 3-D DFT.
+ * ------------------------------------------------------------------------ */
+typedef struct {
+    int kind; /* 0 fft, 1 column */
+    int Nx, Ny, Nz;
+    fft_plan *px, *py, *pz;
+    double *invl; /* Nx*Ny*Nz */
+    /* column solver */
+    r2r_plan *rx, *ry;
+    double *cp, *inv_den, *lower, *measure, *icoup, *gauge, *col;
+    double measure_sum, gauge_sum, norm;
+} orc_solver;
+
+void *orc_fft_create(int Nx, int Ny, int Nz, double dx, double dy, double dz) {
+    orc_solver *s = (orc_solver *)calloc(1, sizeof(orc_solver));
+    s->kind = 0;
+    s->Nx = Nx; s->Ny = Ny; s->Nz = Nz;
+    s->px = fft_plan_new(Nx); s->py = fft_plan_new(Ny); s->pz = fft_plan_new(Nz);
+    s->invl = (double *)malloc(sizeof(double) * (size_t)Nx * Ny * Nz);
+    double nrm = 1.0 / (double)((long long)Nx * Ny * Nz);
+    for (int k = 0; k < Nz; k++)
+        for (int j = 0; j < Ny; j++)
+            for (int i = 0; i < Nx; i++) {
+                int mi = i <= Nx / 2 ? i : Nx - i; /* the halved r2c axis: mode min(m, N - m) */
+                double sxv = sin(M_PI * mi / Nx), syv = sin(M_PI * j / Ny), szv = sin(M_PI * k / Nz);
+                double kx2 = 4 / (dx * dx) * (sxv * sxv);
+                double ky2 = 4 / (dy * dy) * (syv * syv);
+                double kz2 = 4 / (dz * dz) * (szv * szv);
+                double l = -((kx2 + ky2) + kz2);
+                s->invl[(size_t)i + (size_t)Nx * (j + (size_t)Ny * k)] = l == 0 ? 0.0 : nrm / l;
+            }
+    return s;
+}
+
+static void apply_axis_c(double *re, double *im, int Nx, int Ny, int Nz, int ax, const fft_plan *p, int sign) {
+    int n = ax == 0 ? Nx : (ax == 1 ? Ny : Nz);
+    ptrdiff_t st = ax == 0 ? 1 : (ax == 1 ? Nx : (ptrdiff_t)Nx * Ny);
+    int na = ax == 0 ? Ny : Nx, nb = ax == 2 ? Ny : Nz;
+    ptrdiff_t sa = ax == 0 ? Nx : 1, sb = ax == 2 ? Nx : (ptrdiff_t)Nx * Ny;
+#pragma omp parallel
+    {
+        double *w = (double *)malloc(sizeof(double) * 4 * (size_t)n);
+#pragma omp for collapse(2) schedule(static)
+        for (int b = 0; b < nb; b++)
+            for (int a = 0; a < na; a++) {
+                size_t base = (size_t)a * sa + (size_t)b * sb;
+                for (int m = 0; m < n; m++) { w[m] = re[base + m * st]; w[n + m] = im[base + m * st]; }
+                cfft(p, w, w + n, sign, w + 2 * n, w + 3 * n);
+                for (int m = 0; m < n; m++) { re[base + m * st] = w[m]; im[base + m * st] = w[n + m]; }
+            }
+        free(w);
+    }
+}
+
+void orc_fft_solve(void *h, double *phi, const double *src) {
+    orc_solver *s = (orc_solver *)h;
+    size_t n = (size_t)s->Nx * s->Ny * s->Nz;
+    double *re = (double *)malloc(sizeof(double) * n), *im = (double *)calloc(n, sizeof(double));
+    memcpy(re, src, sizeof(double) * n);
+    apply_axis_c(re, im, s->Nx, s->Ny, s->Nz, 0, s->px, -1);
+    apply_axis_c(re, im, s->Nx, s->Ny, s->Nz, 1, s->py, -1);
+    apply_axis_c(re, im, s->Nx, s->Ny, s->Nz, 2, s->pz, -1);
+    for (size_t c = 0; c < n; c++) { re[c] *= s->invl[c]; im[c] *= s->invl[c]; }
+    apply_axis_c(re, im, s->Nx, s->Ny, s->Nz, 2, s->pz, +1);
+    apply_axis_c(re, im, s->Nx, s->Ny, s->Nz, 1, s->py, +1);
+    apply_axis_c(re, im, s->Nx, s->Ny, s->Nz, 0, s->px, +1);
+    memcpy(phi, re, sizeof(double) * n);
+    free(re);
+    free(im);
+}
+
+/* ------------------------------------------------------------------------
+ * ColumnTridiagonalSolver, src/PressureSolvers.jl:544-805
+ * ------------------------------------------------------------------------ */
+/* _r2r_eigen, src/PressureSolvers.jl:618-624 */
+static double r2r_eigen(int N, double d, int bc, int p) {
+    double sv = bc == ORC_PERIODIC ? sin(M_PI * (double)(p < N - p ? p : N - p) / N) : sin(M_PI * p / (2.0 * N));
+    return 4 / (d * d) * (sv * sv);
+}
+
+void *orc_column_create(int Nx, int Ny, int Nz, double dx, double dy, int bcx, int bcy, const double *lower,
+                        const double *upper, const double *hweight, const double *measure,
+                        const double *inv_coupling, const double *gauge) {
+    orc_solver *s = (orc_solver *)calloc(1, sizeof(orc_solver));
+    s->kind = 1;
+    s->Nx = Nx; s->Ny = Ny; s->Nz = Nz;
+    int kx = bcx == ORC_PERIODIC ? 0 : 1, ky = bcy == ORC_PERIODIC ? 0 : 1;
+    s->rx = r2r_new(Nx, kx);
+    s->ry = r2r_new(Ny, ky);
+    int sxn = kx == 0 ? 1 : 2, syn = ky == 0 ? 1 : 2;
+    s->norm = 1.0 / (double)((long long)sxn * Nx * syn * Ny);
+    size_t n = (size_t)Nx * Ny * Nz;
+    s->cp = (double *)calloc(n, sizeof(double));
+    s->inv_den = (double *)calloc(n, sizeof(double));
+    s->lower = (double *)malloc(sizeof(double) * Nz);
+    s->measure = (double *)malloc(sizeof(double) * Nz);
+    s->icoup = (double *)malloc(sizeof(double) * (Nz + 1));
+    s->gauge = (double *)malloc(sizeof(double) * Nz);
+    s->col = (double *)malloc(sizeof(double) * Nz);
+    memcpy(s->lower, lower, sizeof(double) * Nz);
+    memcpy(s->measure, measure, sizeof(double) * Nz);
+    memcpy(s->icoup, inv_coupling, sizeof(double) * (Nz + 1));
+    memcpy(s->gauge, gauge, sizeof(double) * Nz);
+    double *lx = (double *)malloc(sizeof(double) * Nx), *ly = (double *)malloc(sizeof(double) * Ny);
+    for (int p = 0; p < Nx; p++) lx[p] = r2r_eigen(Nx, dx, bcx, p);
+    for (int q = 0; q < Ny; q++) ly[q] = r2r_eigen(Ny, dy, bcy, q);
+    for (int q = 0; q < Ny; q++)
+        for (int p = 0; p < Nx; p++) {
+            if (p == 0 && q == 0) continue;
+            double lam = lx[p] + ly[q];
+            double cprev = 0.0;
+            for (int k = 0; k < Nz; k++) {
+                double b = -(lower[k] + upper[k]) - lam * hweight[k];
+                double den = b - lower[k] * cprev;
+                size_t c = (size_t)p + (size_t)Nx * (q + (size_t)Ny * k);
+                s->inv_den[c] = 1 / den;
+                cprev = upper[k] / den;
+                s->cp[c] = cprev;
+            }
+        }
+    double ms = 0, gs = 0;
+    for (int k = 0; k < Nz; k++) ms += measure[k];
+    for (int k = 0; k < Nz; k++) gs += gauge[k];
+    s->measure_sum = ms;
+    s->gauge_sum = gs;
+    free(lx);
+    free(ly);
+    return s;
+}
+
+static void xy_transform(orc_solver *s, double *b, int inverse) {
+    int Nx = s->Nx, Ny = s->Ny, Nz = s->Nz;
+    int nmax = 2 * (Nx > Ny ? Nx : Ny);
+#pragma omp parallel
+    {
+        double *w = (double *)malloc(sizeof(double) * 4 * (size_t)nmax);
+#pragma omp for schedule(static)
+        for (int k = 0; k < Nz; k++) {
+            double *lev = b + (size_t)Nx * Ny * k;
+            for (int j = 0; j < Ny; j++) {
+                if (inverse) r2r_inv(s->rx, lev + (size_t)Nx * j, 1, w);
+                else r2r_fwd(s->rx, lev + (size_t)Nx * j, 1, w);
+            }
+            for (int i = 0; i < Nx; i++) {
+                if (inverse) r2r_inv(s->ry, lev + i, Nx, w);
+                else r2r_fwd(s->ry, lev + i, Nx, w);
+            }
+        }
+        free(w);
+    }
+}
+
+/* _mean_mode!, src/PressureSolvers.jl:780-805 */
+static void mean_mode(orc_solver *s) {
+    int Nz = s->Nz;
+    double *col = s->col, *m = s->measure, *io = s->icoup, *g = s->gauge;
+    double mb = 0.0;
+    for (int k = 0; k < Nz; k++) mb += col[k] * m[k];
+    mb /= s->measure_sum;
+    double F = 0.0, phk = 0.0;
+    double prev = col[0] - mb;
+    col[0] = 0.0;
+    for (int k = 1; k < Nz; k++) {
+        F += prev * m[k - 1];
+        phk += F * io[k];
+        prev = col[k] - mb;
+        col[k] = phk;
+    }
+    double mphi = 0.0;
+    for (int k = 0; k < Nz; k++) mphi += col[k] * g[k];
+    double sh = mphi / s->gauge_sum;
+    for (int k = 0; k < Nz; k++) col[k] -= sh;
+}
+
+void orc_column_solve(void *h, double *phi, const double *src) {
+    orc_solver *s = (orc_solver *)h;
+    int Nx = s->Nx, Ny = s->Ny, Nz = s->Nz;
+    size_t plane = (size_t)Nx * Ny, n = plane * Nz;
+    double *b = (double *)malloc(sizeof(double) * n);
+    memcpy(b, src, sizeof(double) * n);
+    xy_transform(s, b, 0);
+    for (int k = 0; k < Nz; k++) s->col[k] = b[plane * k];
+    /* _thomas_slab!, src/PressureSolvers.jl:754-772 */
+#pragma omp parallel for schedule(static)
+    for (int q = 0; q < Ny; q++) {
+        for (int pp = 0; pp < Nx; pp++) b[pp + (size_t)Nx * q] *= s->inv_den[pp + (size_t)Nx * q];
+        for (int k = 1; k < Nz; k++) {
+            double a = s->lower[k];
+            for (int pp = 0; pp < Nx; pp++) {
+                size_t c = pp + (size_t)Nx * q + plane * k;
+                b[c] = (b[c] - a * b[c - plane]) * s->inv_den[c];
+            }
+        }
+        for (int k = Nz - 2; k >= 0; k--)
+            for (int pp = 0; pp < Nx; pp++) {
+                size_t c = pp + (size_t)Nx * q + plane * k;
+                b[c] -= s->cp[c] * b[c + plane];
+            }
+    }
+    mean_mode(s);
+    for (int k = 0; k < Nz; k++) b[plane * k] = s->col[k];
+    xy_transform(s, b, 1);
+    for (size_t c = 0; c < n; c++) phi[c] = b[c] * s->norm;
+    free(b);
+}
+
+void orc_solver_destroy(void *h) {
+    orc_solver *s = (orc_solver *)h;
+    if (!s) return;
+    fft_plan_free(s->px); fft_plan_free(s->py); fft_plan_free(s->pz);
+    r2r_free(s->rx); r2r_free(s->ry);
+    free(s->invl); free(s->cp); free(s->inv_den); free(s->lower); free(s->measure);
+    free(s->icoup); free(s->gauge); free(s->col);
+    free(s);
+}
+
+/* ------------------------------------------------------------------------
+ * The composed SSP-RK3 projection step.
+ *
+ * Follows _euler_projection! (src/Simulations.jl:531-650) in its pass form
+ * with a momentum_advection operator and the scalar advection operator of
+ * the chosen scheme, _project_velocity! (:711-744), step_rk3!'s stage
+ * combinations (:854-904), _advance_scalar! (:682-694) and the halo shell
+ * combination (_shell_map!, :1249-1286).
+ * ------------------------------------------------------------------------ */
+typedef struct {
+    orc_grid g;
+    orc_step_params sp;
+    orc_solver *solver;
+    size_t n, nint;
+    double *tu, *tv, *tw, *tt, *u0, *v0, *w0, *t0, *phi, *buf, *pbuf;
+} orc_stepper;
+
+void *orc_stepper_create(const orc_grid *g, const orc_step_params *sp, void *solver) {
+    orc_stepper *s = (orc_stepper *)calloc(1, sizeof(orc_stepper));
+    s->g = *g;
+    s->sp = *sp;
+    s->solver = (orc_solver *)solver;
+    size_t sx = g->Nx + 2 * g->H, sy = g->Ny + 2 * g->H, sz = g->Nz + 2 * g->H;
+    s->n = sx * sy * sz;
+    s->nint = (size_t)g->Nx * g->Ny * g->Nz;
+    double **arrs[] = {&s->tu, &s->tv, &s->tw, &s->tt, &s->u0, &s->v0, &s->w0, &s->t0, &s->phi};
+    for (int a = 0; a < 9; a++) *arrs[a] = (double *)calloc(s->n, sizeof(double));
+    s->buf = (double *)calloc(s->nint, sizeof(double));
+    s->pbuf = (double *)calloc(s->nint, sizeof(double));
+    return s;
+}
+
+void orc_stepper_destroy(void *h) {
+    orc_stepper *s = (orc_stepper *)h;
+    if (!s) return;
+    free(s->tu); free(s->tv); free(s->tw); free(s->tt); free(s->u0); free(s->v0); free(s->w0);
+    free(s->t0); free(s->phi); free(s->buf); free(s->pbuf);
+    free(s);
+}
+
+/* is (i, j, k) an interior point */
+static inline int interior(const orc_grid *g, int i, int j, int k) {
+    int H = g->H;
+    return i > H && i <= H + g->Nx && j > H && j <= H + g->Ny && k > H && k <= H + g->Nz;
+}
+
+/* One projected forward-Euler substage; stage = 0 (none), 2 (3/4, 1/4), 3 (1/3, 2/3). */
+static void euler_projection(orc_stepper *s, double *u, double *v, double *w, double *th, double *p, double dt,
+                             int stage) {
+    const orc_grid *g = &s->g;
+    const orc_step_params *sp = &s->sp;
+    GRID_DIMS(g);
+    int H = g->H;
+    double ca = 0, cb = 1;
+    if (stage == 2) { ca = 0.75; cb = 0.25; }
+    if (stage == 3) { double inv3 = 1.0 / 3; ca = inv3; cb = 2 * inv3; }
+
+    /* 1. halo fill of the state (fill_state_halos!, :1028) */
+    orc_fill_halos(u, g, ORC_FX, sp->bcx, sp->bcy, sp->bcz);
+    orc_fill_halos(v, g, ORC_FY, sp->bcx, sp->bcy, sp->bcz);
+    orc_fill_halos(w, g, ORC_FZ, sp->bcx, sp->bcy, sp->bcz);
+    orc_fill_halos(th, g, ORC_CC, sp->tbcx, sp->tbcy, sp->tbcz);
+    /* _keep_start! by copy in the first RK3 substage (:940-946) */
+    if (stage == 1) {
+        memcpy(s->u0, u, sizeof(double) * s->n);
+        memcpy(s->v0, v, sizeof(double) * s->n);
+        memcpy(s->w0, w, sizeof(double) * s->n);
+        memcpy(s->t0, th, sizeof(double) * s->n);
+    }
+    /* 2.-3. tendencies from zero: momentum advection, scalar advection, viscosity */
+    memset(s->tu, 0, sizeof(double) * s->n);
+    memset(s->tv, 0, sizeof(double) * s->n);
+    memset(s->tw, 0, sizeof(double) * s->n);
+    memset(s->tt, 0, sizeof(double) * s->n);
+    orc_advect_momentum(s->tu, s->tv, s->tw, u, v, w, g, sp->mom_scheme, sp->weno_p);
+    orc_advect_scalar(s->tt, u, v, w, th, g, sp->scal_scheme, sp->weno_p);
+    int nthreads_dummy = 0;
+    (void)nthreads_dummy;
+#pragma omp parallel for schedule(static)
+    for (int k = H + 1; k <= H + g->Nz; k++)
+        for (int j = H + 1; j <= H + g->Ny; j++)
+            for (int i = H + 1; i <= H + g->Nx; i++) {
+                size_t c = ID(i, j, k);
+                if (sp->nu != 0) {
+                    /* axpy!(t, ν, ∇²f): t + ν·lap (src/Simulations.jl:1180-1184) */
+                    s->tu[c] = s->tu[c] + sp->nu * lap_at(g, sx, sy, i, j, k, u, ORC_FX);
+                    s->tv[c] = s->tv[c] + sp->nu * lap_at(g, sx, sy, i, j, k, v, ORC_FY);
+                    s->tw[c] = s->tw[c] + sp->nu * lap_at(g, sx, sy, i, j, k, w, ORC_FZ);
+                    s->tt[c] = s->tt[c] + sp->nu * lap_at(g, sx, sy, i, j, k, th, ORC_CC);
+                }
+                /* 5. buoyancy, _b_buoyancy!, src/FiniteVolume.jl:838-844 */
+                if (sp->theta0 != 0) {
+                    double coef = sp->g_accel / sp->theta0;
+                    double tf = m_zface(g, k, th[ID(i, j, k - 1)], th[ID(i, j, k)]);
+                    s->tw[c] += coef * (tf - sp->theta0);
+                }
+            }
+    /* 6a. surface flux, _b_surface_flux!, src/FiniteVolume.jl:417-439 */
+    if (sp->flux_bot != 0 || sp->flux_top != 0) {
+        int kb = H + 1, kt = H + g->Nz;
+        double fb = sp->flux_bot * m_idz(g, kb), ft = sp->flux_top * m_idz(g, kt);
+        for (int j = H + 1; j <= H + g->Ny; j++)
+            for (int i = H + 1; i <= H + g->Nx; i++) {
+                s->tt[ID(i, j, kb)] += fb;
+                s->tt[ID(i, j, kt)] -= ft;
+            }
+    }
+    /* 7. predictor u* = u + Δt F over the interior */
+#pragma omp parallel for schedule(static)
+    for (int k = H + 1; k <= H + g->Nz; k++)
+        for (int j = H + 1; j <= H + g->Ny; j++)
+            for (int i = H + 1; i <= H + g->Nx; i++) {
+                size_t c = ID(i, j, k);
+                u[c] = u[c] + dt * s->tu[c];
+                v[c] = v[c] + dt * s->tv[c];
+                w[c] = w[c] + dt * s->tw[c];
+            }
+    /* 8.-10. _project_velocity! */
+    orc_fill_halos(u, g, ORC_FX, sp->bcx, sp->bcy, sp->bcz);
+    orc_fill_halos(v, g, ORC_FY, sp->bcx, sp->bcy, sp->bcz);
+    orc_fill_halos(w, g, ORC_FZ, sp->bcx, sp->bcy, sp->bcz);
+    orc_divergence_into(s->buf, u, v, w, g);
+    if (s->solver->kind == 0) orc_fft_solve(s->solver, s->pbuf, s->buf);
+    else orc_column_solve(s->solver, s->pbuf, s->buf);
+    /* the solver writes φ's interior only; halos keep their values */
+    for (int k = 1; k <= g->Nz; k++)
+        for (int j = 1; j <= g->Ny; j++)
+            for (int i = 1; i <= g->Nx; i++)
+                s->phi[ID(i + H, j + H, k + H)] =
+                    s->pbuf[(size_t)(i - 1) + (size_t)g->Nx * ((size_t)(j - 1) + (size_t)g->Ny * (k - 1))];
+    /* φ's halo with the velocity BCs at a cell centre */
+    orc_fill_halos(s->phi, g, ORC_CC, sp->bcx, sp->bcy, sp->bcz);
+    const double *ph = s->phi;
+    /* _b_correct!, src/FiniteVolume.jl:1823-1833 (with p = φ/Δt) */
+#pragma omp parallel for schedule(static)
+    for (int k = H + 1; k <= H + g->Nz; k++)
+        for (int j = H + 1; j <= H + g->Ny; j++)
+            for (int i = H + 1; i <= H + g->Nx; i++) {
+                size_t c = ID(i, j, k);
+                double pc = ph[c];
+                double un = u[c] - (pc - ph[ID(i - 1, j, k)]) * (1.0 / g->dx);
+                double vn = v[c] - (pc - ph[ID(i, j - 1, k)]) * (1.0 / g->dy);
+                double wn = w[c] - (pc - ph[ID(i, j, k - 1)]) * m_idzf(g, k);
+                if (stage >= 2) {
+                    un = ca * s->u0[c] + cb * un;
+                    vn = ca * s->v0[c] + cb * vn;
+                    wn = ca * s->w0[c] + cb * wn;
+                }
+                u[c] = un; v[c] = vn; w[c] = wn;
+                p[c] = pc / dt;
+            }
+    /* 11. _advance_scalar!: interior axpy (+ combination), halo shell combined */
+#pragma omp parallel for schedule(static)
+    for (int k = 1; k <= sz; k++)
+        for (int j = 1; j <= sy; j++)
+            for (int i = 1; i <= sx; i++) {
+                size_t c = ID(i, j, k);
+                int in = interior(g, i, j, k);
+                if (in) {
+                    double y = th[c] + dt * s->tt[c];
+                    th[c] = stage >= 2 ? ca * s->t0[c] + cb * y : y;
+                } else if (stage >= 2) {
+                    th[c] = ca * s->t0[c] + cb * th[c];
+                    u[c] = ca * s->u0[c] + cb * u[c];
+                    v[c] = ca * s->v0[c] + cb * v[c];
+                    w[c] = ca * s->w0[c] + cb * w[c];
+                }
+            }
+}
+
+void orc_step_rk3(void *h, double *u, double *v, double *w, double *th, double *p, double dt) {
+    orc_stepper *s = (orc_stepper *)h;
+    euler_projection(s, u, v, w, th, p, dt, 1);
+    euler_projection(s, u, v, w, th, p, dt, 2);
+    euler_projection(s, u, v, w, th, p, dt, 3);
+}
+
+void orc_step_euler(void *h, double *u, double *v, double *w, double *th, double *p, double dt) {
+    orc_stepper *s = (orc_stepper *)h;
+    euler_projection(s, u, v, w, th, p, dt, 0);
+}
+
+/* ------------------------------------------------------------------------
+ * Multigrid, src/Multigrid.jl
+ * ------------------------------------------------------------------------ */
+typedef struct {
+    int nx, ny, nz;
+    double dx, dy, dz;
+    double *phi, *b, *r, *tmp;
+    double *cx, *cy, *cz; /* (nx+1,ny,nz), (nx,ny+1,nz), (nx,ny,nz+1) or NULL */
+} mg_level;
+
+typedef struct {
+    int nlev;
+    mg_level L[32];
+    int bcx, bcy, bcz, singular, nu1, nu2, smoother;
+    double omega;
+    double *rhs_scale;
+    /* pcg workspace (interior-sized) */
+    double *x, *bb, *pr, *pz, *pp, *pAp, *tmpin;
+} orc_mg;
+
+#define LID(L, i, j, k) ((size_t)(i) + (size_t)((L)->nx + 2) * ((size_t)(j) + (size_t)((L)->ny + 2) * (size_t)(k)))
+/* 0-based storage with the unit halo: interior 1..n */
+
+/* fill_unit_halo!, src/PressureSolvers.jl:890-896: x, then y, then z */
+static void unit_halo(mg_level *L, double *a, int bcx, int bcy, int bcz) {
+    int nx = L->nx, ny = L->ny, nz = L->nz;
+    for (int k = 0; k < nz + 2; k++)
+        for (int j = 0; j < ny + 2; j++) {
+            if (bcx == ORC_PERIODIC) { a[LID(L, 0, j, k)] = a[LID(L, nx, j, k)]; a[LID(L, nx + 1, j, k)] = a[LID(L, 1, j, k)]; }
+            else { a[LID(L, 0, j, k)] = a[LID(L, 1, j, k)]; a[LID(L, nx + 1, j, k)] = a[LID(L, nx, j, k)]; }
+        }
+    for (int k = 0; k < nz + 2; k++)
+        for (int i = 0; i < nx + 2; i++) {
+            if (bcy == ORC_PERIODIC) { a[LID(L, i, 0, k)] = a[LID(L, i, ny, k)]; a[LID(L, i, ny + 1, k)] = a[LID(L, i, 1, k)]; }
+            else { a[LID(L, i, 0, k)] = a[LID(L, i, 1, k)]; a[LID(L, i, ny + 1, k)] = a[LID(L, i, ny, k)]; }
+        }
+    for (int j = 0; j < ny + 2; j++)
+        for (int i = 0; i < nx + 2; i++) {
+            if (bcz == ORC_PERIODIC) { a[LID(L, i, j, 0)] = a[LID(L, i, j, nz)]; a[LID(L, i, j, nz + 1)] = a[LID(L, i, j, 1)]; }
+            else { a[LID(L, i, j, 0)] = a[LID(L, i, j, 1)]; a[LID(L, i, j, nz + 1)] = a[LID(L, i, j, nz)]; }
+        }
+}
+
+#define CXI(L, i, j, k) ((size_t)(i) + (size_t)((L)->nx + 1) * ((size_t)(j) + (size_t)(L)->ny * (size_t)(k)))
+#define CYI(L, i, j, k) ((size_t)(i) + (size_t)(L)->nx * ((size_t)(j) + (size_t)((L)->ny + 1) * (size_t)(k)))
+#define CZI(L, i, j, k) ((size_t)(i) + (size_t)(L)->nx * ((size_t)(j) + (size_t)(L)->ny * (size_t)(k)))
+
+/* _Aφ, src/Multigrid.jl:171-192; (i, j, k) the 1-based interior-local index */
+static inline double mg_A(const mg_level *L, const double *f, int i, int j, int k) {
+    double ix2 = 1.0 / (L->dx * L->dx), iy2 = 1.0 / (L->dy * L->dy), iz2 = 1.0 / (L->dz * L->dz);
+    double c = f[LID(L, i, j, k)];
+    if (!L->cx) {
+        return (((f[LID(L, i + 1, j, k)] - 2 * c) + f[LID(L, i - 1, j, k)]) * ix2 +
+                ((f[LID(L, i, j + 1, k)] - 2 * c) + f[LID(L, i, j - 1, k)]) * iy2) +
+               ((f[LID(L, i, j, k + 1)] - 2 * c) + f[LID(L, i, j, k - 1)]) * iz2;
+    }
+    int ii = i - 1, jj = j - 1, kk = k - 1;
+    double fxp = L->cx[CXI(L, ii + 1, jj, kk)], fxm = L->cx[CXI(L, ii, jj, kk)];
+    double fyp = L->cy[CYI(L, ii, jj + 1, kk)], fym = L->cy[CYI(L, ii, jj, kk)];
+    double fzp = L->cz[CZI(L, ii, jj, kk + 1)], fzm = L->cz[CZI(L, ii, jj, kk)];
+    return ((fxp * (f[LID(L, i + 1, j, k)] - c) - fxm * (c - f[LID(L, i - 1, j, k)])) * ix2 +
+            (fyp * (f[LID(L, i, j + 1, k)] - c) - fym * (c - f[LID(L, i, j - 1, k)])) * iy2) +
+           (fzp * (f[LID(L, i, j, k + 1)] - c) - fzm * (c - f[LID(L, i, j, k - 1)])) * iz2;
+}
+
+/* canonical interior sum of a level array (plane_partials! + ordered_sum) */
+static double mg_isum(const mg_level *L, const double *a) {
+    int sx = L->nx + 2, sy = L->ny + 2;
+    /* ID is 1-based: interior storage 2..n+1 */
+    return orc_interior_sum(a, sx, sy, 2, L->nx, 2, L->ny, 2, L->nz);
+}
+static double mg_isum_sq(const mg_level *L, const double *a) {
+    int nx = L->nx, ny = L->ny, nz = L->nz;
+    double *part = (double *)malloc(sizeof(double) * nz);
+    for (int k = 1; k <= nz; k++) {
+        double l[8] = {0};
+        int nb = nx >> 3;
+        for (int j = 1; j <= ny; j++) {
+            int i = 1;
+            for (int b = 0; b < nb; b++) {
+                for (int r = 0; r < 8; r++) { double x = a[LID(L, i + r, j, k)]; l[r] += x * x; }
+                i += 8;
+            }
+            for (; i <= nx; i++) { double x = a[LID(L, i, j, k)]; l[0] += x * x; }
+        }
+        part[k - 1] = ((l[0] + l[1]) + (l[2] + l[3])) + ((l[4] + l[5]) + (l[6] + l[7]));
+    }
+    double s = 0;
+    for (int k = 0; k < nz; k++) s += part[k];
+    free(part);
+    return s;
+}
+static void mg_shift(mg_level *L, double *a, double m) {
+    for (int k = 1; k <= L->nz; k++)
+        for (int j = 1; j <= L->ny; j++)
+            for (int i = 1; i <= L->nx; i++) a[LID(L, i, j, k)] -= m;
+}
+static void mg_project_mean(mg_level *L, double *a) {
+    double m = mg_isum(L, a) / (double)((long long)L->nx * L->ny * L->nz);
+    mg_shift(L, a, m);
+}
+
+/* _rbgs_sweeps!, src/Multigrid.jl:345-355 (bodies :307-337) */
+static void mg_rbgs(orc_mg *s, mg_level *L, int nsweeps) {
+    double ix2 = 1.0 / (L->dx * L->dx), iy2 = 1.0 / (L->dy * L->dy), iz2 = 1.0 / (L->dz * L->dz);
+    double diagc = -2 * ((ix2 + iy2) + iz2);
+    int imax = (L->nx % 2 == 0) ? 1 << 30 : L->nx + 1;
+    for (int sw = 0; sw < nsweeps; sw++)
+        for (int colour = 0; colour <= 1; colour++) {
+            unit_halo(L, L->phi, s->bcx, s->bcy, s->bcz);
+            /* storage index (1-based) i = 2ih + ((j + k + colour) & 1), j,k in 2..n+1 */
+            for (int k = 2; k <= L->nz + 1; k++)
+                for (int j = 2; j <= L->ny + 1; j++)
+                    for (int ih = 1; ih <= (L->nx + 1) / 2; ih++) {
+                        int i = 2 * ih + ((j + k + colour) & 1);
+                        if (i > imax) continue;
+                        int i0 = i - 1, j0 = j - 1, k0 = k - 1; /* 0-based storage */
+                        double *f = L->phi;
+                        if (!L->cx) {
+                            double off = ((f[LID(L, i0 + 1, j0, k0)] + f[LID(L, i0 - 1, j0, k0)]) * ix2 +
+                                          (f[LID(L, i0, j0 + 1, k0)] + f[LID(L, i0, j0 - 1, k0)]) * iy2) +
+                                         (f[LID(L, i0, j0, k0 + 1)] + f[LID(L, i0, j0, k0 - 1)]) * iz2;
+                            f[LID(L, i0, j0, k0)] = (L->b[LID(L, i0, j0, k0)] - off) / diagc;
+                        } else {
+                            int ii = i0 - 1, jj = j0 - 1, kk = k0 - 1;
+                            double fxp = L->cx[CXI(L, ii + 1, jj, kk)], fxm = L->cx[CXI(L, ii, jj, kk)];
+                            double fyp = L->cy[CYI(L, ii, jj + 1, kk)], fym = L->cy[CYI(L, ii, jj, kk)];
+                            double fzp = L->cz[CZI(L, ii, jj, kk + 1)], fzm = L->cz[CZI(L, ii, jj, kk)];
+                            double diag = -(((fxp + fxm) * ix2 + (fyp + fym) * iy2) + (fzp + fzm) * iz2);
+                            if (fabs(diag) < DBL_EPSILON) continue;
+                            double off = ((fxp * f[LID(L, i0 + 1, j0, k0)] + fxm * f[LID(L, i0 - 1, j0, k0)]) * ix2 +
+                                          (fyp * f[LID(L, i0, j0 + 1, k0)] + fym * f[LID(L, i0, j0 - 1, k0)]) * iy2) +
+                                         (fzp * f[LID(L, i0, j0, k0 + 1)] + fzm * f[LID(L, i0, j0, k0 - 1)]) * iz2;
+                            f[LID(L, i0, j0, k0)] = (L->b[LID(L, i0, j0, k0)] - off) / diag;
+                        }
+                    }
+        }
+}
+
+/* Damped Jacobi: φ ← φ + ω (b − Aφ)/diag, a fully blocked cell skipped.
+ * (No reference counterpart: the smoother the multigrid preconditioner of
+ * the cut-cell solve uses.) */
+static void mg_jacobi(orc_mg *s, mg_level *L, int nsweeps) {
+    double ix2 = 1.0 / (L->dx * L->dx), iy2 = 1.0 / (L->dy * L->dy), iz2 = 1.0 / (L->dz * L->dz);
+    size_t n = (size_t)(L->nx + 2) * (L->ny + 2) * (L->nz + 2);
+    for (int sw = 0; sw < nsweeps; sw++) {
+        unit_halo(L, L->phi, s->bcx, s->bcy, s->bcz);
+        memcpy(L->tmp, L->phi, sizeof(double) * n);
+        for (int k = 1; k <= L->nz; k++)
+            for (int j = 1; j <= L->ny; j++)
+                for (int i = 1; i <= L->nx; i++) {
+                    double diag;
+                    if (!L->cx) diag = -2 * ((ix2 + iy2) + iz2);
+                    else {
+                        int ii = i - 1, jj = j - 1, kk = k - 1;
+                        diag = -(((L->cx[CXI(L, ii + 1, jj, kk)] + L->cx[CXI(L, ii, jj, kk)]) * ix2 +
+                                  (L->cy[CYI(L, ii, jj + 1, kk)] + L->cy[CYI(L, ii, jj, kk)]) * iy2) +
+                                 (L->cz[CZI(L, ii, jj, kk + 1)] + L->cz[CZI(L, ii, jj, kk)]) * iz2);
+                    }
+                    if (fabs(diag) < DBL_EPSILON) continue;
+                    double r = L->b[LID(L, i, j, k)] - mg_A(L, L->tmp, i, j, k);
+                    L->phi[LID(L, i, j, k)] = L->tmp[LID(L, i, j, k)] + s->omega * (r / diag);
+                }
+    }
+}
+
+static void mg_smooth(orc_mg *s, mg_level *L, int nsweeps) {
+    if (s->smoother == 1) mg_jacobi(s, L, nsweeps);
+    else mg_rbgs(s, L, nsweeps);
+}
+
+static void mg_residual(orc_mg *s, mg_level *L) {
+    unit_halo(L, L->phi, s->bcx, s->bcy, s->bcz);
+    for (int k = 1; k <= L->nz; k++)
+        for (int j = 1; j <= L->ny; j++)
+            for (int i = 1; i <= L->nx; i++) L->r[LID(L, i, j, k)] = L->b[LID(L, i, j, k)] - mg_A(L, L->phi, i, j, k);
+}
+
+/* _restrict_body!, src/Multigrid.jl:381-392 */
+static void mg_restrict(mg_level *C, mg_level *F, double m) {
+    double eighth = 1.0 / 8;
+    for (int K = 1; K <= C->nz; K++)
+        for (int J = 1; J <= C->ny; J++)
+            for (int I = 1; I <= C->nx; I++) {
+                int i = 2 * I - 1, j = 2 * J - 1, k = 2 * K - 1; /* 0-based fine storage base */
+                const double *r = F->r;
+                double s = (((((((r[LID(F, i, j, k)] - m) + (r[LID(F, i + 1, j, k)] - m)) + (r[LID(F, i, j + 1, k)] - m)) +
+                               (r[LID(F, i + 1, j + 1, k)] - m)) + (r[LID(F, i, j, k + 1)] - m)) +
+                             (r[LID(F, i + 1, j, k + 1)] - m)) + (r[LID(F, i, j + 1, k + 1)] - m)) +
+                           (r[LID(F, i + 1, j + 1, k + 1)] - m);
+                C->b[LID(C, I, J, K)] = s * eighth;
+            }
+}
+
+/* _prolong_body!, src/Multigrid.jl:415-432, weights _prolong_w (:435-441) */
+static void mg_prolong_add(orc_mg *s, mg_level *F, mg_level *C) {
+    unit_halo(C, C->phi, s->bcx, s->bcy, s->bcz);
+    double n = 3.0 / 4, f = 1.0 / 4;
+    double wv[8] = {n * n * n, f * n * n, n * f * n, f * f * n, n * n * f, f * n * f, n * f * f, f * f * f};
+    for (int k = 1; k <= F->nz; k++)
+        for (int j = 1; j <= F->ny; j++)
+            for (int i = 1; i <= F->nx; i++) {
+                /* i, j, k: 1-based interior indices (the reference's is - 1) */
+                int Ih = ((i + 1) >> 1), Ix = Ih + ((i & 1) ? -1 : 1);
+                int Jh = ((j + 1) >> 1), Jy = Jh + ((j & 1) ? -1 : 1);
+                int Kh = ((k + 1) >> 1), Kz = Kh + ((k & 1) ? -1 : 1);
+                /* reference (1-based storage): Ih = (i+1)>>1 + 1; here 0-based storage drops the +1 */
+                const double *P = C->phi;
+                double acc = 0.0;
+                acc += wv[0] * P[LID(C, Ih, Jh, Kh)];
+                acc += wv[1] * P[LID(C, Ix, Jh, Kh)];
+                acc += wv[2] * P[LID(C, Ih, Jy, Kh)];
+                acc += wv[3] * P[LID(C, Ix, Jy, Kh)];
+                acc += wv[4] * P[LID(C, Ih, Jh, Kz)];
+                acc += wv[5] * P[LID(C, Ix, Jh, Kz)];
+                acc += wv[6] * P[LID(C, Ih, Jy, Kz)];
+                acc += wv[7] * P[LID(C, Ix, Jy, Kz)];
+                F->phi[LID(F, i, j, k)] += acc;
+            }
+}
+
+/* _restrict_coeff, src/Multigrid.jl:465-496 */
+static void mg_restrict_coeff(const mg_level *F, mg_level *C) {
+    int cnx = C->nx, cny = C->ny, cnz = C->nz;
+    int fx1 = F->nx + 1, fy1 = F->ny + 1, fz1 = F->nz + 1;
+    double quarter = 1.0 / 4;
+#define MIN(a, b) ((a) < (b) ? (a) : (b))
+    for (int K = 1; K <= cnz; K++)
+        for (int J = 1; J <= cny; J++)
+            for (int I = 1; I <= cnx + 1; I++) {
+                int i = 2 * I - 1;
+                double sm = 0.0;
+                for (int dk = 0; dk <= 1; dk++)
+                    for (int dj = 0; dj <= 1; dj++)
+                        sm += F->cx[CXI(F, MIN(i, fx1) - 1, MIN(2 * J - 1 + dj, F->ny) - 1, MIN(2 * K - 1 + dk, F->nz) - 1)];
+                C->cx[CXI(C, I - 1, J - 1, K - 1)] = sm * quarter;
+            }
+    for (int K = 1; K <= cnz; K++)
+        for (int J = 1; J <= cny + 1; J++)
+            for (int I = 1; I <= cnx; I++) {
+                int j = 2 * J - 1;
+                double sm = 0.0;
+                for (int dk = 0; dk <= 1; dk++)
+                    for (int di = 0; di <= 1; di++)
+                        sm += F->cy[CYI(F, MIN(2 * I - 1 + di, F->nx) - 1, MIN(j, fy1) - 1, MIN(2 * K - 1 + dk, F->nz) - 1)];
+                C->cy[CYI(C, I - 1, J - 1, K - 1)] = sm * quarter;
+            }
+    for (int K = 1; K <= cnz + 1; K++)
+        for (int J = 1; J <= cny; J++)
+            for (int I = 1; I <= cnx; I++) {
+                int k = 2 * K - 1;
+                double sm = 0.0;
+                for (int dj = 0; dj <= 1; dj++)
+                    for (int di = 0; di <= 1; di++)
+                        sm += F->cz[CZI(F, MIN(2 * I - 1 + di, F->nx) - 1, MIN(2 * J - 1 + dj, F->ny) - 1, MIN(k, fz1) - 1)];
+                C->cz[CZI(C, I - 1, J - 1, K - 1)] = sm * quarter;
+            }
+#undef MIN
+}
+
+static void level_alloc(mg_level *L, int nx, int ny, int nz, double dx, double dy, double dz, int coeff) {
+    L->nx = nx; L->ny = ny; L->nz = nz; L->dx = dx; L->dy = dy; L->dz = dz;
+    size_t n = (size_t)(nx + 2) * (ny + 2) * (nz + 2);
+    L->phi = (double *)calloc(n, sizeof(double));
+    L->b = (double *)calloc(n, sizeof(double));
+    L->r = (double *)calloc(n, sizeof(double));
+    L->tmp = (double *)calloc(n, sizeof(double));
+    if (coeff) {
+        L->cx = (double *)calloc((size_t)(nx + 1) * ny * nz, sizeof(double));
+        L->cy = (double *)calloc((size_t)nx * (ny + 1) * nz, sizeof(double));
+        L->cz = (double *)calloc((size_t)nx * ny * (nz + 1), sizeof(double));
+    }
+}
+
+void *orc_mg_create(int nx, int ny, int nz, double dx, double dy, double dz, int bcx, int bcy, int bcz,
+                    const double *cx, const double *cy, const double *cz, const double *rhs_scale, int nu1,
+                    int nu2, int min_size, int smoother, double omega) {
+    orc_mg *s = (orc_mg *)calloc(1, sizeof(orc_mg));
+    s->bcx = bcx; s->bcy = bcy; s->bcz = bcz;
+    s->nu1 = nu1; s->nu2 = nu2; s->smoother = smoother; s->omega = omega;
+    s->singular = 1; /* periodic or Neumann on every axis */
+    int coeff = cx != NULL;
+    level_alloc(&s->L[0], nx, ny, nz, dx, dy, dz, coeff);
+    if (coeff) {
+        memcpy(s->L[0].cx, cx, sizeof(double) * (size_t)(nx + 1) * ny * nz);
+        memcpy(s->L[0].cy, cy, sizeof(double) * (size_t)nx * (ny + 1) * nz);
+        memcpy(s->L[0].cz, cz, sizeof(double) * (size_t)nx * ny * (nz + 1));
+    }
+    s->nlev = 1;
+    /* the coarsening rule of the constructor, src/Multigrid.jl:576-590 */
+    while (s->nlev < 32) {
+        mg_level *F = &s->L[s->nlev - 1];
+        int can_x = F->nx % 2 == 0 && F->nx / 2 >= min_size;
+        int can_y = F->ny % 2 == 0 && F->ny / 2 >= min_size;
+        int can_z = F->nz % 2 == 0 && F->nz / 2 >= min_size;
+        if (!(can_x && can_y && can_z)) break;
+        mg_level *C = &s->L[s->nlev];
+        level_alloc(C, F->nx / 2, F->ny / 2, F->nz / 2, F->dx * 2, F->dy * 2, F->dz * 2, coeff);
+        if (coeff) mg_restrict_coeff(F, C);
+        s->nlev++;
+    }
+    if (rhs_scale) {
+        s->rhs_scale = (double *)malloc(sizeof(double) * nz);
+        memcpy(s->rhs_scale, rhs_scale, sizeof(double) * nz);
+    }
+    size_t ni = (size_t)nx * ny * nz;
+    double **ws[] = {&s->x, &s->bb, &s->pr, &s->pz, &s->pp, &s->pAp, &s->tmpin};
+    for (int a = 0; a < 7; a++) *ws[a] = (double *)calloc(ni, sizeof(double));
+    return s;
+}
+
+void orc_mg_destroy(void *h) {
+    orc_mg *s = (orc_mg *)h;
+    if (!s) return;
+    for (int l = 0; l < s->nlev; l++) {
+        mg_level *L = &s->L[l];
+        free(L->phi); free(L->b); free(L->r); free(L->tmp); free(L->cx); free(L->cy); free(L->cz);
+    }
+    free(s->rhs_scale); free(s->x); free(s->bb); free(s->pr); free(s->pz); free(s->pp); free(s->pAp); free(s->tmpin);
+    free(s);
+}
+
+int orc_mg_levels(void *h) { return ((orc_mg *)h)->nlev; }
+
+/* mg_vcycle!, src/Multigrid.jl:706-739; _solve_coarsest! :685-689 */
+static void mg_vcycle(orc_mg *s, int lev) {
+    mg_level *L = &s->L[lev];
+    if (lev == s->nlev - 1) {
+        mg_rbgs(s, L, 50);
+        if (s->singular) mg_project_mean(L, L->phi);
+        return;
+    }
+    mg_level *C = &s->L[lev + 1];
+    mg_smooth(s, L, s->nu1);
+    if (s->singular) mg_project_mean(L, L->phi);
+    mg_residual(s, L);
+    double m = s->singular ? mg_isum(L, L->r) / (double)((long long)L->nx * L->ny * L->nz) : 0.0;
+    mg_restrict(C, L, m);
+    if (s->singular) mg_project_mean(C, C->b);
+    memset(C->phi, 0, sizeof(double) * (size_t)(C->nx + 2) * (C->ny + 2) * (C->nz + 2));
+    mg_vcycle(s, lev + 1);
+    mg_prolong_add(s, L, C);
+    if (s->singular) mg_project_mean(L, L->phi);
+    mg_smooth(s, L, s->nu2);
+    if (s->singular) mg_project_mean(L, L->phi);
+}
+
+/* _load_rhs!, src/Multigrid.jl:764-775: copy in with its canonical sum, then the shift */
+static void mg_load_rhs(orc_mg *s, const double *rhs) {
+    mg_level *L = &s->L[0];
+    int nx = L->nx, ny = L->ny, nz = L->nz;
+    for (int k = 1; k <= nz; k++)
+        for (int j = 1; j <= ny; j++)
+            for (int i = 1; i <= nx; i++)
+                L->b[LID(L, i, j, k)] = rhs[(size_t)(i - 1) + (size_t)nx * ((j - 1) + (size_t)ny * (k - 1))];
+    if (s->singular) {
+        double sm = orc_interior_sum(rhs, nx, ny, 1, nx, 1, ny, 1, nz);
+        mg_shift(L, L->b, sm / (double)((long long)nx * ny * nz));
+    }
+}
+/* _store_solution!, src/Multigrid.jl:789-793 */
+static void mg_store(orc_mg *s, double *out) {
+    mg_level *L = &s->L[0];
+    int nx = L->nx, ny = L->ny, nz = L->nz;
+    double m = s->singular ? mg_isum(L, L->phi) / (double)((long long)nx * ny * nz) : 0.0;
+    for (int k = 1; k <= nz; k++)
+        for (int j = 1; j <= ny; j++)
+            for (int i = 1; i <= nx; i++) {
+                double v = L->phi[LID(L, i, j, k)] - m;
+                L->phi[LID(L, i, j, k)] = v;
+                out[(size_t)(i - 1) + (size_t)nx * ((j - 1) + (size_t)ny * (k - 1))] = v;
+            }
+}
+
+int orc_mg_solve(void *h, double *x, const double *rhs_in, double tol, int maxcycles, double *rel_out) {
+    orc_mg *s = (orc_mg *)h;
+    mg_level *L = &s->L[0];
+    int nx = L->nx, ny = L->ny, nz = L->nz;
+    size_t ni = (size_t)nx * ny * nz;
+    memcpy(s->tmpin, rhs_in, sizeof(double) * ni);
+    if (s->rhs_scale)
+        for (int k = 0; k < nz; k++)
+            for (size_t c = 0; c < (size_t)nx * ny; c++) s->tmpin[c + (size_t)nx * ny * k] *= s->rhs_scale[k];
+    mg_load_rhs(s, s->tmpin);
+    memset(L->phi, 0, sizeof(double) * (size_t)(nx + 2) * (ny + 2) * (nz + 2));
+    double bn = sqrt(mg_isum_sq(L, L->b));
+    if (bn == 0) { memset(x, 0, sizeof(double) * ni); *rel_out = 0; return 0; }
+    double rel = 1.0;
+    int ncyc = 0;
+    for (int cyc = 1; cyc <= maxcycles; cyc++) {
+        mg_vcycle(s, 0);
+        mg_residual(s, L);
+        double rn;
+        if (s->singular) {
+            double m = mg_isum(L, L->r) / (double)ni;
+            mg_shift(L, L->r, m);
+            rn = sqrt(mg_isum_sq(L, L->r));
+        } else rn = sqrt(mg_isum_sq(L, L->r));
+        rel = rn / bn;
+        ncyc = cyc;
+        if (rel < tol) break;
+    }
+    mg_store(s, x);
+    *rel_out = rel;
+    return ncyc;
+}
+
+/* _apply_fine!, src/Multigrid.jl:911-917 */
+void orc_mg_apply(void *h, double *out, const double *xin) {
+    orc_mg *s = (orc_mg *)h;
+    mg_level *L = &s->L[0];
+    int nx = L->nx, ny = L->ny, nz = L->nz;
+    for (int k = 1; k <= nz; k++)
+        for (int j = 1; j <= ny; j++)
+            for (int i = 1; i <= nx; i++)
+                L->phi[LID(L, i, j, k)] = xin[(size_t)(i - 1) + (size_t)nx * ((j - 1) + (size_t)ny * (k - 1))];
+    unit_halo(L, L->phi, s->bcx, s->bcy, s->bcz);
+    for (int k = 1; k <= nz; k++)
+        for (int j = 1; j <= ny; j++)
+            for (int i = 1; i <= nx; i++)
+                out[(size_t)(i - 1) + (size_t)nx * ((j - 1) + (size_t)ny * (k - 1))] = mg_A(L, L->phi, i, j, k);
+}
+
+/* _mg_precond!, src/Multigrid.jl:922-932 (one cycle) */
+static void mg_precond(orc_mg *s, double *z, const double *r) {
+    mg_level *L = &s->L[0];
+    mg_load_rhs(s, r);
+    memset(L->phi, 0, sizeof(double) * (size_t)(L->nx + 2) * (L->ny + 2) * (L->nz + 2));
+    mg_vcycle(s, 0);
+    mg_store(s, z);
+}
+
+/* canonical 4-tuple sums over an interior-sized array pair (r², r·z, z, r) */
+static void rz4(const double *r, const double *z, int nx, int ny, int nz, double out[4]) {
+    double tot[4] = {0, 0, 0, 0};
+    for (int k = 0; k < nz; k++) {
+        double l[4][8];
+        memset(l, 0, sizeof(l));
+        int nb = nx >> 3;
+        for (int j = 0; j < ny; j++) {
+            size_t row = (size_t)nx * (j + (size_t)ny * k);
+            int i = 0;
+            for (int b = 0; b < nb; b++) {
+                for (int q = 0; q < 8; q++) {
+                    double rv = r[row + i + q], zv = z[row + i + q];
+                    l[0][q] += rv * rv; l[1][q] += rv * zv; l[2][q] += zv; l[3][q] += rv;
+                }
+                i += 8;
+            }
+            for (; i < nx; i++) {
+                double rv = r[row + i], zv = z[row + i];
+                l[0][0] += rv * rv; l[1][0] += rv * zv; l[2][0] += zv; l[3][0] += rv;
+            }
+        }
+        for (int c = 0; c < 4; c++)
+            tot[c] += ((l[c][0] + l[c][1]) + (l[c][2] + l[c][3])) + ((l[c][4] + l[c][5]) + (l[c][6] + l[c][7]));
+    }
+    for (int c = 0; c < 4; c++) out[c] = tot[c];
+}
+static double dot_c(const double *a, const double *b, int nx, int ny, int nz) {
+    double tot = 0;
+    for (int k = 0; k < nz; k++) {
+        double l[8] = {0};
+        int nb = nx >> 3;
+        for (int j = 0; j < ny; j++) {
+            size_t row = (size_t)nx * (j + (size_t)ny * k);
+            int i = 0;
+            for (int bb = 0; bb < nb; bb++) {
+                for (int q = 0; q < 8; q++) l[q] += a[row + i + q] * b[row + i + q];
+                i += 8;
+            }
+            for (; i < nx; i++) l[0] += a[row + i] * b[row + i];
+        }
+        tot += ((l[0] + l[1]) + (l[2] + l[3])) + ((l[4] + l[5]) + (l[6] + l[7]));
+    }
+    return tot;
+}
+
+/* MGPreconditionedCG solve! (src/Multigrid.jl:955-972) through pcg!
+ * (src/Backends.jl:1526-1589) with a function operator and preconditioner,
+ * project_mean, definite = :negative, zero guess. */
+int orc_mgpcg_solve(void *h, double *xout, const double *rhs_in, double tol, int maxit, int *iters_out,
+                    double *resid_out, double *bnorm_out) {
+    orc_mg *s = (orc_mg *)h;
+    mg_level *L = &s->L[0];
+    int nx = L->nx, ny = L->ny, nz = L->nz;
+    size_t ni = (size_t)nx * ny * nz;
+    double n = (double)ni;
+    double *x = s->x, *b = s->bb, *r = s->pr, *z = s->pz, *p = s->pp, *Ap = s->pAp;
+    memcpy(b, rhs_in, sizeof(double) * ni);
+    if (s->rhs_scale)
+        for (int k = 0; k < nz; k++)
+            for (size_t c = 0; c < (size_t)nx * ny; c++) b[c + (size_t)nx * ny * k] *= s->rhs_scale[k];
+    int pm = s->singular;
+    double mb = pm ? orc_interior_sum(b, nx, ny, 1, nx, 1, ny, 1, nz) / n : 0.0;
+    memset(x, 0, sizeof(double) * ni);
+    /* _pcg_init_body with a non-pointwise M: z = 0 */
+    double b2 = 0;
+    for (size_t c = 0; c < ni; c++) { r[c] = b[c] - mb; z[c] = 0.0; }
+    {
+        double t4[4];
+        rz4(r, z, nx, ny, nz, t4);
+        b2 = t4[0]; /* (b − mb)² = r² at a zero guess */
+    }
+    double bnorm = sqrt(b2);
+    *bnorm_out = bnorm;
+    if (bnorm == 0) {
+        memset(xout, 0, sizeof(double) * ni);
+        *iters_out = 0; *resid_out = 0;
+        return 3;
+    }
+    mg_precond(s, z, r);
+    double t4[4];
+    rz4(r, z, nx, ny, nz, t4);
+    double r2 = t4[0], rz = t4[1], Sz = t4[2], Sr = t4[3];
+    double m = pm ? Sz / n : 0.0;
+    if (pm) rz -= m * Sr;
+    for (size_t c = 0; c < ni; c++) p[c] = z[c] - m;
+    double rnorm = sqrt(r2);
+    int iters = 0, status = rnorm <= tol * bnorm ? 0 : 1;
+    while (status != 0 && iters < maxit) {
+        orc_mg_apply(s, Ap, p);
+        double pAp = dot_c(p, Ap, nx, ny, nz);
+        if (!(fabs(pAp) > DBL_EPSILON * fabs(rz))) { status = 2; break; }
+        double alpha = rz / pAp;
+        iters++;
+        for (size_t c = 0; c < ni; c++) {
+            x[c] = x[c] + alpha * p[c];
+            r[c] = r[c] - alpha * Ap[c];
+        }
+        rz4(r, z, nx, ny, nz, t4);
+        rnorm = sqrt(t4[0]);
+        if (rnorm <= tol * bnorm) { status = 0; break; }
+        mg_precond(s, z, r);
+        rz4(r, z, nx, ny, nz, t4);
+        double rzn = t4[1];
+        Sz = t4[2]; Sr = t4[3];
+        m = pm ? Sz / n : 0.0;
+        if (pm) rzn -= m * Sr;
+        double beta = rzn / rz;
+        for (size_t c = 0; c < ni; c++) p[c] = (z[c] - m) + beta * p[c];
+        rz = rzn;
+    }
+    if (pm) {
+        double mx = orc_interior_sum(x, nx, ny, 1, nx, 1, ny, 1, nz) / n;
+        for (size_t c = 0; c < ni; c++) x[c] = x[c] - mx;
+    }
+    memcpy(xout, x, sizeof(double) * ni);
+    *iters_out = iters;
+    *resid_out = rnorm;
+    return status;
+}
+
+/* ------------------------------------------------------------------------
+ * Cut-cell geometry from a heightmap: cut_cell_geometry_from_heightmap
+ * (src/CutCell.jl:372-396) through cut_cell_geometry (:213-312) and
+ * _finalize_geometry (:398-480; the small-cell merge map and the normals are
+ * not computed: the Poisson operator needs vol_eff and the apertures only).
+ * ------------------------------------------------------------------------ */
+void orc_cutcell_heightmap(const orc_grid *g, const double *hfull, int nsub, double vol_min, double *vol,
+                           double *vol_eff, double *ax, double *ay, double *az) {
+    GRID_DIMS(g);
+    int n1 = sx, n2 = sy, n3 = sz, H = g->H;
+    double invn = 1.0 / (double)nsub;
+    double hx = g->dx * invn, hy = g->dy * invn, hz = g->dz * invn;
+    double volnorm = 1.0 / (double)(nsub * nsub * nsub), areanorm = 1.0 / (double)(nsub * nsub);
+    /* mesh origin 0; _xw(mesh, i) = x0 + (i - Hx - 1) dx (west face of storage cell i) */
+#define XW(i) (((i) - H - 1) * g->dx)
+#define YW(j) (((j) - H - 1) * g->dy)
+#define ZW(k) (((k) - H - 1) * g->dz)
+#define COLI(x, d, n) ({ int _c = (int)floor((x) / (d)) + H + 1; _c < 1 ? 1 : (_c > (n) ? (n) : _c); })
+#define LS(x, y, z) ((z) - hfull[(size_t)(COLI(x, g->dx, n1) - 1) + (size_t)n1 * (size_t)(COLI(y, g->dy, n2) - 1)])
+#pragma omp parallel for schedule(static)
+    for (int k = 1; k <= n3; k++)
+        for (int j = 1; j <= n2; j++)
+            for (int i = 1; i <= n1; i++) {
+                double xb = XW(i), yb = YW(j), zb = ZW(k);
+                int cnt = 0;
+                for (int c = 1; c <= nsub; c++) {
+                    double zs = zb + (c - 0.5) * hz;
+                    for (int b = 1; b <= nsub; b++) {
+                        double ys = yb + (b - 0.5) * hy;
+                        for (int a = 1; a <= nsub; a++) {
+                            double xs = xb + (a - 0.5) * hx;
+                            cnt += LS(xs, ys, zs) > 0;
+                        }
+                    }
+                }
+                vol[ID(i, j, k)] = cnt * volnorm;
+                cnt = 0;
+                for (int c = 1; c <= nsub; c++) {
+                    double zs = zb + (c - 0.5) * hz;
+                    for (int b = 1; b <= nsub; b++) cnt += LS(xb, yb + (b - 0.5) * hy, zs) > 0;
+                }
+                ax[ID(i, j, k)] = cnt * areanorm;
+                cnt = 0;
+                for (int c = 1; c <= nsub; c++) {
+                    double zs = zb + (c - 0.5) * hz;
+                    for (int a = 1; a <= nsub; a++) cnt += LS(xb + (a - 0.5) * hx, yb, zs) > 0;
+                }
+                ay[ID(i, j, k)] = cnt * areanorm;
+                cnt = 0;
+                for (int b = 1; b <= nsub; b++) {
+                    double ys = yb + (b - 0.5) * hy;
+                    for (int a = 1; a <= nsub; a++) cnt += LS(xb + (a - 0.5) * hx, ys, zb) > 0;
+                }
+                az[ID(i, j, k)] = cnt * areanorm;
+            }
+#undef LS
+#undef COLI
+#undef XW
+#undef YW
+#undef ZW
+    size_t n = (size_t)n1 * n2 * n3;
+    unsigned char *solid = (unsigned char *)malloc(n);
+    for (size_t c = 0; c < n; c++) solid[c] = !(vol[c] > 0);
+    for (int k = 1; k <= n3; k++)
+        for (int j = 1; j <= n2; j++) { ax[ID(1, j, k)] = 0; }
+    for (int k = 1; k <= n3; k++)
+        for (int i = 1; i <= n1; i++) { ay[ID(i, 1, k)] = 0; }
+    for (int j = 1; j <= n2; j++)
+        for (int i = 1; i <= n1; i++) { az[ID(i, j, 1)] = 0; }
+    for (int sweep = 0; sweep < 16; sweep++) {
+        int changed = 0;
+        for (int k = 1; k <= n3; k++)
+            for (int j = 1; j <= n2; j++)
+                for (int i = 2; i <= n1; i++)
+                    if (solid[ID(i - 1, j, k)] || solid[ID(i, j, k)]) ax[ID(i, j, k)] = 0;
+        for (int k = 1; k <= n3; k++)
+            for (int j = 2; j <= n2; j++)
+                for (int i = 1; i <= n1; i++)
+                    if (solid[ID(i, j - 1, k)] || solid[ID(i, j, k)]) ay[ID(i, j, k)] = 0;
+        for (int k = 2; k <= n3; k++)
+            for (int j = 1; j <= n2; j++)
+                for (int i = 1; i <= n1; i++)
+                    if (solid[ID(i, j, k - 1)] || solid[ID(i, j, k)]) az[ID(i, j, k)] = 0;
+        for (int k = 1; k <= n3; k++)
+            for (int j = 1; j <= n2; j++)
+                for (int i = 1; i <= n1; i++) {
+                    size_t c = ID(i, j, k);
+                    if (solid[c]) continue;
+                    double open = (ax[c] + ay[c]) + az[c];
+                    if (i < n1) open += ax[ID(i + 1, j, k)];
+                    if (j < n2) open += ay[ID(i, j + 1, k)];
+                    if (k < n3) open += az[ID(i, j, k + 1)];
+                    if (!(open > 0)) { solid[c] = 1; vol[c] = 0; changed = 1; }
+                }
+        if (!changed) break;
+    }
+    for (size_t c = 0; c < n; c++) vol_eff[c] = solid[c] ? 0.0 : (vol[c] < vol_min ? vol_min : vol[c]);
+    free(solid);
+}

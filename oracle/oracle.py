@@ -1,0 +1,291 @@
+"""ctypes wrapper of the CPU oracle (liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() as
+the checker and by bench.py's cpu_baseline leg. The product (aeolus_hip)
+never imports this module.
+
+The grid metrics here are an independent restatement of the reference's
+StretchedGrid constructor (src/StretchedGrids.jl:319-388) and level_metrics
+(:570-600), so the product's own metric construction is checked too.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liboracle.so")
+
+CC, FX, FY, FZ = 0, 1, 2, 3
+PERIODIC, NOFLUX, WALL = 0, 1, 2
+CENTERED, UPWIND1, WENO5, WENO_JS5, WENO_Z5, WENO_Z7 = 0, 1, 2, 3, 4, 5
+SCHEMES = {"centered": CENTERED, "upwind": UPWIND1, "weno5": WENO5, "weno_js5": WENO_JS5, "weno_z5": WENO_Z5,
+           "weno_z7": WENO_Z7}
+BCS = {"periodic": PERIODIC, "noflux": NOFLUX, "wall": WALL}
+
+
+class OrcGrid(ctypes.Structure):
+    _fields_ = [("Nx", ctypes.c_int), ("Ny", ctypes.c_int), ("Nz", ctypes.c_int), ("H", ctypes.c_int),
+                ("dx", ctypes.c_double), ("dy", ctypes.c_double), ("dz", ctypes.c_double),
+                ("stretched", ctypes.c_int)] + [(n, ctypes.POINTER(ctypes.c_double)) for n in
+                                                ("idzc", "idzf", "wlo", "whi", "alo", "ahi")]
+
+
+class OrcStep(ctypes.Structure):
+    _fields_ = [("mom_scheme", ctypes.c_int), ("scal_scheme", ctypes.c_int), ("weno_p", ctypes.c_int),
+                ("nu", ctypes.c_double), ("theta0", ctypes.c_double), ("g_accel", ctypes.c_double),
+                ("flux_bot", ctypes.c_double), ("flux_top", ctypes.c_double),
+                ("bcx", ctypes.c_int), ("bcy", ctypes.c_int), ("bcz", ctypes.c_int),
+                ("tbcx", ctypes.c_int), ("tbcy", ctypes.c_int), ("tbcz", ctypes.c_int), ("solver", ctypes.c_int)]
+
+
+def build():
+    """Compile liboracle.so with its Makefile (gcc, OpenMP, no FMA contraction)."""
+    r = subprocess.run(["make", "-C", HERE, "-s", "liboracle.so"], capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("oracle build failed:\n" + r.stderr)
+    return LIB
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        L = ctypes.CDLL(LIB)
+        P = ctypes.POINTER
+        d, i, vp, pd = ctypes.c_double, ctypes.c_int, ctypes.c_void_p, P(ctypes.c_double)
+        pg = P(OrcGrid)
+        sig = {
+            "orc_weno_z5_left": (d, [d] * 5 + [i]),
+            "orc_weno_z7_left": (d, [d] * 7 + [i]),
+            "orc_weno_js5_left": (d, [d] * 6),
+            "orc_weno5_classic_left": (d, [d] * 5),
+            "orc_fill_halos": (None, [pd, pg, i, i, i, i]),
+            "orc_advect_scalar": (None, [pd, pd, pd, pd, pd, pg, i, i]),
+            "orc_advect_momentum": (None, [pd, pd, pd, pd, pd, pd, pg, i, i]),
+            "orc_laplacian": (None, [pd, pd, pg, i]),
+            "orc_divergence_into": (None, [pd, pd, pd, pd, pg]),
+            "orc_max_divergence": (d, [pd, pd, pd, pg]),
+            "orc_max_adv_rate": (d, [pd, pd, pd, pg]),
+            "orc_interior_sum": (d, [pd, i, i, i, i, i, i, i, i]),
+            "orc_fft_create": (vp, [i, i, i, d, d, d]),
+            "orc_fft_solve": (None, [vp, pd, pd]),
+            "orc_column_create": (vp, [i, i, i, d, d, i, i, pd, pd, pd, pd, pd, pd]),
+            "orc_column_solve": (None, [vp, pd, pd]),
+            "orc_solver_destroy": (None, [vp]),
+            "orc_stepper_create": (vp, [pg, P(OrcStep), vp]),
+            "orc_stepper_destroy": (None, [vp]),
+            "orc_step_rk3": (None, [vp, pd, pd, pd, pd, pd, d]),
+            "orc_step_euler": (None, [vp, pd, pd, pd, pd, pd, d]),
+            "orc_mg_create": (vp, [i, i, i, d, d, d, i, i, i, pd, pd, pd, pd, i, i, i, i, d]),
+            "orc_mg_destroy": (None, [vp]),
+            "orc_mg_levels": (i, [vp]),
+            "orc_mg_solve": (i, [vp, pd, pd, d, i, pd]),
+            "orc_mgpcg_solve": (i, [vp, pd, pd, d, i, P(i), pd, pd]),
+            "orc_mg_apply": (None, [vp, pd, pd]),
+            "orc_cutcell_heightmap": (None, [pg, pd, i, d, pd, pd, pd, pd, pd]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def ptr(a):
+    if a is None:
+        return None
+    assert a.dtype == np.float64 and (a.flags.f_contiguous or a.ndim == 1), "oracle arrays: FP64, Fortran order"
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+# ---------------------------------------------------------------------------
+# grids
+# ---------------------------------------------------------------------------
+def stretched_metrics(zf_interior, Nz, H):
+    """Storage-level (idzc, idzf, wlo, whi, alo, ahi) of a StretchedGrid with
+    (:custom, zf): _extend_faces (src/StretchedGrids.jl:319-337), the
+    constructor's dzc/zc/dzf (:375-388) and level_metrics (:570-600)."""
+    n = Nz + 2 * H
+    zf = np.zeros(n + 1)
+    zf[H:H + Nz + 1] = zf_interior
+    for k in range(H - 1, -1, -1):
+        zf[k] = 2 * zf[k + 1] - zf[k + 2]
+    for k in range(H + Nz + 1, n + 1):
+        zf[k] = 2 * zf[k - 1] - zf[k - 2]
+    dzc = np.array([zf[k + 1] - zf[k] for k in range(n)])
+    zc = np.array([(zf[k] + zf[k + 1]) / 2 for k in range(n)])
+    dzf = np.zeros(n + 1)
+    dzf[0] = dzc[0]
+    for k in range(1, n):
+        dzf[k] = zc[k] - zc[k - 1]
+    dzf[n] = dzc[n - 1]
+    idzc = np.array([1 / x for x in dzc])
+    idzf = np.array([1 / x for x in dzf])
+    wlo = np.zeros(n); whi = np.zeros(n); alo = np.zeros(n); ahi = np.zeros(n)
+    for k in range(1, n):
+        wlo[k] = dzc[k] / (dzc[k - 1] + dzc[k])
+        whi[k] = dzc[k - 1] / (dzc[k - 1] + dzc[k])
+        s = dzc[k - 1] + dzc[k]
+        alo[k] = dzc[k - 1] / s
+        ahi[k] = dzc[k] / s
+    return dict(idzc=idzc, idzf=idzf, wlo=wlo, whi=whi, alo=alo, ahi=ahi, dzc=dzc, dzf=dzf, zf=zf, zc=zc)
+
+
+class Grid:
+    """Oracle-side grid: uniform (zf None) or stretched (interior faces zf)."""
+
+    def __init__(self, size, extent, halo=3, zf=None):
+        self.Nx, self.Ny, self.Nz = size
+        self.H = halo
+        self.dx = extent[0] / self.Nx
+        self.dy = extent[1] / self.Ny
+        self.dz = extent[2] / self.Nz
+        self.storage = (self.Nx + 2 * halo, self.Ny + 2 * halo, self.Nz + 2 * halo)
+        self.zf = None if zf is None else np.asarray(zf, dtype=np.float64)
+        self.m = None if zf is None else stretched_metrics(self.zf, self.Nz, halo)
+        g = OrcGrid()
+        g.Nx, g.Ny, g.Nz, g.H = self.Nx, self.Ny, self.Nz, halo
+        g.dx, g.dy, g.dz = self.dx, self.dy, self.dz
+        g.stretched = 0 if zf is None else 1
+        if zf is not None:
+            for name in ("idzc", "idzf", "wlo", "whi", "alo", "ahi"):
+                setattr(g, name, ptr(self.m[name]))
+        self.c = g
+
+    @property
+    def interior(self):
+        H = self.H
+        return (slice(H, H + self.Nx), slice(H, H + self.Ny), slice(H, H + self.Nz))
+
+    def interior_dz(self):
+        if self.m is None:
+            return np.full(self.Nz, self.dz), np.full(self.Nz + 1, self.dz)
+        H = self.H
+        return self.m["dzc"][H:H + self.Nz].copy(), self.m["dzf"][H:H + self.Nz + 1].copy()
+
+    def zeros(self):
+        return np.zeros(self.storage, order="F")
+
+
+def fill_halos(g, a, loc, bcs):
+    lib().orc_fill_halos(ptr(a), ctypes.byref(g.c), loc, *[BCS.get(b, b) for b in bcs])
+    return a
+
+
+def advect_scalar(g, tend, u, v, w, phi, scheme, p=2):
+    lib().orc_advect_scalar(ptr(tend), ptr(u), ptr(v), ptr(w), ptr(phi), ctypes.byref(g.c), SCHEMES[scheme], p)
+    return tend
+
+
+def advect_momentum(g, tu, tv, tw, u, v, w, scheme, p=2):
+    lib().orc_advect_momentum(ptr(tu), ptr(tv), ptr(tw), ptr(u), ptr(v), ptr(w), ctypes.byref(g.c), SCHEMES[scheme], p)
+    return tu, tv, tw
+
+
+def laplacian(g, f, loc):
+    out = g.zeros()
+    lib().orc_laplacian(ptr(out), ptr(f), ctypes.byref(g.c), loc)
+    return out
+
+
+def divergence_interior(g, u, v, w):
+    buf = np.zeros((g.Nx, g.Ny, g.Nz), order="F")
+    lib().orc_divergence_into(ptr(buf), ptr(u), ptr(v), ptr(w), ctypes.byref(g.c))
+    return buf
+
+
+def max_divergence(g, u, v, w):
+    return lib().orc_max_divergence(ptr(u), ptr(v), ptr(w), ctypes.byref(g.c))
+
+
+def interior_sum(a):
+    """Canonical 8-lane plane sum of a 3-D Fortran array (src/Backends.jl:703-750)."""
+    a = np.asfortranarray(a, dtype=np.float64)
+    nx, ny, nz = a.shape
+    return lib().orc_interior_sum(ptr(a), nx, ny, 1, nx, 1, ny, 1, nz)
+
+
+# ---------------------------------------------------------------------------
+# solvers
+# ---------------------------------------------------------------------------
+class Solver:
+    def __init__(self, h, kind, size):
+        self.h = h
+        self.kind = kind
+        self.size = size
+
+    def solve(self, src):
+        src = np.asfortranarray(src, dtype=np.float64)
+        phi = np.zeros(self.size, order="F")
+        (lib().orc_fft_solve if self.kind == 0 else lib().orc_column_solve)(self.h, ptr(phi), ptr(src))
+        return phi
+
+    def __del__(self):
+        try:
+            lib().orc_solver_destroy(self.h)
+        except Exception:
+            pass
+
+
+def fft_solver(g):
+    return Solver(lib().orc_fft_create(g.Nx, g.Ny, g.Nz, g.dx, g.dy, g.dz), 0, (g.Nx, g.Ny, g.Nz))
+
+
+def column_solver(size, dxy, lower, upper, hweight, measure, inv_coupling, gauge, bcx="periodic", bcy="periodic"):
+    vs = [np.ascontiguousarray(v, dtype=np.float64) for v in (lower, upper, hweight, measure, inv_coupling, gauge)]
+    h = lib().orc_column_create(size[0], size[1], size[2], dxy[0], dxy[1], BCS[bcx], BCS[bcy], *[ptr(v) for v in vs])
+    s = Solver(h, 1, tuple(size))
+    s._keep = vs
+    return s
+
+
+def stretched_column_solver(g, bcx="periodic", bcy="periodic"):
+    """ColumnTridiagonalSolver(T, N, dxy, dzc, dzf) (src/PressureSolvers.jl:685-700)."""
+    dzc, dzf = g.interior_dz()
+    Nz = g.Nz
+    lower = np.zeros(Nz)
+    upper = np.zeros(Nz)
+    for k in range(Nz):
+        if k > 0:
+            lower[k] = 1 / (dzc[k] * dzf[k])
+        if k < Nz - 1:
+            upper[k] = 1 / (dzc[k] * dzf[k + 1])
+    return column_solver((g.Nx, g.Ny, Nz), (g.dx, g.dy), lower, upper, np.ones(Nz), dzc, dzf, dzc, bcx, bcy)
+
+
+class Stepper:
+    def __init__(self, g, scheme="centered", weno_p=2, nu=0.0, theta0=0.0, g_accel=9.81, flux_bot=0.0, flux_top=0.0,
+                 bcs=("periodic",) * 3, theta_bcs=None, solver=None):
+        theta_bcs = theta_bcs or bcs
+        if solver is None:
+            solver = fft_solver(g) if all(b == "periodic" for b in bcs) else stretched_column_solver(g)
+        self.solver = solver
+        sp = OrcStep()
+        sp.mom_scheme = sp.scal_scheme = SCHEMES[scheme]
+        sp.weno_p = weno_p
+        sp.nu, sp.theta0, sp.g_accel, sp.flux_bot, sp.flux_top = nu, theta0, g_accel, flux_bot, flux_top
+        sp.bcx, sp.bcy, sp.bcz = (BCS[b] for b in bcs)
+        sp.tbcx, sp.tbcy, sp.tbcz = (BCS[b] for b in theta_bcs)
+        sp.solver = solver.kind
+        self.g = g
+        self.sp = sp
+        self.h = lib().orc_stepper_create(ctypes.byref(g.c), ctypes.byref(sp), solver.h)
+
+    def rk3(self, u, v, w, th, p, dt):
+        lib().orc_step_rk3(self.h, ptr(u), ptr(v), ptr(w), ptr(th), ptr(p), dt)
+
+    def euler(self, u, v, w, th, p, dt):
+        lib().orc_step_euler(self.h, ptr(u), ptr(v), ptr(w), ptr(th), ptr(p), dt)
+
+    def __del__(self):
+        try:
+            lib().orc_stepper_destroy(self.h)
+        except Exception:
+            pass

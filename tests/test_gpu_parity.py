@@ -1,0 +1,229 @@
+"""GPU parity: the HIP kernels through the C ABI against the CPU oracle.
+
+FP64 results must agree with the oracle to 1e-12 relative (max |gpu - oracle|
+/ max |oracle|); integer/copy work (halo fills) bit for bit. FP32 results
+state their error against the FP64 oracle.
+"""
+import numpy as np
+import pytest
+
+from helpers import rand_storage, rel_err
+
+pytestmark = pytest.mark.gpu
+
+TOL64 = 1e-12
+SCHEMES = ["centered", "upwind", "weno5", "weno_js5", "weno_z5", "weno_z7"]
+
+
+def grids(hip, orc, size, extent, H=3, stretched=False):
+    from aeolus_hip.fields import faces_geometric
+    zf = faces_geometric(size[2], 0.0, extent[2], 1.05) if stretched else None
+    return hip.Grid(size, extent, halo=H, zf=zf), orc.Grid(size, extent, halo=H, zf=zf)
+
+
+def up(hip, a, prec=0):
+    from aeolus_hip.fields import DeviceArray
+    return DeviceArray.from_numpy(a if prec == 0 else a.astype(np.float32), prec)
+
+
+def test_grid_metrics_match(hip, orc):
+    from aeolus_hip.workloads import dcbl_faces
+    zf, Lz = dcbl_faces(24)
+    g, og = hip.Grid((16, 16, 24), (320.0, 320.0, Lz), halo=3, zf=zf), orc.Grid((16, 16, 24), (320.0, 320.0, Lz), 3, zf)
+    idzc, idzf = g.metrics()
+    assert np.array_equal(idzc, og.m["idzc"])
+    assert np.array_equal(idzf, og.m["idzf"])
+
+
+@pytest.mark.parametrize("loc", [0, 1, 2, 3])
+@pytest.mark.parametrize("bcs", [("periodic",) * 3, ("periodic", "noflux", "wall"), ("noflux", "periodic", "noflux")])
+def test_halo_fill_bit_exact(hip, orc, loc, bcs):
+    g, og = grids(hip, orc, (12, 10, 9), (1.0, 1.0, 1.0))
+    a = rand_storage(og.storage, 10 + loc)
+    d = up(hip, a)
+    hip.fill_halos(g, d, loc, bcs)
+    orc.fill_halos(og, a, loc, bcs)
+    assert np.array_equal(d.download(), a)
+
+
+def _fields(orc, og, seed, bcs):
+    u, v, w, phi = (rand_storage(og.storage, seed + s) for s in range(4))
+    for a, loc in ((u, orc.FX), (v, orc.FY), (w, orc.FZ), (phi, orc.CC)):
+        orc.fill_halos(og, a, loc, bcs)
+    return u, v, w, phi
+
+
+@pytest.mark.parametrize("stretched", [False, True])
+@pytest.mark.parametrize("scheme", SCHEMES)
+def test_advect_scalar_parity(hip, orc, scheme, stretched):
+    H = 4 if scheme == "weno_z7" else 3
+    g, og = grids(hip, orc, (70, 21, 37), (3.0, 1.0, 2.0), H, stretched)
+    bcs = ("periodic", "periodic", "wall" if stretched else "periodic")
+    u, v, w, phi = _fields(orc, og, 100, bcs)
+    t0 = rand_storage(og.storage, 99)
+    t = t0.copy(order="F")
+    orc.advect_scalar(og, t, u, v, w, phi, scheme)
+    dt_ = up(hip, t0)
+    hip.advection_scheme(scheme, g)(dt_, up(hip, u), up(hip, v), up(hip, w), up(hip, phi))
+    got = dt_.download()
+    I = og.interior
+    assert rel_err(got[I], t[I]) <= TOL64
+    assert np.array_equal(got, t) or rel_err(got, t) <= TOL64
+
+
+@pytest.mark.parametrize("stretched", [False, True])
+@pytest.mark.parametrize("scheme", SCHEMES)
+def test_advect_momentum_parity(hip, orc, scheme, stretched):
+    H = 4 if scheme == "weno_z7" else 3
+    g, og = grids(hip, orc, (66, 13, 35), (2.0, 1.0, 3.0), H, stretched)
+    bcs = ("periodic", "periodic", "wall" if stretched else "periodic")
+    u, v, w, _ = _fields(orc, og, 200, bcs)
+    tu0, tv0, tw0 = (rand_storage(og.storage, s) for s in (1, 2, 3))
+    tu, tv, tw = (a.copy(order="F") for a in (tu0, tv0, tw0))
+    orc.advect_momentum(og, tu, tv, tw, u, v, w, scheme)
+    dtu, dtv, dtw = up(hip, tu0), up(hip, tv0), up(hip, tw0)
+    hip.advection_scheme(scheme, g).momentum(dtu, dtv, dtw, up(hip, u), up(hip, v), up(hip, w))
+    I = og.interior
+    for d, ref in ((dtu, tu), (dtv, tv), (dtw, tw)):
+        assert rel_err(d.download()[I], ref[I]) <= TOL64
+
+
+@pytest.mark.parametrize("scheme", ["weno_z5", "weno5"])
+def test_advect_fp32_error_stated(hip, orc, scheme):
+    # FP32 tendencies against the FP64 oracle on smooth data: stated bound 1e-5
+    g, og = grids(hip, orc, (64, 32, 16), (2 * np.pi, np.pi, np.pi / 2))
+    H = 3
+    x = (np.arange(og.storage[0]) - H + 0.5) * og.dx
+    y = (np.arange(og.storage[1]) - H + 0.5) * og.dy
+    z = (np.arange(og.storage[2]) - H + 0.5) * og.dz
+    X, Y, Z = np.meshgrid(x, y, z, indexing="ij")
+    u = np.asfortranarray(1.0 + 0.5 * np.sin(Y) * np.cos(Z))
+    v = np.asfortranarray(0.3 * np.cos(X))
+    w = np.asfortranarray(0.2 * np.sin(X + Y))
+    phi = np.asfortranarray(np.sin(X) * np.cos(2 * Y) + 0.1 * np.cos(4 * Z))
+    t = og.zeros()
+    orc.advect_scalar(og, t, u, v, w, phi, scheme)
+    dt_ = up(hip, og.zeros(), 1)
+    hip.advection_scheme(scheme, g)(dt_, up(hip, u, 1), up(hip, v, 1), up(hip, w, 1), up(hip, phi, 1))
+    err = rel_err(dt_.download()[og.interior], t[og.interior])
+    print(f"FP32 {scheme} tendency error vs FP64 oracle: {err:.2e}")
+    assert err < 1e-5
+
+
+@pytest.mark.parametrize("size", [(16, 8, 32), (64, 64, 64), (128, 32, 16)])
+def test_fft_solver_parity(hip, orc, size):
+    extent = (1.0, 2.0, 3.0)
+    g, og = grids(hip, orc, size, extent, H=1)
+    src = np.asfortranarray(np.random.default_rng(size[0]).normal(size=size))
+    ref = orc.fft_solver(og).solve(src)
+    got = hip.solve_interior(hip.FFTPoissonSolver(g), src)
+    assert rel_err(got, ref) <= TOL64
+
+
+@pytest.mark.parametrize("stretched", [False, True])
+@pytest.mark.parametrize("size", [(16, 8, 12), (64, 32, 40), (256, 64, 9)])
+def test_column_solver_parity(hip, orc, size, stretched):
+    extent = (100.0, 60.0, 80.0)
+    g, og = grids(hip, orc, size, extent, H=1, stretched=stretched)
+    src = np.asfortranarray(np.random.default_rng(size[2]).normal(size=size))
+    ref = orc.stretched_column_solver(og).solve(src)
+    got = hip.solve_interior(hip.solver_for(g), src)
+    assert rel_err(got, ref) <= TOL64
+    it, res = hip.solver_stats(hip.solver_for(g))
+    assert it == 0 and np.isnan(res)  # a direct solver: (0, NaN)
+
+
+def test_density_weighted_column_parity(hip, orc):
+    Nx, Ny, Nz = 32, 16, 20
+    dz = 50.0
+    zc = (np.arange(Nz) + 0.5) * dz
+    zf = np.arange(Nz + 1) * dz
+    w_c, w_f = np.exp(-zc / 8000.0), np.exp(-zf / 8000.0)
+    s = hip.ColumnTridiagonalSolver.density_weighted((Nx, Ny, Nz), (100.0, 100.0), dz, w_c, w_f)
+    lower = np.array([w_f[k] / dz ** 2 if k > 0 else 0 for k in range(Nz)])
+    upper = np.array([w_f[k + 1] / dz ** 2 if k < Nz - 1 else 0 for k in range(Nz)])
+    o = orc.column_solver((Nx, Ny, Nz), (100.0, 100.0), lower, upper, w_c, np.full(Nz, dz ** 2), 1 / w_f,
+                          np.ones(Nz))
+    src = np.asfortranarray(np.random.default_rng(3).normal(size=(Nx, Ny, Nz)))
+    assert rel_err(hip.solve_interior(s, src), o.solve(src)) <= TOL64
+
+
+def _run_step(hip, orc, kind, scheme, steps, prec=0, N=None, euler=False):
+    from aeolus_hip import workloads as W
+    if kind == "tgv":
+        N = N or 32
+        g = W.tgv_grid(N)
+        og = orc.Grid((N, N, N), (2 * np.pi,) * 3, halo=3)
+        state = W.tgv_state(g)
+        phys = dict(W.TGV_PHYS)
+        dt = 0.5 * (2 * np.pi / N) / 3.0
+    else:
+        Nxy, Nz = N or (32, 24)
+        zf, Lz = W.dcbl_faces(Nz)
+        g = hip.Grid((Nxy, Nxy, Nz), (20.0 * Nxy, 20.0 * Nxy, Lz), halo=3, zf=zf)
+        og = orc.Grid((Nxy, Nxy, Nz), (20.0 * Nxy, 20.0 * Nxy, Lz), halo=3, zf=zf)
+        state = W.dcbl_state(g)
+        rng = np.random.default_rng(5)
+        for a in state[:2]:
+            a[g.interior] += 0.5 * rng.uniform(-1, 1, size=a[g.interior].shape)
+        phys = dict(W.DCBL_PHYS)
+        dt = 0.5
+    sim = hip.BoussinesqSimulation(g, scheme=scheme, precision=prec, **phys)
+    sim.set(*state)
+    ost = orc.Stepper(og, scheme=scheme, **{k: v for k, v in phys.items()})
+    u, v, w, th = (np.asfortranarray(a.copy()) for a in state)
+    p = og.zeros()
+    for _ in range(steps):
+        if euler:
+            sim.step(dt)
+            ost.euler(u, v, w, th, p, dt)
+        else:
+            sim.step_rk3(dt)
+            ost.rk3(u, v, w, th, p, dt)
+    return sim.state(), dict(u=u, v=v, w=w, theta=th, p=p), og
+
+
+@pytest.mark.parametrize("kind,scheme", [("tgv", "weno5"), ("tgv", "centered"), ("tgv", "weno_z5"),
+                                         ("dcbl", "weno_z5"), ("dcbl", "centered")])
+def test_rk3_step_parity_fp64(hip, orc, kind, scheme):
+    got, ref, og = _run_step(hip, orc, kind, scheme, steps=2)
+    for n in ("u", "v", "w", "theta", "p"):
+        assert rel_err(got[n], ref[n]) <= TOL64, n
+
+
+def test_euler_step_parity_fp64(hip, orc):
+    got, ref, og = _run_step(hip, orc, "dcbl", "weno_z5", steps=2, euler=True)
+    for n in ("u", "v", "w", "theta", "p"):
+        assert rel_err(got[n], ref[n]) <= TOL64, n
+
+
+def test_rk3_step_fp32_error_stated(hip, orc):
+    # FP32 WENO-Z tendencies with the FP64 projection: error against the FP64 oracle
+    got, ref, og = _run_step(hip, orc, "dcbl", "weno_z5", steps=2, prec=1)
+    I = og.interior
+    errs = {n: rel_err(got[n][I], ref[n][I]) for n in ("u", "v", "w")}
+    th_err = np.abs(got["theta"][I] - ref["theta"][I]).max()
+    print(f"FP32 DCBL step error vs FP64 oracle: {errs}, theta abs {th_err:.2e} K")
+    assert max(errs.values()) < 1e-4 and th_err < 1e-3
+
+
+def test_dcbl_full_size_properties(hip):
+    # 256x256x128 DCBL, one RK3 step from rest (FP64): solenoidal, w = 0 on the
+    # walls, θ budget equals the surface heat input
+    from aeolus_hip import workloads as W
+    g = W.dcbl_grid()
+    sim = hip.BoussinesqSimulation(g, scheme="weno_z5", **W.DCBL_PHYS)
+    state = W.dcbl_state(g)
+    sim.set(*state)
+    dzc, _ = g.interior_dz()
+    I = g.interior
+    m0 = (state[3][I] * dzc).sum()
+    dt = 0.5
+    sim.step_rk3(dt)
+    st = sim.state()
+    m1 = (st["theta"][I] * dzc).sum()
+    assert abs((m1 - m0) - 0.1 * dt * g.Nx * g.Ny) < 1e-9 * abs(m0)
+    assert sim.max_divergence() < 1e-12
+    w = sim.w.download()
+    assert np.all(w[:, :, g.H] == 0)
+    assert np.isfinite(st["p"][I]).all()
