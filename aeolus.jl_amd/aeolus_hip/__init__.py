@@ -10,13 +10,13 @@ from . import _lib
 from ._lib import AeolusError, init, sync
 from .advection import advection_scheme, advect_weno_z, momentum_advect_weno_z, required_halo
 from .fields import DeviceArray, Grid, faces_geometric, faces_uniform, fill_halos
-from .pressure import (ColumnTridiagonalSolver, FFTPoissonSolver, solve, solve_interior, solve_source_buffer,
-                       solver_for, solver_stats, source_buffer)
+from .pressure import (ColumnTridiagonalSolver, FFTPoissonSolver, MGPreconditionedCG, MultigridPoissonSolver, solve,
+                       solve_interior, solve_source_buffer, solver_for, solver_stats, source_buffer, stretched_operator)
 from .simulation import BoussinesqSimulation
 
 __all__ = [
     "AeolusError", "init", "sync", "advection_scheme", "advect_weno_z", "momentum_advect_weno_z", "required_halo",
     "DeviceArray", "Grid", "faces_geometric", "faces_uniform", "fill_halos", "ColumnTridiagonalSolver",
-    "FFTPoissonSolver", "solve", "solve_interior", "solve_source_buffer", "solver_for", "solver_stats",
+    "FFTPoissonSolver", "MGPreconditionedCG", "MultigridPoissonSolver", "stretched_operator", "solve", "solve_interior", "solve_source_buffer", "solver_for", "solver_stats",
     "source_buffer", "BoussinesqSimulation",
 ]

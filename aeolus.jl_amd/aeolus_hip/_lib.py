@@ -9,7 +9,8 @@ import os
 import threading
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libaeolus_hip.so")
+# AEOLUS_HIP_LIB selects another build of the library (the same ABI)
+LIB_PATH = os.environ.get("AEOLUS_HIP_LIB") or os.path.join(_HERE, "libaeolus_hip.so")
 
 F64, F32 = 0, 1
 CELL, FACE_X, FACE_Y, FACE_Z = 0, 1, 2, 3
@@ -54,6 +55,7 @@ SIGNATURES = {
     "aeo_memcpy_h2d": [_vp, _vp, _sz],
     "aeo_memcpy_d2h": [_vp, _vp, _sz],
     "aeo_memset": [_vp, _i, _sz],
+    "aeo_memcpy_d2d": [_vp, _vp, _sz],
     "aeo_grid_create": [_i, _i, _i, _i, _d, _d, _d, _pd, _pvp],
     "aeo_grid_destroy": [_vp],
     "aeo_grid_metrics": [_vp, _pd, _pd],
@@ -62,6 +64,11 @@ SIGNATURES = {
     "aeo_advect_momentum": [_vp, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp],
     "aeo_poisson_fft_create": [_i, _i, _i, _d, _d, _d, _pvp],
     "aeo_poisson_column_create": [_i, _i, _i, _d, _d, _i, _i, _pd, _pd, _pd, _pd, _pd, _pd, _pvp],
+    "aeo_poisson_mgpcg_create": [_i, _i, _i, _d, _d, _d, _i, _i, _i, _pd, _pd, _pd, _pd, _i, _i, _i, _d, _i, _d,
+                                 _i, _pvp],
+    "aeo_poisson_mg_create": [_i, _i, _i, _d, _d, _d, _i, _i, _i, _pd, _pd, _pd, _pd, _i, _i, _d, _i, _pvp],
+    "aeo_poisson_levels": [_vp, _pi],
+    "aeo_poisson_apply": [_vp, _vp, _vp],
     "aeo_poisson_source_buffer": [_vp, _pvp],
     "aeo_poisson_solve_source_buffer": [_vp, _vp],
     "aeo_poisson_solve": [_vp, _i, _vp, _vp, _i],

@@ -1,0 +1,125 @@
+"""Cut-cell geometry of flat-roofed buildings and its pressure operator (host setup).
+
+cut_cell_geometry_from_heightmap (src/CutCell.jl:372-396) through
+cut_cell_geometry's sub-cell sampling (:213-312) and _finalize_geometry's
+consistency sweeps and small-cell floor (:398-480), vectorised in numpy:
+the level set is φ(x, y, z) = z − h(column), φ > 0 fluid. The small-cell
+merge map and the surface normals are not built (the projection's operator
+needs the volumes and apertures only).
+
+The pressure operator of the projection is D·G (aperture power 2,
+cutcell_laplacian!, :735-781); in the multigrid's face-coefficient form the
+symmetric operator vol_eff·L has c = a² on every face (0 on blocked faces)
+and the source vol_eff·(D u) is the aperture-weighted face-flux balance.
+"""
+import numpy as np
+
+
+def _col(x, d, H, n):
+    """The storage column containing x (1-based, clamped): idx in src/CutCell.jl:385."""
+    return np.clip(np.floor(x / d).astype(np.int64) + H + 1, 1, n)
+
+
+def heightmap_geometry(N, spacing, H, height, nsub=4, vol_min=1e-3):
+    """(vol, vol_eff, ax, ay, az) on the storage (Fortran arrays), for interior
+    size N = (Nx, Ny, Nz), spacing (dx, dy, dz), halo H, origin 0 and `height`
+    of shape (Nx, Ny) (edge-replicated into the halo) or the storage (sx, sy)."""
+    Nx, Ny, Nz = N
+    dx, dy, dz = spacing
+    sx, sy, sz = Nx + 2 * H, Ny + 2 * H, Nz + 2 * H
+    height = np.asarray(height, dtype=np.float64)
+    if height.shape == (sx, sy):
+        hfull = height
+    elif height.shape == (Nx, Ny):
+        ii = np.clip(np.arange(1, sx + 1) - H, 1, Nx) - 1
+        jj = np.clip(np.arange(1, sy + 1) - H, 1, Ny) - 1
+        hfull = height[np.ix_(ii, jj)]
+    else:
+        raise ValueError(f"height must be {Nx}x{Ny} or {sx}x{sy}, got {height.shape}")
+    invn = 1.0 / nsub
+    hx, hy, hz = dx * invn, dy * invn, dz * invn
+    volnorm, areanorm = 1.0 / nsub ** 3, 1.0 / nsub ** 2
+    i = np.arange(1, sx + 1)
+    j = np.arange(1, sy + 1)
+    k = np.arange(1, sz + 1)
+    xb = (i - H - 1) * dx          # _xw
+    yb = (j - H - 1) * dy
+    zb = (k - H - 1) * dz
+    sub = np.arange(1, nsub + 1) - 0.5
+    xs = xb[:, None] + sub[None, :] * hx       # (sx, nsub)
+    ys = yb[:, None] + sub[None, :] * hy
+    zs = zb[:, None] + sub[None, :] * hz       # (sz, nsub)
+    cxs = _col(xs, dx, H, sx)                  # columns of the sub-samples
+    cys = _col(ys, dy, H, sy)
+    cxf = _col(xb, dx, H, sx)                  # columns at the faces
+    cyf = _col(yb, dy, H, sy)
+    vol = np.zeros((sx, sy, sz))
+    ax = np.zeros((sx, sy, sz))
+    ay = np.zeros((sx, sy, sz))
+    az = np.zeros((sx, sy, sz))
+    # counts of fluid sub-samples: a loop over the horizontal sub-samples, vectorised over cells
+    for a in range(nsub):
+        for b in range(nsub):
+            hab = hfull[np.ix_(cxs[:, a] - 1, cys[:, b] - 1)]            # (sx, sy)
+            for c in range(nsub):
+                vol += zs[None, None, :, c] - hab[:, :, None] > 0
+            az += zb[None, None, :] - hab[:, :, None] > 0
+        hxa = hfull[np.ix_(cxf - 1, cys[:, a] - 1)]                        # x-face columns
+        hya = hfull[np.ix_(cxs[:, a] - 1, cyf - 1)]                        # y-face columns
+        for c in range(nsub):
+            ax += zs[None, None, :, c] - hxa[:, :, None] > 0
+            ay += zs[None, None, :, c] - hya[:, :, None] > 0
+    vol *= volnorm
+    ax *= areanorm
+    ay *= areanorm
+    az *= areanorm
+    # _finalize_geometry
+    solid = ~(vol > 0)
+    ax[0, :, :] = 0.0
+    ay[:, 0, :] = 0.0
+    az[:, :, 0] = 0.0
+    for _ in range(16):
+        ax[1:, :, :][solid[:-1, :, :] | solid[1:, :, :]] = 0.0
+        ay[:, 1:, :][solid[:, :-1, :] | solid[:, 1:, :]] = 0.0
+        az[:, :, 1:][solid[:, :, :-1] | solid[:, :, 1:]] = 0.0
+        opn = (ax + ay) + az
+        opn[:-1, :, :] += ax[1:, :, :]
+        opn[:, :-1, :] += ay[:, 1:, :]
+        opn[:, :, :-1] += az[:, :, 1:]
+        newly = ~solid & ~(opn > 0)
+        if not newly.any():
+            break
+        solid |= newly
+        vol[newly] = 0.0
+    vol_eff = np.where(solid, 0.0, np.where(vol < vol_min, vol_min, vol))
+    F = lambda a: np.asfortranarray(a)
+    return F(vol), F(vol_eff), F(ax), F(ay), F(az)
+
+
+def mg_coefficients(ax, ay, az, N, H):
+    """Face coefficients a² of the D·G operator on the interior faces:
+    cx (Nx+1, Ny, Nz), cy (Nx, Ny+1, Nz), cz (Nx, Ny, Nz+1)."""
+    Nx, Ny, Nz = N
+    I = slice(H, H + Nx)
+    J = slice(H, H + Ny)
+    K = slice(H, H + Nz)
+    cx = ax[H:H + Nx + 1, J, K] ** 2
+    cy = ay[I, H:H + Ny + 1, K] ** 2
+    cz = az[I, J, H:H + Nz + 1] ** 2
+    return np.asfortranarray(cx), np.asfortranarray(cy), np.asfortranarray(cz)
+
+
+def cutcell_source(u, v, w, ax, ay, az, vol_eff, N, H, spacing):
+    """vol_eff · (D u): the aperture-weighted face-flux balance of the face
+    velocities (cutcell_divergence!, src/CutCell.jl:602-626, times the cell
+    volume fraction), zero in solid cells, over the interior."""
+    Nx, Ny, Nz = N
+    dx, dy, dz = spacing
+    I = slice(H, H + Nx); J = slice(H, H + Ny); K = slice(H, H + Nz)
+    Ip = slice(H + 1, H + Nx + 1); Jp = slice(H + 1, H + Ny + 1); Kp = slice(H + 1, H + Nz + 1)
+    fx = (ax[Ip, J, K] * u[Ip, J, K] - ax[I, J, K] * u[I, J, K]) / dx
+    fy = (ay[I, Jp, K] * v[I, Jp, K] - ay[I, J, K] * v[I, J, K]) / dy
+    fz = (az[I, J, Kp] * w[I, J, Kp] - az[I, J, K] * w[I, J, K]) / dz
+    b = (fx + fy) + fz
+    b[~(vol_eff[I, J, K] > 0)] = 0.0
+    return np.asfortranarray(b)

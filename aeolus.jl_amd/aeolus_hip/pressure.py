@@ -108,6 +108,103 @@ class ColumnTridiagonalSolver(PressureSolver):
         return cls(size, dxy, lower, upper, wc, np.full(Nz, dz ** 2), 1.0 / wf, np.ones(Nz), **kw)
 
 
+_BCK = {"periodic": L.PERIODIC, "neumann": L.NOFLUX, "noflux": L.NOFLUX}
+
+
+def _coeffs(coeff, size):
+    if coeff is None:
+        return None, None, None, ()
+    nx, ny, nz = size
+    cx, cy, cz = (np.asfortranarray(np.asarray(c, dtype=np.float64)) for c in coeff)
+    if cx.shape != (nx + 1, ny, nz) or cy.shape != (nx, ny + 1, nz) or cz.shape != (nx, ny, nz + 1):
+        raise ValueError("face coefficients must be (nx+1, ny, nz), (nx, ny+1, nz), (nx, ny, nz+1)")
+    return _pd(cx), _pd(cy), _pd(cz), (cx, cy, cz)
+
+
+class MGPreconditionedCG(PressureSolver):
+    """MGPreconditionedCG (src/Multigrid.jl:879-972): FP64 CG preconditioned by
+    one V-cycle, the mean projected (every axis periodic or Neumann).
+
+    size, spacing: interior cells and (dx, dy, dz) of the uniform operator;
+    coeff: (cx, cy, cz) face coefficients of ∇·(c∇φ) or None; rhs_scale:
+    per-level source factor (the stretched operator); smoother "rbgs" (the
+    reference's) or "jacobi" (damped, weight omega); vcycle "f32" or "f64".
+    """
+
+    kind = "mgpcg"
+
+    def __init__(self, size, spacing, bcs=("periodic", "periodic", "neumann"), coeff=None, rhs_scale=None, nu1=2,
+                 nu2=2, smoother="rbgs", omega=0.8, vcycle="f64", tol=1e-10, maxit=200):
+        nx, ny, nz = size
+        pcx, pcy, pcz, keep = _coeffs(coeff, size)
+        rs = None if rhs_scale is None else np.ascontiguousarray(np.asarray(rhs_scale, dtype=np.float64))
+        if rs is not None and len(rs) != nz:
+            raise ValueError(f"rhs_scale needs one entry per level ({nz})")
+        h = ctypes.c_void_p()
+        L.check(L.init().aeo_poisson_mgpcg_create(
+            nx, ny, nz, float(spacing[0]), float(spacing[1]), float(spacing[2]), *[_BCK[b] for b in bcs],
+            pcx, pcy, pcz, None if rs is None else _pd(rs), nu1, nu2, {"rbgs": 0, "jacobi": 1}[smoother],
+            float(omega), {"f64": L.F64, "f32": L.F32}[vcycle], float(tol), int(maxit), ctypes.byref(h)))
+        self._keep = (keep, rs)
+        self.axis_bcs = tuple(bcs)
+        super().__init__(h.value, (nx, ny, nz))
+
+    @classmethod
+    def stretched(cls, grid, **kw):
+        """MGPreconditionedCG(g::StretchedGrid) (src/Multigrid.jl:949-953) through
+        the row-scaled uniform operator of _stretched_operator (:641-658)."""
+        size, spacing, coeff, sc = stretched_operator(grid)
+        return cls(size, spacing, coeff=coeff, rhs_scale=sc, **kw)
+
+    def levels(self):
+        n = ctypes.c_int()
+        L.check(L.init().aeo_poisson_levels(self.handle, ctypes.byref(n)))
+        return n.value
+
+    def apply(self, out, x):
+        """out = A x (the fine operator) on FP64 interior-sized DeviceArrays."""
+        L.check(L.init().aeo_poisson_apply(self.handle, L.dptr(out), L.dptr(x)))
+        return out
+
+
+class MultigridPoissonSolver(MGPreconditionedCG):
+    """MultigridPoissonSolver (src/Multigrid.jl:531-861): V-cycles to tol (FP64)."""
+
+    kind = "mg"
+
+    def __init__(self, size, spacing, bcs=("periodic", "periodic", "neumann"), coeff=None, rhs_scale=None, nu1=2,
+                 nu2=2, tol=1e-10, maxcycles=50):
+        nx, ny, nz = size
+        pcx, pcy, pcz, keep = _coeffs(coeff, size)
+        rs = None if rhs_scale is None else np.ascontiguousarray(np.asarray(rhs_scale, dtype=np.float64))
+        h = ctypes.c_void_p()
+        L.check(L.init().aeo_poisson_mg_create(
+            nx, ny, nz, float(spacing[0]), float(spacing[1]), float(spacing[2]), *[_BCK[b] for b in bcs],
+            pcx, pcy, pcz, None if rs is None else _pd(rs), nu1, nu2, float(tol), int(maxcycles), ctypes.byref(h)))
+        self._keep = (keep, rs)
+        self.axis_bcs = tuple(bcs)
+        PressureSolver.__init__(self, h.value, (nx, ny, nz))
+
+
+def stretched_operator(grid):
+    """_stretched_operator (src/Multigrid.jl:641-658): the equivalent uniform grid
+    of spacing Δ = min Δz_c, face coefficients s_k (x, y) and Δ/Δz_f (z), and
+    the row scale s_k = Δz_c[k]/Δ."""
+    dzc, dzf = grid.interior_dz()
+    D = dzc.min()
+    nx, ny, nz = grid.Nx, grid.Ny, grid.Nz
+    sc = dzc / D
+    cx = np.zeros((nx + 1, ny, nz), order="F")
+    cy = np.zeros((nx, ny + 1, nz), order="F")
+    cz = np.zeros((nx, ny, nz + 1), order="F")
+    for k in range(nz):
+        cx[:, :, k] = sc[k]
+        cy[:, :, k] = sc[k]
+    for k in range(nz + 1):
+        cz[:, :, k] = D / dzf[k]
+    return (nx, ny, nz), (grid.dx, grid.dy, (nz * D) / nz), (cx, cy, cz), sc
+
+
 def solver_for(grid):
     """The default solver of a grid (_default_solver, src/Simulations.jl:1137-1144, and
     StretchedPoissonSolver, src/stretched/poisson.jl:78-103) for periodic x-y."""
@@ -134,10 +231,13 @@ def solve(p, source, solver, H):
 
 
 def solver_stats(solver):
+    """(iterations, residual): (0, NaN) for the direct solvers, (CG iterations,
+    ‖r‖/‖b‖) for MGPreconditionedCG, (cycles, relative residual) for multigrid."""
     it = ctypes.c_int()
     r = ctypes.c_double()
     st = ctypes.c_int()
     L.check(L.init().aeo_poisson_stats(solver.handle, ctypes.byref(it), ctypes.byref(r), ctypes.byref(st)))
+    solver.last_status = L.STATUS_NAMES.get(st.value, st.value)
     return it.value, r.value
 
 

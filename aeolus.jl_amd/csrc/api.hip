@@ -129,6 +129,9 @@ extern "C" int aeo_grid_create(int Nx, int Ny, int Nz, int H, double dx, double 
     if (H < 1) return fail(AEO_ERR_ARG, "halo must be >= 1, got %d", H);
     if (Nx < H || Ny < H || Nz <= H) return fail(AEO_ERR_ARG, "interior (%d,%d,%d) must exceed the halo %d", Nx, Ny, Nz, H);
     if (!(dx > 0) || !(dy > 0)) return fail(AEO_ERR_ARG, "dx, dy must be positive");
+    // the kernels address storage arrays with 32-bit offsets
+    if ((double)(Nx + 2 * H) * (Ny + 2 * H) * (Nz + 2 * H) * 8.0 >= 2147483648.0)
+        return fail(AEO_ERR_UNSUPPORTED, "storage (%d+2H)(%d+2H)(%d+2H) exceeds 2 GiB per FP64 array", Nx, Ny, Nz);
     auto g = std::make_unique<aeo_grid>();
     g->Nx = Nx; g->Ny = Ny; g->Nz = Nz; g->H = H; g->dx = dx; g->dy = dy; g->dz = dz;
     const int n = Nz + 2 * H;
@@ -266,6 +269,23 @@ extern "C" int aeo_memcpy_d2h(void *dst, const void *src, size_t bytes) {
     HIPCHK(hipStreamSynchronize(g_ctx->stream));
     return AEO_OK;
 }
+__global__ void copy_kernel(uint4 *__restrict__ dst, const uint4 *__restrict__ src, size_t n) {
+    for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x)
+        dst[t] = src[t];
+}
+
+extern "C" int aeo_memcpy_d2d(void *dst, const void *src, size_t bytes) {
+    NEED_CTX();
+    if (!dst || !src) return fail(AEO_ERR_ARG, "null argument");
+    if (bytes % 16 || ((uintptr_t)dst | (uintptr_t)src) % 16)
+        return fail(AEO_ERR_ARG, "aeo_memcpy_d2d: 16-byte aligned pointers and sizes");
+    Timed t("copy");
+    size_t n = bytes / 16;
+    hipLaunchKernelGGL(copy_kernel, dim3(4096), dim3(256), 0, g_ctx->stream, (uint4 *)dst, (const uint4 *)src, n);
+    HIPCHK(hipGetLastError());
+    return AEO_OK;
+}
+
 extern "C" int aeo_memset(void *dptr, int value, size_t bytes) {
     NEED_CTX();
     HIPCHK(hipMemsetAsync(dptr, value, bytes, g_ctx->stream));
@@ -355,10 +375,13 @@ extern "C" int aeo_advect_momentum(aeo_grid *g, int precision, int scheme, int p
 // pressure solvers
 // ---------------------------------------------------------------------------
 struct aeo_poisson {
-    int kind = 0;  // 0 fft, 1 column
+    int kind = 0;  // 0 fft, 1 column, 2 MG-preconditioned CG, 3 multigrid cycles
     int Nx, Ny, Nz;
     bool periodic_x, periodic_y, periodic_z;
     SpectralPlan *sp = nullptr;
+    MGPlan *mg = nullptr;
+    double tol = 0;
+    int maxit = 0;
     double *src = nullptr, *phi = nullptr;  // interior-sized FP64
     int iters = 0, status = AEO_DIRECT;
     double resid = NAN;
@@ -428,6 +451,27 @@ extern "C" int aeo_poisson_source_buffer(aeo_poisson *h, double **dptr) {
 
 static int poisson_run(aeo_poisson *h, double *phi) {
     Timed t("poisson");
+    if (h->kind == 2) {
+        int it = 0;
+        double res = 0, bn = 0;
+        int stt = mg_pcg_solve(h->mg, g_ctx->stream, phi, h->src, h->tol, h->maxit, &it, &res, &bn);
+        HIPCHK(hipGetLastError());
+        h->iters = it;
+        // solver_stats of MGPreconditionedCG: ‖r‖/‖b‖ (src/Multigrid.jl:965-966)
+        h->resid = stt == AEO_ZERO_RHS ? 0.0 : res / bn;
+        h->status = stt;
+        return AEO_OK;
+    }
+    if (h->kind == 3) {
+        int cyc = 0;
+        double rel = 0;
+        int stt = mg_cycles_solve(h->mg, g_ctx->stream, phi, h->src, h->tol, h->maxit, &cyc, &rel);
+        HIPCHK(hipGetLastError());
+        h->iters = cyc;
+        h->resid = rel;
+        h->status = stt;
+        return AEO_OK;
+    }
     int rc = h->kind == 0 ? spectral_fft_solve(h->sp, g_ctx->stream, phi, h->src)
                           : spectral_column_solve(h->sp, g_ctx->stream, phi, h->src);
     if (rc) return fail(AEO_ERR_STATE, "solver not set up");
@@ -435,6 +479,69 @@ static int poisson_run(aeo_poisson *h, double *phi) {
     h->iters = 0;
     h->resid = NAN;
     h->status = AEO_DIRECT;
+    return AEO_OK;
+}
+
+static int mg_common_create(int kind, int nx, int ny, int nz, double dx, double dy, double dz, int bc_x, int bc_y,
+                            int bc_z, const double *cx, const double *cy, const double *cz, const double *rhs_scale,
+                            int nu1, int nu2, int smoother, double omega, int vprec, double tol, int maxit,
+                            aeo_poisson **out) {
+    NEED_CTX();
+    if (!out) return fail(AEO_ERR_ARG, "null output handle");
+    if (nx < 1 || ny < 1 || nz < 1) return fail(AEO_ERR_ARG, "size must be positive");
+    for (int bc : {bc_x, bc_y, bc_z})
+        if (bc != AEO_PERIODIC && bc != AEO_NOFLUX)
+            return fail(AEO_ERR_ARG, "Multigrid: unsupported BC (use periodic or Neumann)");
+    if ((cx == nullptr) != (cy == nullptr) || (cx == nullptr) != (cz == nullptr))
+        return fail(AEO_ERR_ARG, "coefficients: give all of cx, cy, cz or none");
+    if (smoother != 0 && smoother != 1) return fail(AEO_ERR_ARG, "smoother: 0 red-black Gauss-Seidel, 1 Jacobi");
+    if (vprec != AEO_F64 && vprec != AEO_F32) return fail(AEO_ERR_ARG, "bad V-cycle precision");
+    if (kind == 3 && vprec != AEO_F64) return fail(AEO_ERR_UNSUPPORTED, "plain multigrid cycles run in FP64");
+    if (nu1 < 0 || nu2 < 0 || maxit < 0 || !(tol >= 0)) return fail(AEO_ERR_ARG, "bad sweep count, tolerance or cap");
+    if ((double)nx * ny * nz * 8.0 >= 2147483648.0) return fail(AEO_ERR_UNSUPPORTED, "grid exceeds 2 GiB per FP64 array");
+    auto h = std::make_unique<aeo_poisson>();
+    h->kind = kind; h->Nx = nx; h->Ny = ny; h->Nz = nz;
+    h->periodic_x = bc_x == AEO_PERIODIC; h->periodic_y = bc_y == AEO_PERIODIC; h->periodic_z = bc_z == AEO_PERIODIC;
+    h->tol = tol; h->maxit = maxit;
+    char err[512] = "";
+    HIPCHK(hipStreamSynchronize(g_ctx->stream));
+    h->mg = mg_create(nx, ny, nz, dx, dy, dz, bc_x, bc_y, bc_z, cx, cy, cz, rhs_scale, nu1, nu2, smoother, omega,
+                      vprec == AEO_F64 ? 0 : 1, 4, err, sizeof err);
+    if (!h->mg) return fail(AEO_ERR_HIP, "%s", err);
+    int rc = poisson_alloc(h.get());
+    if (rc) return rc;
+    *out = h.release();
+    return AEO_OK;
+}
+
+extern "C" int aeo_poisson_mgpcg_create(int nx, int ny, int nz, double dx, double dy, double dz, int bc_x, int bc_y,
+                                        int bc_z, const double *cx, const double *cy, const double *cz,
+                                        const double *rhs_scale, int nu1, int nu2, int smoother, double omega,
+                                        int vcycle_precision, double tol, int maxit, aeo_poisson **out) {
+    return mg_common_create(2, nx, ny, nz, dx, dy, dz, bc_x, bc_y, bc_z, cx, cy, cz, rhs_scale, nu1, nu2, smoother,
+                            omega, vcycle_precision, tol, maxit, out);
+}
+
+extern "C" int aeo_poisson_mg_create(int nx, int ny, int nz, double dx, double dy, double dz, int bc_x, int bc_y,
+                                     int bc_z, const double *cx, const double *cy, const double *cz,
+                                     const double *rhs_scale, int nu1, int nu2, double tol, int maxcycles,
+                                     aeo_poisson **out) {
+    return mg_common_create(3, nx, ny, nz, dx, dy, dz, bc_x, bc_y, bc_z, cx, cy, cz, rhs_scale, nu1, nu2, 0, 1.0,
+                            AEO_F64, tol, maxcycles, out);
+}
+
+extern "C" int aeo_poisson_levels(aeo_poisson *h, int *levels) {
+    if (!h || !levels) return fail(AEO_ERR_ARG, "null argument");
+    *levels = h->mg ? mg_levels(h->mg) : 1;
+    return AEO_OK;
+}
+
+extern "C" int aeo_poisson_apply(aeo_poisson *h, double *out, const double *x) {
+    NEED_CTX();
+    if (!h || !out || !x) return fail(AEO_ERR_ARG, "null argument");
+    if (!h->mg) return fail(AEO_ERR_UNSUPPORTED, "aeo_poisson_apply: multigrid solvers only");
+    mg_apply(h->mg, g_ctx->stream, out, x);
+    HIPCHK(hipGetLastError());
     return AEO_OK;
 }
 
@@ -484,6 +591,7 @@ extern "C" int aeo_poisson_destroy(aeo_poisson *h) {
     if (!h) return AEO_OK;
     if (g_ctx) hipStreamSynchronize(g_ctx->stream);
     spectral_destroy(h->sp);
+    mg_destroy(h->mg);
     hipFree(h->src);
     hipFree(h->phi);
     delete h;

@@ -53,14 +53,33 @@ struct Metrics {
     }
 };
 
-__device__ __forceinline__ size_t lin(int i, int j, int k, int sx, int sy) {
-    return (size_t)i + (size_t)sx * ((size_t)j + (size_t)sy * (size_t)k);
+// 32-bit element offsets: every storage array of the library holds fewer
+// than 2^31 bytes (checked when a grid is created), so loads take the form
+// uniform base pointer + 32-bit per-lane offset.
+__device__ __forceinline__ uint32_t lin(int i, int j, int k, int sx, int sy) {
+    return (uint32_t)i + (uint32_t)sx * ((uint32_t)j + (uint32_t)sy * (uint32_t)k);
 }
 
 // ---------------------------------------------------------------------------
 // Reconstructions, in the reference's evaluation order (no FMA contraction:
 // the library is built with -ffp-contract=off).
 // ---------------------------------------------------------------------------
+// RN(x/3) without a division: q = RN(x·r), r = RN(1/3); the remainder
+// x − 3q is exact in one FMA and RN(q + (x − 3q)·r) is RN(x/3). x/3 is never
+// a rounding midpoint and lies at least ulp/6 from one, while the corrected
+// sum is within ~2^-54 ulp of x/3, so the final rounding is the correct one
+// (finite x above the subnormal range). x/6, x/12, x/24 are this times an
+// exact power of two.
+template <typename T> __device__ __forceinline__ T div3(T x) {
+    const T r = T(1.0 / 3.0);
+    T q = x * r;
+    T e = fma(T(-3), q, x);
+    return fma(e, r, q);
+}
+template <typename T> __device__ __forceinline__ T div6(T x) { return div3<T>(x) * T(0.5); }
+template <typename T> __device__ __forceinline__ T div12(T x) { return div3<T>(x) * T(0.25); }
+template <typename T> __device__ __forceinline__ T div24(T x) { return div3<T>(x) * T(0.125); }
+
 template <typename T> __device__ __forceinline__ T beta3(T a1, T a2) { return a1 * a1 + T(13.0 / 3.0) * (a2 * a2); }
 template <typename T> __device__ __forceinline__ T beta4(T a1, T a2, T a3) {
     return ((a1 * a1 + T(1.0 / 2.0) * (a1 * a3)) + T(13.0 / 3.0) * (a2 * a2)) + T(3129.0 / 80.0) * (a3 * a3);
@@ -75,21 +94,27 @@ template <typename T> __device__ __forceinline__ T b3_p0(T v1, T v2, T v3) {
     return beta3<T>(((T(-3) * v1 + T(4) * v2) - v3) / T(2), ((v1 - T(2) * v2) + v3) / T(2));
 }
 template <typename T> __device__ __forceinline__ T b4_m3(T v1, T v2, T v3, T v4) {
-    return beta4<T>((((T(-7) * v1 + T(33) * v2) - T(69) * v3) + T(43) * v4) / T(24),
-                    (((-v1 + T(4) * v2) - T(5) * v3) + T(2) * v4) / T(2), (((-v1 + T(3) * v2) - T(3) * v3) + v4) / T(6));
+    return beta4<T>(div24<T>(((T(-7) * v1 + T(33) * v2) - T(69) * v3) + T(43) * v4),
+                    (((-v1 + T(4) * v2) - T(5) * v3) + T(2) * v4) / T(2), div6<T>(((-v1 + T(3) * v2) - T(3) * v3) + v4));
 }
 template <typename T> __device__ __forceinline__ T b4_p0(T v1, T v2, T v3, T v4) {
-    return beta4<T>((((T(-43) * v1 + T(69) * v2) - T(33) * v3) + T(7) * v4) / T(24),
-                    (((T(2) * v1 - T(5) * v2) + T(4) * v3) - v4) / T(2), (((-v1 + T(3) * v2) - T(3) * v3) + v4) / T(6));
+    return beta4<T>(div24<T>(((T(-43) * v1 + T(69) * v2) - T(33) * v3) + T(7) * v4),
+                    (((T(2) * v1 - T(5) * v2) + T(4) * v3) - v4) / T(2), div6<T>(((-v1 + T(3) * v2) - T(3) * v3) + v4));
 }
 template <typename T> __device__ __forceinline__ T b4_m2(T v1, T v2, T v3, T v4) {
-    return beta4<T>((((T(5) * v1 - T(27) * v2) + T(15) * v3) + T(7) * v4) / T(24), ((v2 - T(2) * v3) + v4) / T(2),
-                    (((-v1 + T(3) * v2) - T(3) * v3) + v4) / T(6));
+    return beta4<T>(div24<T>(((T(5) * v1 - T(27) * v2) + T(15) * v3) + T(7) * v4), ((v2 - T(2) * v3) + v4) / T(2),
+                    div6<T>(((-v1 + T(3) * v2) - T(3) * v3) + v4));
 }
 template <typename T> __device__ __forceinline__ T b4_m1(T v1, T v2, T v3, T v4) {
-    return beta4<T>((((T(-7) * v1 - T(15) * v2) + T(27) * v3) - T(5) * v4) / T(24), ((v1 - T(2) * v2) + v3) / T(2),
-                    (((-v1 + T(3) * v2) - T(3) * v3) + v4) / T(6));
+    return beta4<T>(div24<T>(((T(-7) * v1 - T(15) * v2) + T(27) * v3) - T(5) * v4), ((v1 - T(2) * v2) + v3) / T(2),
+                    div6<T>(((-v1 + T(3) * v2) - T(3) * v3) + v4));
 }
+// General divisions of the reconstructions: correctly rounded in FP64 (the
+// reference's arithmetic); the reciprocal-multiply form in FP32, whose
+// results are stated against the FP64 oracle rather than bit-matched.
+template <typename T> __device__ __forceinline__ T qdiv(T a, T b) { return a / b; }
+template <> __device__ __forceinline__ float qdiv<float>(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+
 template <int P, typename T> __device__ __forceinline__ T ipow(T x) {
     if constexpr (P == 2) return x * x;
     else return x;
@@ -98,45 +123,45 @@ template <int P, typename T> __device__ __forceinline__ T ipow(T x) {
 // _weno_z5_left, src/AdvectionSchemes.jl:240-255
 template <int P, typename T>
 __device__ __forceinline__ T weno_z5(T v1, T v2, T v3, T v4, T v5) {
-    T q0 = ((T(2) * v1 - T(7) * v2) + T(11) * v3) / T(6);
-    T q1 = ((-v2 + T(5) * v3) + T(2) * v4) / T(6);
-    T q2 = ((T(2) * v3 + T(5) * v4) - v5) / T(6);
+    T q0 = div6<T>((T(2) * v1 - T(7) * v2) + T(11) * v3);
+    T q1 = div6<T>((-v2 + T(5) * v3) + T(2) * v4);
+    T q2 = div6<T>((T(2) * v3 + T(5) * v4) - v5);
     T b0 = b3_m2<T>(v1, v2, v3), b1 = b3_m1<T>(v2, v3, v4), b2 = b3_p0<T>(v3, v4, v5);
     const T eps = Eps<T>::v * Eps<T>::v;
     T tau = fabs(b0 - b2);
-    T a0 = T(1.0 / 10.0) * (T(1) + ipow<P>(tau / (b0 + eps)));
-    T a1 = T(6.0 / 10.0) * (T(1) + ipow<P>(tau / (b1 + eps)));
-    T a2 = T(3.0 / 10.0) * (T(1) + ipow<P>(tau / (b2 + eps)));
-    return ((a0 * q0 + a1 * q1) + a2 * q2) / ((a0 + a1) + a2);
+    T a0 = T(1.0 / 10.0) * (T(1) + ipow<P>(qdiv<T>(tau, b0 + eps)));
+    T a1 = T(6.0 / 10.0) * (T(1) + ipow<P>(qdiv<T>(tau, b1 + eps)));
+    T a2 = T(3.0 / 10.0) * (T(1) + ipow<P>(qdiv<T>(tau, b2 + eps)));
+    return qdiv<T>((a0 * q0 + a1 * q1) + a2 * q2, (a0 + a1) + a2);
 }
 // _weno_z7_left, src/AdvectionSchemes.jl:284-303
 template <int P, typename T>
 __device__ __forceinline__ T weno_z7(T v1, T v2, T v3, T v4, T v5, T v6, T v7) {
-    T q0 = (((T(-3) * v1 + T(13) * v2) - T(23) * v3) + T(25) * v4) / T(12);
-    T q1 = (((v2 - T(5) * v3) + T(13) * v4) + T(3) * v5) / T(12);
-    T q2 = (((-v3 + T(7) * v4) + T(7) * v5) - v6) / T(12);
-    T q3 = (((T(3) * v4 + T(13) * v5) - T(5) * v6) + v7) / T(12);
+    T q0 = div12<T>(((T(-3) * v1 + T(13) * v2) - T(23) * v3) + T(25) * v4);
+    T q1 = div12<T>(((v2 - T(5) * v3) + T(13) * v4) + T(3) * v5);
+    T q2 = div12<T>(((-v3 + T(7) * v4) + T(7) * v5) - v6);
+    T q3 = div12<T>(((T(3) * v4 + T(13) * v5) - T(5) * v6) + v7);
     T b0 = b4_m3<T>(v1, v2, v3, v4), b1 = b4_m2<T>(v2, v3, v4, v5);
     T b2 = b4_m1<T>(v3, v4, v5, v6), b3 = b4_p0<T>(v4, v5, v6, v7);
     const T eps = Eps<T>::v * Eps<T>::v;
     T tau = fabs(((b0 - b1) - b2) + b3);
-    T a0 = T(1.0 / 35.0) * (T(1) + ipow<P>(tau / (b0 + eps)));
-    T a1 = T(12.0 / 35.0) * (T(1) + ipow<P>(tau / (b1 + eps)));
-    T a2 = T(18.0 / 35.0) * (T(1) + ipow<P>(tau / (b2 + eps)));
-    T a3 = T(4.0 / 35.0) * (T(1) + ipow<P>(tau / (b3 + eps)));
-    return (((a0 * q0 + a1 * q1) + a2 * q2) + a3 * q3) / (((a0 + a1) + a2) + a3);
+    T a0 = T(1.0 / 35.0) * (T(1) + ipow<P>(qdiv<T>(tau, b0 + eps)));
+    T a1 = T(12.0 / 35.0) * (T(1) + ipow<P>(qdiv<T>(tau, b1 + eps)));
+    T a2 = T(18.0 / 35.0) * (T(1) + ipow<P>(qdiv<T>(tau, b2 + eps)));
+    T a3 = T(4.0 / 35.0) * (T(1) + ipow<P>(qdiv<T>(tau, b3 + eps)));
+    return qdiv<T>(((a0 * q0 + a1 * q1) + a2 * q2) + a3 * q3, ((a0 + a1) + a2) + a3);
 }
 // weno_js5_left, src/AdvectionSchemes.jl:319-342 (ε = 1e-6)
 template <typename T>
 __device__ __forceinline__ T weno_js5(T v1, T v2, T v3, T v4, T v5) {
-    T q0 = ((T(2) * v1 - T(7) * v2) + T(11) * v3) / T(6);
-    T q1 = ((-v2 + T(5) * v3) + T(2) * v4) / T(6);
-    T q2 = ((T(2) * v3 + T(5) * v4) - v5) / T(6);
+    T q0 = div6<T>((T(2) * v1 - T(7) * v2) + T(11) * v3);
+    T q1 = div6<T>((-v2 + T(5) * v3) + T(2) * v4);
+    T q2 = div6<T>((T(2) * v3 + T(5) * v4) - v5);
     T b0 = b3_m2<T>(v1, v2, v3), b1 = b3_m1<T>(v2, v3, v4), b2 = b3_p0<T>(v3, v4, v5);
     const T e = T(1e-6);
     T d0 = b0 + e, d1 = b1 + e, d2 = b2 + e;
-    T a0 = T(1.0 / 10.0) / (d0 * d0), a1 = T(6.0 / 10.0) / (d1 * d1), a2 = T(3.0 / 10.0) / (d2 * d2);
-    return ((a0 * q0 + a1 * q1) + a2 * q2) / ((a0 + a1) + a2);
+    T a0 = qdiv<T>(T(1.0 / 10.0), d0 * d0), a1 = qdiv<T>(T(6.0 / 10.0), d1 * d1), a2 = qdiv<T>(T(3.0 / 10.0), d2 * d2);
+    return qdiv<T>((a0 * q0 + a1 * q1) + a2 * q2, (a0 + a1) + a2);
 }
 // FiniteVolume._weno5_left, src/FiniteVolume.jl:1582-1596
 template <typename T>
@@ -152,9 +177,9 @@ __device__ __forceinline__ T weno5_classic(T fm2, T fm1, T fc, T fp1, T fp2) {
     T b2 = T(13.0 / 12.0) * (t2 * t2) + T(1.0 / 4.0) * (s2 * s2);
     const T e = T(1e-6);
     T d0 = b0 + e, d1 = b1 + e, d2 = b2 + e;
-    T a0 = T(1.0 / 10.0) / (d0 * d0), a1 = T(6.0 / 10.0) / (d1 * d1), a2 = T(3.0 / 10.0) / (d2 * d2);
+    T a0 = qdiv<T>(T(1.0 / 10.0), d0 * d0), a1 = qdiv<T>(T(6.0 / 10.0), d1 * d1), a2 = qdiv<T>(T(3.0 / 10.0), d2 * d2);
     T s = (a0 + a1) + a2;
-    return ((a0 * q0 + a1 * q1) + a2 * q2) / s;
+    return qdiv<T>((a0 * q0 + a1 * q1) + a2 * q2, s);
 }
 
 // Stencil half-width of a scheme: the face between cells c and c+1 reads
@@ -169,20 +194,29 @@ template <> struct SchemeInfo<SCH_WENO_Z7> { static constexpr int R = 4; };
 // vel >= 0, else the mirrored right-biased one, into one left
 // reconstruction (_upwind_value, src/AdvectionSchemes.jl:659-666).
 template <int S, int P, typename T>
-__device__ __forceinline__ T face_value(const T *__restrict__ d, long base, long stride, int c, T vel) {
+__device__ __forceinline__ T face_value(const T *__restrict__ d, uint32_t base, uint32_t stride, int c, T vel) {
     if constexpr (S == SCH_CENTERED) {
-        return (d[base + (long)c * stride] + d[base + (long)(c + 1) * stride]) / T(2);
+        return (d[base + (uint32_t)c * stride] + d[base + (uint32_t)(c + 1) * stride]) / T(2);
     } else if constexpr (S == SCH_UPWIND1) {
-        return vel >= T(0) ? d[base + (long)c * stride] : d[base + (long)(c + 1) * stride];
+        return vel >= T(0) ? d[base + (uint32_t)c * stride] : d[base + (uint32_t)(c + 1) * stride];
     } else {
         constexpr int R = SchemeInfo<S>::R;
-        T a[2 * R];
-#pragma unroll
-        for (int m = 0; m < 2 * R; m++) a[m] = d[base + (long)(c - R + 1 + m) * stride];
+        const uint32_t o0 = base + (uint32_t)(c - R + 1) * stride;
         const bool up = vel >= T(0);
+        // the left-biased stencil (vel >= 0) or the mirrored right-biased one,
+        // selected pair by pair (t[m] = up ? a[m] : a[2R-1-m])
         T t[2 * R - 1];
-#pragma unroll
-        for (int m = 0; m < 2 * R - 1; m++) t[m] = up ? a[m] : a[2 * R - 1 - m];
+        if constexpr (R == 3) {
+            const T a0 = d[o0], a1 = d[o0 + stride], a2 = d[o0 + 2 * stride], a3 = d[o0 + 3 * stride],
+                    a4 = d[o0 + 4 * stride], a5 = d[o0 + 5 * stride];
+            t[0] = up ? a0 : a5; t[1] = up ? a1 : a4; t[2] = up ? a2 : a3; t[3] = up ? a3 : a2; t[4] = up ? a4 : a1;
+        } else {
+            const T a0 = d[o0], a1 = d[o0 + stride], a2 = d[o0 + 2 * stride], a3 = d[o0 + 3 * stride],
+                    a4 = d[o0 + 4 * stride], a5 = d[o0 + 5 * stride], a6 = d[o0 + 6 * stride],
+                    a7 = d[o0 + 7 * stride];
+            t[0] = up ? a0 : a7; t[1] = up ? a1 : a6; t[2] = up ? a2 : a5; t[3] = up ? a3 : a4;
+            t[4] = up ? a4 : a3; t[5] = up ? a5 : a2; t[6] = up ? a6 : a1;
+        }
         if constexpr (S == SCH_WENO5) return weno5_classic<T>(t[0], t[1], t[2], t[3], t[4]);
         else if constexpr (S == SCH_WENO_JS5) return weno_js5<T>(t[0], t[1], t[2], t[3], t[4]);
         else if constexpr (S == SCH_WENO_Z5) return weno_z5<P, T>(t[0], t[1], t[2], t[3], t[4]);

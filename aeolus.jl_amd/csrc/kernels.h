@@ -53,4 +53,17 @@ int spectral_column_setup(SpectralPlan *p, double dx, double dy, const double *l
                           const double *gauge);
 int spectral_column_solve(SpectralPlan *p, hipStream_t st, double *phi, const double *src);
 
+// ---- multigrid and MG-preconditioned CG (mg.hip) ----
+struct MGPlan;
+MGPlan *mg_create(int nx, int ny, int nz, double dx, double dy, double dz, int bcx, int bcy, int bcz, const double *cx,
+                  const double *cy, const double *cz, const double *rhs_scale, int nu1, int nu2, int smoother,
+                  double omega, int prec, int min_size, char *err, size_t errlen);
+void mg_destroy(MGPlan *P);
+int mg_levels(const MGPlan *P);
+int mg_pcg_solve(MGPlan *P, hipStream_t st, double *xout, const double *bin, double tol, int maxit, int *iters_out,
+                 double *resid_out, double *bnorm_out);
+int mg_cycles_solve(MGPlan *P, hipStream_t st, double *xout, const double *bin, double tol, int maxcycles,
+                    int *cycles_out, double *rel_out);
+void mg_apply(MGPlan *P, hipStream_t st, double *out, const double *x);
+
 }  // namespace aeo

@@ -1,0 +1,838 @@
+// Geometric multigrid and the MG-preconditioned CG (gfx950).
+//
+// MultigridPoissonSolver / MGPreconditionedCG (src/Multigrid.jl:531-972) with
+// the CG loop of pcg! (src/Backends.jl:1526-1589): cell-centred V-cycle,
+// red-black Gauss-Seidel (the reference's smoother) or damped Jacobi,
+// 8-cell mean restriction with the residual mean subtracted inside,
+// trilinear (3/4, 1/4) prolongation, 4-face mean coefficient coarsening, 50
+// red-black sweeps on the coarsest level, and the null-space (mean)
+// projection after each smoothing, residual and prolongation.
+//
+// Device design: level arrays are interior-sized; the unit halo of the
+// reference (periodic wrap or mirror) is an index map in the kernels, so no
+// halo launches. Every mean projection stays on the device (block partial
+// sums, one finalize launch writing the mean, the shift reads it), so a
+// V-cycle runs without a host synchronisation. The coarsest level runs its
+// 50 sweeps in one workgroup with the level in LDS. The V-cycle runs in T =
+// float or double; the outer CG is FP64 (its residual and inner products).
+// The reductions are deterministic (a fixed tree per launch geometry); their
+// order differs from the reference's 8-lane plane order, so iterates agree
+// with the CPU restatement to round-off, not bit for bit.
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "kernels.h"
+
+namespace aeo {
+
+constexpr int RB = 256;  // threads of the reduction / elementwise launches
+
+__device__ __forceinline__ int wrapc(int i, int n, int bc) {
+    if (i < 0) return bc == BC_PERIODIC ? i + n : 0;
+    if (i >= n) return bc == BC_PERIODIC ? i - n : n - 1;
+    return i;
+}
+
+template <typename T>
+struct LevelD {
+    int nx, ny, nz;
+    T ix2, iy2, iz2;
+    T *phi, *b, *r, *tmp;
+    const T *cx, *cy, *cz;  // face coefficients or null (the constant operator)
+    int bcx, bcy, bcz;
+};
+
+template <typename T>
+__device__ __forceinline__ uint32_t lidx(const LevelD<T> &L, int i, int j, int k) {
+    return (uint32_t)i + (uint32_t)L.nx * ((uint32_t)j + (uint32_t)L.ny * (uint32_t)k);
+}
+
+// (Aφ) at interior (i, j, k): _Aφ, src/Multigrid.jl:171-192
+template <typename T>
+__device__ __forceinline__ T applyA(const LevelD<T> &L, const T *__restrict__ f, int i, int j, int k) {
+    const T c = f[lidx(L, i, j, k)];
+    const T xp = f[lidx(L, wrapc(i + 1, L.nx, L.bcx), j, k)], xm = f[lidx(L, wrapc(i - 1, L.nx, L.bcx), j, k)];
+    const T yp = f[lidx(L, i, wrapc(j + 1, L.ny, L.bcy), k)], ym = f[lidx(L, i, wrapc(j - 1, L.ny, L.bcy), k)];
+    const T zp = f[lidx(L, i, j, wrapc(k + 1, L.nz, L.bcz))], zm = f[lidx(L, i, j, wrapc(k - 1, L.nz, L.bcz))];
+    if (!L.cx) {
+        return (((xp - T(2) * c) + xm) * L.ix2 + ((yp - T(2) * c) + ym) * L.iy2) + ((zp - T(2) * c) + zm) * L.iz2;
+    }
+    const uint32_t nx1 = L.nx + 1, ny1 = L.ny + 1;
+    const T fxp = L.cx[(uint32_t)(i + 1) + nx1 * ((uint32_t)j + (uint32_t)L.ny * k)];
+    const T fxm = L.cx[(uint32_t)i + nx1 * ((uint32_t)j + (uint32_t)L.ny * k)];
+    const T fyp = L.cy[(uint32_t)i + (uint32_t)L.nx * ((uint32_t)(j + 1) + ny1 * k)];
+    const T fym = L.cy[(uint32_t)i + (uint32_t)L.nx * ((uint32_t)j + ny1 * k)];
+    const T fzp = L.cz[lidx(L, i, j, k + 1)];
+    const T fzm = L.cz[lidx(L, i, j, k)];
+    return ((fxp * (xp - c) - fxm * (c - xm)) * L.ix2 + (fyp * (yp - c) - fym * (c - ym)) * L.iy2) +
+           (fzp * (zp - c) - fzm * (c - zm)) * L.iz2;
+}
+
+template <typename T>
+__device__ __forceinline__ T diagA(const LevelD<T> &L, int i, int j, int k) {
+    if (!L.cx) return T(-2) * ((L.ix2 + L.iy2) + L.iz2);
+    const uint32_t nx1 = L.nx + 1, ny1 = L.ny + 1;
+    const T fxp = L.cx[(uint32_t)(i + 1) + nx1 * ((uint32_t)j + (uint32_t)L.ny * k)];
+    const T fxm = L.cx[(uint32_t)i + nx1 * ((uint32_t)j + (uint32_t)L.ny * k)];
+    const T fyp = L.cy[(uint32_t)i + (uint32_t)L.nx * ((uint32_t)(j + 1) + ny1 * k)];
+    const T fym = L.cy[(uint32_t)i + (uint32_t)L.nx * ((uint32_t)j + ny1 * k)];
+    const T fzp = L.cz[lidx(L, i, j, k + 1)];
+    const T fzm = L.cz[lidx(L, i, j, k)];
+    return -(((fxp + fxm) * L.ix2 + (fyp + fym) * L.iy2) + (fzp + fzm) * L.iz2);
+}
+
+// ---- block reduction: partial[blockIdx] = Σ over the block of v (FP64) ------
+__device__ __forceinline__ double block_sum(double v, double *sh) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) sh[w] = v;
+    __syncthreads();
+    double s = 0.0;
+    if (threadIdx.x == 0) {
+        const int nw = (blockDim.x + 63) >> 6;
+        for (int q = 0; q < nw; q++) s += sh[q];
+    }
+    __syncthreads();
+    return s;
+}
+
+// out[0] = Σ partials (fixed order) * scale; `n` partials
+__global__ void finalize_kernel(double *__restrict__ out, const double *__restrict__ partials, int n, double scale) {
+    __shared__ double sh[RB / 64];
+    double v = 0.0;
+    for (int q = threadIdx.x; q < n; q += blockDim.x) v += partials[q];
+    double s = block_sum(v, sh);
+    if (threadIdx.x == 0) out[0] = s * scale;
+}
+
+// several sums at once: partials laid out [comp][n]
+__global__ void finalize_multi_kernel(double *__restrict__ out, const double *__restrict__ partials, int n, int ncomp) {
+    __shared__ double sh[RB / 64];
+    for (int c = 0; c < ncomp; c++) {
+        double v = 0.0;
+        for (int q = threadIdx.x; q < n; q += blockDim.x) v += partials[(size_t)c * n + q];
+        double s = block_sum(v, sh);
+        if (threadIdx.x == 0) out[c] = s;
+    }
+}
+
+// ---- V-cycle kernels --------------------------------------------------------
+// one colour of the red-black Gauss-Seidel sweep (_rbgs_body!, src/Multigrid.jl:307-337):
+// cell (i, j, k) has colour (i + j + k + 1) & 1 in 0-based interior indices
+// (the reference's (i + j + k) & 1 in 1-based unit-halo storage)
+template <typename T>
+__global__ void rbgs_kernel(LevelD<T> L, int colour) {
+    const int ih = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y, k = blockIdx.z;
+    const int i = 2 * ih + ((j + k + 1 + colour) & 1);
+    if (ih >= (L.nx + 1) / 2 || i >= L.nx) return;
+    const T *f = L.phi;
+    const T xp = f[lidx(L, wrapc(i + 1, L.nx, L.bcx), j, k)], xm = f[lidx(L, wrapc(i - 1, L.nx, L.bcx), j, k)];
+    const T yp = f[lidx(L, i, wrapc(j + 1, L.ny, L.bcy), k)], ym = f[lidx(L, i, wrapc(j - 1, L.ny, L.bcy), k)];
+    const T zp = f[lidx(L, i, j, wrapc(k + 1, L.nz, L.bcz))], zm = f[lidx(L, i, j, wrapc(k - 1, L.nz, L.bcz))];
+    T diag, off;
+    if (!L.cx) {
+        diag = T(-2) * ((L.ix2 + L.iy2) + L.iz2);
+        off = ((xp + xm) * L.ix2 + (yp + ym) * L.iy2) + (zp + zm) * L.iz2;
+    } else {
+        const uint32_t nx1 = L.nx + 1, ny1 = L.ny + 1;
+        const T fxp = L.cx[(uint32_t)(i + 1) + nx1 * ((uint32_t)j + (uint32_t)L.ny * k)];
+        const T fxm = L.cx[(uint32_t)i + nx1 * ((uint32_t)j + (uint32_t)L.ny * k)];
+        const T fyp = L.cy[(uint32_t)i + (uint32_t)L.nx * ((uint32_t)(j + 1) + ny1 * k)];
+        const T fym = L.cy[(uint32_t)i + (uint32_t)L.nx * ((uint32_t)j + ny1 * k)];
+        const T fzp = L.cz[lidx(L, i, j, k + 1)];
+        const T fzm = L.cz[lidx(L, i, j, k)];
+        diag = -(((fxp + fxm) * L.ix2 + (fyp + fym) * L.iy2) + (fzp + fzm) * L.iz2);
+        if (fabs(diag) < Eps<T>::v) return;  // a fully blocked cell
+        off = ((fxp * xp + fxm * xm) * L.ix2 + (fyp * yp + fym * ym) * L.iy2) + (fzp * zp + fzm * zm) * L.iz2;
+    }
+    L.phi[lidx(L, i, j, k)] = (L.b[lidx(L, i, j, k)] - off) / diag;
+}
+
+// damped Jacobi from `src` into `dst`: φ ← φ + ω (b − Aφ)/diag (a fully blocked cell keeps φ)
+template <typename T>
+__global__ void jacobi_kernel(LevelD<T> L, const T *__restrict__ src, T *__restrict__ dst, T omega) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y, k = blockIdx.z;
+    if (i >= L.nx) return;
+    const uint32_t c = lidx(L, i, j, k);
+    const T d = diagA(L, i, j, k);
+    const T x = src[c];
+    if (fabs(d) < Eps<T>::v) { dst[c] = x; return; }
+    const T r = L.b[c] - applyA(L, src, i, j, k);
+    dst[c] = x + omega * (r / d);
+}
+
+// r = b − Aφ (_res_body!, src/Multigrid.jl:266-269), with the block sums of r
+template <typename T>
+__global__ void residual_kernel(LevelD<T> L, double *__restrict__ partials) {
+    __shared__ double sh[RB / 64];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y, k = blockIdx.z;
+    double v = 0.0;
+    if (i < L.nx) {
+        const uint32_t c = lidx(L, i, j, k);
+        const T r = L.b[c] - applyA(L, L.phi, i, j, k);
+        L.r[c] = r;
+        v = (double)r;
+    }
+    double s = block_sum(v, sh);
+    if (threadIdx.x == 0) partials[blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z)] = s;
+}
+
+// coarse b = the mean of the 8 fine (r − m) (_restrict_body!, src/Multigrid.jl:381-392), with block sums
+template <typename T>
+__global__ void restrict_kernel(LevelD<T> C, LevelD<T> F, const double *__restrict__ mptr, double *__restrict__ partials) {
+    __shared__ double sh[RB / 64];
+    const int I = blockIdx.x * blockDim.x + threadIdx.x;
+    const int J = blockIdx.y, K = blockIdx.z;
+    double v = 0.0;
+    if (I < C.nx) {
+        const T m = mptr ? (T)mptr[0] : T(0);
+        const int i = 2 * I, j = 2 * J, k = 2 * K;
+        const T *r = F.r;
+        T s = (((((((r[lidx(F, i, j, k)] - m) + (r[lidx(F, i + 1, j, k)] - m)) + (r[lidx(F, i, j + 1, k)] - m)) +
+                   (r[lidx(F, i + 1, j + 1, k)] - m)) + (r[lidx(F, i, j, k + 1)] - m)) +
+                 (r[lidx(F, i + 1, j, k + 1)] - m)) + (r[lidx(F, i, j + 1, k + 1)] - m)) +
+               (r[lidx(F, i + 1, j + 1, k + 1)] - m);
+        T bval = s * T(1.0 / 8);
+        C.b[lidx(C, I, J, K)] = bval;
+        v = (double)bval;
+    }
+    double s = block_sum(v, sh);
+    if (threadIdx.x == 0) partials[blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z)] = s;
+}
+
+// φ += trilinear correction (_prolong_body!, src/Multigrid.jl:415-432), with block sums of φ
+template <typename T>
+__global__ void prolong_kernel(LevelD<T> F, LevelD<T> C, double *__restrict__ partials) {
+    __shared__ double sh[RB / 64];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y, k = blockIdx.z;
+    double v = 0.0;
+    if (i < F.nx) {
+        // 1-based fine index i+1: coarse cell (i+1+1)>>1 (1-based) and the far neighbour
+        const int Ih = (i >> 1), Ix = Ih + (((i + 1) & 1) ? -1 : 1);
+        const int Jh = (j >> 1), Jy = Jh + (((j + 1) & 1) ? -1 : 1);
+        const int Kh = (k >> 1), Kz = Kh + (((k + 1) & 1) ? -1 : 1);
+        const int ix = wrapc(Ix, C.nx, C.bcx), jy = wrapc(Jy, C.ny, C.bcy), kz = wrapc(Kz, C.nz, C.bcz);
+        const T n = T(3) / T(4), f = T(1) / T(4);
+        const T *P = C.phi;
+        T acc = T(0);
+        acc += (n * n * n) * P[lidx(C, Ih, Jh, Kh)];
+        acc += (f * n * n) * P[lidx(C, ix, Jh, Kh)];
+        acc += (n * f * n) * P[lidx(C, Ih, jy, Kh)];
+        acc += (f * f * n) * P[lidx(C, ix, jy, Kh)];
+        acc += (n * n * f) * P[lidx(C, Ih, Jh, kz)];
+        acc += (f * n * f) * P[lidx(C, ix, Jh, kz)];
+        acc += (n * f * f) * P[lidx(C, Ih, jy, kz)];
+        acc += (f * f * f) * P[lidx(C, ix, jy, kz)];
+        const uint32_t c = lidx(F, i, j, k);
+        const T val = F.phi[c] + acc;
+        F.phi[c] = val;
+        v = (double)val;
+    }
+    double s = block_sum(v, sh);
+    if (threadIdx.x == 0) partials[blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z)] = s;
+}
+
+// a ← a − m (m on the device), with the block sums of the result when asked
+template <typename T>
+__global__ void shift_kernel(T *__restrict__ a, int n, const double *__restrict__ mptr) {
+    const T m = (T)mptr[0];
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) a[t] = a[t] - m;
+}
+
+// block sums of an array (for a mean)
+template <typename T>
+__global__ void sum_kernel(const T *__restrict__ a, int n, double *__restrict__ partials) {
+    __shared__ double sh[RB / 64];
+    double v = 0.0;
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) v += (double)a[t];
+    double s = block_sum(v, sh);
+    if (threadIdx.x == 0) partials[blockIdx.x] = s;
+}
+
+template <typename T>
+__global__ void sumsq_kernel(const T *__restrict__ a, int n, double *__restrict__ partials) {
+    __shared__ double sh[RB / 64];
+    double v = 0.0;
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) v += (double)a[t] * (double)a[t];
+    double s = block_sum(v, sh);
+    if (threadIdx.x == 0) partials[blockIdx.x] = s;
+}
+
+template <typename T>
+__global__ void copyout_kernel(double *__restrict__ out, T *__restrict__ phi, int n, const double *__restrict__ mptr) {
+    const double m = mptr[0];
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
+        const T v = (T)((double)phi[t] - m);
+        phi[t] = v;
+        out[t] = (double)v;
+    }
+}
+
+template <typename T>
+__global__ void zero_kernel(T *__restrict__ a, int n) {
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) a[t] = T(0);
+}
+
+// The coarsest level in one workgroup: 50 red-black sweeps with φ and b in LDS,
+// then the mean projection (_solve_coarsest!, src/Multigrid.jl:685-689).
+template <typename T>
+__global__ void __launch_bounds__(1024) coarsest_kernel(LevelD<T> L, int nsweeps, int singular) {
+    extern __shared__ unsigned char smem_raw[];
+    T *sphi = reinterpret_cast<T *>(smem_raw);
+    const int n = L.nx * L.ny * L.nz;
+    T *sb = sphi + n;
+    __shared__ double sh[1024 / 64];
+    for (int t = threadIdx.x; t < n; t += blockDim.x) { sphi[t] = L.phi[t]; sb[t] = L.b[t]; }
+    __syncthreads();
+    LevelD<T> S = L;
+    S.phi = sphi;
+    S.b = sb;
+    for (int sw = 0; sw < nsweeps; sw++) {
+        for (int colour = 0; colour <= 1; colour++) {
+            for (int t = threadIdx.x; t < n; t += blockDim.x) {
+                const int i = t % L.nx, j = (t / L.nx) % L.ny, k = t / (L.nx * L.ny);
+                if (((i + j + k + 1) & 1) != colour) continue;
+                const T d = diagA(S, i, j, k);
+                if (L.cx && fabs(d) < Eps<T>::v) continue;
+                // (b − off)/diag with off the neighbour sum of the reference's form
+                const T xp = sphi[lidx(S, wrapc(i + 1, S.nx, S.bcx), j, k)], xm = sphi[lidx(S, wrapc(i - 1, S.nx, S.bcx), j, k)];
+                const T yp = sphi[lidx(S, i, wrapc(j + 1, S.ny, S.bcy), k)], ym = sphi[lidx(S, i, wrapc(j - 1, S.ny, S.bcy), k)];
+                const T zp = sphi[lidx(S, i, j, wrapc(k + 1, S.nz, S.bcz))], zm = sphi[lidx(S, i, j, wrapc(k - 1, S.nz, S.bcz))];
+                T o;
+                if (!L.cx) {
+                    o = ((xp + xm) * L.ix2 + (yp + ym) * L.iy2) + (zp + zm) * L.iz2;
+                } else {
+                    const uint32_t nx1 = L.nx + 1, ny1 = L.ny + 1;
+                    const T fxp = L.cx[(uint32_t)(i + 1) + nx1 * ((uint32_t)j + (uint32_t)L.ny * k)];
+                    const T fxm = L.cx[(uint32_t)i + nx1 * ((uint32_t)j + (uint32_t)L.ny * k)];
+                    const T fyp = L.cy[(uint32_t)i + (uint32_t)L.nx * ((uint32_t)(j + 1) + ny1 * k)];
+                    const T fym = L.cy[(uint32_t)i + (uint32_t)L.nx * ((uint32_t)j + ny1 * k)];
+                    const T fzp = L.cz[lidx(L, i, j, k + 1)];
+                    const T fzm = L.cz[lidx(L, i, j, k)];
+                    o = ((fxp * xp + fxm * xm) * L.ix2 + (fyp * yp + fym * ym) * L.iy2) + (fzp * zp + fzm * zm) * L.iz2;
+                }
+                sphi[t] = (sb[t] - o) / d;
+            }
+            __syncthreads();
+        }
+    }
+    if (singular) {
+        double v = 0.0;
+        for (int t = threadIdx.x; t < n; t += blockDim.x) v += (double)sphi[t];
+        double s = block_sum(v, sh);
+        __shared__ T mean;
+        if (threadIdx.x == 0) mean = (T)(s / n);
+        __syncthreads();
+        for (int t = threadIdx.x; t < n; t += blockDim.x) sphi[t] = sphi[t] - mean;
+        __syncthreads();
+    }
+    for (int t = threadIdx.x; t < n; t += blockDim.x) L.phi[t] = sphi[t];
+}
+
+// ---- the outer FP64 CG kernels (pcg!, src/Backends.jl:1526-1589) ------------
+// Ap = A p on the fine level (FP64), with the block sums of p·Ap
+__global__ void cg_apply_kernel(LevelD<double> L, const double *__restrict__ p, double *__restrict__ Ap,
+                                double *__restrict__ partials) {
+    __shared__ double sh[RB / 64];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y, k = blockIdx.z;
+    double v = 0.0;
+    if (i < L.nx) {
+        const uint32_t c = lidx(L, i, j, k);
+        const double a = applyA(L, p, i, j, k);
+        Ap[c] = a;
+        v = p[c] * a;
+    }
+    double s = block_sum(v, sh);
+    if (threadIdx.x == 0) partials[blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z)] = s;
+}
+
+// x += αp; r −= αAp; block sums of (r², r)  (α = num/den from the device)
+__global__ void cg_update_kernel(double *__restrict__ x, double *__restrict__ r, const double *__restrict__ p,
+                                 const double *__restrict__ Ap, double num, const double *__restrict__ den, int n,
+                                 double *__restrict__ partials, int nblk) {
+    __shared__ double sh[RB / 64];
+    const double alpha = den ? num / den[0] : 0.0;
+    double s2 = 0.0, s1 = 0.0;
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
+        x[t] = x[t] + alpha * p[t];
+        const double rv = r[t] - alpha * Ap[t];
+        r[t] = rv;
+        s2 += rv * rv;
+        s1 += rv;
+    }
+    double a = block_sum(s2, sh);
+    double b = block_sum(s1, sh);
+    if (threadIdx.x == 0) { partials[blockIdx.x] = a; partials[nblk + blockIdx.x] = b; }
+}
+
+// the preconditioner's right-hand side: level-0 b = T(r − mean r)  (_load_rhs!, src/Multigrid.jl:764-775)
+template <typename T>
+__global__ void load_rhs_kernel(T *__restrict__ b, const double *__restrict__ r, int n, const double *__restrict__ mptr) {
+    const double m = mptr[0];
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) b[t] = (T)(r[t] - m);
+}
+
+// z = φ − mean φ (_store_solution!, :789-793), then the block sums of (r·z, z, r)
+template <typename T>
+__global__ void store_rz_kernel(double *__restrict__ z, const T *__restrict__ phi, const double *__restrict__ r, int n,
+                                const double *__restrict__ mptr, double *__restrict__ partials, int nblk) {
+    __shared__ double sh[RB / 64];
+    const double m = mptr[0];
+    double srz = 0.0, sz = 0.0, sr = 0.0;
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
+        const double zv = (double)phi[t] - m;
+        z[t] = zv;
+        srz += r[t] * zv;
+        sz += zv;
+        sr += r[t];
+    }
+    double a = block_sum(srz, sh), b = block_sum(sz, sh), c = block_sum(sr, sh);
+    if (threadIdx.x == 0) { partials[blockIdx.x] = a; partials[nblk + blockIdx.x] = b; partials[2 * nblk + blockIdx.x] = c; }
+}
+
+// p = (z − m) + βp  (_pcg_dir_body!, src/Backends.jl:1413-1417); first: p = z − m
+__global__ void cg_dir_kernel(double *__restrict__ p, const double *__restrict__ z, int n, double m, double beta,
+                              int first) {
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x)
+        p[t] = first ? z[t] - m : (z[t] - m) + beta * p[t];
+}
+
+__global__ void scale_rows_kernel(double *__restrict__ a, const double *__restrict__ sc, int nxy, int n) {
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) a[t] = a[t] * sc[t / nxy];
+}
+
+__global__ void fill_kernel(double *__restrict__ a, int n, double v) {
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) a[t] = v;
+}
+__global__ void shift_copy_kernel(double *__restrict__ out, const double *__restrict__ x, int n, double m) {
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) out[t] = x[t] - m;
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+struct MGLevelH {
+    int nx, ny, nz;
+    double dx, dy, dz;
+    void *phi = nullptr, *b = nullptr, *r = nullptr, *tmp = nullptr;
+    void *cx = nullptr, *cy = nullptr, *cz = nullptr;
+};
+
+struct MGPlan {
+    int nlev = 0;
+    MGLevelH L[24];
+    int bcx, bcy, bcz, nu1, nu2, smoother, prec;  // prec 0 double, 1 float
+    double omega;
+    bool coeff = false;
+    // FP64 fine operator for the CG
+    double *cx64 = nullptr, *cy64 = nullptr, *cz64 = nullptr;
+    double *rhs_scale = nullptr;
+    // CG arrays and reduction scratch
+    double *x = nullptr, *r = nullptr, *z = nullptr, *p = nullptr, *Ap = nullptr;
+    double *part = nullptr, *scal = nullptr;  // scal: device scalars
+    int npart = 0;
+};
+
+static int ilog2c(int n) { int l = 0; while ((1 << l) < n) l++; return l; }
+
+template <typename T>
+static LevelD<T> devlevel(const MGPlan *P, int l) {
+    const MGLevelH &h = P->L[l];
+    LevelD<T> L;
+    L.nx = h.nx; L.ny = h.ny; L.nz = h.nz;
+    L.ix2 = T(1.0 / (h.dx * h.dx)); L.iy2 = T(1.0 / (h.dy * h.dy)); L.iz2 = T(1.0 / (h.dz * h.dz));
+    L.phi = (T *)h.phi; L.b = (T *)h.b; L.r = (T *)h.r; L.tmp = (T *)h.tmp;
+    L.cx = (const T *)h.cx; L.cy = (const T *)h.cy; L.cz = (const T *)h.cz;
+    L.bcx = P->bcx; L.bcy = P->bcy; L.bcz = P->bcz;
+    return L;
+}
+
+static LevelD<double> finelevel64(const MGPlan *P) {
+    const MGLevelH &h = P->L[0];
+    LevelD<double> L;
+    L.nx = h.nx; L.ny = h.ny; L.nz = h.nz;
+    L.ix2 = 1.0 / (h.dx * h.dx); L.iy2 = 1.0 / (h.dy * h.dy); L.iz2 = 1.0 / (h.dz * h.dz);
+    L.phi = L.b = L.r = L.tmp = nullptr;
+    L.cx = P->cx64; L.cy = P->cy64; L.cz = P->cz64;
+    L.bcx = P->bcx; L.bcy = P->bcy; L.bcz = P->bcz;
+    return L;
+}
+
+static dim3 grid3(int nx, int ny, int nz) { return dim3((nx + RB - 1) / RB, ny, nz); }
+static int nblocks3(int nx, int ny, int nz) { return ((nx + RB - 1) / RB) * ny * nz; }
+static int flat_blocks(int n) { int b = (n + RB - 1) / RB; return b < 2048 ? b : 2048; }
+
+// the mean of the interior of `a` (level l) into scal[slot], via partials
+template <typename T>
+static void mean_to(MGPlan *P, hipStream_t st, const T *a, int n, int slot) {
+    int nb = flat_blocks(n);
+    hipLaunchKernelGGL(sum_kernel<T>, dim3(nb), dim3(RB), 0, st, a, n, P->part);
+    hipLaunchKernelGGL(finalize_kernel, dim3(1), dim3(RB), 0, st, P->scal + slot, P->part, nb, 1.0 / n);
+}
+
+template <typename T>
+static void project_mean(MGPlan *P, hipStream_t st, int l) {
+    const MGLevelH &h = P->L[l];
+    int n = h.nx * h.ny * h.nz;
+    mean_to<T>(P, st, (const T *)h.phi, n, 4);
+    hipLaunchKernelGGL(shift_kernel<T>, dim3(flat_blocks(n)), dim3(RB), 0, st, (T *)h.phi, n, P->scal + 4);
+}
+
+template <typename T>
+static void smooth(MGPlan *P, hipStream_t st, int l, int nsweeps) {
+    MGLevelH &h = P->L[l];
+    LevelD<T> L = devlevel<T>(P, l);
+    if (P->smoother == 0) {
+        dim3 g(((h.nx + 1) / 2 + RB - 1) / RB, h.ny, h.nz);
+        for (int s = 0; s < nsweeps; s++)
+            for (int colour = 0; colour <= 1; colour++) hipLaunchKernelGGL(rbgs_kernel<T>, g, dim3(RB), 0, st, L, colour);
+    } else {
+        for (int s = 0; s < nsweeps; s++) {
+            hipLaunchKernelGGL(jacobi_kernel<T>, grid3(h.nx, h.ny, h.nz), dim3(RB), 0, st, L, (const T *)h.phi,
+                               (T *)h.tmp, (T)P->omega);
+            std::swap(h.phi, h.tmp);
+            L = devlevel<T>(P, l);
+        }
+    }
+}
+
+template <typename T>
+static void vcycle(MGPlan *P, hipStream_t st, int l) {
+    MGLevelH &h = P->L[l];
+    const int n = h.nx * h.ny * h.nz;
+    if (l == P->nlev - 1) {
+        size_t lds = sizeof(T) * 2 * (size_t)n;
+        if (n <= 4096) {
+            hipLaunchKernelGGL(coarsest_kernel<T>, dim3(1), dim3(1024), lds, st, devlevel<T>(P, l), 50, 1);
+        } else {
+            int sm = P->smoother;
+            P->smoother = 0;
+            smooth<T>(P, st, l, 50);
+            P->smoother = sm;
+            project_mean<T>(P, st, l);
+        }
+        return;
+    }
+    MGLevelH &c = P->L[l + 1];
+    const int nc = c.nx * c.ny * c.nz;
+    smooth<T>(P, st, l, P->nu1);
+    project_mean<T>(P, st, l);
+    // residual, its mean, and the restriction of (r − m)
+    LevelD<T> L = devlevel<T>(P, l), C = devlevel<T>(P, l + 1);
+    int nb = nblocks3(h.nx, h.ny, h.nz);
+    hipLaunchKernelGGL(residual_kernel<T>, grid3(h.nx, h.ny, h.nz), dim3(RB), 0, st, L, P->part);
+    hipLaunchKernelGGL(finalize_kernel, dim3(1), dim3(RB), 0, st, P->scal + 5, P->part, nb, 1.0 / n);
+    int nbc = nblocks3(c.nx, c.ny, c.nz);
+    hipLaunchKernelGGL(restrict_kernel<T>, grid3(c.nx, c.ny, c.nz), dim3(RB), 0, st, C, L, (const double *)(P->scal + 5),
+                       P->part);
+    hipLaunchKernelGGL(finalize_kernel, dim3(1), dim3(RB), 0, st, P->scal + 6, P->part, nbc, 1.0 / nc);
+    hipLaunchKernelGGL(shift_kernel<T>, dim3(flat_blocks(nc)), dim3(RB), 0, st, (T *)c.b, nc, P->scal + 6);
+    hipLaunchKernelGGL(zero_kernel<T>, dim3(flat_blocks(nc)), dim3(RB), 0, st, (T *)c.phi, nc);
+    vcycle<T>(P, st, l + 1);
+    // prolong and correct (with the sum of the corrected φ), project
+    L = devlevel<T>(P, l);
+    C = devlevel<T>(P, l + 1);
+    hipLaunchKernelGGL(prolong_kernel<T>, grid3(h.nx, h.ny, h.nz), dim3(RB), 0, st, L, C, P->part);
+    hipLaunchKernelGGL(finalize_kernel, dim3(1), dim3(RB), 0, st, P->scal + 4, P->part, nb, 1.0 / n);
+    hipLaunchKernelGGL(shift_kernel<T>, dim3(flat_blocks(n)), dim3(RB), 0, st, (T *)h.phi, n, P->scal + 4);
+    smooth<T>(P, st, l, P->nu2);
+    project_mean<T>(P, st, l);
+}
+
+// coefficient coarsening, _restrict_coeff (src/Multigrid.jl:465-496), on the host in FP64
+static void restrict_coeff(const std::vector<double> &cx, const std::vector<double> &cy, const std::vector<double> &cz,
+                           int fnx, int fny, int fnz, std::vector<double> &Cx, std::vector<double> &Cy,
+                           std::vector<double> &Cz, int cnx, int cny, int cnz) {
+    auto FX = [&](int i, int j, int k) { return cx[(size_t)(i - 1) + (size_t)(fnx + 1) * ((j - 1) + (size_t)fny * (k - 1))]; };
+    auto FY = [&](int i, int j, int k) { return cy[(size_t)(i - 1) + (size_t)fnx * ((j - 1) + (size_t)(fny + 1) * (k - 1))]; };
+    auto FZ = [&](int i, int j, int k) { return cz[(size_t)(i - 1) + (size_t)fnx * ((j - 1) + (size_t)fny * (k - 1))]; };
+    Cx.assign((size_t)(cnx + 1) * cny * cnz, 0);
+    Cy.assign((size_t)cnx * (cny + 1) * cnz, 0);
+    Cz.assign((size_t)cnx * cny * (cnz + 1), 0);
+    const double quarter = 1.0 / 4;
+    auto mn = [](int a, int b) { return a < b ? a : b; };
+    for (int K = 1; K <= cnz; K++)
+        for (int J = 1; J <= cny; J++)
+            for (int I = 1; I <= cnx + 1; I++) {
+                int i = 2 * I - 1;
+                double s = 0;
+                for (int dk = 0; dk <= 1; dk++)
+                    for (int dj = 0; dj <= 1; dj++) s += FX(mn(i, fnx + 1), mn(2 * J - 1 + dj, fny), mn(2 * K - 1 + dk, fnz));
+                Cx[(size_t)(I - 1) + (size_t)(cnx + 1) * ((J - 1) + (size_t)cny * (K - 1))] = s * quarter;
+            }
+    for (int K = 1; K <= cnz; K++)
+        for (int J = 1; J <= cny + 1; J++)
+            for (int I = 1; I <= cnx; I++) {
+                int j = 2 * J - 1;
+                double s = 0;
+                for (int dk = 0; dk <= 1; dk++)
+                    for (int di = 0; di <= 1; di++) s += FY(mn(2 * I - 1 + di, fnx), mn(j, fny + 1), mn(2 * K - 1 + dk, fnz));
+                Cy[(size_t)(I - 1) + (size_t)cnx * ((J - 1) + (size_t)(cny + 1) * (K - 1))] = s * quarter;
+            }
+    for (int K = 1; K <= cnz + 1; K++)
+        for (int J = 1; J <= cny; J++)
+            for (int I = 1; I <= cnx; I++) {
+                int k = 2 * K - 1;
+                double s = 0;
+                for (int dj = 0; dj <= 1; dj++)
+                    for (int di = 0; di <= 1; di++) s += FZ(mn(2 * I - 1 + di, fnx), mn(2 * J - 1 + dj, fny), mn(k, fnz + 1));
+                Cz[(size_t)(I - 1) + (size_t)cnx * ((J - 1) + (size_t)cny * (K - 1))] = s * quarter;
+            }
+}
+
+template <typename T>
+static hipError_t upload_as(void **dst, const std::vector<double> &h) {
+    std::vector<T> t(h.begin(), h.end());
+    hipError_t e = hipMalloc(dst, sizeof(T) * t.size());
+    if (e != hipSuccess) return e;
+    return hipMemcpy(*dst, t.data(), sizeof(T) * t.size(), hipMemcpyHostToDevice);
+}
+
+MGPlan *mg_create(int nx, int ny, int nz, double dx, double dy, double dz, int bcx, int bcy, int bcz, const double *cx,
+                  const double *cy, const double *cz, const double *rhs_scale, int nu1, int nu2, int smoother,
+                  double omega, int prec, int min_size, char *err, size_t errlen) {
+    auto fail = [&](const char *m) { snprintf(err, errlen, "%s", m); return (MGPlan *)nullptr; };
+    MGPlan *P = new MGPlan();
+    P->bcx = bcx; P->bcy = bcy; P->bcz = bcz; P->nu1 = nu1; P->nu2 = nu2; P->smoother = smoother;
+    P->omega = omega; P->prec = prec;
+    P->coeff = cx != nullptr;
+    const size_t esz = prec == 0 ? 8 : 4;
+    std::vector<double> hx, hy, hz;
+    if (P->coeff) {
+        hx.assign(cx, cx + (size_t)(nx + 1) * ny * nz);
+        hy.assign(cy, cy + (size_t)nx * (ny + 1) * nz);
+        hz.assign(cz, cz + (size_t)nx * ny * (nz + 1));
+        if (upload_as<double>((void **)&P->cx64, hx) || upload_as<double>((void **)&P->cy64, hy) ||
+            upload_as<double>((void **)&P->cz64, hz))
+            return fail("mg: allocation failed");
+    }
+    int lx = nx, ly = ny, lz = nz;
+    double ddx = dx, ddy = dy, ddz = dz;
+    while (true) {
+        MGLevelH &h = P->L[P->nlev];
+        h.nx = lx; h.ny = ly; h.nz = lz; h.dx = ddx; h.dy = ddy; h.dz = ddz;
+        size_t n = (size_t)lx * ly * lz;
+        void **arrs[4] = {&h.phi, &h.b, &h.r, &h.tmp};
+        for (auto a : arrs) {
+            if (hipMalloc(a, esz * n) != hipSuccess) return fail("mg: allocation failed");
+            hipMemset(*a, 0, esz * n);
+        }
+        if (P->coeff) {
+            hipError_t e1 = prec == 0 ? upload_as<double>(&h.cx, hx) : upload_as<float>(&h.cx, hx);
+            hipError_t e2 = prec == 0 ? upload_as<double>(&h.cy, hy) : upload_as<float>(&h.cy, hy);
+            hipError_t e3 = prec == 0 ? upload_as<double>(&h.cz, hz) : upload_as<float>(&h.cz, hz);
+            if (e1 || e2 || e3) return fail("mg: allocation failed");
+        }
+        P->nlev++;
+        // the coarsening rule, src/Multigrid.jl:576-590
+        bool cxo = lx % 2 == 0 && lx / 2 >= min_size, cyo = ly % 2 == 0 && ly / 2 >= min_size,
+             czo = lz % 2 == 0 && lz / 2 >= min_size;
+        if (!(cxo && cyo && czo) || P->nlev >= 24) break;
+        int nlx = lx / 2, nly = ly / 2, nlz = lz / 2;
+        if (P->coeff) {
+            std::vector<double> Cx, Cy, Cz;
+            restrict_coeff(hx, hy, hz, lx, ly, lz, Cx, Cy, Cz, nlx, nly, nlz);
+            hx.swap(Cx); hy.swap(Cy); hz.swap(Cz);
+        }
+        lx = nlx; ly = nly; lz = nlz;
+        ddx *= 2; ddy *= 2; ddz *= 2;
+    }
+    size_t n0 = (size_t)nx * ny * nz;
+    double **cg[5] = {&P->x, &P->r, &P->z, &P->p, &P->Ap};
+    for (auto a : cg) {
+        if (hipMalloc(a, 8 * n0) != hipSuccess) return fail("mg: allocation failed");
+        hipMemset(*a, 0, 8 * n0);
+    }
+    P->npart = nblocks3(nx, ny, nz);
+    if (P->npart < 3 * 2048) P->npart = 3 * 2048;
+    if (hipMalloc(&P->part, sizeof(double) * P->npart) != hipSuccess) return fail("mg: allocation failed");
+    if (hipMalloc(&P->scal, sizeof(double) * 16) != hipSuccess) return fail("mg: allocation failed");
+    hipMemset(P->scal, 0, sizeof(double) * 16);
+    if (rhs_scale) {
+        std::vector<double> rs(rhs_scale, rhs_scale + nz);
+        if (upload_as<double>((void **)&P->rhs_scale, rs)) return fail("mg: allocation failed");
+    }
+    hipDeviceSynchronize();
+    return P;
+}
+
+void mg_destroy(MGPlan *P) {
+    if (!P) return;
+    for (int l = 0; l < P->nlev; l++) {
+        MGLevelH &h = P->L[l];
+        hipFree(h.phi); hipFree(h.b); hipFree(h.r); hipFree(h.tmp); hipFree(h.cx); hipFree(h.cy); hipFree(h.cz);
+    }
+    hipFree(P->cx64); hipFree(P->cy64); hipFree(P->cz64); hipFree(P->rhs_scale);
+    hipFree(P->x); hipFree(P->r); hipFree(P->z); hipFree(P->p); hipFree(P->Ap);
+    hipFree(P->part); hipFree(P->scal);
+    delete P;
+}
+
+int mg_levels(const MGPlan *P) { return P->nlev; }
+
+// z = M⁻¹ r: one V-cycle from zero, the reference's _mg_precond! (src/Multigrid.jl:922-932);
+// returns with the block sums of (r·z, z, r) finalised into scal[0..2]
+template <typename T>
+static void precond(MGPlan *P, hipStream_t st) {
+    MGLevelH &h = P->L[0];
+    const int n = h.nx * h.ny * h.nz;
+    const int fb = flat_blocks(n);
+    // the mean of r, then b = r − m
+    mean_to<double>(P, st, P->r, n, 7);
+    hipLaunchKernelGGL(load_rhs_kernel<T>, dim3(fb), dim3(RB), 0, st, (T *)h.b, (const double *)P->r, n, P->scal + 7);
+    hipLaunchKernelGGL(zero_kernel<T>, dim3(fb), dim3(RB), 0, st, (T *)h.phi, n);
+    vcycle<T>(P, st, 0);
+    mean_to<T>(P, st, (const T *)h.phi, n, 8);
+    hipLaunchKernelGGL(store_rz_kernel<T>, dim3(fb), dim3(RB), 0, st, P->z, (const T *)h.phi, (const double *)P->r, n,
+                       P->scal + 8, P->part, fb);
+    hipLaunchKernelGGL(finalize_multi_kernel, dim3(1), dim3(RB), 0, st, P->scal, P->part, fb, 3);
+}
+
+static double read_scal(hipStream_t st, const double *d) {
+    double v = 0;
+    hipMemcpyAsync(&v, d, sizeof(double), hipMemcpyDeviceToHost, st);
+    hipStreamSynchronize(st);
+    return v;
+}
+static void read_scals(hipStream_t st, const double *d, double *h, int n) {
+    hipMemcpyAsync(h, d, sizeof(double) * n, hipMemcpyDeviceToHost, st);
+    hipStreamSynchronize(st);
+}
+
+// MGPreconditionedCG solve!: pcg! with project_mean, definite = :negative,
+// zero guess; b is the FP64 interior-sized source (scaled rows when rhs_scale)
+int mg_pcg_solve(MGPlan *P, hipStream_t st, double *xout, const double *bin, double tol, int maxit, int *iters_out,
+                 double *resid_out, double *bnorm_out) {
+    MGLevelH &h = P->L[0];
+    const int n = h.nx * h.ny * h.nz;
+    const int fb = flat_blocks(n);
+    const double dn = (double)n;
+    LevelD<double> F = finelevel64(P);
+    double *b = P->Ap;  // the scaled source lives in Ap until the first apply
+    hipMemcpyAsync(b, bin, sizeof(double) * n, hipMemcpyDeviceToDevice, st);
+    if (P->rhs_scale)
+        hipLaunchKernelGGL(scale_rows_kernel, dim3(fb), dim3(RB), 0, st, b, (const double *)P->rhs_scale, h.nx * h.ny, n);
+    // mb = mean(b); r = b − mb; x = 0; ‖b − mb‖
+    mean_to<double>(P, st, b, n, 9);
+    double mb = read_scal(st, P->scal + 9);
+    hipLaunchKernelGGL(shift_copy_kernel, dim3(fb), dim3(RB), 0, st, P->r, (const double *)b, n, mb);
+    hipLaunchKernelGGL(fill_kernel, dim3(fb), dim3(RB), 0, st, P->x, n, 0.0);
+    hipLaunchKernelGGL(fill_kernel, dim3(fb), dim3(RB), 0, st, P->z, n, 0.0);
+    // b2 = Σ r² (z = 0): the update kernel with α = 0
+    hipLaunchKernelGGL(cg_update_kernel, dim3(fb), dim3(RB), 0, st, P->x, P->r, (const double *)P->z,
+                       (const double *)P->z, 0.0, (const double *)nullptr, n, P->part, fb);
+    hipLaunchKernelGGL(finalize_multi_kernel, dim3(1), dim3(RB), 0, st, P->scal + 12, P->part, fb, 2);
+    double b2 = read_scal(st, P->scal + 12);
+    double bnorm = sqrt(b2);
+    *bnorm_out = bnorm;
+    if (bnorm == 0) {
+        hipLaunchKernelGGL(fill_kernel, dim3(fb), dim3(RB), 0, st, xout, n, 0.0);
+        *iters_out = 0;
+        *resid_out = 0;
+        return 3;
+    }
+    if (P->prec == 0) precond<double>(P, st); else precond<float>(P, st);
+    double s3[3];
+    read_scals(st, P->scal, s3, 3);
+    double r2 = b2, rz = s3[0], Sz = s3[1], Sr = s3[2];
+    double m = Sz / dn;
+    rz -= m * Sr;
+    hipLaunchKernelGGL(cg_dir_kernel, dim3(fb), dim3(RB), 0, st, P->p, (const double *)P->z, n, m, 0.0, 1);
+    double rnorm = sqrt(r2);
+    int iters = 0, status = rnorm <= tol * bnorm ? 0 : 1;
+    int nb3 = nblocks3(h.nx, h.ny, h.nz);
+    while (status != 0 && iters < maxit) {
+        hipLaunchKernelGGL(cg_apply_kernel, grid3(h.nx, h.ny, h.nz), dim3(RB), 0, st, F, (const double *)P->p, P->Ap,
+                           P->part);
+        hipLaunchKernelGGL(finalize_kernel, dim3(1), dim3(RB), 0, st, P->scal + 11, P->part, nb3, 1.0);
+        double pAp = read_scal(st, P->scal + 11);
+        if (!(fabs(pAp) > DBL_EPSILON * fabs(rz))) { status = 2; break; }
+        iters++;
+        hipLaunchKernelGGL(cg_update_kernel, dim3(fb), dim3(RB), 0, st, P->x, P->r, (const double *)P->p,
+                           (const double *)P->Ap, rz, (const double *)(P->scal + 11), n, P->part, fb);
+        hipLaunchKernelGGL(finalize_multi_kernel, dim3(1), dim3(RB), 0, st, P->scal + 12, P->part, fb, 2);
+        double s2[2];
+        read_scals(st, P->scal + 12, s2, 2);
+        rnorm = sqrt(s2[0]);
+        if (rnorm <= tol * bnorm) { status = 0; break; }
+        if (P->prec == 0) precond<double>(P, st); else precond<float>(P, st);
+        read_scals(st, P->scal, s3, 3);
+        double rzn = s3[0];
+        Sz = s3[1]; Sr = s3[2];
+        m = Sz / dn;
+        rzn -= m * Sr;
+        double beta = rzn / rz;
+        hipLaunchKernelGGL(cg_dir_kernel, dim3(fb), dim3(RB), 0, st, P->p, (const double *)P->z, n, m, beta, 0);
+        rz = rzn;
+    }
+    // the solution's mean removed
+    mean_to<double>(P, st, P->x, n, 13);
+    double mx = read_scal(st, P->scal + 13);
+    hipLaunchKernelGGL(shift_copy_kernel, dim3(fb), dim3(RB), 0, st, xout, (const double *)P->x, n, mx);
+    *iters_out = iters;
+    *resid_out = rnorm;
+    return status;
+}
+
+// plain multigrid cycles (mg_solve!, src/Multigrid.jl:801-845) with FP64 levels
+int mg_cycles_solve(MGPlan *P, hipStream_t st, double *xout, const double *bin, double tol, int maxcycles,
+                    int *cycles_out, double *rel_out) {
+    if (P->prec != 0) return -1;
+    MGLevelH &h = P->L[0];
+    const int n = h.nx * h.ny * h.nz;
+    const int fb = flat_blocks(n);
+    hipMemcpyAsync(P->r, bin, sizeof(double) * n, hipMemcpyDeviceToDevice, st);
+    if (P->rhs_scale)
+        hipLaunchKernelGGL(scale_rows_kernel, dim3(fb), dim3(RB), 0, st, P->r, (const double *)P->rhs_scale, h.nx * h.ny, n);
+    // _load_rhs!: b = rhs − mean(rhs); φ = 0; ‖b‖
+    mean_to<double>(P, st, P->r, n, 7);
+    hipLaunchKernelGGL(load_rhs_kernel<double>, dim3(fb), dim3(RB), 0, st, (double *)h.b, (const double *)P->r, n,
+                       P->scal + 7);
+    hipLaunchKernelGGL(zero_kernel<double>, dim3(fb), dim3(RB), 0, st, (double *)h.phi, n);
+    hipLaunchKernelGGL(sumsq_kernel<double>, dim3(fb), dim3(RB), 0, st, (const double *)h.b, n, P->part);
+    hipLaunchKernelGGL(finalize_kernel, dim3(1), dim3(RB), 0, st, P->scal, P->part, fb, 1.0);
+    const double bn = sqrt(read_scal(st, P->scal));
+    if (bn == 0) {
+        hipLaunchKernelGGL(fill_kernel, dim3(fb), dim3(RB), 0, st, xout, n, 0.0);
+        *cycles_out = 0;
+        *rel_out = 0;
+        return 0;
+    }
+    double rel = 1;
+    int ncyc = 0;
+    const int nb3 = nblocks3(h.nx, h.ny, h.nz);
+    for (int cyc = 1; cyc <= maxcycles; cyc++) {
+        vcycle<double>(P, st, 0);
+        // _residual_norm!: r, its mean removed, ‖r‖
+        LevelD<double> L = devlevel<double>(P, 0);
+        hipLaunchKernelGGL(residual_kernel<double>, grid3(h.nx, h.ny, h.nz), dim3(RB), 0, st, L, P->part);
+        hipLaunchKernelGGL(finalize_kernel, dim3(1), dim3(RB), 0, st, P->scal + 5, P->part, nb3, 1.0 / n);
+        hipLaunchKernelGGL(shift_kernel<double>, dim3(fb), dim3(RB), 0, st, (double *)h.r, n, P->scal + 5);
+        hipLaunchKernelGGL(sumsq_kernel<double>, dim3(fb), dim3(RB), 0, st, (const double *)h.r, n, P->part);
+        hipLaunchKernelGGL(finalize_kernel, dim3(1), dim3(RB), 0, st, P->scal, P->part, fb, 1.0);
+        rel = sqrt(read_scal(st, P->scal)) / bn;
+        ncyc = cyc;
+        if (rel < tol) break;
+    }
+    // _store_solution!
+    mean_to<double>(P, st, (const double *)h.phi, n, 8);
+    hipLaunchKernelGGL(copyout_kernel<double>, dim3(fb), dim3(RB), 0, st, xout, (double *)h.phi, n, P->scal + 8);
+    *cycles_out = ncyc;
+    *rel_out = rel;
+    return rel < tol ? 0 : 1;
+}
+
+// apply the fine FP64 operator (_apply_fine!, src/Multigrid.jl:911-917)
+void mg_apply(MGPlan *P, hipStream_t st, double *out, const double *x) {
+    MGLevelH &h = P->L[0];
+    hipLaunchKernelGGL(cg_apply_kernel, grid3(h.nx, h.ny, h.nz), dim3(RB), 0, st, finelevel64(P), x, out, P->part);
+}
+
+}  // namespace aeo

@@ -135,7 +135,7 @@ __device__ __forceinline__ T flux_x(const Fields<T> &f, const Metrics<T> &m, int
         vel = m.stretched ? m.m_zflux(k, lo, hi) : (lo + hi) / T(2);
     }
     const T *d = field_ptr<F>(f);
-    return vel * face_value<S, P, T>(d, (long)lin(0, j, k, sx, sy), 1, c, vel);
+    return vel * face_value<S, P, T>(d, lin(0, j, k, sx, sy), 1u, c, vel);
 }
 
 template <int F, int S, int P, typename T>
@@ -150,7 +150,7 @@ __device__ __forceinline__ T flux_y(const Fields<T> &f, const Metrics<T> &m, int
         vel = m.stretched ? m.m_zflux(k, lo, hi) : (lo + hi) / T(2);
     }
     const T *d = field_ptr<F>(f);
-    return vel * face_value<S, P, T>(d, (long)lin(i, 0, k, sx, sy), (long)sx, c, vel);
+    return vel * face_value<S, P, T>(d, lin(i, 0, k, sx, sy), (uint32_t)sx, c, vel);
 }
 
 template <int F, int S, int P, typename T>
@@ -162,7 +162,7 @@ __device__ __forceinline__ T flux_z(const Fields<T> &f, const Metrics<T> &m, int
     else if constexpr (F == F_V) vel = (f.w[lin(i, j - 1, c + 1, sx, sy)] + f.w[lin(i, j, c + 1, sx, sy)]) / T(2);
     else vel = (f.w[lin(i, j, c, sx, sy)] + f.w[lin(i, j, c + 1, sx, sy)]) / T(2);
     const T *d = field_ptr<F>(f);
-    return vel * face_value<S, P, T>(d, (long)lin(i, j, 0, sx, sy), (long)sx * sy, c, vel);
+    return vel * face_value<S, P, T>(d, lin(i, j, 0, sx, sy), (uint32_t)sx * (uint32_t)sy, c, vel);
 }
 
 // _lap_at, src/FiniteVolume.jl:271-283
@@ -215,10 +215,13 @@ template <int MODE> struct ModeFields { static constexpr int lo = F_U, hi = F_S;
 template <> struct ModeFields<MODE_SCALAR> { static constexpr int lo = F_S, hi = F_S; };
 template <> struct ModeFields<MODE_MOMENTUM> { static constexpr int lo = F_U, hi = F_W; };
 
-template <int MODE, int S, int P, typename T, int BY>
+template <int MODE, int S, int P, typename T, int BY, int STR>
 __global__ void __launch_bounds__(MW *BY) adv_march_kernel(MarchArgs<T> a) {
     constexpr int F0 = ModeFields<MODE>::lo, F1 = ModeFields<MODE>::hi, NF = F1 - F0 + 1;
-    const Metrics<T> &m = a.m;
+    // the grid kind as a compile-time constant: the metric accessors fold to
+    // UniformZ or LevelZ
+    Metrics<T> m = a.m;
+    m.stretched = STR;
     const int H = m.H;
     const int lane = threadIdx.x, ty = threadIdx.y;
     const int i0 = H + blockIdx.x * MW;
@@ -274,6 +277,10 @@ __global__ void __launch_bounds__(MW *BY) adv_march_kernel(MarchArgs<T> a) {
                 else if (F == F_W) { fxw[q] = flux_x<F_W, S, P, T>(a.in, m, i - 1, j, k); fys[q] = flux_y<F_W, S, P, T>(a.in, m, i, j - 1, k); fzt[q] = flux_z<F_W, S, P, T>(a.in, m, i, j, k); }
                 else { fxw[q] = flux_x<F_S, S, P, T>(a.in, m, i - 1, j, k); fys[q] = flux_y<F_S, S, P, T>(a.in, m, i, j - 1, k); fzt[q] = flux_z<F_S, S, P, T>(a.in, m, i, j, k); }
                 ys[buf][q][ty][lane] = fys[q];
+                // one field at a time: the scheduler does not interleave the
+                // reconstructions of different fields (shorter live ranges,
+                // fewer VGPRs, more waves per SIMD)
+                __builtin_amdgcn_sched_barrier(0);
             }
         }
 #pragma unroll
@@ -295,7 +302,7 @@ __global__ void __launch_bounds__(MW *BY) adv_march_kernel(MarchArgs<T> a) {
                     else fyn[F - F0] = flux_y<F_S, S, P, T>(a.in, m, i, j, k);
                 }
             }
-            const size_t c = lin(i, j, k, m.sx, m.sy);
+            const uint32_t c = lin(i, j, k, m.sx, m.sy);
 #pragma unroll
             for (int F = F0; F <= F1; F++) {
                 const int q = F - F0;
@@ -342,7 +349,10 @@ static void launch_march_sp(hipStream_t st, const MarchArgs<T> &a) {
     constexpr int BY = 4;
     const Metrics<T> &m = a.m;
     dim3 grid((m.Nx + MW - 1) / MW, (m.Ny + BY - 1) / BY, (m.Nz + a.kc - 1) / a.kc);
-    hipLaunchKernelGGL((adv_march_kernel<MODE, S, P, T, BY>), grid, dim3(MW, BY), 0, st, a);
+    if (m.stretched)
+        hipLaunchKernelGGL((adv_march_kernel<MODE, S, P, T, BY, 1>), grid, dim3(MW, BY), 0, st, a);
+    else
+        hipLaunchKernelGGL((adv_march_kernel<MODE, S, P, T, BY, 0>), grid, dim3(MW, BY), 0, st, a);
 }
 
 template <int MODE, typename T>

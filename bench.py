@@ -1,0 +1,217 @@
+"""Benchmark: one full SSP-RK3 projection step of the dry convective boundary
+layer at 256 x 256 x 128 on one MI355X.
+
+Workload (the reference's config of the metric, restated): Boussinesq,
+periodic x-y, walls in z, stretched z (Δz 10 m at the surface growing 2 %
+per level), θ = 300 K + 3 K/km above 1 km with seeded ±0.1 K noise in the
+lowest 10 levels, surface heat flux 0.1 K m/s; WENO-Z5 advection of momentum
+and θ, FFT-x-y + FP64 Thomas-z projection at every substage. Synthetic seeded
+input, inputs resident in HBM before the timed region.
+
+Prints ONE JSON line: value = cell-updates/s of the FP64 step (the
+reference's arithmetic), with the FP32-WENO form, the Poisson solve time,
+the roofline of the dominant kernel (the fused tendency kernel) and the CPU
+oracle timed on the host cores beside it.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for p in (ROOT, os.path.join(ROOT, "aeolus.jl_amd")):
+    if p not in sys.path:
+        sys.path.insert(0, p)
+
+import numpy as np  # noqa: E402
+
+METRIC = "cell-updates/s per RK3 step + Poisson ms to 1e-8 residual (learned vs MG-PCG)"
+HBM_PEAK_GBS = 8000.0
+NX, NY, NZ = 256, 256, 128
+DT = 0.5
+
+
+def tendency_bytes_per_launch(esz):
+    """Compulsory HBM bytes of one launch of the fused tendency kernel over the
+    interior: reads u, v, w, θ, writes u*, v*, w*, θ' (8 arrays), plus θ0 in
+    the combined substages 2 and 3 (9 arrays); averaged over the 3 launches of
+    a step: (8 + 9 + 9) / 3 arrays."""
+    return NX * NY * NZ * esz * (8 + 9 + 9) / 3.0
+
+
+def read_traffic():
+    """HBM bytes per tendency launch from the committed rocprofv3 --pmc summary
+    (FETCH_SIZE + WRITE_SIZE per dispatch, scaled by the copy-kernel factor)."""
+    path = os.path.join(ROOT, "profiles", "tendency_dcbl_256x256x128_pmc.txt")
+    if not os.path.exists(path):
+        return None
+    vals = {}
+    for line in open(path):
+        if "=" in line and not line.lstrip().startswith("#"):
+            k, v = line.split("=", 1)
+            try:
+                vals[k.strip()] = float(v.split()[0])
+            except ValueError:
+                pass
+    if "traffic_bytes_per_launch" in vals:
+        return vals["traffic_bytes_per_launch"]
+    return None
+
+
+def run_gpu(steps, warmup, precision):
+    import aeolus_hip as hip
+    from aeolus_hip import _lib as L
+    from aeolus_hip import workloads as W
+
+    hip.init(0)
+    g = W.dcbl_grid((NX, NY, NZ))
+    sim = hip.BoussinesqSimulation(g, scheme="weno_z5", precision=precision, **W.DCBL_PHYS)
+    sim.set(*W.dcbl_state(g))
+    lib = L.init()
+    for _ in range(warmup):
+        sim.step_rk3(DT)
+    hip.sync()
+    L.check(lib.aeo_profile_enable(1))
+    L.check(lib.aeo_profile_reset())
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        sim.step_rk3(DT)
+    hip.sync()
+    t1 = time.perf_counter()
+    prof = {}
+    import ctypes
+    for name in ("tendency", "poisson", "halo", "divergence", "correct", "shell"):
+        c = ctypes.c_int()
+        ms = ctypes.c_double()
+        L.check(lib.aeo_profile_get(name.encode(), ctypes.byref(c), ctypes.byref(ms)))
+        prof[name] = (c.value, ms.value)
+    L.check(lib.aeo_profile_enable(0))
+    div = sim.max_divergence()
+    return (t1 - t0) / steps, prof, div
+
+
+def poisson_ms(reps=20):
+    """The direct column solve (exact to round-off) at the workload's size."""
+    import aeolus_hip as hip
+    from aeolus_hip import workloads as W
+    g = W.dcbl_grid((NX, NY, NZ))
+    s = hip.solver_for(g)
+    src = np.random.default_rng(1).normal(size=(NX, NY, NZ))
+    hip.solve_interior(s, src)
+    phi = hip.DeviceArray((NX, NY, NZ))
+    hip.sync()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        hip.solve_source_buffer(phi, s)
+    hip.sync()
+    return (time.perf_counter() - t0) / reps * 1e3
+
+
+def cpu_baseline(threads):
+    """The CPU oracle (FP64, OpenMP) on a bounded sample of the same workload:
+    the full 256x256x128 grid, one warm-up step and two timed RK3 steps."""
+    os.environ["OMP_NUM_THREADS"] = str(threads)
+    from oracle import oracle as orc
+    from aeolus_hip import workloads as W
+    from aeolus_hip.fields import Grid
+    zf, Lz = W.dcbl_faces(NZ)
+    extent = (20.0 * NX, 20.0 * NY, Lz)
+    og = orc.Grid((NX, NY, NZ), extent, 3, zf)
+    g = Grid((NX, NY, NZ), extent, halo=3, zf=zf)
+    u, v, w, th = W.dcbl_state(g)
+    p = og.zeros()
+    st = orc.Stepper(og, scheme="weno_z5", **W.DCBL_PHYS)
+    st.rk3(u, v, w, th, p, DT)
+    n = 2
+    t0 = time.perf_counter()
+    for _ in range(n):
+        st.rk3(u, v, w, th, p, DT)
+    dt = (time.perf_counter() - t0) / n
+    return NX * NY * NZ / dt, dt
+
+
+def profile_pass(steps, warmup):
+    import ctypes
+    import aeolus_hip as hip
+    from aeolus_hip import _lib as L
+    t, _, _ = run_gpu(steps, warmup, 0)
+    nbytes = 1 << 30
+    a = hip.DeviceArray((nbytes // 8,))
+    b = hip.DeviceArray((nbytes // 8,))
+    L.check(L.init().aeo_memcpy_d2d(ctypes.c_void_p(b.ptr), ctypes.c_void_p(a.ptr), nbytes))
+    hip.sync()
+    print(json.dumps({"profile_pass": True, "ms_per_step": t * 1e3, "copy_bytes": nbytes}), flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--profile-pass", action="store_true",
+                    help="a short FP64 run for rocprofv3 counter passes: the steps, then one 1 GiB copy kernel "
+                         "(the calibration of FETCH_SIZE / WRITE_SIZE)")
+    args = ap.parse_args()
+    if args.profile_pass:
+        profile_pass(args.steps, args.warmup)
+        return
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus})",
+              file=sys.stderr)
+        sys.exit(2)
+    rank = int(os.environ.get("RANK", "0"))
+
+    t64, prof64, div64 = run_gpu(args.steps, args.warmup, 0)
+    t32, prof32, div32 = run_gpu(args.steps, args.warmup, 1)
+    pms = poisson_ms()
+    cells = NX * NY * NZ
+
+    n_t, ms_t = prof64["tendency"]
+    avg_launch_s = ms_t / max(n_t, 1) / 1e3
+    alg = tendency_bytes_per_launch(8)
+    achieved = alg / avg_launch_s / 1e9
+    traffic = read_traffic()
+
+    cpu = None
+    if rank == 0 and not args.no_cpu:
+        threads = args.cpu_threads or min(os.cpu_count() or 1, 16)
+        rate, tstep = cpu_baseline(threads)
+        cpu = {"value": rate, "unit": "cell-updates/s", "cores": threads, "kind": "port",
+               "sample": f"CPU oracle (FP64, OpenMP), DCBL 256x256x128 WENO-Z5 RK3 step, 1 warm-up + 2 timed "
+                         f"steps, {tstep:.2f} s/step"}
+
+    breakdown = {k: round(v[1] / args.steps, 4) for k, v in prof64.items()}
+    out = {
+        "metric": METRIC,
+        "value": cells / t64,
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": t64 * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "dcbl 256x256x128 stretched-z, WENO-Z5 momentum+theta, SSP-RK3, "
+                               "FFT-xy + FP64 Thomas-z projection per substage",
+                   "grid": [NX, NY, NZ], "dt": DT, "parallelism": "single-gpu"},
+        "fp32_weno": {"value": cells / t32, "ms_per_step": t32 * 1e3, "dtype": "f32 tendencies, f64 projection"},
+        "poisson_ms": {"direct_column_256x256x128": pms, "mgpcg": None, "learned_fcg": None},
+        "max_divergence_after_step": div64,
+        "ms_per_step_by_kernel_f64": breakdown,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": "adv_march_kernel<MODE_STEP, WENO-Z5, p=2, double> (fused tendency+predictor)",
+                     "avg_launch_ms": avg_launch_s * 1e3, "algorithmic_bytes_per_launch": alg},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

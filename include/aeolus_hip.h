@@ -68,6 +68,9 @@ int aeo_free(void *dptr);
 int aeo_memcpy_h2d(void *dst, const void *src, size_t bytes);
 int aeo_memcpy_d2h(void *dst, const void *src, size_t bytes);
 int aeo_memset(void *dptr, int value, size_t bytes);
+/* device-to-device copy by a plain streaming copy kernel (16-byte aligned);
+ * also the reference kernel for HBM-traffic counter calibration */
+int aeo_memcpy_d2d(void *dst, const void *src, size_t bytes);
 
 /* ---- grids ---------------------------------------------------------------
  * RegularGrid (src/Grids.jl:98-140) when zf_interior is NULL (uniform dz);
@@ -109,6 +112,31 @@ int aeo_poisson_column_create(int Nx, int Ny, int Nz, double dx, double dy, int 
                               const double *lower, const double *upper, const double *hweight,
                               const double *measure, const double *inv_coupling, const double *gauge,
                               aeo_poisson **h);
+/* MGPreconditionedCG(mesh; coeff_x/y/z, rhs_scale, nu1, nu2, tol, maxit)
+ * (src/Multigrid.jl:879-972): pcg! with one V-cycle as the preconditioner,
+ * the mean projected, FP64 outer iteration. cx (nx+1, ny, nz), cy (nx,
+ * ny+1, nz), cz (nx, ny, nz+1) are host face coefficients of ∇·(c∇φ) (all
+ * NULL: the constant Laplacian); rhs_scale (nz, or NULL) scales the source
+ * rows (the stretched operator, :632-658). smoother 0: red-black
+ * Gauss-Seidel (the reference's), 1: damped Jacobi with weight omega.
+ * vcycle_precision AEO_F64 or AEO_F32 (the V-cycle's arithmetic). BCs:
+ * AEO_PERIODIC or AEO_NOFLUX per axis. stats: (CG iterations, ‖r‖/‖b‖,
+ * status). */
+int aeo_poisson_mgpcg_create(int nx, int ny, int nz, double dx, double dy, double dz, int bc_x, int bc_y, int bc_z,
+                             const double *cx, const double *cy, const double *cz, const double *rhs_scale, int nu1,
+                             int nu2, int smoother, double omega, int vcycle_precision, double tol, int maxit,
+                             aeo_poisson **h);
+/* MultigridPoissonSolver(mesh; ..., tol, maxcycles) (src/Multigrid.jl:531-861):
+ * V-cycles (red-black Gauss-Seidel, FP64) until ‖r‖/‖b‖ < tol.
+ * stats: (cycles, relative residual, status). */
+int aeo_poisson_mg_create(int nx, int ny, int nz, double dx, double dy, double dz, int bc_x, int bc_y, int bc_z,
+                          const double *cx, const double *cy, const double *cz, const double *rhs_scale, int nu1,
+                          int nu2, double tol, int maxcycles, aeo_poisson **h);
+/* mg_level_count (src/Multigrid.jl:628): the hierarchy depth (1 for direct solvers) */
+int aeo_poisson_levels(aeo_poisson *h, int *levels);
+/* _apply_fine! (src/Multigrid.jl:911-917): out = A x on interior-sized FP64
+ * device arrays (multigrid solvers) */
+int aeo_poisson_apply(aeo_poisson *h, double *out, const double *x);
 /* source_buffer(solver) (src/PressureSolvers.jl:57-69): the device FP64
  * interior-sized (Nx, Ny, Nz) array the caller may write the source into */
 int aeo_poisson_source_buffer(aeo_poisson *h, double **dptr);

@@ -22,7 +22,7 @@ PERIODIC, NOFLUX, WALL = 0, 1, 2
 CENTERED, UPWIND1, WENO5, WENO_JS5, WENO_Z5, WENO_Z7 = 0, 1, 2, 3, 4, 5
 SCHEMES = {"centered": CENTERED, "upwind": UPWIND1, "weno5": WENO5, "weno_js5": WENO_JS5, "weno_z5": WENO_Z5,
            "weno_z7": WENO_Z7}
-BCS = {"periodic": PERIODIC, "noflux": NOFLUX, "wall": WALL}
+BCS = {"periodic": PERIODIC, "noflux": NOFLUX, "neumann": NOFLUX, "wall": WALL}
 
 
 class OrcGrid(ctypes.Structure):

@@ -1,0 +1,178 @@
+"""GPU parity of the multigrid solvers against the CPU oracle.
+
+The reductions of the GPU (a fixed tree) and of the oracle (the reference's
+8-lane plane order) differ in rounding, so CG iterates agree to round-off:
+the FP64 MG-PCG solution within 1e-9 relative of the oracle's, the same
+iteration count within one; the FP32 V-cycle converges to the same
+tolerance and its solution is stated against the FP64 oracle.
+"""
+import numpy as np
+import pytest
+
+from helpers import rel_err
+
+pytestmark = pytest.mark.gpu
+
+
+def cutcell_coeffs(N, block=((4, 10), (8, 14), (0, 6))):
+    """test/support/multigrid_cases.jl:13-27 (0-based half-open ranges)."""
+    Nx, Ny, Nz = N
+    cx = np.ones((Nx + 1, Ny, Nz)); cy = np.ones((Nx, Ny + 1, Nz)); cz = np.ones((Nx, Ny, Nz + 1))
+    (i0, i1), (j0, j1), (k0, k1) = block
+    cx[i0:i1 + 1, j0:j1, k0:k1] = 0
+    cy[i0:i1, j0:j1 + 1, k0:k1] = 0
+    cz[i0:i1, j0:j1, k0:k1 + 1] = 0
+    cx[3, :, 0:7] *= 0.3
+    cy[:, 7, 0:7] *= 0.6
+    cz[:, :, 7] *= 0.8
+    return [np.asfortranarray(c) for c in (cx, cy, cz)]
+
+
+def cutcell_source(N, seed, block=((4, 10), (8, 14), (0, 6))):
+    b = np.random.default_rng(seed).normal(size=N)
+    solid = np.zeros(N, bool)
+    (i0, i1), (j0, j1), (k0, k1) = block
+    solid[i0:i1, j0:j1, k0:k1] = True
+    b[solid] = 0
+    b[~solid] -= b[~solid].sum() / (~solid).sum()
+    return np.asfortranarray(b)
+
+
+def oracle_mgpcg(orc, N, spacing, bcs, coeff, rhs_scale, src, tol, maxit, smoother=0, omega=0.8):
+    import ctypes
+    L = orc.lib()
+    cs = [None] * 3 if coeff is None else [orc.ptr(np.asfortranarray(c)) for c in coeff]
+    rs = None if rhs_scale is None else orc.ptr(np.ascontiguousarray(rhs_scale))
+    h = L.orc_mg_create(*N, *spacing, *[orc.BCS[b] for b in bcs], *cs, rs, 2, 2, 4, smoother, omega)
+    x = np.zeros(N, order="F")
+    it = ctypes.c_int()
+    res = ctypes.c_double()
+    bn = ctypes.c_double()
+    st = L.orc_mgpcg_solve(h, orc.ptr(x), orc.ptr(np.asfortranarray(src)), tol, maxit, ctypes.byref(it),
+                           ctypes.byref(res), ctypes.byref(bn))
+    nlev = L.orc_mg_levels(h)
+    L.orc_mg_destroy(h)
+    return x, it.value, res.value / bn.value, st, nlev
+
+
+CASES = {
+    "uniform(periodic,neumann)": dict(N=(32, 32, 16), spacing=(1000 / 32, 1000 / 32, 200 / 16),
+                                      bcs=("periodic", "periodic", "neumann"), coeff=None),
+    "uniform(neumann,neumann)": dict(N=(32, 16, 16), spacing=(1.0, 1.0, 1.0), bcs=("neumann",) * 3, coeff=None),
+    "cutcell": dict(N=(16, 16, 16), spacing=(1000 / 16, 1000 / 16, 200 / 16),
+                    bcs=("periodic", "periodic", "neumann"), coeff="cutcell"),
+}
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_mgpcg_fp64_parity(hip, orc, name):
+    c = CASES[name]
+    N = c["N"]
+    coeff = cutcell_coeffs(N) if c["coeff"] == "cutcell" else None
+    src = cutcell_source(N, 303) if coeff is not None else np.asfortranarray(np.random.default_rng(303).normal(size=N))
+    tol = 1e-10
+    x_ref, it_ref, rel_ref, st_ref, nlev = oracle_mgpcg(orc, N, c["spacing"], c["bcs"], coeff, None, src, tol, 200)
+    s = hip.MGPreconditionedCG(N, c["spacing"], bcs=c["bcs"], coeff=coeff, tol=tol, maxit=200, vcycle="f64")
+    assert s.levels() == nlev
+    x = hip.solve_interior(s, src)
+    it, rel = hip.solver_stats(s)
+    print(f"{name}: oracle {it_ref} its rel {rel_ref:.2e}; gpu {it} its rel {rel:.2e}")
+    assert st_ref == 0 and s.last_status == "converged"
+    assert abs(it - it_ref) <= 1
+    assert rel <= tol
+    assert rel_err(x, x_ref) < 1e-8
+
+
+def test_mgpcg_stretched_fp64_parity(hip, orc):
+    from aeolus_hip.fields import faces_geometric
+    N = (32, 32, 32)
+    zf = faces_geometric(32, 0.0, 2000.0, 1.04)
+    g = hip.Grid(N, (1000.0, 1000.0, 2000.0), halo=3, zf=zf)
+    size, spacing, coeff, sc = hip.stretched_operator(g)
+    src = np.asfortranarray(np.random.default_rng(7).normal(size=N))
+    x_ref, it_ref, rel_ref, _, _ = oracle_mgpcg(orc, size, spacing, ("periodic", "periodic", "neumann"), coeff, sc,
+                                                src, 1e-10, 200)
+    s = hip.MGPreconditionedCG.stretched(g, tol=1e-10, maxit=200)
+    x = hip.solve_interior(s, src)
+    it, rel = hip.solver_stats(s)
+    assert abs(it - it_ref) <= 1 and rel <= 1e-10
+    assert rel_err(x, x_ref) < 1e-8
+
+
+def test_mgpcg_solves_the_stretched_operator(hip, orc):
+    # the stretched MG-PCG and the direct column solver invert the same operator
+    from aeolus_hip.fields import faces_geometric
+    N = (32, 32, 32)
+    zf = faces_geometric(32, 0.0, 2000.0, 1.04)
+    g = hip.Grid(N, (1000.0, 1000.0, 2000.0), halo=3, zf=zf)
+    dzc, _ = g.interior_dz()
+    src = np.random.default_rng(11).normal(size=N)
+    src -= (src * dzc).sum() / (dzc.sum() * N[0] * N[1])
+    src = np.asfortranarray(src)
+    x_direct = hip.solve_interior(hip.solver_for(g), src)
+    x_mg = hip.solve_interior(hip.MGPreconditionedCG.stretched(g, tol=1e-12, maxit=300), src)
+    # gauges differ (volume-weighted vs plain mean): compare up to a constant
+    d = (x_mg - x_direct)
+    assert np.abs(d - d.mean()).max() < 1e-8 * np.abs(x_direct).max()
+
+
+@pytest.mark.parametrize("smoother", ["rbgs", "jacobi"])
+def test_mgpcg_fp32_vcycle(hip, orc, smoother):
+    # FP32 V-cycle inside the FP64 CG: the same tolerance, the solution stated
+    # against the FP64 oracle (the problem's conditioning times 1e-8)
+    N = (32, 32, 16)
+    spacing = (2.0, 2.0, 2.0)
+    coeff = cutcell_coeffs(N)
+    src = cutcell_source(N, 5)
+    x_ref, it_ref, _, _, _ = oracle_mgpcg(orc, N, spacing, ("periodic", "periodic", "neumann"), coeff, None, src,
+                                          1e-12, 400)
+    s = hip.MGPreconditionedCG(N, spacing, coeff=coeff, tol=1e-8, maxit=400, vcycle="f32", smoother=smoother)
+    x = hip.solve_interior(s, src)
+    it, rel = hip.solver_stats(s)
+    err = rel_err(x, x_ref)
+    print(f"fp32 V-cycle ({smoother}): {it} its, rel {rel:.2e}, error vs FP64 oracle {err:.2e} (oracle {it_ref} its)")
+    assert s.last_status == "converged" and rel <= 1e-8
+    assert err < 1e-6
+
+
+def test_mg_cycles_parity(hip, orc):
+    import ctypes
+    N = (32, 32, 16)
+    spacing = (1000 / 32, 1000 / 32, 200 / 16)
+    src = np.asfortranarray(np.random.default_rng(303).normal(size=N))
+    L = orc.lib()
+    h = L.orc_mg_create(*N, *spacing, 0, 0, 1, None, None, None, None, 2, 2, 4, 0, 1.0)
+    x_ref = np.zeros(N, order="F")
+    rel = ctypes.c_double()
+    cyc_ref = L.orc_mg_solve(h, orc.ptr(x_ref), orc.ptr(src), 1e-10, 50, ctypes.byref(rel))
+    L.orc_mg_destroy(h)
+    s = hip.MultigridPoissonSolver(N, spacing, tol=1e-10, maxcycles=50)
+    x = hip.solve_interior(s, src)
+    cyc, r = hip.solver_stats(s)
+    assert abs(cyc - cyc_ref) <= 1 and r < 1e-10
+    assert rel_err(x, x_ref) < 1e-8
+
+
+def test_mgpcg_operator_identity(hip):
+    # the returned x satisfies A x = b − mean(b) to the tolerance, recomputed in FP64
+    N = (64, 64, 32)
+    spacing = (2.0, 2.0, 2.0)
+    coeff = cutcell_coeffs(N, block=((10, 30), (12, 40), (0, 20)))
+    src = cutcell_source(N, 9, block=((10, 30), (12, 40), (0, 20)))
+    s = hip.MGPreconditionedCG(N, spacing, coeff=coeff, tol=1e-8, maxit=400, vcycle="f32", smoother="jacobi")
+    x = hip.solve_interior(s, src)
+    dx = hip.DeviceArray(N)
+    dx.upload(x)
+    out = hip.DeviceArray(N)
+    s.apply(out, dx)
+    Ax = out.download()
+    b = src - src.mean()
+    assert np.linalg.norm(Ax - b) <= 1.01e-8 * np.linalg.norm(b)
+
+
+def test_mgpcg_zero_rhs(hip):
+    N = (16, 16, 8)
+    s = hip.MGPreconditionedCG(N, (1.0, 1.0, 1.0), tol=1e-10)
+    x = hip.solve_interior(s, np.zeros(N))
+    it, rel = hip.solver_stats(s)
+    assert s.last_status == "zero_rhs" and it == 0 and np.all(x == 0)

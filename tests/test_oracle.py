@@ -373,3 +373,100 @@ def test_dcbl_budget_and_walls(orc):
         orc.fill_halos(g, a, loc, ("periodic", "periodic", "wall"))
     assert np.all(w[:, :, 3] == 0)
     assert orc.max_divergence(g, u, v, w) < 1e-12 * max(1.0, np.abs(u).max() / g.dx)
+
+
+# ---------------------------------------------------------------------------
+# multigrid (CPU oracle): the reference's test/multigrid.jl checks
+# ---------------------------------------------------------------------------
+def _orc_mg(orc, N, spacing, bcs, coeff=None, rhs_scale=None, smoother=0):
+    L = orc.lib()
+    cs = [None] * 3 if coeff is None else [orc.ptr(np.asfortranarray(c)) for c in coeff]
+    rs = None if rhs_scale is None else orc.ptr(np.ascontiguousarray(rhs_scale))
+    return L.orc_mg_create(*N, *spacing, *[orc.BCS[b] for b in bcs], *cs, rs, 2, 2, 4, smoother, 0.8)
+
+
+def test_mg_manufactured_and_levels(orc):
+    # test/multigrid.jl:10-31: ≥ 3 levels, recovery of a manufactured solution
+    import ctypes
+    N = (32, 32, 16)
+    sp = (1.0 / 32, 1.0 / 32, 0.5 / 16)
+    h = _orc_mg(orc, N, sp, ("periodic", "periodic", "neumann"))
+    L = orc.lib()
+    assert L.orc_mg_levels(h) >= 3
+    x = (np.arange(32) + 0.5) / 32
+    z = (np.arange(16) + 0.5) * sp[2]
+    X, Y, Z = np.meshgrid(x, x, z, indexing="ij")
+    phi = np.asfortranarray(np.sin(2 * np.pi * X) * np.cos(2 * np.pi * Y) * np.cos(np.pi * Z / 0.5))
+    phi -= phi.mean()
+    b = np.zeros(N, order="F")
+    L.orc_mg_apply(h, orc.ptr(b), orc.ptr(phi))
+    out = np.zeros(N, order="F")
+    rel = ctypes.c_double()
+    cyc = L.orc_mg_solve(h, orc.ptr(out), orc.ptr(b), 1e-12, 60, ctypes.byref(rel))
+    assert rel.value < 1e-12 and cyc < 30
+    assert np.abs(out - phi).max() < 1e-9
+    L.orc_mg_destroy(h)
+
+
+def test_mgpcg_matches_column_solver(orc):
+    # MG-CG and the direct column solver invert the same operator (test/multigrid.jl:23-31)
+    import ctypes
+    N = (16, 16, 8)
+    g = orc.Grid(N, (16.0, 16.0, 8.0), halo=1)
+    src = np.asfortranarray(np.random.default_rng(1).normal(size=N))
+    src -= src.mean()
+    x_dir = orc.stretched_column_solver(g).solve(src)
+    h = _orc_mg(orc, N, (1.0, 1.0, 1.0), ("periodic", "periodic", "neumann"))
+    L = orc.lib()
+    x = np.zeros(N, order="F")
+    it = ctypes.c_int(); res = ctypes.c_double(); bn = ctypes.c_double()
+    st = L.orc_mgpcg_solve(h, orc.ptr(x), orc.ptr(src), 1e-12, 100, ctypes.byref(it), ctypes.byref(res),
+                           ctypes.byref(bn))
+    L.orc_mg_destroy(h)
+    assert st == 0
+    assert rel_err(x, x_dir) < 1e-9
+
+
+def test_mgpcg_rhs_scale_invariance(orc):
+    # test/multigrid.jl:75-83: scaling the source scales the solution
+    import ctypes
+    N = (16, 16, 8)
+    src = np.asfortranarray(np.random.default_rng(2).normal(size=N))
+    sols = []
+    for s in (1.0, 1e3):
+        h = _orc_mg(orc, N, (1.0, 1.0, 1.0), ("periodic", "periodic", "neumann"))
+        x = np.zeros(N, order="F")
+        it = ctypes.c_int(); res = ctypes.c_double(); bn = ctypes.c_double()
+        orc.lib().orc_mgpcg_solve(h, orc.ptr(x), orc.ptr(np.asfortranarray(s * src)), 1e-12, 100, ctypes.byref(it),
+                                  ctypes.byref(res), ctypes.byref(bn))
+        orc.lib().orc_mg_destroy(h)
+        sols.append(x / s)
+    assert rel_err(sols[1], sols[0]) < 1e-8
+
+
+def test_cutcell_geometry_matches_oracle(orc):
+    # the product's numpy builder (host setup) against the oracle's C restatement, bit for bit
+    from aeolus_hip.cutcell import heightmap_geometry
+    N, H = (24, 20, 16), 2
+    g = orc.Grid(N, (48.0, 40.0, 32.0), halo=H)
+    rng = np.random.default_rng(3)
+    h = np.zeros((N[0], N[1]))
+    for _ in range(5):
+        i0, j0 = rng.integers(0, 18), rng.integers(0, 14)
+        h[i0:i0 + rng.integers(2, 6), j0:j0 + rng.integers(2, 6)] = rng.uniform(3.0, 25.0)
+    vol, ve, ax, ay, az = heightmap_geometry(N, (2.0, 2.0, 2.0), H, h, nsub=4)
+    sx, sy, sz = g.storage
+    ii = np.clip(np.arange(1, sx + 1) - H, 1, N[0]) - 1
+    jj = np.clip(np.arange(1, sy + 1) - H, 1, N[1]) - 1
+    hfull = np.asfortranarray(h[np.ix_(ii, jj)])
+    out = [np.zeros(g.storage, order="F") for _ in range(5)]
+    orc.lib().orc_cutcell_heightmap(ctypes_byref(g), orc.ptr(hfull), 4, 1e-3, *[orc.ptr(a) for a in out])
+    for a, b in zip((vol, ve, ax, ay, az), out):
+        assert np.array_equal(a, b)
+    # a flat roof at 5.0 m cuts the cells [4, 6) m to half volume
+    assert vol[H + 0, H + 0, H + 2] in (0.0, 0.5, 1.0)
+
+
+def ctypes_byref(g):
+    import ctypes
+    return ctypes.byref(g.c)
