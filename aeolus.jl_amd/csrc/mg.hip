@@ -120,13 +120,13 @@ __global__ void finalize_multi_kernel(double *__restrict__ out, const double *__
 
 // ---- V-cycle kernels --------------------------------------------------------
 // one colour of the red-black Gauss-Seidel sweep (_rbgs_body!, src/Multigrid.jl:307-337):
-// cell (i, j, k) has colour (i + j + k + 1) & 1 in 0-based interior indices
-// (the reference's (i + j + k) & 1 in 1-based unit-halo storage)
+// colour c updates the cells with (i + j + k + c) even in 0-based interior
+// indices (the reference's i = 2ih + ((j + k + c) & 1) in unit-halo storage)
 template <typename T>
 __global__ void rbgs_kernel(LevelD<T> L, int colour) {
     const int ih = blockIdx.x * blockDim.x + threadIdx.x;
     const int j = blockIdx.y, k = blockIdx.z;
-    const int i = 2 * ih + ((j + k + 1 + colour) & 1);
+    const int i = 2 * ih + ((j + k + colour) & 1);
     if (ih >= (L.nx + 1) / 2 || i >= L.nx) return;
     const T *f = L.phi;
     const T xp = f[lidx(L, wrapc(i + 1, L.nx, L.bcx), j, k)], xm = f[lidx(L, wrapc(i - 1, L.nx, L.bcx), j, k)];
@@ -297,7 +297,7 @@ __global__ void __launch_bounds__(1024) coarsest_kernel(LevelD<T> L, int nsweeps
         for (int colour = 0; colour <= 1; colour++) {
             for (int t = threadIdx.x; t < n; t += blockDim.x) {
                 const int i = t % L.nx, j = (t / L.nx) % L.ny, k = t / (L.nx * L.ny);
-                if (((i + j + k + 1) & 1) != colour) continue;
+                if (((i + j + k) & 1) != colour) continue;
                 const T d = diagA(S, i, j, k);
                 if (L.cx && fabs(d) < Eps<T>::v) continue;
                 // (b − off)/diag with off the neighbour sum of the reference's form

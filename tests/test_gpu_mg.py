@@ -125,11 +125,16 @@ def test_mgpcg_fp32_vcycle(hip, orc, smoother):
     coeff = cutcell_coeffs(N)
     src = cutcell_source(N, 5)
     x_ref, it_ref, _, _, _ = oracle_mgpcg(orc, N, spacing, ("periodic", "periodic", "neumann"), coeff, None, src,
-                                          1e-12, 400)
+                                          1e-12, 400, smoother=int(smoother == "jacobi"))
     s = hip.MGPreconditionedCG(N, spacing, coeff=coeff, tol=1e-8, maxit=400, vcycle="f32", smoother=smoother)
     x = hip.solve_interior(s, src)
     it, rel = hip.solver_stats(s)
-    err = rel_err(x, x_ref)
+    # the fully blocked cells' values are not fixed by the equations (only by
+    # the preconditioner's rounding): compare the fluid cells up to a constant
+    fluid = np.ones(N, bool)
+    fluid[4:10, 8:14, 0:6] = False
+    d = x[fluid] - x_ref[fluid]
+    err = np.abs(d - d.mean()).max() / np.abs(x_ref[fluid]).max()
     print(f"fp32 V-cycle ({smoother}): {it} its, rel {rel:.2e}, error vs FP64 oracle {err:.2e} (oracle {it_ref} its)")
     assert s.last_status == "converged" and rel <= 1e-8
     assert err < 1e-6
@@ -176,3 +181,27 @@ def test_mgpcg_zero_rhs(hip):
     x = hip.solve_interior(s, np.zeros(N))
     it, rel = hip.solver_stats(s)
     assert s.last_status == "zero_rhs" and it == 0 and np.all(x == 0)
+
+
+@pytest.mark.parametrize("coeff", ["uniform", "cutcell"])
+def test_single_vcycle_parity(hip, orc, coeff):
+    # one V-cycle from zero (the preconditioner itself): the red-black colour
+    # order, transfers and coarsest solve of the oracle, to round-off
+    import ctypes
+    N = (16, 16, 16)
+    spacing = (1000 / 16, 1000 / 16, 200 / 16)
+    cs = cutcell_coeffs(N) if coeff == "cutcell" else None
+    src = cutcell_source(N, 17) if cs is not None else np.asfortranarray(np.random.default_rng(17).normal(size=N))
+    L = orc.lib()
+    cp = [None] * 3 if cs is None else [orc.ptr(c) for c in cs]
+    h = L.orc_mg_create(*N, *spacing, 0, 0, 1, *cp, None, 2, 2, 4, 0, 1.0)
+    x_ref = np.zeros(N, order="F")
+    rel = ctypes.c_double()
+    assert L.orc_mg_solve(h, orc.ptr(x_ref), orc.ptr(src), 1e-30, 1, ctypes.byref(rel)) == 1
+    L.orc_mg_destroy(h)
+    s = hip.MultigridPoissonSolver(N, spacing, coeff=cs, tol=1e-30, maxcycles=1)
+    x = hip.solve_interior(s, src)
+    cyc, r = hip.solver_stats(s)
+    assert cyc == 1
+    assert rel_err(x, x_ref) < 1e-12
+    assert abs(r - rel.value) < 1e-10 * rel.value

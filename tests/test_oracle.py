@@ -5,6 +5,7 @@ coefficient, a transposed stencil, a wrong metric, a sign) fails it. These
 run on the CPU only (no GPU).
 """
 import math
+import os
 
 import numpy as np
 import pytest
@@ -470,3 +471,29 @@ def test_cutcell_geometry_matches_oracle(orc):
 def ctypes_byref(g):
     import ctypes
     return ctypes.byref(g.c)
+
+
+# ---------------------------------------------------------------------------
+# golden vectors: WENO face values in exact rational arithmetic from the
+# polynomial / smoothness-indicator definitions (tests/golden/make_golden.py)
+# ---------------------------------------------------------------------------
+def test_weno_faces_match_exact_rational_golden(orc):
+    import json
+    path = os.path.join(os.path.dirname(__file__), "golden", "weno_faces.json")
+    cases = json.load(open(path))["cases"]
+    L = orc.lib()
+    worst = 0.0
+    for c in cases:
+        v = [float.fromhex(x) for x in c["v7"]]
+        scale = max(abs(x) for x in v)
+        got = {
+            "weno_z5_p1": L.orc_weno_z5_left(*v[1:6], 1),
+            "weno_z5_p2": L.orc_weno_z5_left(*v[1:6], 2),
+            "weno_z7_p2": L.orc_weno_z7_left(*v, 2),
+            "weno_js5": L.orc_weno_js5_left(*v[1:6], 1e-6),
+        }
+        for k, g in got.items():
+            e = abs(g - float.fromhex(c[k])) / scale
+            worst = max(worst, e)
+            assert e < 1e-13, (k, v, g, float.fromhex(c[k]))
+    print(f"worst WENO face error vs exact rational: {worst:.2e} (relative to the stencil's max)")
