@@ -123,74 +123,67 @@ __device__ __forceinline__ const T *field_ptr(const Fields<T> &f) {
     else return f.s;
 }
 
-template <int F, int S, int P, typename T>
-__device__ __forceinline__ T flux_x(const Fields<T> &f, const Metrics<T> &m, int c, int j, int k) {
-    const int sx = m.sx, sy = m.sy;
-    T vel;
-    if constexpr (F == F_S) vel = f.u[lin(c + 1, j, k, sx, sy)];
-    else if constexpr (F == F_U) vel = (f.u[lin(c, j, k, sx, sy)] + f.u[lin(c + 1, j, k, sx, sy)]) / T(2);
-    else if constexpr (F == F_V) vel = (f.u[lin(c + 1, j - 1, k, sx, sy)] + f.u[lin(c + 1, j, k, sx, sy)]) / T(2);
-    else {
-        T lo = f.u[lin(c + 1, j, k - 1, sx, sy)], hi = f.u[lin(c + 1, j, k, sx, sy)];
-        vel = m.stretched ? m.m_zflux(k, lo, hi) : (lo + hi) / T(2);
-    }
-    const T *d = field_ptr<F>(f);
-    return vel * face_value<S, P, T>(d, lin(0, j, k, sx, sy), 1u, c, vel);
-}
-
-template <int F, int S, int P, typename T>
-__device__ __forceinline__ T flux_y(const Fields<T> &f, const Metrics<T> &m, int i, int c, int k) {
-    const int sx = m.sx, sy = m.sy;
-    T vel;
-    if constexpr (F == F_S) vel = f.v[lin(i, c + 1, k, sx, sy)];
-    else if constexpr (F == F_U) vel = (f.v[lin(i - 1, c + 1, k, sx, sy)] + f.v[lin(i, c + 1, k, sx, sy)]) / T(2);
-    else if constexpr (F == F_V) vel = (f.v[lin(i, c, k, sx, sy)] + f.v[lin(i, c + 1, k, sx, sy)]) / T(2);
-    else {
-        T lo = f.v[lin(i, c + 1, k - 1, sx, sy)], hi = f.v[lin(i, c + 1, k, sx, sy)];
-        vel = m.stretched ? m.m_zflux(k, lo, hi) : (lo + hi) / T(2);
-    }
-    const T *d = field_ptr<F>(f);
-    return vel * face_value<S, P, T>(d, lin(i, 0, k, sx, sy), (uint32_t)sx, c, vel);
-}
-
-template <int F, int S, int P, typename T>
-__device__ __forceinline__ T flux_z(const Fields<T> &f, const Metrics<T> &m, int i, int j, int c) {
-    const int sx = m.sx, sy = m.sy;
-    T vel;
-    if constexpr (F == F_S) vel = f.w[lin(i, j, c + 1, sx, sy)];
-    else if constexpr (F == F_U) vel = (f.w[lin(i - 1, j, c + 1, sx, sy)] + f.w[lin(i, j, c + 1, sx, sy)]) / T(2);
-    else if constexpr (F == F_V) vel = (f.w[lin(i, j - 1, c + 1, sx, sy)] + f.w[lin(i, j, c + 1, sx, sy)]) / T(2);
-    else vel = (f.w[lin(i, j, c, sx, sy)] + f.w[lin(i, j, c + 1, sx, sy)]) / T(2);
-    const T *d = field_ptr<F>(f);
-    return vel * face_value<S, P, T>(d, lin(i, j, 0, sx, sy), (uint32_t)sx * (uint32_t)sy, c, vel);
-}
-
-// _lap_at, src/FiniteVolume.jl:271-283
+// byte strides of the storage along x, y, z
+struct BStride {
+    uint32_t x, y, z;
+};
 template <typename T>
-__device__ __forceinline__ T lap_at(const T *__restrict__ f, const Metrics<T> &m, int i, int j, int k, int loc) {
-    const int sx = m.sx, sy = m.sy;
-    T c = f[lin(i, j, k, sx, sy)];
-    return (((f[lin(i + 1, j, k, sx, sy)] - T(2) * c) + f[lin(i - 1, j, k, sx, sy)]) * m.idx2 +
-            ((f[lin(i, j + 1, k, sx, sy)] - T(2) * c) + f[lin(i, j - 1, k, sx, sy)]) * m.idy2) +
-           m.m_lapz(loc, k, f[lin(i, j, k - 1, sx, sy)], c, f[lin(i, j, k + 1, sx, sy)]);
+__device__ __forceinline__ BStride bstrides(const Metrics<T> &m) {
+    return BStride{(uint32_t)sizeof(T), (uint32_t)sizeof(T) * (uint32_t)m.sx,
+                   (uint32_t)sizeof(T) * (uint32_t)m.sx * (uint32_t)m.sy};
+}
+
+// The flux of field F through the minus face (west, south or bottom for DIR
+// = 0, 1, 2) of the cell at byte offset cb on level k.
+template <int DIR, int F, int S, int P, typename T>
+__device__ __forceinline__ T flux_minus(const Fields<T> &f, const Metrics<T> &m, uint32_t cb, int k,
+                                        const BStride &b) {
+    const uint32_t bd = DIR == 0 ? b.x : DIR == 1 ? b.y : b.z;
+    const T *vp = DIR == 0 ? f.u : DIR == 1 ? f.v : f.w;  // the velocity normal to the face
+    T vel;
+    if constexpr (F == F_S) {
+        vel = ldo(vp, cb);
+    } else if constexpr (F == DIR) {
+        vel = (ldo(vp, cb - bd) + ldo(vp, cb)) / T(2);
+    } else {
+        // the normal velocity averaged along the transported component's axis
+        const uint32_t bf = F == F_U ? b.x : F == F_V ? b.y : b.z;
+        const T lo = ldo(vp, cb - bf), hi = ldo(vp, cb);
+        if constexpr (F == F_W) vel = m.m_zflux(k, lo, hi);
+        else vel = (lo + hi) / T(2);
+    }
+    constexpr int R = SchemeInfo<S>::R;
+    return vel * face_value<S, P, T>(field_ptr<F>(f), cb - (uint32_t)R * bd, bd, vel);
+}
+
+// _lap_at, src/FiniteVolume.jl:271-283 (f at byte offset cb, level k)
+template <typename T>
+__device__ __forceinline__ T lap_at(const T *__restrict__ f, const Metrics<T> &m, uint32_t cb, int k, int loc,
+                                    const BStride &b) {
+    const T c = ldo(f, cb);
+    return (((ldo(f, cb + b.x) - T(2) * c) + ldo(f, cb - b.x)) * m.idx2 +
+            ((ldo(f, cb + b.y) - T(2) * c) + ldo(f, cb - b.y)) * m.idy2) +
+           m.m_lapz(loc, k, ldo(f, cb - b.z), c, ldo(f, cb + b.z));
 }
 
 // ---------------------------------------------------------------------------
 // The marching advection kernel.
 //
 // A block is 64 x BY threads: each wave is one j-row of 64 cells along x,
-// and every thread marches up a chunk of KC levels. Each face flux is
+// and every thread marches up a chunk of KC <= KCM levels. Each face flux is
 // computed once (the reference evaluates every interior face from both of
 // its cells; both evaluations are the same expression on the same values, so
 // sharing one evaluation gives the same bits):
 //   z: the top-face flux of level k is the bottom-face flux of level k+1
 //      (carried in a register);
 //   x: each lane computes its west face, the east face comes from the next
-//      lane; the east face of a wave's last cell is computed for all KC
-//      levels of the chunk in one batched pass before the march (lane l does
-//      level k0 + l) and kept in LDS;
+//      lane; the east faces of a wave's last cell for the chunk's levels are
+//      computed before the march (lane l does level k0 + l) and kept in LDS;
 //   y: each wave computes its south faces, the north face comes from the
-//      next wave through LDS; the block's top row computes its north face.
+//      next wave through LDS; the north faces of the block's top row for the
+//      chunk's levels are computed before the march by all the block's
+//      threads (thread (lane, ty) does levels k0 + ty, k0 + ty + BY, ...),
+//      so that no wave does more face evaluations per level than the others.
 // ---------------------------------------------------------------------------
 enum Mode { MODE_SCALAR = 0, MODE_MOMENTUM = 1, MODE_STEP = 2 };
 
@@ -206,147 +199,160 @@ struct MarchArgs {
     T nu, dt, theta0, coef, fb, ft;
     T ca, cb;
     int combine, has_visc, has_buoy, has_sflux;
-    int kc;  // levels per chunk (<= 64)
+    int kc;  // levels per chunk (<= KCM)
 };
 
-constexpr int MW = 64;  // lanes per wave (gfx950)
+constexpr int MW = 64;   // lanes per wave (gfx950)
+constexpr int KCM = 16;  // levels per chunk
 
 template <int MODE> struct ModeFields { static constexpr int lo = F_U, hi = F_S; };
 template <> struct ModeFields<MODE_SCALAR> { static constexpr int lo = F_S, hi = F_S; };
 template <> struct ModeFields<MODE_MOMENTUM> { static constexpr int lo = F_U, hi = F_W; };
 
+// flux_minus with the field as a (loop-unrolled, hence constant) argument
+template <int DIR, int S, int P, typename T>
+__device__ __forceinline__ T flux_f(int F, const Fields<T> &f, const Metrics<T> &m, uint32_t co, int k,
+                                    const BStride &b) {
+    if (F == F_U) return flux_minus<DIR, F_U, S, P, T>(f, m, co, k, b);
+    if (F == F_V) return flux_minus<DIR, F_V, S, P, T>(f, m, co, k, b);
+    if (F == F_W) return flux_minus<DIR, F_W, S, P, T>(f, m, co, k, b);
+    return flux_minus<DIR, F_S, S, P, T>(f, m, co, k, b);
+}
+
 template <int MODE, int S, int P, typename T, int BY, int STR>
 __global__ void __launch_bounds__(MW *BY) adv_march_kernel(MarchArgs<T> a) {
     constexpr int F0 = ModeFields<MODE>::lo, F1 = ModeFields<MODE>::hi, NF = F1 - F0 + 1;
+    constexpr uint32_t ES = sizeof(T);
     // the grid kind as a compile-time constant: the metric accessors fold to
     // UniformZ or LevelZ
     Metrics<T> m = a.m;
     m.stretched = STR;
+    const BStride b = bstrides(m);
     const int H = m.H;
     const int lane = threadIdx.x, ty = threadIdx.y;
-    const int i0 = H + blockIdx.x * MW;
-    const int i = i0 + lane, j = H + blockIdx.y * BY + ty;
-    const int k0 = H + blockIdx.z * a.kc;
+    const int bx = blockIdx.x, by = blockIdx.y, bz = blockIdx.z;
+    const int i0 = H + bx * MW;
+    const int jb = H + by * BY;
+    const int i = i0 + lane, j = jb + ty;
+    const int k0 = H + bz * a.kc;
     const int k1 = min(k0 + a.kc, H + m.Nz);
     const int ilast = min(i0 + MW - 1, H + m.Nx - 1);  // the last cell of this wave's row
-    const int jlast = min(H + (int)(blockIdx.y + 1) * BY - 1, H + m.Ny - 1);
+    const int jlast = min(jb + BY - 1, H + m.Ny - 1);  // the block's top row
     const bool row_ok = j < H + m.Ny;
     const bool ok = row_ok && i < H + m.Nx;
 
-    __shared__ T xb[NF][BY][MW];      // east boundary x-flux of each row, per level of the chunk
-    __shared__ T ys[2][NF][BY][MW];   // south y-flux, double buffered over k
+    __shared__ T xb[NF][BY][KCM];  // east boundary x-flux of each row, per level of the chunk
+    __shared__ T yn[NF][KCM][MW];  // north y-flux of the top row, per level of the chunk
+    __shared__ T xs[NF][BY][MW];   // west x-flux of the current level
+    __shared__ T ys[NF][BY][MW];   // south y-flux of the current level
 
-    // batched east-boundary x-fluxes: lane l does level k0 + l
+    // batched east-boundary x-fluxes (the west faces of cell ilast + 1): lane l does level k0 + l
     if (row_ok) {
         const int kb = k0 + lane;
         if (kb < k1) {
+            const uint32_t ce = lin(ilast + 1, j, kb, m.sx, m.sy) * ES;
 #pragma unroll
             for (int F = F0; F <= F1; F++) {
-                T v;
-                if (F == F_U) v = flux_x<F_U, S, P, T>(a.in, m, ilast, j, kb);
-                else if (F == F_V) v = flux_x<F_V, S, P, T>(a.in, m, ilast, j, kb);
-                else if (F == F_W) v = flux_x<F_W, S, P, T>(a.in, m, ilast, j, kb);
-                else v = flux_x<F_S, S, P, T>(a.in, m, ilast, j, kb);
-                xb[F - F0][ty][lane] = v;
+                xb[F - F0][ty][lane] = flux_f<0, S, P, T>(F, a.in, m, ce, kb, b);
+                __builtin_amdgcn_sched_barrier(0);
             }
         }
     }
-
+    // batched north y-fluxes of the top row (the south faces of row jlast + 1):
+    // thread (lane, ty) does levels k0 + ty + BY·n
+    if (i < H + m.Nx) {
+        for (int kk = k0 + ty; kk < k1; kk += BY) {
+            const uint32_t cn = lin(i, jlast + 1, kk, m.sx, m.sy) * ES;
+#pragma unroll
+            for (int F = F0; F <= F1; F++) {
+                yn[F - F0][kk - k0][lane] = flux_f<1, S, P, T>(F, a.in, m, cn, kk, b);
+                __builtin_amdgcn_sched_barrier(0);
+            }
+        }
+    }
     // bottom-face z-flux of the first level
     T fzb[NF];
     if (ok) {
+        const uint32_t c0 = lin(i, j, k0, m.sx, m.sy) * ES;
 #pragma unroll
         for (int F = F0; F <= F1; F++) {
-            if (F == F_U) fzb[F - F0] = flux_z<F_U, S, P, T>(a.in, m, i, j, k0 - 1);
-            else if (F == F_V) fzb[F - F0] = flux_z<F_V, S, P, T>(a.in, m, i, j, k0 - 1);
-            else if (F == F_W) fzb[F - F0] = flux_z<F_W, S, P, T>(a.in, m, i, j, k0 - 1);
-            else fzb[F - F0] = flux_z<F_S, S, P, T>(a.in, m, i, j, k0 - 1);
+            fzb[F - F0] = flux_f<2, S, P, T>(F, a.in, m, c0, k0, b);
+            __builtin_amdgcn_sched_barrier(0);
         }
     }
     __syncthreads();
 
-    int buf = 0;
     for (int k = k0; k < k1; k++) {
-        T fxw[NF], fxe[NF], fys[NF], fyn[NF], fzt[NF];
+        // the cell's byte offset, made opaque to the compiler: otherwise loop
+        // strength reduction keeps one 64-bit pointer per load site live
+        // across the loop (about 100 VGPRs); every address below is this
+        // offset plus a uniform stride multiple
+        uint32_t co = lin(i, j, k, m.sx, m.sy) * ES;
+        asm volatile("" : "+v"(co));
+        // the faces of this level, one at a time: west and south fluxes to
+        // LDS, the top z-flux in a register (the next level's bottom flux)
+        T fzt[NF];
         if (ok) {
 #pragma unroll
             for (int F = F0; F <= F1; F++) {
                 const int q = F - F0;
-                if (F == F_U) { fxw[q] = flux_x<F_U, S, P, T>(a.in, m, i - 1, j, k); fys[q] = flux_y<F_U, S, P, T>(a.in, m, i, j - 1, k); fzt[q] = flux_z<F_U, S, P, T>(a.in, m, i, j, k); }
-                else if (F == F_V) { fxw[q] = flux_x<F_V, S, P, T>(a.in, m, i - 1, j, k); fys[q] = flux_y<F_V, S, P, T>(a.in, m, i, j - 1, k); fzt[q] = flux_z<F_V, S, P, T>(a.in, m, i, j, k); }
-                else if (F == F_W) { fxw[q] = flux_x<F_W, S, P, T>(a.in, m, i - 1, j, k); fys[q] = flux_y<F_W, S, P, T>(a.in, m, i, j - 1, k); fzt[q] = flux_z<F_W, S, P, T>(a.in, m, i, j, k); }
-                else { fxw[q] = flux_x<F_S, S, P, T>(a.in, m, i - 1, j, k); fys[q] = flux_y<F_S, S, P, T>(a.in, m, i, j - 1, k); fzt[q] = flux_z<F_S, S, P, T>(a.in, m, i, j, k); }
-                ys[buf][q][ty][lane] = fys[q];
-                // one field at a time: the scheduler does not interleave the
-                // reconstructions of different fields (shorter live ranges,
-                // fewer VGPRs, more waves per SIMD)
+                xs[q][ty][lane] = flux_f<0, S, P, T>(F, a.in, m, co, k, b);
+                __builtin_amdgcn_sched_barrier(0);
+                ys[q][ty][lane] = flux_f<1, S, P, T>(F, a.in, m, co, k, b);
+                __builtin_amdgcn_sched_barrier(0);
+                fzt[q] = flux_f<2, S, P, T>(F, a.in, m, co + b.z, k + 1, b);
                 __builtin_amdgcn_sched_barrier(0);
             }
         }
-#pragma unroll
-        for (int q = 0; q < NF; q++) {
-            T nb = __shfl_down(ok ? fxw[q] : T(0), 1);
-            fxe[q] = (i == ilast) ? xb[q][ty][k - k0] : nb;
-        }
         __syncthreads();
         if (ok) {
-            if (j < jlast) {
-#pragma unroll
-                for (int q = 0; q < NF; q++) fyn[q] = ys[buf][q][ty + 1][lane];
-            } else {
-#pragma unroll
-                for (int F = F0; F <= F1; F++) {
-                    if (F == F_U) fyn[F - F0] = flux_y<F_U, S, P, T>(a.in, m, i, j, k);
-                    else if (F == F_V) fyn[F - F0] = flux_y<F_V, S, P, T>(a.in, m, i, j, k);
-                    else if (F == F_W) fyn[F - F0] = flux_y<F_W, S, P, T>(a.in, m, i, j, k);
-                    else fyn[F - F0] = flux_y<F_S, S, P, T>(a.in, m, i, j, k);
-                }
-            }
-            const uint32_t c = lin(i, j, k, m.sx, m.sy);
 #pragma unroll
             for (int F = F0; F <= F1; F++) {
                 const int q = F - F0;
+                const T fxw = xs[q][ty][lane], fys = ys[q][ty][lane];
+                const T fxe = (i == ilast) ? xb[q][ty][k - k0] : xs[q][ty][lane + 1 < MW ? lane + 1 : lane];
+                const T fyn = (j < jlast) ? ys[q][ty + 1 < BY ? ty + 1 : ty][lane] : yn[q][k - k0][lane];
                 const T idzF = (F == F_W) ? m.m_idzf(k) : m.m_idz(k);
-                const T adv = ((fxe[q] - fxw[q]) * m.idx + (fyn[q] - fys[q]) * m.idy) + (fzt[q] - fzb[q]) * idzF;
+                const T adv = ((fxe - fxw) * m.idx + (fyn - fys) * m.idy) + (fzt[q] - fzb[q]) * idzF;
                 if constexpr (MODE == MODE_SCALAR) {
-                    a.ts[c] = a.ts[c] - adv;
+                    sto(a.ts, co, ldo(a.ts, co) - adv);
                 } else if constexpr (MODE == MODE_MOMENTUM) {
-                    if (F == F_U) a.tu[c] = a.tu[c] - adv;
-                    else if (F == F_V) a.tv[c] = a.tv[c] - adv;
-                    else a.tw[c] = a.tw[c] - adv;
+                    T *tp = F == F_U ? a.tu : F == F_V ? a.tv : a.tw;
+                    sto(tp, co, ldo(tp, co) - adv);
                 } else {
                     // the tendency from zero (+ ν∇²f), then buoyancy / surface flux
                     T t = T(0) - adv;
                     const T *fp = F == F_U ? a.in.u : F == F_V ? a.in.v : F == F_W ? a.in.w : a.in.s;
                     const int loc = F == F_U ? LOC_FX : F == F_V ? LOC_FY : F == F_W ? LOC_FZ : LOC_CC;
-                    if (a.has_visc) t = t + a.nu * lap_at<T>(fp, m, i, j, k, loc);
+                    if (a.has_visc) t = t + a.nu * lap_at<T>(fp, m, co, k, loc, b);
                     if (F == F_W && a.has_buoy) {
-                        T tf = m.m_zface(k, a.in.s[lin(i, j, k - 1, m.sx, m.sy)], a.in.s[c]);
+                        const T tf = m.m_zface(k, ldo(a.in.s, co - b.z), ldo(a.in.s, co));
                         t += a.coef * (tf - a.theta0);
                     }
                     if (F == F_S && a.has_sflux) {
                         if (k == H) t += a.fb;
                         if (k == H + m.Nz - 1) t -= a.ft;
                     }
-                    const T x = fp[c];
-                    if (F == F_U) a.uo[c] = x + a.dt * t;
-                    else if (F == F_V) a.vo[c] = x + a.dt * t;
-                    else if (F == F_W) a.wo[c] = x + a.dt * t;
+                    const T x = ldo(fp, co);
+                    if (F == F_U) sto(a.uo, co, x + a.dt * t);
+                    else if (F == F_V) sto(a.vo, co, x + a.dt * t);
+                    else if (F == F_W) sto(a.wo, co, x + a.dt * t);
                     else {
-                        T y = x + a.dt * t;
-                        a.so[c] = a.combine ? a.ca * a.s0[c] + a.cb * y : y;
+                        const T y = x + a.dt * t;
+                        sto(a.so, co, a.combine ? a.ca * ldo(a.s0, co) + a.cb * y : y);
                     }
                 }
                 fzb[q] = fzt[q];
+                __builtin_amdgcn_sched_barrier(0);
             }
         }
-        buf ^= 1;
+        __syncthreads();  // the level's fluxes are read before the next level writes them
     }
 }
 
 template <int MODE, int S, int P, typename T>
 static void launch_march_sp(hipStream_t st, const MarchArgs<T> &a) {
-    constexpr int BY = 4;
+    constexpr int BY = 8;
     const Metrics<T> &m = a.m;
     dim3 grid((m.Nx + MW - 1) / MW, (m.Ny + BY - 1) / BY, (m.Nz + a.kc - 1) / a.kc);
     if (m.stretched)
@@ -381,7 +387,7 @@ int launch_advect(hipStream_t st, const Metrics<T> &m, int mode, int scheme, int
     a.m = m;
     a.in = Fields<T>{u, v, w, s};
     a.tu = tu; a.tv = tv; a.tw = tw; a.ts = ts;
-    a.kc = m.Nz < 32 ? m.Nz : 32;
+    a.kc = m.Nz < KCM ? m.Nz : KCM;
     if (mode == MODE_SCALAR) return launch_march_mode<MODE_SCALAR, T>(st, a, scheme, p);
     return launch_march_mode<MODE_MOMENTUM, T>(st, a, scheme, p);
 }
@@ -406,7 +412,7 @@ int launch_tendency_step(hipStream_t st, const Metrics<T> &m, int scheme, int p,
     a.fb = T(ph.flux_bot * ph.idz_bot);
     a.ft = T(ph.flux_top * ph.idz_top);
     stage_coeffs(stage, &ph, &a.ca, &a.cb, &a.combine);
-    a.kc = m.Nz < 32 ? m.Nz : 32;
+    a.kc = m.Nz < KCM ? m.Nz : KCM;
     return launch_march_mode<MODE_STEP, T>(st, a, scheme, p);
 }
 template int launch_tendency_step<double>(hipStream_t, const Metrics<double> &, int, int, const double *,
