@@ -10,6 +10,7 @@ from . import _lib
 from ._lib import AeolusError, init, sync
 from .advection import advection_scheme, advect_weno_z, momentum_advect_weno_z, required_halo
 from .fields import DeviceArray, Grid, faces_geometric, faces_uniform, fill_halos
+from . import learned, workloads
 from .pressure import (ColumnTridiagonalSolver, FFTPoissonSolver, MGPreconditionedCG, MultigridPoissonSolver, solve,
                        solve_interior, solve_source_buffer, solver_for, solver_stats, source_buffer, stretched_operator)
 from .simulation import BoussinesqSimulation

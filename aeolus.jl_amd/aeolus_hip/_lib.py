@@ -80,6 +80,15 @@ SIGNATURES = {
     "aeo_step_euler": [_vp, _vp, _vp, _vp, _vp, _vp, _d],
     "aeo_max_divergence": [_vp, _i, _vp, _vp, _vp, _pd],
     "aeo_max_advective_rate": [_vp, _i, _vp, _vp, _vp, _pd],
+    "aeo_unet_create": [_i, _i, _i, _i, ctypes.c_ulonglong, _pvp],
+    "aeo_unet_destroy": [_vp],
+    "aeo_unet_num_params": [_vp, ctypes.POINTER(ctypes.c_longlong)],
+    "aeo_unet_get_params": [_vp, _vp],
+    "aeo_unet_set_params": [_vp, _vp],
+    "aeo_unet_forward": [_vp, _vp, _vp, _vp],
+    "aeo_unet_gradients": [_vp, _vp, _vp, _vp, _vp, ctypes.POINTER(ctypes.c_float)],
+    "aeo_unet_train_step": [_vp, _vp, _vp, _vp, ctypes.c_float, ctypes.POINTER(ctypes.c_float)],
+    "aeo_poisson_fcg_create": [_vp, _vp, _vp, _i, _i, _d, _i, _pvp],
     "aeo_profile_enable": [_i],
     "aeo_profile_reset": [],
     "aeo_profile_get": [ctypes.c_char_p, _pi, _pd],

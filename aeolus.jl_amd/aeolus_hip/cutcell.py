@@ -48,50 +48,67 @@ def heightmap_geometry(N, spacing, H, height, nsub=4, vol_min=1e-3):
     sub = np.arange(1, nsub + 1) - 0.5
     xs = xb[:, None] + sub[None, :] * hx       # (sx, nsub)
     ys = yb[:, None] + sub[None, :] * hy
-    zs = zb[:, None] + sub[None, :] * hz       # (sz, nsub)
     cxs = _col(xs, dx, H, sx)                  # columns of the sub-samples
     cys = _col(ys, dy, H, sy)
     cxf = _col(xb, dx, H, sx)                  # columns at the faces
     cyf = _col(yb, dy, H, sy)
-    vol = np.zeros((sx, sy, sz))
-    ax = np.zeros((sx, sy, sz))
-    ay = np.zeros((sx, sy, sz))
-    az = np.zeros((sx, sy, sz))
+    # levels whose cells or bottom faces can be cut: below the tallest roof
+    # (every sample above it is fluid; the sweeps below cannot reach past
+    # the first all-fluid level, whose cells keep open faces)
+    hmax = float(hfull.max())
+    kc = int(np.searchsorted(zb, hmax, side="right")) + 1
+    kc = min(max(kc, 1), sz)
+    zs = zb[:kc, None] + sub[None, :] * hz     # (kc, nsub)
+    zbk = zb[:kc]
+    vol = np.ones((sx, sy, sz))
+    ax = np.ones((sx, sy, sz))
+    ay = np.ones((sx, sy, sz))
+    az = np.ones((sx, sy, sz))
+    vk = np.zeros((sx, sy, kc))
+    axk = np.zeros((sx, sy, kc))
+    ayk = np.zeros((sx, sy, kc))
+    azk = np.zeros((sx, sy, kc))
     # counts of fluid sub-samples: a loop over the horizontal sub-samples, vectorised over cells
     for a in range(nsub):
         for b in range(nsub):
             hab = hfull[np.ix_(cxs[:, a] - 1, cys[:, b] - 1)]            # (sx, sy)
             for c in range(nsub):
-                vol += zs[None, None, :, c] - hab[:, :, None] > 0
-            az += zb[None, None, :] - hab[:, :, None] > 0
+                vk += zs[None, None, :, c] - hab[:, :, None] > 0
+            azk += zbk[None, None, :] - hab[:, :, None] > 0
         hxa = hfull[np.ix_(cxf - 1, cys[:, a] - 1)]                        # x-face columns
         hya = hfull[np.ix_(cxs[:, a] - 1, cyf - 1)]                        # y-face columns
         for c in range(nsub):
-            ax += zs[None, None, :, c] - hxa[:, :, None] > 0
-            ay += zs[None, None, :, c] - hya[:, :, None] > 0
-    vol *= volnorm
-    ax *= areanorm
-    ay *= areanorm
-    az *= areanorm
-    # _finalize_geometry
-    solid = ~(vol > 0)
+            axk += zs[None, None, :, c] - hxa[:, :, None] > 0
+            ayk += zs[None, None, :, c] - hya[:, :, None] > 0
+    vol[:, :, :kc] = vk * volnorm
+    ax[:, :, :kc] = axk * areanorm
+    ay[:, :, :kc] = ayk * areanorm
+    az[:, :, :kc] = azk * areanorm
+    del vk, axk, ayk, azk
+    # _finalize_geometry (on the levels that can hold solid cells, plus one)
+    ke = min(kc + 1, sz)
+    V, AX, AY, AZ = vol[:, :, :ke], ax[:, :, :ke], ay[:, :, :ke], az[:, :, :ke]
+    solid = ~(V > 0)
     ax[0, :, :] = 0.0
     ay[:, 0, :] = 0.0
     az[:, :, 0] = 0.0
     for _ in range(16):
-        ax[1:, :, :][solid[:-1, :, :] | solid[1:, :, :]] = 0.0
-        ay[:, 1:, :][solid[:, :-1, :] | solid[:, 1:, :]] = 0.0
-        az[:, :, 1:][solid[:, :, :-1] | solid[:, :, 1:]] = 0.0
-        opn = (ax + ay) + az
-        opn[:-1, :, :] += ax[1:, :, :]
-        opn[:, :-1, :] += ay[:, 1:, :]
-        opn[:, :, :-1] += az[:, :, 1:]
+        AX[1:, :, :][solid[:-1, :, :] | solid[1:, :, :]] = 0.0
+        AY[:, 1:, :][solid[:, :-1, :] | solid[:, 1:, :]] = 0.0
+        AZ[:, :, 1:][solid[:, :, :-1] | solid[:, :, 1:]] = 0.0
+        opn = (AX + AY) + AZ
+        opn[:-1, :, :] += AX[1:, :, :]
+        opn[:, :-1, :] += AY[:, 1:, :]
+        opn[:, :, :-1] += AZ[:, :, 1:]
+        if ke < sz:
+            opn[:, :, -1] += az[:, :, ke]
         newly = ~solid & ~(opn > 0)
         if not newly.any():
             break
         solid |= newly
-        vol[newly] = 0.0
-    vol_eff = np.where(solid, 0.0, np.where(vol < vol_min, vol_min, vol))
+        V[newly] = 0.0
+    vol_eff = np.where(vol < vol_min, vol_min, vol)
+    vol_eff[:, :, :ke][solid] = 0.0
     F = lambda a: np.asfortranarray(a)
     return F(vol), F(vol_eff), F(ax), F(ay), F(az)
 

@@ -73,3 +73,80 @@ def dcbl_state(grid, seed=DCBL_SEED):
 DCBL_PHYS = dict(nu=0.0, theta0=300.0, g_accel=9.81, flux_bot=0.1, flux_top=0.0,
                  bcs=("periodic", "periodic", "wall"), theta_bcs=("periodic", "periodic", "noflux"))
 TGV_PHYS = dict(nu=1.0 / 1600, theta0=0.0, bcs=("periodic",) * 3)
+
+
+# ---------------------------------------------------------------------------
+# cut-cell Poisson workload: seeded flat-roofed buildings and the divergence
+# of a seeded Gaussian random velocity with a k^-5/3 energy spectrum
+# (the source construction of test/support/multigrid_cases.jl:29-36 on a
+# building heightmap, src/CutCell.jl:372-396)
+# ---------------------------------------------------------------------------
+CUTCELL_SEED = 512192
+
+
+def buildings_heightmap(N, spacing, n=64, seed=CUTCELL_SEED, footprint=(10.0, 40.0), height=(10.0, 80.0)):
+    """(Nx, Ny) roof heights of n rectangular buildings: sides U[footprint] m,
+    heights U[height] m, positions uniform (wrapping periodically); where
+    buildings overlap the taller roof wins."""
+    Nx, Ny = N[0], N[1]
+    dx, dy = spacing[0], spacing[1]
+    rng = np.random.default_rng(seed)
+    h = np.zeros((Nx, Ny))
+    for _ in range(n):
+        wx, wy = rng.uniform(*footprint, size=2)
+        x0, y0 = rng.uniform(0, Nx * dx), rng.uniform(0, Ny * dy)
+        ht = rng.uniform(*height)
+        ix = (int(x0 // dx) + np.arange(max(1, int(round(wx / dx))))) % Nx
+        iy = (int(y0 // dy) + np.arange(max(1, int(round(wy / dy))))) % Ny
+        blk = np.ix_(ix, iy)
+        h[blk] = np.maximum(h[blk], ht)
+    return h
+
+
+def spectral_velocity(N, seed, slope=-5.0 / 3.0):
+    """u, v, w of shape N (periodic, zero mean, unit RMS each): Gaussian
+    random fields whose energy spectrum E(k) = 4πk²|û(k)|² follows k^slope."""
+    import scipy.fft as sfft
+    rng = np.random.default_rng(seed)
+    kx = np.fft.fftfreq(N[0])[:, None, None]
+    ky = np.fft.fftfreq(N[1])[None, :, None]
+    kz = np.fft.rfftfreq(N[2])[None, None, :]
+    k = np.sqrt(kx * kx + ky * ky + kz * kz)
+    amp = np.zeros_like(k)
+    nz = k > 0
+    amp[nz] = k[nz] ** ((slope - 2.0) / 2.0)
+    out = []
+    for _ in range(3):
+        f = sfft.rfftn(rng.standard_normal(N), workers=-1) * amp
+        a = sfft.irfftn(f, s=N, workers=-1)
+        a -= a.mean()
+        a /= np.sqrt((a * a).mean())
+        out.append(a)
+    return out
+
+
+def cutcell_workload(N=(512, 512, 192), spacing=2.0, n_buildings=64, seed=CUTCELL_SEED, H=1, nsub=4,
+                     footprint=(10.0, 40.0), height=(10.0, 80.0)):
+    """The standalone cut-cell Poisson problem: face coefficients a² of the
+    D·G operator (cx, cy, cz), the fluid mask and the source vol_eff·(D u),
+    projected to zero mean over the fluid (solid cells 0). Roofs below the
+    domain top keep the fluid connected (one null-space constant)."""
+    from .cutcell import cutcell_source, heightmap_geometry, mg_coefficients
+    sp = (spacing, spacing, spacing)
+    heights = buildings_heightmap(N, sp, n=n_buildings, seed=seed, footprint=footprint, height=height)
+    vol, vol_eff, ax, ay, az = heightmap_geometry(N, sp, H, heights, nsub=nsub)
+    del vol
+    coeff = mg_coefficients(ax, ay, az, N, H)
+    u0, v0, w0 = spectral_velocity(N, seed + 1)
+    st = tuple(n + 2 * H for n in N)
+    u, v, w = (np.zeros(st, order="F") for _ in range(3))
+    for dst, a in ((u, u0), (v, v0), (w, w0)):
+        dst[H:H + N[0], H:H + N[1], H:H + N[2]] = a
+        dst[H + N[0], H:H + N[1], H:H + N[2]] = a[0]           # the east face of the last cell (periodic)
+        dst[H:H + N[0], H + N[1], H:H + N[2]] = a[:, 0]        # the north face of the last row
+    b = cutcell_source(u, v, w, ax, ay, az, vol_eff, N, H, sp)
+    fluid = vol_eff[H:H + N[0], H:H + N[1], H:H + N[2]] > 0
+    b[fluid] -= b[fluid].mean()
+    b[~fluid] = 0.0
+    return dict(coeff=coeff, source=np.asfortranarray(b), mask=np.asfortranarray(fluid.astype(np.float64)),
+                heights=heights)

@@ -374,8 +374,9 @@ extern "C" int aeo_advect_momentum(aeo_grid *g, int precision, int scheme, int p
 // ---------------------------------------------------------------------------
 // pressure solvers
 // ---------------------------------------------------------------------------
+struct aeo_unet;
 struct aeo_poisson {
-    int kind = 0;  // 0 fft, 1 column, 2 MG-preconditioned CG, 3 multigrid cycles
+    int kind = 0;  // 0 fft, 1 column, 2 MG-preconditioned CG, 3 multigrid cycles, 4 flexible CG (learned)
     int Nx, Ny, Nz;
     bool periodic_x, periodic_y, periodic_z;
     SpectralPlan *sp = nullptr;
@@ -385,6 +386,12 @@ struct aeo_poisson {
     double *src = nullptr, *phi = nullptr;  // interior-sized FP64
     int iters = 0, status = AEO_DIRECT;
     double resid = NAN;
+    // flexible CG: the multigrid solver whose operator and V-cycle it uses
+    aeo_poisson *base = nullptr;
+    aeo_unet *net = nullptr;
+    const double *mask = nullptr;
+    int mode = 0, use_guess = 0;
+    FCGState *fcg = nullptr;
 };
 
 static int poisson_alloc(aeo_poisson *h) {
@@ -449,8 +456,24 @@ extern "C" int aeo_poisson_source_buffer(aeo_poisson *h, double **dptr) {
     return AEO_OK;
 }
 
+struct aeo_unet {
+    UNet *N = nullptr;
+};
+
 static int poisson_run(aeo_poisson *h, double *phi) {
     Timed t("poisson");
+    if (h->kind == 4) {
+        int it = 0;
+        double res = 0, bn = 0;
+        int stt = fcg_solve(h->fcg, h->base->mg, h->net ? h->net->N : nullptr, h->mask, h->mode, h->use_guess,
+                            g_ctx->stream, phi, h->src, h->tol, h->maxit, &it, &res, &bn);
+        if (stt < 0) return fail(AEO_ERR_HIP, "flexible CG: a network pass failed");
+        HIPCHK(hipGetLastError());
+        h->iters = it;
+        h->resid = stt == AEO_ZERO_RHS ? 0.0 : res / bn;
+        h->status = stt;
+        return AEO_OK;
+    }
     if (h->kind == 2) {
         int it = 0;
         double res = 0, bn = 0;
@@ -592,9 +615,98 @@ extern "C" int aeo_poisson_destroy(aeo_poisson *h) {
     if (g_ctx) hipStreamSynchronize(g_ctx->stream);
     spectral_destroy(h->sp);
     mg_destroy(h->mg);
+    fcg_destroy(h->fcg);
     hipFree(h->src);
     hipFree(h->phi);
     delete h;
+    return AEO_OK;
+}
+
+// ---------------------------------------------------------------------------
+// the learned preconditioner
+// ---------------------------------------------------------------------------
+extern "C" int aeo_unet_create(int nx, int ny, int nz, int in_channels, unsigned long long seed, aeo_unet **out) {
+    NEED_CTX();
+    if (!out) return fail(AEO_ERR_ARG, "null output handle");
+    char err[512] = "";
+    HIPCHK(hipStreamSynchronize(g_ctx->stream));
+    UNet *N = unet_create(nx, ny, nz, in_channels, seed, err, sizeof err);
+    if (!N) return fail(AEO_ERR_ARG, "%s", err);
+    auto h = new aeo_unet();
+    h->N = N;
+    *out = h;
+    return AEO_OK;
+}
+extern "C" int aeo_unet_destroy(aeo_unet *h) {
+    if (!h) return AEO_OK;
+    if (g_ctx) hipStreamSynchronize(g_ctx->stream);
+    unet_destroy(h->N);
+    delete h;
+    return AEO_OK;
+}
+extern "C" int aeo_unet_num_params(aeo_unet *h, long long *n) {
+    if (!h || !n) return fail(AEO_ERR_ARG, "null argument");
+    *n = (long long)unet_num_params(h->N);
+    return AEO_OK;
+}
+extern "C" int aeo_unet_get_params(aeo_unet *h, float *host) {
+    NEED_CTX();
+    if (!h || !host) return fail(AEO_ERR_ARG, "null argument");
+    HIPCHK(hipStreamSynchronize(g_ctx->stream));
+    if (unet_get_params(h->N, host)) return fail(AEO_ERR_HIP, "copy of the parameters failed");
+    return AEO_OK;
+}
+extern "C" int aeo_unet_set_params(aeo_unet *h, const float *host) {
+    NEED_CTX();
+    if (!h || !host) return fail(AEO_ERR_ARG, "null argument");
+    HIPCHK(hipStreamSynchronize(g_ctx->stream));
+    if (unet_set_params(h->N, g_ctx->stream, host)) return fail(AEO_ERR_HIP, "copy of the parameters failed");
+    return AEO_OK;
+}
+extern "C" int aeo_unet_forward(aeo_unet *h, const double *src, const double *mask, double *out) {
+    NEED_CTX();
+    if (!h || !src || !out) return fail(AEO_ERR_ARG, "null argument");
+    Timed t("unet");
+    if (unet_forward(h->N, g_ctx->stream, src, mask, out)) return fail(AEO_ERR_HIP, "U-Net forward failed");
+    HIPCHK(hipGetLastError());
+    return AEO_OK;
+}
+extern "C" int aeo_unet_gradients(aeo_unet *h, const double *src, const double *mask, const double *target,
+                                  float *grad_host, float *loss) {
+    NEED_CTX();
+    if (!h || !src || !target || !loss) return fail(AEO_ERR_ARG, "null argument");
+    if (unet_gradients(h->N, g_ctx->stream, src, mask, target, grad_host, loss))
+        return fail(AEO_ERR_HIP, "U-Net backward failed");
+    return AEO_OK;
+}
+extern "C" int aeo_unet_train_step(aeo_unet *h, const double *src, const double *mask, const double *target, float lr,
+                                   float *loss) {
+    NEED_CTX();
+    if (!h || !src || !target) return fail(AEO_ERR_ARG, "null argument");
+    Timed t("unet_train");
+    if (unet_train_step(h->N, g_ctx->stream, src, mask, target, lr, loss)) return fail(AEO_ERR_HIP, "training step failed");
+    return AEO_OK;
+}
+extern "C" int aeo_poisson_fcg_create(aeo_poisson *base, aeo_unet *net, const double *mask, int mode, int use_guess,
+                                      double tol, int maxit, aeo_poisson **out) {
+    NEED_CTX();
+    if (!base || !out) return fail(AEO_ERR_ARG, "null argument");
+    if (!base->mg || base->kind != 2) return fail(AEO_ERR_ARG, "flexible CG: the base solver must be an MG-preconditioned CG");
+    if (mode < 0 || mode > 2) return fail(AEO_ERR_ARG, "mode: 0 network, 1 network + V-cycle, 2 V-cycle");
+    if ((mode != 2 || use_guess) && !net) return fail(AEO_ERR_ARG, "flexible CG: this mode needs the network");
+    if (!(tol >= 0) || maxit < 0) return fail(AEO_ERR_ARG, "bad tolerance or cap");
+    auto h = std::make_unique<aeo_poisson>();
+    h->kind = 4; h->Nx = base->Nx; h->Ny = base->Ny; h->Nz = base->Nz;
+    h->periodic_x = base->periodic_x; h->periodic_y = base->periodic_y; h->periodic_z = base->periodic_z;
+    h->base = base; h->net = net; h->mask = mask; h->mode = mode; h->use_guess = use_guess;
+    h->tol = tol; h->maxit = maxit;
+    if (net && (unet_dims_match(net->N, base->Nx, base->Ny, base->Nz) == 0))
+        return fail(AEO_ERR_ARG, "flexible CG: the network's grid differs from the solver's");
+    h->fcg = fcg_create(base->Nx * base->Ny * base->Nz);
+    if (!h->fcg) return fail(AEO_ERR_HIP, "flexible CG: allocation failed");
+    int rc = poisson_alloc(h.get());
+    if (rc) return rc;
+    *out = h.release();
     return AEO_OK;
 }
 

@@ -236,34 +236,61 @@ __global__ void thomas_kernel(double2 *__restrict__ hat, double *__restrict__ cp
         return;
     }
     const double lam = lx[kx] + ly[ky];
+    // the sweeps take TB levels at a time: the loads of a batch are issued
+    // together (the recurrence itself is sequential; with one column per
+    // thread there are few waves, so memory latency is hidden by batching)
+    constexpr int TB = 8;
     double cprev = 0.0;
     double2 dprev = make_double2(0, 0);
-    for (int k = 0; k < Nz; k++) {
-        const double b = -(lower[k] + upper[k]) - lam * hweight[k];
-        const double den = b - lower[k] * cprev;
-        const double inv_den = 1 / den;
-        cprev = upper[k] / den;
-        cp[c0 + plane * k] = cprev;
-        double2 v = hat[c0 + plane * k];
-        if (k == 0) {
-            v.x = v.x * inv_den;
-            v.y = v.y * inv_den;
-        } else {
-            const double a = lower[k];
-            v.x = (v.x - a * dprev.x) * inv_den;
-            v.y = (v.y - a * dprev.y) * inv_den;
+    for (int kb = 0; kb < Nz; kb += TB) {
+        double2 v[TB];
+#pragma unroll
+        for (int q = 0; q < TB; q++)
+            if (kb + q < Nz) v[q] = hat[c0 + plane * (kb + q)];
+#pragma unroll
+        for (int q = 0; q < TB; q++) {
+            const int k = kb + q;
+            if (k < Nz) {
+                const double b = -(lower[k] + upper[k]) - lam * hweight[k];
+                const double den = b - lower[k] * cprev;
+                const double inv_den = 1 / den;
+                cprev = upper[k] / den;
+                cp[c0 + plane * k] = cprev;
+                if (k == 0) {
+                    v[q].x = v[q].x * inv_den;
+                    v[q].y = v[q].y * inv_den;
+                } else {
+                    const double a = lower[k];
+                    v[q].x = (v[q].x - a * dprev.x) * inv_den;
+                    v[q].y = (v[q].y - a * dprev.y) * inv_den;
+                }
+                dprev = v[q];
+            }
         }
-        hat[c0 + plane * k] = v;
-        dprev = v;
+#pragma unroll
+        for (int q = 0; q < TB; q++)
+            if (kb + q < Nz) hat[c0 + plane * (kb + q)] = v[q];
     }
     double2 nxt = dprev;
-    for (int k = Nz - 2; k >= 0; k--) {
-        double2 v = hat[c0 + plane * k];
-        const double c = cp[c0 + plane * k];
-        v.x -= c * nxt.x;
-        v.y -= c * nxt.y;
-        hat[c0 + plane * k] = v;
-        nxt = v;
+    for (int kt = Nz - 2; kt >= 0; kt -= TB) {
+        double2 v[TB];
+        double c[TB];
+#pragma unroll
+        for (int q = 0; q < TB; q++)
+            if (kt - q >= 0) {
+                v[q] = hat[c0 + plane * (kt - q)];
+                c[q] = cp[c0 + plane * (kt - q)];
+            }
+#pragma unroll
+        for (int q = 0; q < TB; q++)
+            if (kt - q >= 0) {
+                v[q].x -= c[q] * nxt.x;
+                v[q].y -= c[q] * nxt.y;
+                nxt = v[q];
+            }
+#pragma unroll
+        for (int q = 0; q < TB; q++)
+            if (kt - q >= 0) hat[c0 + plane * (kt - q)] = v[q];
     }
 }
 

@@ -65,5 +65,27 @@ int mg_pcg_solve(MGPlan *P, hipStream_t st, double *xout, const double *bin, dou
 int mg_cycles_solve(MGPlan *P, hipStream_t st, double *xout, const double *bin, double tol, int maxcycles,
                     int *cycles_out, double *rel_out);
 void mg_apply(MGPlan *P, hipStream_t st, double *out, const double *x);
+// z = one V-cycle of the preconditioner on r (its mean removed), z mean-free (FP64)
+void mg_precond_apply(MGPlan *P, hipStream_t st, double *z, const double *r);
+
+// ---- the learned preconditioner (unet.hip) ----
+struct UNet;
+UNet *unet_create(int nx, int ny, int nz, int inc, uint64_t seed, char *err, size_t errlen);
+void unet_destroy(UNet *N);
+size_t unet_num_params(const UNet *N);
+int unet_dims_match(const UNet *N, int nx, int ny, int nz);
+int unet_get_params(UNet *N, float *host);
+int unet_set_params(UNet *N, hipStream_t st, const float *host);
+int unet_forward(UNet *N, hipStream_t st, const double *src, const double *mask, double *out);
+int unet_gradients(UNet *N, hipStream_t st, const double *src, const double *mask, const double *target,
+                   float *grad_host, float *loss);
+int unet_train_step(UNet *N, hipStream_t st, const double *src, const double *mask, const double *target, float lr,
+                    float *loss);
+struct FCGState;
+FCGState *fcg_create(int n);
+void fcg_destroy(FCGState *F);
+int fcg_solve(FCGState *F, MGPlan *P, UNet *N, const double *mask, int mode, int use_guess, hipStream_t st,
+              double *xout, const double *bin, double tol, int maxit, int *iters_out, double *resid_out,
+              double *bnorm_out);
 
 }  // namespace aeo

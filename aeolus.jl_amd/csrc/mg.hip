@@ -829,6 +829,16 @@ int mg_cycles_solve(MGPlan *P, hipStream_t st, double *xout, const double *bin, 
     return rel < tol ? 0 : 1;
 }
 
+// one V-cycle of the preconditioner on r (its mean removed first), z its
+// mean-free result (_mg_precond!, src/Multigrid.jl:922-932); r and z may alias
+void mg_precond_apply(MGPlan *P, hipStream_t st, double *z, const double *r) {
+    MGLevelH &h = P->L[0];
+    const size_t n = (size_t)h.nx * h.ny * h.nz;
+    if (r != P->r) hipMemcpyAsync(P->r, r, sizeof(double) * n, hipMemcpyDeviceToDevice, st);
+    if (P->prec == 0) precond<double>(P, st); else precond<float>(P, st);
+    hipMemcpyAsync(z, P->z, sizeof(double) * n, hipMemcpyDeviceToDevice, st);
+}
+
 // apply the fine FP64 operator (_apply_fine!, src/Multigrid.jl:911-917)
 void mg_apply(MGPlan *P, hipStream_t st, double *out, const double *x) {
     MGLevelH &h = P->L[0];

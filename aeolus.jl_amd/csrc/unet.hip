@@ -1,0 +1,970 @@
+// The learned preconditioner: a 3-level 3-D U-Net on gfx950 MFMA, trained on
+// the GPU by its own forward and backward kernels, and the flexible CG that
+// uses it (no reference counterpart: the reference has no network; its
+// nearest hook is src/ML.jl:40-56, and the solver it accelerates is
+// MGPreconditionedCG, src/Multigrid.jl:879-972).
+//
+// Network (c0 = 16 channels at full resolution):
+//   enc0: conv 3³ in→16, conv 16→16 (skip0)         full resolution
+//   pool: 2×2×2 mean
+//   enc1: conv 16→32, conv 32→32 (skip1)            1/2
+//   pool
+//   mid:  conv 32→64, conv 64→64                    1/4
+//   up ×2 (nearest), concat [up, skip1] (96 channels)
+//   dec1: conv 96→32, conv 32→32                    1/2
+//   up ×2, concat [up, skip0] (48 channels)
+//   dec0: conv 48→16, conv 16→16                    full
+//   head: 1×1 conv 16→1 (FP32)
+// every 3³ convolution has a bias and a ReLU; x and y are periodic, z is
+// zero-padded. Input channels: the source scaled to unit RMS, and optionally
+// the fluid mask; the output is multiplied back by the source's RMS.
+//
+// Arithmetic: activations and weights in BF16, FP32 accumulation
+// (v_mfma_f32_16x16x32_bf16), FP32 master weights, gradients and Adam.
+// Layout: channels-last [voxel][channel], voxel = x + nx·(y + ny·z); channel
+// counts padded to multiples of 8 (the input to 8).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+
+#include "kernels.h"
+
+namespace aeo {
+
+typedef __bf16 bf16;
+typedef __attribute__((ext_vector_type(8))) __bf16 bf16x8;
+typedef __attribute__((ext_vector_type(4))) float f32x4;
+typedef __attribute__((ext_vector_type(4))) unsigned int u32x4;
+
+struct Vol {
+    int nx, ny, nz;
+};
+__device__ __forceinline__ int wrapi(int a, int n) { return a < 0 ? a + n : (a >= n ? a - n : a); }
+__device__ __forceinline__ bf16x8 zero8() { return __builtin_bit_cast(bf16x8, u32x4{0u, 0u, 0u, 0u}); }
+
+constexpr int UB = 256;  // threads per block of the elementwise kernels
+
+// ---------------------------------------------------------------------------
+// 3×3×3 convolution as an implicit GEMM on MFMA:
+//   out[v][ooff + n] = act(bias[n] + Σ_tap Σ_c in[nb(v, tap)][c] · W[tap][c][n])
+// M = voxels (64 per wave: four 16-row tiles), N = COUT, K = 27·CIN, ordered
+// k = tap·CIN + c; chunks of 32 (lane group g of 16 lanes holds k = 32·ch +
+// 8g .. +7, i.e. 8 channels of one tap: one 16-byte load). Weights in the B
+// layout wb[k/8][n][k%8], zero-filled up to NCH·32 rows. The same kernel with
+// the flipped, transposed weights is the backward pass for the input.
+// Lane layout of v_mfma_f32_16x16x32_bf16: A row = lane % 16, k = 8·(lane/16)
+// .. +7; B column = lane % 16, same k; D[4·(lane/16) + i][lane % 16].
+// ---------------------------------------------------------------------------
+template <int CIN, int COUT>
+__global__ void __launch_bounds__(256) conv3_kernel(const bf16 *__restrict__ in, const bf16 *__restrict__ wb,
+                                                    const float *__restrict__ bias, bf16 *__restrict__ out, int ocs,
+                                                    int ooff, Vol g, int relu) {
+    constexpr int TN = COUT / 16, TM = 4;
+    constexpr int NCH = (27 * CIN + 31) / 32;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int nxy = g.nx * g.ny;
+    const int nvox = nxy * g.nz;
+    const int vbase = (blockIdx.x * 4 + wave) * 64;
+    if (vbase >= nvox) return;  // no block-level synchronisation below
+    const int r = lane & 15, grp = lane >> 4;
+    int vx[TM], vy[TM], vz[TM];
+#pragma unroll
+    for (int t = 0; t < TM; t++) {
+        const int v = vbase + 16 * t + r;
+        vz[t] = v < nvox ? v / nxy : -1000;  // out-of-range rows read zeros
+        const int rem = v - (v / nxy) * nxy;
+        vy[t] = rem / g.nx;
+        vx[t] = rem - vy[t] * g.nx;
+    }
+    f32x4 acc[TM][TN];
+#pragma unroll
+    for (int t = 0; t < TM; t++)
+#pragma unroll
+        for (int u = 0; u < TN; u++) acc[t][u] = f32x4{0.f, 0.f, 0.f, 0.f};
+
+    for (int ch = 0; ch < NCH; ch++) {
+        const int k0 = ch * 32 + grp * 8;
+        const int tap = k0 / CIN, c0 = k0 - tap * CIN;
+        bf16x8 b[TN];
+#pragma unroll
+        for (int u = 0; u < TN; u++)
+            b[u] = *reinterpret_cast<const bf16x8 *>(wb + ((size_t)(k0 >> 3) * COUT + 16 * u + r) * 8);
+        bf16x8 a[TM];
+        const int dx = tap % 3 - 1, dy = (tap / 3) % 3 - 1, dz = tap / 9 - 1;
+#pragma unroll
+        for (int t = 0; t < TM; t++) {
+            const int z = vz[t] + dz;
+            if (tap < 27 && z >= 0 && z < g.nz) {
+                const int x = wrapi(vx[t] + dx, g.nx), y = wrapi(vy[t] + dy, g.ny);
+                a[t] = *reinterpret_cast<const bf16x8 *>(in + ((size_t)(x + g.nx * (y + g.ny * z)) * CIN + c0));
+            } else {
+                a[t] = zero8();
+            }
+        }
+#pragma unroll
+        for (int t = 0; t < TM; t++)
+#pragma unroll
+            for (int u = 0; u < TN; u++) acc[t][u] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[t], b[u], acc[t][u], 0, 0, 0);
+    }
+#pragma unroll
+    for (int t = 0; t < TM; t++)
+#pragma unroll
+        for (int u = 0; u < TN; u++) {
+            const int n = 16 * u + r;
+            const float bn = bias ? bias[n] : 0.f;
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const int v = vbase + 16 * t + 4 * grp + i;
+                if (v < nvox) {
+                    float y = acc[t][u][i] + bn;
+                    if (relu) y = fmaxf(y, 0.f);
+                    out[(size_t)v * ocs + ooff + n] = (bf16)y;
+                }
+            }
+        }
+}
+
+// ---------------------------------------------------------------------------
+// Weight gradient of a 3³ convolution: part[blk, tap, grp][c][n] =
+// Σ_{v in the block's range, v ≡ grp} in[nb(v, tap)][c] · dy[v][n], FP32.
+// Grid (blocks, 27). VB voxels at a time are staged in LDS as FP32; each
+// thread owns a 4×4 (c, n) tile; the G = 256 / (CIN·COUT/16) thread groups
+// take interleaved voxels and write separate partials.
+// ---------------------------------------------------------------------------
+template <int CIN, int COUT>
+__global__ void __launch_bounds__(256) wgrad_kernel(const bf16 *__restrict__ in, const bf16 *__restrict__ dy,
+                                                    float *__restrict__ part, Vol g, int vpb) {
+    constexpr int NT = (CIN / 4) * (COUT / 4);
+    constexpr int G = 256 / NT;
+    constexpr int VB = 32;
+    __shared__ float sx[VB][CIN + 1];
+    __shared__ float sd[VB][COUT + 1];
+    const int t = threadIdx.x, grp = t / NT, tt = t - grp * NT;
+    const int ci = (tt / (COUT / 4)) * 4, co = (tt % (COUT / 4)) * 4;
+    const int tap = blockIdx.y;
+    const int dx = tap % 3 - 1, dy_ = (tap / 3) % 3 - 1, dz = tap / 9 - 1;
+    const int nxy = g.nx * g.ny, nvox = nxy * g.nz;
+    const int vs = blockIdx.x * vpb, ve = min(vs + vpb, nvox);
+    float acc[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; a++)
+#pragma unroll
+        for (int b = 0; b < 4; b++) acc[a][b] = 0.f;
+    for (int v0 = vs; v0 < ve; v0 += VB) {
+        for (int e = t; e < VB * CIN; e += 256) {
+            const int vv = e / CIN, c = e - vv * CIN, v = v0 + vv;
+            float val = 0.f;
+            if (v < ve) {
+                const int z = v / nxy, rem = v - z * nxy, y = rem / g.nx, x = rem - y * g.nx;
+                const int zz = z + dz;
+                if (zz >= 0 && zz < g.nz) {
+                    const int xx = wrapi(x + dx, g.nx), yy = wrapi(y + dy_, g.ny);
+                    val = (float)in[(size_t)(xx + g.nx * (yy + g.ny * zz)) * CIN + c];
+                }
+            }
+            sx[vv][c] = val;
+        }
+        for (int e = t; e < VB * COUT; e += 256) {
+            const int vv = e / COUT, n = e - vv * COUT, v = v0 + vv;
+            sd[vv][n] = v < ve ? (float)dy[(size_t)v * COUT + n] : 0.f;
+        }
+        __syncthreads();
+        if (grp < G) {
+            for (int vv = grp; vv < VB; vv += G) {
+                float xa[4], da[4];
+#pragma unroll
+                for (int a = 0; a < 4; a++) xa[a] = sx[vv][ci + a];
+#pragma unroll
+                for (int b = 0; b < 4; b++) da[b] = sd[vv][co + b];
+#pragma unroll
+                for (int a = 0; a < 4; a++)
+#pragma unroll
+                    for (int b = 0; b < 4; b++) acc[a][b] = fmaf(xa[a], da[b], acc[a][b]);
+            }
+        }
+        __syncthreads();
+    }
+    if (grp < G) {
+        float *p = part + ((size_t)(blockIdx.x * 27 + tap) * G + grp) * (CIN * COUT);
+#pragma unroll
+        for (int a = 0; a < 4; a++)
+#pragma unroll
+            for (int b = 0; b < 4; b++) p[(ci + a) * COUT + co + b] = acc[a][b];
+    }
+}
+
+// grad[tap][c][n] (c < cin, the logical input channels) = Σ over blocks and groups
+__global__ void wgrad_reduce_kernel(const float *__restrict__ part, float *__restrict__ grad, int nblk, int G,
+                                    int cinp, int cin, int cout) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    const int per = cin * cout;
+    if (e >= 27 * per) return;
+    const int tap = e / per, rem = e - tap * per, c = rem / cout, n = rem - c * cout;
+    double s = 0.0;
+    for (int b = 0; b < nblk; b++)
+        for (int q = 0; q < G; q++) s += part[((size_t)(b * 27 + tap) * G + q) * (cinp * cout) + c * cout + n];
+    grad[e] = (float)s;
+}
+
+// per-block column sums of dy[v][n] (bias gradients)
+template <int COUT>
+__global__ void __launch_bounds__(256) colsum_kernel(const bf16 *__restrict__ dy, float *__restrict__ part, int nvox,
+                                                     int vpb) {
+    __shared__ float sh[256];
+    constexpr int R = 256 / COUT;
+    const int t = threadIdx.x, n = t % COUT, rr = t / COUT;
+    const int vs = blockIdx.x * vpb, ve = min(vs + vpb, nvox);
+    float s = 0.f;
+    for (int v = vs + rr; v < ve; v += R) s += (float)dy[(size_t)v * COUT + n];
+    sh[t] = s;
+    __syncthreads();
+    if (rr == 0) {
+        for (int q = 1; q < R; q++) s += sh[q * COUT + n];
+        part[(size_t)blockIdx.x * COUT + n] = s;
+    }
+}
+__global__ void colsum_reduce_kernel(const float *__restrict__ part, float *__restrict__ out, int nblk, int cout) {
+    const int n = threadIdx.x;
+    if (n >= cout) return;
+    double s = 0.0;
+    for (int b = 0; b < nblk; b++) s += part[(size_t)b * cout + n];
+    out[n] = (float)s;
+}
+
+// ---------------------------------------------------------------------------
+// pooling, upsampling, skips and ReLU masks
+// ---------------------------------------------------------------------------
+// coarse voxel of fine voxel v (fine grid g)
+__device__ __forceinline__ int parent_of(int v, Vol g) {
+    const int nxy = g.nx * g.ny;
+    const int z = v / nxy, rem = v - z * nxy, y = rem / g.nx, x = rem - y * g.nx;
+    return (x >> 1) + (g.nx >> 1) * ((y >> 1) + (g.ny >> 1) * (z >> 1));
+}
+// fine voxel of child q (0..7) of coarse voxel p (fine grid g)
+__device__ __forceinline__ int child_of(int p, int q, Vol g) {
+    const int cx = g.nx >> 1, cy = g.ny >> 1;
+    const int z = p / (cx * cy), rem = p - z * cx * cy, y = rem / cx, x = rem - y * cx;
+    return (2 * x + (q & 1)) + g.nx * ((2 * y + ((q >> 1) & 1)) + g.ny * (2 * z + (q >> 2)));
+}
+
+// out[p][c] = mean of the 8 children in[child][ioff + c] (fine grid g)
+__global__ void pool_kernel(const bf16 *__restrict__ in, int ics, int ioff, bf16 *__restrict__ out, int C, Vol g) {
+    const int nc = (g.nx >> 1) * (g.ny >> 1) * (g.nz >> 1);
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (size_t)nc * C) return;
+    const int p = (int)(e / C), c = (int)(e - (size_t)p * C);
+    float s = 0.f;
+    for (int q = 0; q < 8; q++) s += (float)in[(size_t)child_of(p, q, g) * ics + ioff + c];
+    out[e] = (bf16)(s * 0.125f);
+}
+// out[v][ooff + c] = in[parent(v)][c] (fine grid g)
+__global__ void up_kernel(const bf16 *__restrict__ in, bf16 *__restrict__ out, int ocs, int ooff, int C, Vol g) {
+    const int nf = g.nx * g.ny * g.nz;
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (size_t)nf * C) return;
+    const int v = (int)(e / C), c = (int)(e - (size_t)v * C);
+    out[(size_t)v * ocs + ooff + c] = in[(size_t)parent_of(v, g) * C + c];
+}
+// out[v][c] = y[v][yoff + c] > 0 ? gin[v][goff + c] : 0
+__global__ void relu_mask_kernel(const bf16 *gin, int gcs, int goff, const bf16 *__restrict__ y, int ycs,
+                                 int yoff, bf16 *out, int C, int nvox) {
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (size_t)nvox * C) return;
+    const int v = (int)(e / C), c = (int)(e - (size_t)v * C);
+    out[e] = (float)y[(size_t)v * ycs + yoff + c] > 0.f ? gin[(size_t)v * gcs + goff + c] : (bf16)0.f;
+}
+// the upsampling's backward with the ReLU mask of its input (coarse y [p][C]):
+// out[p][c] = y[p][c] > 0 ? Σ_children gcat[child][goff + c] : 0 (fine grid g)
+__global__ void up_back_kernel(const bf16 *__restrict__ gcat, int gcs, int goff, const bf16 *__restrict__ y,
+                               bf16 *__restrict__ out, int C, Vol g) {
+    const int nc = (g.nx >> 1) * (g.ny >> 1) * (g.nz >> 1);
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (size_t)nc * C) return;
+    const int p = (int)(e / C), c = (int)(e - (size_t)p * C);
+    float s = 0.f;
+    for (int q = 0; q < 8; q++) s += (float)gcat[(size_t)child_of(p, q, g) * gcs + goff + c];
+    out[e] = (float)y[e] > 0.f ? (bf16)s : (bf16)0.f;
+}
+// the skip's gradient: the pooling's backward plus the concatenation's slice,
+// masked by the skip's ReLU: out[v][c] = y[v][off + c] > 0 ? gp[parent][c]/8 + gcat[v][off + c] : 0
+__global__ void skip_back_kernel(const bf16 *__restrict__ gp, const bf16 *__restrict__ gcat,
+                                 const bf16 *__restrict__ cat, int cs, int off, bf16 *__restrict__ out, int C, Vol g) {
+    const int nf = g.nx * g.ny * g.nz;
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (size_t)nf * C) return;
+    const int v = (int)(e / C), c = (int)(e - (size_t)v * C);
+    const size_t s = (size_t)v * cs + off + c;
+    const float gv = (float)gp[(size_t)parent_of(v, g) * C + c] * 0.125f + (float)gcat[s];
+    out[e] = (float)cat[s] > 0.f ? (bf16)gv : (bf16)0.f;
+}
+
+// ---------------------------------------------------------------------------
+// input, head, loss
+// ---------------------------------------------------------------------------
+// Σ x² per block (FP64)
+__global__ void sumsq64_kernel(const double *__restrict__ x, int n, double *__restrict__ part) {
+    __shared__ double sh[UB];
+    double s = 0.0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) s += x[i] * x[i];
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = UB / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+}
+// scal[0] = 1/rms (0 when the input is zero), scal[1] = rms
+__global__ void rms_finalize_kernel(const double *__restrict__ part, int nb, int n, double *__restrict__ scal) {
+    if (threadIdx.x != 0) return;
+    double s = 0.0;
+    for (int b = 0; b < nb; b++) s += part[b];
+    const double rms = sqrt(s / n);
+    scal[0] = rms > 0 ? 1.0 / rms : 0.0;
+    scal[1] = rms;
+}
+// x0[v][0] = src·s, x0[v][1] = mask (when given), other channels 0
+__global__ void pack_kernel(const double *__restrict__ src, const double *__restrict__ mask,
+                            const double *__restrict__ scal, bf16 *__restrict__ x0, int nvox) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= nvox) return;
+    const double s = scal[0];
+    bf16x8 o = zero8();
+    o[0] = (bf16)(float)(src[v] * s);
+    if (mask) o[1] = (bf16)(float)mask[v];
+    *reinterpret_cast<bf16x8 *>(x0 + (size_t)v * 8) = o;
+}
+// out[v] = (bh + Σ_c wh[c]·a[v][c]) · rms  (FP64 output, the source's scale restored)
+__global__ void head_kernel(const bf16 *__restrict__ a, const float *__restrict__ wh, const double *__restrict__ scal,
+                            double *__restrict__ out, float *__restrict__ yout, int nvox) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= nvox) return;
+    const bf16x8 a0 = *reinterpret_cast<const bf16x8 *>(a + (size_t)v * 16);
+    const bf16x8 a1 = *reinterpret_cast<const bf16x8 *>(a + (size_t)v * 16 + 8);
+    float y = wh[16];
+#pragma unroll
+    for (int c = 0; c < 8; c++) y = fmaf(wh[c], (float)a0[c], y);
+#pragma unroll
+    for (int c = 0; c < 8; c++) y = fmaf(wh[8 + c], (float)a1[c], y);
+    if (yout) yout[v] = y;
+    if (out) out[v] = (double)y * scal[1];
+}
+// loss = mean((y − t·s)²); dy = 2 (y − t·s) / n; head gradients and g9 = dy·wh ⊙ (a > 0)
+__global__ void __launch_bounds__(256) head_back_kernel(const float *__restrict__ y, const double *__restrict__ tgt,
+                                                        const double *__restrict__ scal, const bf16 *__restrict__ a,
+                                                        const float *__restrict__ wh, bf16 *__restrict__ g9,
+                                                        double *__restrict__ part, int nvox) {
+    // part per block: [0] Σ loss terms, [1..16] Σ dy·a[c], [17] Σ dy
+    __shared__ double sh[18][256];
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    double acc[18];
+    for (int q = 0; q < 18; q++) acc[q] = 0.0;
+    if (v < nvox) {
+        const float d = y[v] - (float)(tgt[v] * scal[0]);
+        const float dy = 2.f * d / (float)nvox;
+        acc[0] = (double)d * d;
+        acc[17] = dy;
+        bf16x8 o0 = zero8(), o1 = zero8();
+        for (int c = 0; c < 16; c++) {
+            const float ac = (float)a[(size_t)v * 16 + c];
+            acc[1 + c] = (double)dy * ac;
+            const float gv = ac > 0.f ? dy * wh[c] : 0.f;
+            if (c < 8) o0[c] = (bf16)gv; else o1[c - 8] = (bf16)gv;
+        }
+        *reinterpret_cast<bf16x8 *>(g9 + (size_t)v * 16) = o0;
+        *reinterpret_cast<bf16x8 *>(g9 + (size_t)v * 16 + 8) = o1;
+    }
+    for (int q = 0; q < 18; q++) sh[q][threadIdx.x] = acc[q];
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w)
+            for (int q = 0; q < 18; q++) sh[q][threadIdx.x] += sh[q][threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x < 18) part[(size_t)blockIdx.x * 18 + threadIdx.x] = sh[threadIdx.x][0];
+}
+// loss (scal[2]) and the head gradients (grad: wh[16], bh)
+__global__ void head_reduce_kernel(const double *__restrict__ part, int nb, int nvox, double *__restrict__ scal,
+                                   float *__restrict__ ghead) {
+    const int q = threadIdx.x;
+    if (q >= 18) return;
+    double s = 0.0;
+    for (int b = 0; b < nb; b++) s += part[(size_t)b * 18 + q];
+    if (q == 0) scal[2] = s / nvox;
+    else ghead[q - 1] = (float)s;  // q-1 = 0..15: wh, 16: bh
+}
+
+// ---------------------------------------------------------------------------
+// parameters: Adam and the BF16 weight layouts
+// ---------------------------------------------------------------------------
+__global__ void adam_kernel(float *__restrict__ w, float *__restrict__ m, float *__restrict__ v,
+                            const float *__restrict__ g, int n, float lr, float b1, float b2, float c1, float c2,
+                            float eps) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float gi = g[i];
+    const float mi = b1 * m[i] + (1.f - b1) * gi;
+    const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    w[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+}
+// forward B layout: wf[k/8][n][k%8], k = tap·cinp + c (c < cin, else 0; rows past 27·cinp 0)
+// backward-data B layout (the flipped, transposed kernel, a conv cout → cinp):
+// wt[k'/8][c][k'%8], k' = tap'·cout + n, value W[26 − tap'][c][n]
+__global__ void wlayout_kernel(const float *__restrict__ W, bf16 *__restrict__ wf, bf16 *__restrict__ wt, int cin,
+                               int cinp, int cout, int kf, int kt) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < kf * cout) {
+        const int k = e / cout, n = e - k * cout;
+        const int tap = k / cinp, c = k - tap * cinp;
+        const float val = (tap < 27 && c < cin) ? W[(tap * cin + c) * cout + n] : 0.f;
+        wf[((size_t)(k >> 3) * cout + n) * 8 + (k & 7)] = (bf16)val;
+    }
+    if (wt && e < kt * cinp) {
+        const int k = e / cinp, c = e - k * cinp;
+        const int tap = k / cout, n = k - tap * cout;
+        const float val = (tap < 27 && c < cin) ? W[((26 - tap) * cin + c) * cout + n] : 0.f;
+        wt[((size_t)(k >> 3) * cinp + c) * 8 + (k & 7)] = (bf16)val;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+struct UConv {
+    int cin, cinp, cout;
+    size_t woff, boff;  // offsets in the FP32 parameter vector
+    bf16 *wf = nullptr, *wt = nullptr;
+    int kf, kt;  // K rows (padded to chunks of 32) of the two layouts
+};
+
+struct UNet {
+    int nx, ny, nz, inc;
+    Vol g0, g1, g2;
+    int n0, n1, n2;
+    UConv L[10];
+    size_t nparam, headoff;
+    float *W = nullptr, *G = nullptr, *M = nullptr, *V = nullptr;
+    int step = 0;
+    // activations (BF16, channels-last)
+    bf16 *x0, *a0, *cat0, *a8, *a9, *p0, *a2, *cat1, *a6, *a7, *p1, *a4, *a5;
+    // gradients
+    bf16 *g0b16, *g1b16, *gcat0, *g1l, *g2l, *gcat1, *gl2a, *gl2b, *gp1, *gp0;
+    float *y;           // head output (scaled space)
+    double *scal;       // [0] 1/rms, [1] rms, [2] loss
+    double *dpart;      // reduction partials (FP64)
+    float *fpart;       // weight-gradient partials (FP32)
+    size_t fpart_n;
+};
+
+#define UCHK(x)                                  \
+    do {                                         \
+        if ((x) != hipSuccess) return -2;        \
+    } while (0)
+
+static int nblk(size_t n, int b = UB) { return (int)((n + b - 1) / b); }
+
+template <int CIN, int COUT>
+static void conv_launch(hipStream_t st, const bf16 *in, const bf16 *wb, const float *bias, bf16 *out, int ocs,
+                        int ooff, Vol g, int relu) {
+    const int nv = g.nx * g.ny * g.nz;
+    hipLaunchKernelGGL((conv3_kernel<CIN, COUT>), dim3((nv + 255) / 256), dim3(256), 0, st, in, wb, bias, out, ocs, ooff,
+                       g, relu);
+}
+
+// dispatch on (cin, cout) to the instantiated convolutions
+static int conv_run(hipStream_t st, int cin, int cout, const bf16 *in, const bf16 *wb, const float *bias, bf16 *out,
+                    int ocs, int ooff, Vol g, int relu) {
+#define CASE(a, b)                                                              \
+    if (cin == a && cout == b) {                                                \
+        conv_launch<a, b>(st, in, wb, bias, out, ocs, ooff, g, relu);           \
+        return 0;                                                               \
+    }
+    CASE(8, 16) CASE(16, 16) CASE(16, 32) CASE(32, 32) CASE(32, 64) CASE(64, 64) CASE(96, 32) CASE(48, 16)
+    CASE(32, 16) CASE(64, 32) CASE(32, 96) CASE(16, 48)
+#undef CASE
+    return -1;
+}
+
+template <int CIN, int COUT>
+static int wgrad_launch(hipStream_t st, UNet *N, const bf16 *in, const bf16 *dy, Vol g, float *grad, int cin) {
+    constexpr int G = 256 / ((CIN / 4) * (COUT / 4));
+    const int nv = g.nx * g.ny * g.nz;
+    int nb = (nv + 8191) / 8192;
+    if (nb > 64) nb = 64;
+    const int vpb = ((nv + nb - 1) / nb + 31) / 32 * 32;
+    nb = (nv + vpb - 1) / vpb;
+    if ((size_t)nb * 27 * G * CIN * COUT > N->fpart_n) return -1;
+    hipLaunchKernelGGL((wgrad_kernel<CIN, COUT>), dim3(nb, 27), dim3(256), 0, st, in, dy, N->fpart, g, vpb);
+    hipLaunchKernelGGL(wgrad_reduce_kernel, dim3(nblk((size_t)27 * cin * COUT)), dim3(UB), 0, st, N->fpart, grad, nb, G,
+                       CIN, cin, COUT);
+    return 0;
+}
+static int wgrad_run(hipStream_t st, UNet *N, int cinp, int cin, int cout, const bf16 *in, const bf16 *dy, Vol g,
+                     float *grad) {
+#define CASE(a, b) \
+    if (cinp == a && cout == b) return wgrad_launch<a, b>(st, N, in, dy, g, grad, cin);
+    CASE(8, 16) CASE(16, 16) CASE(16, 32) CASE(32, 32) CASE(32, 64) CASE(64, 64) CASE(96, 32) CASE(48, 16)
+#undef CASE
+    return -1;
+}
+template <int COUT>
+static void colsum_launch(hipStream_t st, UNet *N, const bf16 *dy, int nv, float *out) {
+    int nb = (nv + 16383) / 16384;
+    if (nb > 256) nb = 256;
+    const int vpb = (nv + nb - 1) / nb;
+    nb = (nv + vpb - 1) / vpb;
+    hipLaunchKernelGGL(colsum_kernel<COUT>, dim3(nb), dim3(256), 0, st, dy, N->fpart, nv, vpb);
+    hipLaunchKernelGGL(colsum_reduce_kernel, dim3(1), dim3(64), 0, st, N->fpart, out, nb, COUT);
+}
+static void colsum_run(hipStream_t st, UNet *N, int cout, const bf16 *dy, int nv, float *out) {
+    if (cout == 16) colsum_launch<16>(st, N, dy, nv, out);
+    else if (cout == 32) colsum_launch<32>(st, N, dy, nv, out);
+    else colsum_launch<64>(st, N, dy, nv, out);
+}
+
+static int refresh_layouts(UNet *N, hipStream_t st) {
+    for (int l = 0; l < 10; l++) {
+        UConv &c = N->L[l];
+        const int n = (c.kf * c.cout > c.kt * c.cinp) ? c.kf * c.cout : c.kt * c.cinp;
+        hipLaunchKernelGGL(wlayout_kernel, dim3(nblk(n)), dim3(UB), 0, st, (const float *)(N->W + c.woff), c.wf, c.wt,
+                           c.cin, c.cinp, c.cout, c.kf, c.kt);
+    }
+    return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+void unet_destroy(UNet *N) {
+    if (!N) return;
+    void *ptrs[] = {N->W, N->G, N->M, N->V, N->x0, N->a0, N->cat0, N->a8, N->a9, N->p0, N->a2, N->cat1, N->a6,
+                    N->a7, N->p1, N->a4, N->a5, N->g0b16, N->g1b16, N->gcat0, N->g1l, N->g2l, N->gcat1, N->gl2a,
+                    N->gl2b, N->gp1, N->gp0, N->y, N->scal, N->dpart, N->fpart};
+    for (void *p : ptrs)
+        if (p) (void)hipFree(p);
+    for (auto &c : N->L) {
+        if (c.wf) (void)hipFree(c.wf);
+        if (c.wt) (void)hipFree(c.wt);
+    }
+    delete N;
+}
+
+// He-normal initial weights from a seeded splitmix64 + Box–Muller stream, zero biases
+static void init_params(UNet *N, std::vector<float> &w, uint64_t seed) {
+    uint64_t s = seed ? seed : 0x9E3779B97F4A7C15ull;
+    auto next = [&]() {
+        uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        return z ^ (z >> 31);
+    };
+    auto unif = [&]() { return ((next() >> 11) + 0.5) * (1.0 / 9007199254740992.0); };
+    auto gauss = [&]() { return sqrt(-2.0 * log(unif())) * cos(6.283185307179586 * unif()); };
+    w.assign(N->nparam, 0.f);
+    for (auto &c : N->L) {
+        const double sd = sqrt(2.0 / (27.0 * c.cin));
+        for (size_t i = 0; i < (size_t)27 * c.cin * c.cout; i++) w[c.woff + i] = (float)(sd * gauss());
+    }
+    const double sdh = sqrt(1.0 / 16.0);
+    for (int c = 0; c < 16; c++) w[N->headoff + c] = (float)(sdh * gauss());
+}
+
+UNet *unet_create(int nx, int ny, int nz, int inc, uint64_t seed, char *err, size_t errlen) {
+    if (nx % 4 || ny % 4 || nz % 4 || nx < 4 || ny < 4 || nz < 4) {
+        snprintf(err, errlen, "the U-Net needs every axis divisible by 4 (got %d x %d x %d)", nx, ny, nz);
+        return nullptr;
+    }
+    if (inc != 1 && inc != 2) {
+        snprintf(err, errlen, "in_channels must be 1 (source) or 2 (source and mask)");
+        return nullptr;
+    }
+    if ((size_t)nx * ny * nz * 96 >= ((size_t)1 << 31) * 2) {
+        snprintf(err, errlen, "grid too large for the U-Net's activation buffers");
+        return nullptr;
+    }
+    UNet *N = new UNet();
+    N->nx = nx; N->ny = ny; N->nz = nz; N->inc = inc;
+    N->g0 = Vol{nx, ny, nz}; N->g1 = Vol{nx / 2, ny / 2, nz / 2}; N->g2 = Vol{nx / 4, ny / 4, nz / 4};
+    N->n0 = nx * ny * nz; N->n1 = N->n0 / 8; N->n2 = N->n1 / 8;
+    const int spec[10][3] = {{inc, 8, 16}, {16, 16, 16}, {16, 16, 32}, {32, 32, 32}, {32, 32, 64},
+                             {64, 64, 64}, {96, 96, 32}, {32, 32, 32}, {48, 48, 16}, {16, 16, 16}};
+    size_t off = 0;
+    for (int l = 0; l < 10; l++) {
+        UConv &c = N->L[l];
+        c.cin = spec[l][0]; c.cinp = spec[l][1]; c.cout = spec[l][2];
+        c.woff = off; off += (size_t)27 * c.cin * c.cout;
+        c.boff = off; off += c.cout;
+        c.kf = ((27 * c.cinp + 31) / 32) * 32;
+        c.kt = ((27 * c.cout + 31) / 32) * 32;
+    }
+    N->headoff = off;
+    off += 17;
+    N->nparam = off;
+    bool ok = true;
+    auto A = [&](void **p, size_t bytes) {
+        if (hipMalloc(p, bytes) != hipSuccess) ok = false;
+        else (void)hipMemset(*p, 0, bytes);
+    };
+    const size_t b2 = sizeof(bf16);
+    A((void **)&N->W, off * 4); A((void **)&N->G, off * 4); A((void **)&N->M, off * 4); A((void **)&N->V, off * 4);
+    for (int l = 0; l < 10; l++) {
+        UConv &c = N->L[l];
+        A((void **)&c.wf, (size_t)c.kf * c.cout * b2);
+        if (l > 0) A((void **)&c.wt, (size_t)c.kt * c.cinp * b2);
+    }
+    const size_t n0 = N->n0, n1 = N->n1, n2 = N->n2;
+    A((void **)&N->x0, n0 * 8 * b2); A((void **)&N->a0, n0 * 16 * b2); A((void **)&N->cat0, n0 * 48 * b2);
+    A((void **)&N->a8, n0 * 16 * b2); A((void **)&N->a9, n0 * 16 * b2);
+    A((void **)&N->p0, n1 * 16 * b2); A((void **)&N->a2, n1 * 32 * b2); A((void **)&N->cat1, n1 * 96 * b2);
+    A((void **)&N->a6, n1 * 32 * b2); A((void **)&N->a7, n1 * 32 * b2);
+    A((void **)&N->p1, n2 * 32 * b2); A((void **)&N->a4, n2 * 64 * b2); A((void **)&N->a5, n2 * 64 * b2);
+    A((void **)&N->g0b16, n0 * 16 * b2); A((void **)&N->g1b16, n0 * 16 * b2); A((void **)&N->gcat0, n0 * 48 * b2);
+    A((void **)&N->g1l, n1 * 32 * b2); A((void **)&N->g2l, n1 * 32 * b2); A((void **)&N->gcat1, n1 * 96 * b2);
+    A((void **)&N->gl2a, n2 * 64 * b2); A((void **)&N->gl2b, n2 * 64 * b2); A((void **)&N->gp1, n2 * 32 * b2);
+    A((void **)&N->gp0, n1 * 16 * b2);
+    A((void **)&N->y, n0 * 4);
+    A((void **)&N->scal, 8 * 8);
+    const int nbh = nblk(n0);
+    A((void **)&N->dpart, (size_t)(nbh > 4096 ? nbh : 4096) * 18 * 8);
+    N->fpart_n = (size_t)64 * 27 * 8 * 1024;  // blocks × taps × groups × (cin·cout) bound
+    A((void **)&N->fpart, N->fpart_n * 4);
+    if (!ok) {
+        snprintf(err, errlen, "hipMalloc failed for the U-Net buffers");
+        unet_destroy(N);
+        return nullptr;
+    }
+    std::vector<float> w;
+    init_params(N, w, seed);
+    if (hipMemcpy(N->W, w.data(), off * 4, hipMemcpyHostToDevice) != hipSuccess || refresh_layouts(N, 0) != 0 ||
+        hipDeviceSynchronize() != hipSuccess) {
+        snprintf(err, errlen, "U-Net initialisation failed");
+        unet_destroy(N);
+        return nullptr;
+    }
+    return N;
+}
+
+size_t unet_num_params(const UNet *N) { return N->nparam; }
+int unet_dims_match(const UNet *N, int nx, int ny, int nz) { return N->nx == nx && N->ny == ny && N->nz == nz; }
+int unet_get_params(UNet *N, float *host) {
+    return hipMemcpy(host, N->W, N->nparam * 4, hipMemcpyDeviceToHost) == hipSuccess ? 0 : -2;
+}
+int unet_set_params(UNet *N, hipStream_t st, const float *host) {
+    if (hipMemcpy(N->W, host, N->nparam * 4, hipMemcpyHostToDevice) != hipSuccess) return -2;
+    (void)hipMemset(N->M, 0, N->nparam * 4);
+    (void)hipMemset(N->V, 0, N->nparam * 4);
+    N->step = 0;
+    return refresh_layouts(N, st);
+}
+
+// the forward pass; the source's RMS into scal, the head output into N->y and (if out) out
+int unet_forward(UNet *N, hipStream_t st, const double *src, const double *mask, double *out) {
+    const int n0 = N->n0;
+    hipLaunchKernelGGL(sumsq64_kernel, dim3(256), dim3(UB), 0, st, src, n0, N->dpart);
+    hipLaunchKernelGGL(rms_finalize_kernel, dim3(1), dim3(64), 0, st, N->dpart, 256, n0, N->scal);
+    hipLaunchKernelGGL(pack_kernel, dim3(nblk(n0)), dim3(UB), 0, st, src, N->inc == 2 ? mask : nullptr, N->scal, N->x0,
+                       n0);
+    UConv *L = N->L;
+    const float *W = N->W;
+    int rc = 0;
+    rc |= conv_run(st, 8, 16, N->x0, L[0].wf, W + L[0].boff, N->a0, 16, 0, N->g0, 1);
+    rc |= conv_run(st, 16, 16, N->a0, L[1].wf, W + L[1].boff, N->cat0, 48, 32, N->g0, 1);  // skip0
+    hipLaunchKernelGGL(pool_kernel, dim3(nblk((size_t)N->n1 * 16)), dim3(UB), 0, st, N->cat0, 48, 32, N->p0, 16, N->g0);
+    rc |= conv_run(st, 16, 32, N->p0, L[2].wf, W + L[2].boff, N->a2, 32, 0, N->g1, 1);
+    rc |= conv_run(st, 32, 32, N->a2, L[3].wf, W + L[3].boff, N->cat1, 96, 64, N->g1, 1);  // skip1
+    hipLaunchKernelGGL(pool_kernel, dim3(nblk((size_t)N->n2 * 32)), dim3(UB), 0, st, N->cat1, 96, 64, N->p1, 32, N->g1);
+    rc |= conv_run(st, 32, 64, N->p1, L[4].wf, W + L[4].boff, N->a4, 64, 0, N->g2, 1);
+    rc |= conv_run(st, 64, 64, N->a4, L[5].wf, W + L[5].boff, N->a5, 64, 0, N->g2, 1);
+    hipLaunchKernelGGL(up_kernel, dim3(nblk((size_t)N->n1 * 64)), dim3(UB), 0, st, N->a5, N->cat1, 96, 0, 64, N->g1);
+    rc |= conv_run(st, 96, 32, N->cat1, L[6].wf, W + L[6].boff, N->a6, 32, 0, N->g1, 1);
+    rc |= conv_run(st, 32, 32, N->a6, L[7].wf, W + L[7].boff, N->a7, 32, 0, N->g1, 1);
+    hipLaunchKernelGGL(up_kernel, dim3(nblk((size_t)n0 * 32)), dim3(UB), 0, st, N->a7, N->cat0, 48, 0, 32, N->g0);
+    rc |= conv_run(st, 48, 16, N->cat0, L[8].wf, W + L[8].boff, N->a8, 16, 0, N->g0, 1);
+    rc |= conv_run(st, 16, 16, N->a8, L[9].wf, W + L[9].boff, N->a9, 16, 0, N->g0, 1);
+    hipLaunchKernelGGL(head_kernel, dim3(nblk(n0)), dim3(UB), 0, st, N->a9, W + N->headoff, N->scal, out, N->y, n0);
+    return rc ? -1 : (hipGetLastError() == hipSuccess ? 0 : -2);
+}
+
+// the loss of one sample and the gradient of every parameter into N->G
+int unet_backward(UNet *N, hipStream_t st, const double *target) {
+    const int n0 = N->n0, n1 = N->n1, n2 = N->n2;
+    UConv *L = N->L;
+    float *G = N->G;
+    int rc = 0;
+    const int nbh = nblk(n0);
+    hipLaunchKernelGGL(head_back_kernel, dim3(nbh), dim3(256), 0, st, N->y, target, N->scal, N->a9, N->W + N->headoff,
+                       N->g0b16, N->dpart, n0);
+    hipLaunchKernelGGL(head_reduce_kernel, dim3(1), dim3(64), 0, st, N->dpart, nbh, n0, N->scal, G + N->headoff);
+    // layer 9 (16 → 16 at full resolution): g9 in g0b16
+    auto layer_grads = [&](int l, const bf16 *in, const bf16 *g, Vol gv, int nv) {
+        rc |= wgrad_run(st, N, L[l].cinp, L[l].cin, L[l].cout, in, g, gv, G + L[l].woff);
+        colsum_run(st, N, L[l].cout, g, nv, G + L[l].boff);
+    };
+    layer_grads(9, N->a8, N->g0b16, N->g0, n0);
+    rc |= conv_run(st, 16, 16, N->g0b16, L[9].wt, nullptr, N->g1b16, 16, 0, N->g0, 0);
+    hipLaunchKernelGGL(relu_mask_kernel, dim3(nblk((size_t)n0 * 16)), dim3(UB), 0, st, N->g1b16, 16, 0, N->a8, 16, 0,
+                       N->g1b16, 16, n0);
+    // layer 8 (cat0 48 → 16)
+    layer_grads(8, N->cat0, N->g1b16, N->g0, n0);
+    rc |= conv_run(st, 16, 48, N->g1b16, L[8].wt, nullptr, N->gcat0, 48, 0, N->g0, 0);
+    // up (a7 → cat0[0:32)) backward with a7's mask → g7 (level 1, 32)
+    hipLaunchKernelGGL(up_back_kernel, dim3(nblk((size_t)n1 * 32)), dim3(UB), 0, st, N->gcat0, 48, 0, N->a7, N->g1l, 32,
+                       N->g0);
+    // layer 7 (32 → 32 at 1/2)
+    layer_grads(7, N->a6, N->g1l, N->g1, n1);
+    rc |= conv_run(st, 32, 32, N->g1l, L[7].wt, nullptr, N->g2l, 32, 0, N->g1, 0);
+    hipLaunchKernelGGL(relu_mask_kernel, dim3(nblk((size_t)n1 * 32)), dim3(UB), 0, st, N->g2l, 32, 0, N->a6, 32, 0,
+                       N->g2l, 32, n1);
+    // layer 6 (cat1 96 → 32)
+    layer_grads(6, N->cat1, N->g2l, N->g1, n1);
+    rc |= conv_run(st, 32, 96, N->g2l, L[6].wt, nullptr, N->gcat1, 96, 0, N->g1, 0);
+    // up (a5 → cat1[0:64)) backward with a5's mask → g5 (level 2, 64)
+    hipLaunchKernelGGL(up_back_kernel, dim3(nblk((size_t)n2 * 64)), dim3(UB), 0, st, N->gcat1, 96, 0, N->a5, N->gl2a, 64,
+                       N->g1);
+    // layer 5 (64 → 64 at 1/4)
+    layer_grads(5, N->a4, N->gl2a, N->g2, n2);
+    rc |= conv_run(st, 64, 64, N->gl2a, L[5].wt, nullptr, N->gl2b, 64, 0, N->g2, 0);
+    hipLaunchKernelGGL(relu_mask_kernel, dim3(nblk((size_t)n2 * 64)), dim3(UB), 0, st, N->gl2b, 64, 0, N->a4, 64, 0,
+                       N->gl2b, 64, n2);
+    // layer 4 (p1 32 → 64)
+    layer_grads(4, N->p1, N->gl2b, N->g2, n2);
+    rc |= conv_run(st, 64, 32, N->gl2b, L[4].wt, nullptr, N->gp1, 32, 0, N->g2, 0);
+    // skip1: pool backward + cat1[64:96) slice, masked by skip1 → g3 (level 1, 32)
+    hipLaunchKernelGGL(skip_back_kernel, dim3(nblk((size_t)n1 * 32)), dim3(UB), 0, st, N->gp1, N->gcat1, N->cat1, 96, 64,
+                       N->g1l, 32, N->g1);
+    // layer 3 (a2 32 → 32)
+    layer_grads(3, N->a2, N->g1l, N->g1, n1);
+    rc |= conv_run(st, 32, 32, N->g1l, L[3].wt, nullptr, N->g2l, 32, 0, N->g1, 0);
+    hipLaunchKernelGGL(relu_mask_kernel, dim3(nblk((size_t)n1 * 32)), dim3(UB), 0, st, N->g2l, 32, 0, N->a2, 32, 0,
+                       N->g2l, 32, n1);
+    // layer 2 (p0 16 → 32)
+    layer_grads(2, N->p0, N->g2l, N->g1, n1);
+    rc |= conv_run(st, 32, 16, N->g2l, L[2].wt, nullptr, N->gp0, 16, 0, N->g1, 0);
+    // skip0: pool backward + cat0[32:48) slice, masked by skip0 → g1 (full, 16)
+    hipLaunchKernelGGL(skip_back_kernel, dim3(nblk((size_t)n0 * 16)), dim3(UB), 0, st, N->gp0, N->gcat0, N->cat0, 48, 32,
+                       N->g0b16, 16, N->g0);
+    // layer 1 (a0 16 → 16)
+    layer_grads(1, N->a0, N->g0b16, N->g0, n0);
+    rc |= conv_run(st, 16, 16, N->g0b16, L[1].wt, nullptr, N->g1b16, 16, 0, N->g0, 0);
+    hipLaunchKernelGGL(relu_mask_kernel, dim3(nblk((size_t)n0 * 16)), dim3(UB), 0, st, N->g1b16, 16, 0, N->a0, 16, 0,
+                       N->g1b16, 16, n0);
+    // layer 0 (x0 → 16)
+    layer_grads(0, N->x0, N->g1b16, N->g0, n0);
+    return rc ? -1 : (hipGetLastError() == hipSuccess ? 0 : -2);
+}
+
+int unet_loss(UNet *N, hipStream_t st, float *loss) {
+    double l = 0;
+    if (hipMemcpyAsync(&l, N->scal + 2, 8, hipMemcpyDeviceToHost, st) != hipSuccess) return -2;
+    if (hipStreamSynchronize(st) != hipSuccess) return -2;
+    *loss = (float)l;
+    return 0;
+}
+
+int unet_gradients(UNet *N, hipStream_t st, const double *src, const double *mask, const double *target,
+                   float *grad_host, float *loss) {
+    int rc = unet_forward(N, st, src, mask, nullptr);
+    if (rc) return rc;
+    rc = unet_backward(N, st, target);
+    if (rc) return rc;
+    if (grad_host && hipMemcpyAsync(grad_host, N->G, N->nparam * 4, hipMemcpyDeviceToHost, st) != hipSuccess) return -2;
+    return unet_loss(N, st, loss);
+}
+
+int unet_train_step(UNet *N, hipStream_t st, const double *src, const double *mask, const double *target, float lr,
+                    float *loss) {
+    int rc = unet_forward(N, st, src, mask, nullptr);
+    if (rc) return rc;
+    rc = unet_backward(N, st, target);
+    if (rc) return rc;
+    N->step++;
+    const float b1 = 0.9f, b2 = 0.999f;
+    const float c1 = 1.f - powf(b1, (float)N->step), c2 = 1.f - powf(b2, (float)N->step);
+    hipLaunchKernelGGL(adam_kernel, dim3(nblk(N->nparam)), dim3(UB), 0, st, N->W, N->M, N->V, (const float *)N->G,
+                       (int)N->nparam, lr, b1, b2, c1, c2, 1e-8f);
+    rc = refresh_layouts(N, st);
+    if (rc) return rc;
+    return loss ? unet_loss(N, st, loss) : 0;
+}
+
+// ---------------------------------------------------------------------------
+// Flexible CG (Notay, SIAM J. Sci. Comput. 22 (2000) 1444; truncation 1) on
+// the FP64 operator of a multigrid plan, with the network as initial guess
+// and as (part of) the preconditioner. The network is nonlinear, so the
+// preconditioner varies between iterations: the new direction is
+// orthogonalised against the previous one, β = −(z·w)/(p·w) with w = A p,
+// and α = (p·r)/(p·w). The operator is singular (constants): b's mean is
+// removed and every z and the solution are projected to zero mean.
+//   mode 0: z = N(r)
+//   mode 1: z = N(r) + V(r − A N(r))   (a V-cycle on what the network leaves)
+//   mode 2: z = V(r)
+// ---------------------------------------------------------------------------
+__global__ void fcg_sums_kernel(const double *__restrict__ a, const double *__restrict__ b,
+                                const double *__restrict__ c, const double *__restrict__ d, int n,
+                                double *__restrict__ part) {
+    // part[blk] = (Σ a·b, Σ c·d, Σ a, Σ c)   (b or d null: 1)
+    __shared__ double sh[4][UB];
+    double s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const double ai = a[i], ci = c ? c[i] : 0.0;
+        s0 += ai * (b ? b[i] : 1.0);
+        s2 += ai;
+        if (c) { s1 += ci * (d ? d[i] : 1.0); s3 += ci; }
+    }
+    sh[0][threadIdx.x] = s0; sh[1][threadIdx.x] = s1; sh[2][threadIdx.x] = s2; sh[3][threadIdx.x] = s3;
+    __syncthreads();
+    for (int w = UB / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w)
+            for (int q = 0; q < 4; q++) sh[q][threadIdx.x] += sh[q][threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x < 4) part[blockIdx.x * 4 + threadIdx.x] = sh[threadIdx.x][0];
+}
+__global__ void fcg_finalize_kernel(const double *__restrict__ part, int nb, double *__restrict__ out) {
+    if (threadIdx.x >= 4) return;
+    double s = 0;
+    for (int b = 0; b < nb; b++) s += part[b * 4 + threadIdx.x];
+    out[threadIdx.x] = s;
+}
+// y = a·x + b·y − m   (x may be null: y = b·y − m)
+__global__ void fcg_axpby_kernel(double *__restrict__ y, const double *__restrict__ x, double a, double b, double m,
+                                 int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    y[i] = (x ? a * x[i] : 0.0) + b * y[i] - m;
+}
+// x += α p; r −= α w
+__global__ void fcg_update_kernel(double *__restrict__ x, double *__restrict__ r, const double *__restrict__ p,
+                                  const double *__restrict__ w, double alpha, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    x[i] += alpha * p[i];
+    r[i] -= alpha * w[i];
+}
+
+struct FCGState {
+    int n = 0;
+    double *b = nullptr, *x = nullptr, *r = nullptr, *z = nullptr, *p = nullptr, *w = nullptr, *t = nullptr;
+    double *part = nullptr, *sc = nullptr;
+};
+
+FCGState *fcg_create(int n) {
+    FCGState *F = new FCGState();
+    F->n = n;
+    bool ok = true;
+    for (double **q : {&F->b, &F->x, &F->r, &F->z, &F->p, &F->w, &F->t})
+        if (hipMalloc(q, sizeof(double) * n) != hipSuccess) ok = false;
+    if (hipMalloc(&F->part, sizeof(double) * 4 * 1024) != hipSuccess) ok = false;
+    if (hipMalloc(&F->sc, sizeof(double) * 4) != hipSuccess) ok = false;
+    if (!ok) {
+        fcg_destroy(F);
+        return nullptr;
+    }
+    return F;
+}
+void fcg_destroy(FCGState *F) {
+    if (!F) return;
+    for (double *q : {F->b, F->x, F->r, F->z, F->p, F->w, F->t, F->part, F->sc})
+        if (q) (void)hipFree(q);
+    delete F;
+}
+
+// (Σ a·b, Σ c·d, Σ a, Σ c) to the host
+static void fcg_sums(FCGState *F, hipStream_t st, const double *a, const double *b, const double *c, const double *d,
+                     double out[4]) {
+    const int nb = 1024;
+    hipLaunchKernelGGL(fcg_sums_kernel, dim3(nb), dim3(UB), 0, st, a, b, c, d, F->n, F->part);
+    hipLaunchKernelGGL(fcg_finalize_kernel, dim3(1), dim3(64), 0, st, F->part, nb, F->sc);
+    hipMemcpyAsync(out, F->sc, sizeof(double) * 4, hipMemcpyDeviceToHost, st);
+    hipStreamSynchronize(st);
+}
+static void fcg_project(FCGState *F, hipStream_t st, double *y) {
+    double s[4];
+    fcg_sums(F, st, y, nullptr, nullptr, nullptr, s);
+    hipLaunchKernelGGL(fcg_axpby_kernel, dim3(nblk(F->n)), dim3(UB), 0, st, y, (const double *)nullptr, 0.0, 1.0,
+                       s[2] / F->n, F->n);
+}
+// z = M(r), mean-free
+static int fcg_precond(FCGState *F, MGPlan *P, UNet *N, const double *mask, int mode, hipStream_t st) {
+    if (mode == 2) {
+        mg_precond_apply(P, st, F->z, F->r);
+    } else {
+        int rc = unet_forward(N, st, F->r, mask, F->z);
+        if (rc) return rc;
+        if (mode == 1) {
+            mg_apply(P, st, F->t, F->z);  // t = A z1
+            hipLaunchKernelGGL(fcg_axpby_kernel, dim3(nblk(F->n)), dim3(UB), 0, st, F->t, (const double *)F->r, 1.0,
+                               -1.0, 0.0, F->n);  // t = r − A z1
+            mg_precond_apply(P, st, F->t, F->t);
+            hipLaunchKernelGGL(fcg_axpby_kernel, dim3(nblk(F->n)), dim3(UB), 0, st, F->z, (const double *)F->t, 1.0,
+                               1.0, 0.0, F->n);  // z = z1 + V(t)
+        }
+    }
+    fcg_project(F, st, F->z);
+    return 0;
+}
+
+int fcg_solve(FCGState *F, MGPlan *P, UNet *N, const double *mask, int mode, int use_guess, hipStream_t st,
+              double *xout, const double *bin, double tol, int maxit, int *iters_out, double *resid_out,
+              double *bnorm_out) {
+    const int n = F->n;
+    const int nb = nblk(n);
+    double s[4];
+    // b' = b − mean(b)
+    hipMemcpyAsync(F->b, bin, sizeof(double) * n, hipMemcpyDeviceToDevice, st);
+    fcg_project(F, st, F->b);
+    fcg_sums(F, st, F->b, F->b, nullptr, nullptr, s);
+    const double bnorm = sqrt(s[0]);
+    *bnorm_out = bnorm;
+    if (bnorm == 0) {
+        hipMemsetAsync(xout, 0, sizeof(double) * n, st);
+        *iters_out = 0;
+        *resid_out = 0;
+        return 3;
+    }
+    // x0 = N(b') (projected) and r = b' − A x0, or x0 = 0 and r = b'
+    if (use_guess && N) {
+        int rc = unet_forward(N, st, F->b, mask, F->x);
+        if (rc) return -1;
+        fcg_project(F, st, F->x);
+        mg_apply(P, st, F->w, F->x);
+        hipMemcpyAsync(F->r, F->b, sizeof(double) * n, hipMemcpyDeviceToDevice, st);
+        hipLaunchKernelGGL(fcg_axpby_kernel, dim3(nb), dim3(UB), 0, st, F->r, (const double *)F->w, -1.0, 1.0, 0.0, n);
+    } else {
+        hipMemsetAsync(F->x, 0, sizeof(double) * n, st);
+        hipMemcpyAsync(F->r, F->b, sizeof(double) * n, hipMemcpyDeviceToDevice, st);
+    }
+    fcg_sums(F, st, F->r, F->r, nullptr, nullptr, s);
+    double rnorm = sqrt(s[0]);
+    int iters = 0, status = rnorm <= tol * bnorm ? 0 : 1;
+    if (status != 0) {
+        if (fcg_precond(F, P, N, mask, mode, st)) return -1;
+        hipMemcpyAsync(F->p, F->z, sizeof(double) * n, hipMemcpyDeviceToDevice, st);
+    }
+    while (status != 0 && iters < maxit) {
+        mg_apply(P, st, F->w, F->p);
+        fcg_sums(F, st, F->p, F->w, F->p, F->r, s);  // p·w, p·r
+        const double pw = s[0], pr = s[1];
+        if (!(fabs(pw) > 1e-300) || !std::isfinite(pw)) { status = 2; break; }
+        const double alpha = pr / pw;
+        hipLaunchKernelGGL(fcg_update_kernel, dim3(nb), dim3(UB), 0, st, F->x, F->r, (const double *)F->p,
+                           (const double *)F->w, alpha, n);
+        iters++;
+        fcg_sums(F, st, F->r, F->r, nullptr, nullptr, s);
+        rnorm = sqrt(s[0]);
+        if (rnorm <= tol * bnorm) { status = 0; break; }
+        if (fcg_precond(F, P, N, mask, mode, st)) return -1;
+        fcg_sums(F, st, F->z, F->w, nullptr, nullptr, s);
+        const double beta = -s[0] / pw;
+        hipLaunchKernelGGL(fcg_axpby_kernel, dim3(nb), dim3(UB), 0, st, F->p, (const double *)F->z, 1.0, beta, 0.0, n);
+    }
+    hipMemcpyAsync(xout, F->x, sizeof(double) * n, hipMemcpyDeviceToDevice, st);
+    fcg_project(F, st, xout);
+    *iters_out = iters;
+    *resid_out = rnorm;
+    return status;
+}
+
+}  // namespace aeo

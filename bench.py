@@ -108,6 +108,71 @@ def poisson_ms(reps=20):
     return (time.perf_counter() - t0) / reps * 1e3
 
 
+CUT_N = (512, 512, 192)
+CUT_DX = 2.0
+
+
+def _residual64(solver, x_dev, b):
+    """‖(b − mean b) − A x‖ / ‖b − mean b‖ recomputed in FP64 with the solver's operator."""
+    import aeolus_hip as hip
+    ax = hip.DeviceArray(CUT_N)
+    solver.apply(ax, x_dev)
+    r = (b - b.mean()) - ax.download()
+    return float(np.linalg.norm(r) / np.linalg.norm(b - b.mean()))
+
+
+def poisson_legs(reps=3, weights=None, log=print):
+    """The standalone cut-cell Poisson solve at 512x512x192, Δ = 2 m, 64 seeded
+    buildings, source = divergence of a seeded k^-5/3 velocity: ms per solve
+    to ‖r‖ ≤ 1e-8‖b‖ for MG-PCG (FP32 V-cycle, 2+2 Jacobi; and FP64 V-cycle,
+    red-black Gauss-Seidel) and for the learned flexible CG."""
+    import ctypes
+    import aeolus_hip as hip
+    from aeolus_hip import _lib as L
+    from aeolus_hip import workloads as W
+    from aeolus_hip.learned import FlexibleCG, UNet
+    t0 = time.perf_counter()
+    w = W.cutcell_workload(CUT_N, CUT_DX, n_buildings=64)
+    log(f"cut-cell workload built in {time.perf_counter() - t0:.1f} s")
+    b = w["source"]
+    out = {}
+
+    def run(name, solver):
+        n = CUT_N[0] * CUT_N[1] * CUT_N[2]
+        buf = ctypes.c_void_p(hip.source_buffer(solver))
+        phi = hip.DeviceArray(CUT_N)
+        ms = []
+        for rep in range(reps + 1):
+            L.check(L.init().aeo_memcpy_h2d(buf, np.asfortranarray(b).ctypes.data_as(ctypes.c_void_p), 8 * n))
+            hip.sync()
+            t = time.perf_counter()
+            hip.solve_source_buffer(phi, solver)
+            hip.sync()
+            if rep:
+                ms.append((time.perf_counter() - t) * 1e3)
+        it, rel = hip.solver_stats(solver)
+        base = solver if hasattr(solver, "apply") else solver._keep[0]
+        out[name] = {"ms": float(np.median(ms)), "iterations": it, "residual": rel,
+                     "residual_fp64_recomputed": _residual64(base, phi, b), "status": solver.last_status}
+        log(f"{name}: {out[name]}")
+
+    mg32 = hip.MGPreconditionedCG(CUT_N, (CUT_DX,) * 3, coeff=w["coeff"], tol=1e-8, maxit=500, nu1=2, nu2=2,
+                                  smoother="jacobi", omega=0.8, vcycle="f32")
+    run("mgpcg_f32_jacobi", mg32)
+    mg64 = hip.MGPreconditionedCG(CUT_N, (CUT_DX,) * 3, coeff=w["coeff"], tol=1e-8, maxit=500, nu1=2, nu2=2,
+                                  smoother="rbgs", vcycle="f64")
+    run("mgpcg_f64_rbgs", mg64)
+    del mg64
+    if weights is not None and os.path.exists(weights):
+        net = UNet(CUT_N, in_channels=2, seed=0)
+        net.set_params(np.load(weights, allow_pickle=False))
+        mask = hip.DeviceArray.from_numpy(w["mask"])
+        run("fcg_net_guess_vcycle", FlexibleCG(mg32, net, mask, mode="vcycle", guess=True, tol=1e-8, maxit=500))
+        run("fcg_net_vcycle", FlexibleCG(mg32, net, mask, mode="net+vcycle", guess=True, tol=1e-8, maxit=500))
+        run("fcg_vcycle", FlexibleCG(mg32, None, None, mode="vcycle", guess=False, tol=1e-8, maxit=500))
+    return out
+
+
 def cpu_baseline(threads):
     """The CPU oracle (FP64, OpenMP) on a bounded sample of the same workload:
     the full 256x256x128 grid, one warm-up step and two timed RK3 steps."""
@@ -151,6 +216,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--no-poisson", action="store_true", help="skip the cut-cell Poisson leg")
+    ap.add_argument("--weights", default=os.path.join(ROOT, "aeolus.jl_amd", "aeolus_hip", "weights",
+                                                      "unet_cutcell_dx2.npy"))
     ap.add_argument("--profile-pass", action="store_true",
                     help="a short FP64 run for rocprofv3 counter passes: the steps, then one 1 GiB copy kernel "
                          "(the calibration of FETCH_SIZE / WRITE_SIZE)")
@@ -167,6 +235,7 @@ def main():
     t64, prof64, div64 = run_gpu(args.steps, args.warmup, 0)
     t32, prof32, div32 = run_gpu(args.steps, args.warmup, 1)
     pms = poisson_ms()
+    legs = None if args.no_poisson else poisson_legs(weights=args.weights, log=lambda m: print(m, file=sys.stderr))
     cells = NX * NY * NZ
 
     n_t, ms_t = prof64["tendency"]
@@ -201,7 +270,15 @@ def main():
                                "FFT-xy + FP64 Thomas-z projection per substage",
                    "grid": [NX, NY, NZ], "dt": DT, "parallelism": "single-gpu"},
         "fp32_weno": {"value": cells / t32, "ms_per_step": t32 * 1e3, "dtype": "f32 tendencies, f64 projection"},
-        "poisson_ms": {"direct_column_256x256x128": pms, "mgpcg": None, "learned_fcg": None},
+        "poisson_ms": {
+            "direct_column_256x256x128": pms,
+            "mgpcg": None if legs is None else legs["mgpcg_f32_jacobi"]["ms"],
+            "learned_fcg": None if legs is None or "fcg_net_vcycle" not in legs else min(
+                legs["fcg_net_vcycle"]["ms"], legs["fcg_net_guess_vcycle"]["ms"]),
+            "workload": "cut-cell 512x512x192, dx 2 m, 64 seeded buildings, source = div of a seeded k^-5/3 "
+                        "velocity; to ||r|| <= 1e-8 ||b||",
+            "legs": legs,
+        },
         "max_divergence_after_step": div64,
         "ms_per_step_by_kernel_f64": breakdown,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",

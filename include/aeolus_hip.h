@@ -175,6 +175,42 @@ int aeo_max_divergence(aeo_grid *g, int precision, const void *u, const void *v,
 /* _max_advective_rate (src/TimeIntegration.jl:164-182): dt = C / rate */
 int aeo_max_advective_rate(aeo_grid *g, int precision, const void *u, const void *v, const void *w, double *out);
 
+/* ---- the learned preconditioner ------------------------------------------
+ * No reference counterpart: the reference has no network (its nearest hook
+ * is src/ML.jl:40-56). A 3-level 3-D U-Net on an nx x ny x nz grid (each
+ * divisible by 4): 3x3x3 convolutions with 16/32/64 channels, bias and
+ * ReLU, 2x2x2 mean pooling, nearest upsampling, skip concatenation and a
+ * 1x1 head; periodic in x and y, zero-padded in z. BF16 weights and
+ * activations on MFMA, FP32 accumulation, FP32 master weights and Adam.
+ * The source is normalised by its RMS on the way in and the output scaled
+ * back. in_channels: 1 (source) or 2 (source and fluid mask). */
+typedef struct aeo_unet aeo_unet;
+int aeo_unet_create(int nx, int ny, int nz, int in_channels, unsigned long long seed, aeo_unet **h);
+int aeo_unet_destroy(aeo_unet *h);
+int aeo_unet_num_params(aeo_unet *h, long long *n);
+/* the FP32 master parameters (per 3x3x3 layer W[tap][c][n] then b[n]; the
+ * head's 16 weights and bias last); setting them resets Adam */
+int aeo_unet_get_params(aeo_unet *h, float *host);
+int aeo_unet_set_params(aeo_unet *h, const float *host);
+/* out = network(src, mask): FP64 interior-sized device arrays (mask NULL
+ * for one input channel) */
+int aeo_unet_forward(aeo_unet *h, const double *src, const double *mask, double *out);
+/* the loss mean((net − target/rms(src))²) of one sample and, if grad_host,
+ * its gradient (host FP32, num_params) */
+int aeo_unet_gradients(aeo_unet *h, const double *src, const double *mask, const double *target, float *grad_host,
+                       float *loss);
+/* one Adam step (β = 0.9, 0.999) on one sample; loss may be NULL */
+int aeo_unet_train_step(aeo_unet *h, const double *src, const double *mask, const double *target, float lr,
+                        float *loss);
+/* Flexible CG (Notay 2000, one previous direction kept) on the operator of
+ * an MG-preconditioned CG solver `base` (same grid as the network), with
+ * the network as initial guess (use_guess) and preconditioner. mode 0: the
+ * network; 1: the network, then one V-cycle of `base` on the residual it
+ * leaves; 2: the V-cycle only. The operator's null space (constants) is
+ * projected out. Stats as for MGPreconditionedCG. */
+int aeo_poisson_fcg_create(aeo_poisson *base, aeo_unet *net, const double *mask, int mode, int use_guess, double tol,
+                           int maxit, aeo_poisson **h);
+
 /* ---- kernel timing (no reference counterpart; the timing the benchmark and
  * the roofline read): HIP events around every launch of the named kernels on
  * the library's stream */

@@ -89,6 +89,9 @@ def lib():
             "orc_mgpcg_solve": (i, [vp, pd, pd, d, i, P(i), pd, pd]),
             "orc_mg_apply": (None, [vp, pd, pd]),
             "orc_cutcell_heightmap": (None, [pg, pd, i, d, pd, pd, pd, pd, pd]),
+            "orc_unet_num_params": (ctypes.c_longlong, [i]),
+            "orc_unet_forward": (None, [P(ctypes.c_float), i, i, i, i, pd, pd, i, pd]),
+            "orc_unet_gradients": (None, [P(ctypes.c_float), i, i, i, i, pd, pd, pd, i, pd, pd]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -289,3 +292,41 @@ class Stepper:
             lib().orc_stepper_destroy(self.h)
         except Exception:
             pass
+
+
+# ---------------------------------------------------------------------------
+# the learned preconditioner's network (unet_oracle.c)
+# ---------------------------------------------------------------------------
+def unet_num_params(inc):
+    return int(lib().orc_unet_num_params(inc))
+
+
+def _fptr(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+
+
+def unet_forward(params, src, mask=None, bf16=True):
+    """The network's output for an interior-sized source (and mask), FP64."""
+    params = np.ascontiguousarray(params, dtype=np.float32)
+    src = np.asfortranarray(src, dtype=np.float64)
+    nx, ny, nz = src.shape
+    inc = 1 if mask is None else 2
+    out = np.zeros(src.shape, order="F")
+    m = None if mask is None else ptr(np.asfortranarray(mask, dtype=np.float64))
+    lib().orc_unet_forward(_fptr(params), nx, ny, nz, inc, ptr(src), m, int(bf16), ptr(out))
+    return out
+
+
+def unet_gradients(params, src, target, mask=None, bf16=True):
+    """(loss, gradient) of the mean-square loss against target · (1/rms(src))."""
+    params = np.ascontiguousarray(params, dtype=np.float32)
+    src = np.asfortranarray(src, dtype=np.float64)
+    target = np.asfortranarray(target, dtype=np.float64)
+    nx, ny, nz = src.shape
+    inc = 1 if mask is None else 2
+    grad = np.zeros(unet_num_params(inc))
+    loss = ctypes.c_double()
+    m = None if mask is None else ptr(np.asfortranarray(mask, dtype=np.float64))
+    lib().orc_unet_gradients(_fptr(params), nx, ny, nz, inc, ptr(src), m, ptr(target), int(bf16), ptr(grad),
+                             ctypes.byref(loss))
+    return loss.value, grad

@@ -1,0 +1,123 @@
+"""GPU tests of the learned preconditioner: the U-Net's MFMA forward and its
+backward kernels against the CPU network oracle, training on the GPU, and the
+flexible CG that uses the network.
+
+Tolerances. The GPU stores activations and gradients in BF16 (8-bit
+mantissa) and accumulates in FP32 on MFMA; the oracle rounds to BF16 at the
+same points but accumulates in FP64, so a value near a BF16 rounding boundary
+can land one BF16 ulp (2^-8 relative) apart and carry that through the later
+layers. The forward output is held to 1e-2 relative L2, each layer's
+gradient to 5e-2 relative L2 (with a cosine above 0.998 over all parameters).
+"""
+import numpy as np
+import pytest
+
+from helpers import rel_err
+
+pytestmark = pytest.mark.gpu
+
+
+def _sample(shape, seed):
+    rng = np.random.default_rng(seed)
+    src = rng.standard_normal(shape)
+    mask = (rng.random(shape) > 0.3).astype(np.float64)
+    tgt = rng.standard_normal(shape)
+    return src, mask, tgt
+
+
+def _layer_slices(inc):
+    spec = [(inc, 16), (16, 16), (16, 32), (32, 32), (32, 64), (64, 64), (96, 32), (32, 32), (48, 16), (16, 16)]
+    out, off = [], 0
+    for cin, cout in spec:
+        out.append((slice(off, off + 27 * cin * cout), slice(off + 27 * cin * cout, off + 27 * cin * cout + cout)))
+        off += 27 * cin * cout + cout
+    out.append((slice(off, off + 16), slice(off + 16, off + 17)))
+    return out
+
+
+def test_unet_params_roundtrip(hip):
+    from aeolus_hip.learned import UNet
+    net = UNet((16, 16, 8), in_channels=2, seed=11)
+    p = net.params()
+    assert p.size == net.num_params and np.isfinite(p).all() and np.abs(p).max() > 0
+    q = (p * 0.5).astype(np.float32)
+    net.set_params(q)
+    assert np.array_equal(net.params(), q)
+
+
+@pytest.mark.parametrize("shape,inc", [((16, 16, 8), 2), ((12, 8, 16), 1)])
+def test_unet_forward_matches_oracle(hip, orc, shape, inc):
+    from aeolus_hip.learned import UNet
+    net = UNet(shape, in_channels=inc, seed=5)
+    src, mask, _ = _sample(shape, 1)
+    m = mask if inc == 2 else None
+    out = net(src, m)
+    ref = orc.unet_forward(net.params(), src, m, bf16=True)
+    e = np.linalg.norm(out - ref) / np.linalg.norm(ref)
+    print(f"forward {shape} inc={inc}: relative L2 vs BF16 oracle {e:.2e}")
+    assert e < 1e-2
+
+
+def test_unet_gradients_match_oracle(hip, orc):
+    from aeolus_hip.fields import DeviceArray
+    from aeolus_hip.learned import UNet
+    shape = (16, 16, 8)
+    net = UNet(shape, in_channels=2, seed=7)
+    src, mask, tgt = _sample(shape, 2)
+    d = [DeviceArray.from_numpy(a) for a in (src, mask, tgt)]
+    loss, g = net.gradients(d[0], d[2], d[1])
+    loss_ref, g_ref = orc.unet_gradients(net.params(), src, tgt, mask, bf16=True)
+    assert abs(loss - loss_ref) < 1e-3 * loss_ref
+    cos = float(g @ g_ref) / (np.linalg.norm(g) * np.linalg.norm(g_ref))
+    worst = 0.0
+    for li, (ws, bs) in enumerate(_layer_slices(2)):
+        for sl in (ws, bs):
+            e = np.linalg.norm(g[sl] - g_ref[sl]) / max(np.linalg.norm(g_ref[sl]), 1e-30)
+            worst = max(worst, e)
+            assert e < 5e-2, (li, e)
+    print(f"loss {loss:.6e} (oracle {loss_ref:.6e}); worst layer gradient rel L2 {worst:.2e}; cosine {cos:.6f}")
+    assert cos > 0.998
+
+
+def test_unet_training_reduces_loss(hip):
+    # Adam on a fixed set of Poisson pairs: the mean loss falls
+    from aeolus_hip import MGPreconditionedCG, solve_interior
+    from aeolus_hip import workloads as W
+    from aeolus_hip.fields import DeviceArray
+    from aeolus_hip.learned import UNet, train
+    N = (32, 32, 16)
+    samples = []
+    for s in range(6):
+        w = W.cutcell_workload(N, 2.0, n_buildings=4, seed=100 + s, height=(4.0, 20.0))
+        mg = MGPreconditionedCG(N, (2.0,) * 3, coeff=w["coeff"], tol=1e-10, maxit=300)
+        x = solve_interior(mg, w["source"])
+        fluid = w["mask"] > 0
+        x[~fluid] = 0.0
+        x[fluid] -= x[fluid].mean()
+        samples.append((DeviceArray.from_numpy(w["source"]), DeviceArray.from_numpy(x),
+                        DeviceArray.from_numpy(w["mask"])))
+    net = UNet(N, in_channels=2, seed=1)
+    losses = train(net, samples, epochs=40, lr=2e-3)
+    print(f"training loss: first epoch {losses[0]:.4e}, last {losses[-1]:.4e}")
+    assert losses[-1] < 0.5 * losses[0]
+
+
+def test_fcg_vcycle_mode_matches_mgpcg(hip):
+    # without the network the flexible CG is MG-PCG with a flexible update
+    from aeolus_hip import MGPreconditionedCG, solve_interior, solver_stats
+    from aeolus_hip import workloads as W
+    from aeolus_hip.learned import FlexibleCG
+    N = (32, 32, 16)
+    w = W.cutcell_workload(N, 2.0, n_buildings=4, seed=9, height=(4.0, 20.0))
+    mg = MGPreconditionedCG(N, (2.0,) * 3, coeff=w["coeff"], tol=1e-8, maxit=300, vcycle="f32", smoother="jacobi")
+    x_mg = solve_interior(mg, w["source"])
+    it_mg, r_mg = solver_stats(mg)
+    f = FlexibleCG(mg, None, None, mode="vcycle", guess=False, tol=1e-8, maxit=300)
+    x_f = solve_interior(f, w["source"])
+    it_f, r_f = solver_stats(f)
+    print(f"MG-PCG {it_mg} its ({r_mg:.2e}); flexible CG, V-cycle {it_f} its ({r_f:.2e})")
+    assert f.last_status == "converged" and r_f <= 1e-8
+    assert abs(it_f - it_mg) <= 3
+    fluid = w["mask"] > 0
+    d = x_f[fluid] - x_mg[fluid]
+    assert np.abs(d - d.mean()).max() < 1e-6 * np.abs(x_mg[fluid]).max()
