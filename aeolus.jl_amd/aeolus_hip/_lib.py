@@ -80,6 +80,12 @@ SIGNATURES = {
     "aeo_step_euler": [_vp, _vp, _vp, _vp, _vp, _vp, _d],
     "aeo_max_divergence": [_vp, _i, _vp, _vp, _vp, _pd],
     "aeo_max_advective_rate": [_vp, _i, _vp, _vp, _vp, _pd],
+    "aeo_kinetic_energy": [_vp, _i, _vp, _vp, _vp, _pd],
+    "aeo_cutcell_divergence": [_vp, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
+    "aeo_cutcell_gradient": [_vp, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
+    "aeo_cutcell_laplacian": [_vp, _i, _vp, _vp, _vp, _vp, _vp, _vp, _i],
+    "aeo_cutcell_project_source": [_vp, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
+    "aeo_cutcell_project_correct": [_vp, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _i, _d],
     "aeo_unet_create": [_i, _i, _i, _i, ctypes.c_ulonglong, _pvp],
     "aeo_unet_destroy": [_vp],
     "aeo_unet_num_params": [_vp, ctypes.POINTER(ctypes.c_longlong)],

@@ -140,3 +140,75 @@ def cutcell_source(u, v, w, ax, ay, az, vol_eff, N, H, spacing):
     b = (fx + fy) + fz
     b[~(vol_eff[I, J, K] > 0)] = 0.0
     return np.asfortranarray(b)
+
+
+# ---------------------------------------------------------------------------
+# the operators on the GPU (cutcell.hip through the C ABI)
+# ---------------------------------------------------------------------------
+class CutCellGeometry:
+    """CutCellGeometry (src/CutCell.jl:177-195) on the device: apertures ax,
+    ay, az and the effective volume fractions vol_eff (FP64 storage arrays),
+    built on the host by heightmap_geometry."""
+
+    def __init__(self, grid, height, nsub=4, vol_min=1e-3):
+        from .fields import DeviceArray
+        N = (grid.Nx, grid.Ny, grid.Nz)
+        vol, ve, ax, ay, az = heightmap_geometry(N, (grid.dx, grid.dy, grid.dz), grid.H, height, nsub, vol_min)
+        self.grid = grid
+        self.host = dict(vol=vol, vol_eff=ve, ax=ax, ay=ay, az=az)
+        self.ax, self.ay, self.az, self.vol_eff = (DeviceArray.from_numpy(a) for a in (ax, ay, az, ve))
+
+    def coefficients(self):
+        """Face coefficients a² of vol_eff·D·G for the multigrid solvers."""
+        h = self.host
+        return mg_coefficients(h["ax"], h["ay"], h["az"], (self.grid.Nx, self.grid.Ny, self.grid.Nz), self.grid.H)
+
+    def fluid_mask(self):
+        H, g = self.grid.H, self.grid
+        return np.asfortranarray((self.host["vol_eff"][H:H + g.Nx, H:H + g.Ny, H:H + g.Nz] > 0).astype(np.float64))
+
+    def _geo(self):
+        from . import _lib as L
+        return L.dptr(self.ax), L.dptr(self.ay), L.dptr(self.az)
+
+
+def divergence(div, u, v, w, geom):
+    """cutcell_divergence!(div, u, v, w, geom; redistribute = false) (src/CutCell.jl:602-626)."""
+    from . import _lib as L
+    L.check(L.init().aeo_cutcell_divergence(geom.grid.handle, div.precision, L.dptr(div), L.dptr(u), L.dptr(v),
+                                            L.dptr(w), *geom._geo(), L.dptr(geom.vol_eff)))
+    return div
+
+
+def gradient(gx, gy, gz, p, geom):
+    """cutcell_gradient_x!/_y!/_z! (src/CutCell.jl:666-700); p FP64 with its halo filled."""
+    from . import _lib as L
+    L.check(L.init().aeo_cutcell_gradient(geom.grid.handle, gx.precision, L.dptr(gx), L.dptr(gy), L.dptr(gz),
+                                          L.dptr(p), *geom._geo()))
+    return gx, gy, gz
+
+
+def laplacian(out, f, geom, aperture_power=2):
+    """cutcell_laplacian!(out, f, geom; aperture_power) (src/CutCell.jl:735-781)."""
+    from . import _lib as L
+    L.check(L.init().aeo_cutcell_laplacian(geom.grid.handle, out.precision, L.dptr(out), L.dptr(f), *geom._geo(),
+                                           L.dptr(geom.vol_eff), aperture_power))
+    return out
+
+
+def project(u, v, w, geom, solver, bcs=("periodic", "periodic", "neumann")):
+    """The cut-cell projection: solve vol_eff·D·G φ = vol_eff·D u* with
+    `solver` (an MGPreconditionedCG on geom.coefficients()), then u ← u* − G φ.
+    Returns φ (FP64 interior-sized DeviceArray)."""
+    from . import _lib as L
+    from .fields import DeviceArray
+    from .pressure import solve_source_buffer
+    g = geom.grid
+    L.check(L.init().aeo_cutcell_project_source(g.handle, u.precision, solver.handle, L.dptr(u), L.dptr(v),
+                                                L.dptr(w), *geom._geo(), L.dptr(geom.vol_eff)))
+    phi = DeviceArray((g.Nx, g.Ny, g.Nz))
+    solve_source_buffer(phi, solver)
+    bk = {"periodic": L.PERIODIC, "neumann": L.NOFLUX, "noflux": L.NOFLUX}
+    L.check(L.init().aeo_cutcell_project_correct(g.handle, u.precision, L.dptr(u), L.dptr(v), L.dptr(w), L.dptr(phi),
+                                                 *geom._geo(), *[bk[b] for b in bcs], 1.0))
+    return phi

@@ -51,8 +51,9 @@ class FFTPoissonSolver(PressureSolver):
 
 
 class ColumnTridiagonalSolver(PressureSolver):
-    """ColumnTridiagonalSolver (src/PressureSolvers.jl:544-805): periodic x-y
-    transform + FP64 Thomas in z + mean mode, with the column coefficients
+    """ColumnTridiagonalSolver (src/PressureSolvers.jl:544-805): x-y transform
+    (R2HC on periodic axes, REDFT10 on Neumann ones) + FP64 Thomas in z + mean
+    mode, with the column coefficients
     lower, upper, hweight, measure (Nz) and inv_coupling (Nz+1), gauge (Nz)."""
 
     kind = "column"
@@ -72,6 +73,8 @@ class ColumnTridiagonalSolver(PressureSolver):
         L.check(L.init().aeo_poisson_column_create(Nx, Ny, Nz, float(dxy[0]), float(dxy[1]), bx, by,
                                                    *[_pd(v) for v in vecs], ctypes.byref(h)))
         self._vecs = vecs
+        self.axis_bcs = ("periodic" if bx == L.PERIODIC else "neumann",
+                         "periodic" if by == L.PERIODIC else "neumann", "neumann")
         super().__init__(h.value, (Nx, Ny, Nz))
 
     @classmethod

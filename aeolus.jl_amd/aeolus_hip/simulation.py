@@ -87,6 +87,14 @@ class BoussinesqSimulation:
                                             L.dptr(self.w), ctypes.byref(out)))
         return out.value
 
+    def kinetic_energy(self):
+        """kinetic_energy(sim) (src/Simulations.jl:992-1015): ½ ∫(u² + v² + w²) dV,
+        u, v, w as stored on their faces."""
+        out = ctypes.c_double()
+        L.check(L.init().aeo_kinetic_energy(self.grid.handle, self.precision, L.dptr(self.u), L.dptr(self.v),
+                                            L.dptr(self.w), ctypes.byref(out)))
+        return out.value
+
     def advective_dt(self, C=0.5):
         """dt = C / max Σ|uᵢ|/Δxᵢ (CFLConstraints C_advective, src/TimeIntegration.jl:127-182)."""
         self.fill_velocity_halos()

@@ -431,16 +431,19 @@ extern "C" int aeo_poisson_column_create(int Nx, int Ny, int Nz, double dx, doub
     // src/PressureSolvers.jl:631-636
     if (!(lower[0] == 0 && upper[Nz - 1] == 0))
         return fail(AEO_ERR_ARG, "the bottom and top faces carry no flux: lower[1] = upper[Nz] = 0");
-    if (bc_x != AEO_PERIODIC || bc_y != AEO_PERIODIC)
-        return fail(AEO_ERR_UNSUPPORTED, "column solver: only periodic lateral boundaries in this build");
+    for (int bc : {bc_x, bc_y})
+        if (bc != AEO_PERIODIC && bc != AEO_NOFLUX)
+            return fail(AEO_ERR_ARG, "column solver: lateral boundaries periodic (R2HC) or Neumann (REDFT10)");
     auto h = std::make_unique<aeo_poisson>();
     h->kind = 1; h->Nx = Nx; h->Ny = Ny; h->Nz = Nz;
-    h->periodic_x = h->periodic_y = true;
+    h->periodic_x = bc_x == AEO_PERIODIC;
+    h->periodic_y = bc_y == AEO_PERIODIC;
     h->periodic_z = false;
     char err[512] = "";
     h->sp = spectral_create(Nx, Ny, Nz, g_ctx->stream, err, sizeof err);
     if (!h->sp) return fail(AEO_ERR_UNSUPPORTED, "%s", err);
-    if (spectral_column_setup(h->sp, dx, dy, lower, upper, hweight, measure, inv_coupling, gauge)) {
+    if (spectral_column_setup(h->sp, dx, dy, lower, upper, hweight, measure, inv_coupling, gauge,
+                              bc_x == AEO_NOFLUX, bc_y == AEO_NOFLUX)) {
         spectral_destroy(h->sp);
         return fail(AEO_ERR_HIP, "column solver setup failed");
     }
@@ -620,6 +623,84 @@ extern "C" int aeo_poisson_destroy(aeo_poisson *h) {
     hipFree(h->phi);
     delete h;
     return AEO_OK;
+}
+
+// ---------------------------------------------------------------------------
+// cut-cell operators (src/CutCell.jl:602-781), regular grids
+// ---------------------------------------------------------------------------
+template <typename T>
+static int cutcell_call(aeo_grid *g, int op, T *a, T *b, T *c, const T *f0, const T *f1, const T *f2, double *buf,
+                        const double *phi, int phi_interior, int bcx, int bcy, int bcz, const double *ax,
+                        const double *ay, const double *az, const double *ve, int flag, double s) {
+    if (launch_cutcell<T>(g_ctx->stream, op, g->m64, a, b, c, f0, f1, f2, buf, phi, phi_interior, bcx, bcy, bcz, ax,
+                          ay, az, ve, flag, s))
+        return fail(AEO_ERR_ARG, "bad cut-cell operator");
+    HIPCHK(hipGetLastError());
+    return AEO_OK;
+}
+#define CC_CHECK()                                                                                    \
+    NEED_CTX();                                                                                       \
+    if (!g) return fail(AEO_ERR_ARG, "null grid");                                                    \
+    if (g->stretched) return fail(AEO_ERR_UNSUPPORTED, "cut-cell operators: regular grids only");     \
+    if (precision != AEO_F64 && precision != AEO_F32) return fail(AEO_ERR_ARG, "bad precision %d", precision)
+
+extern "C" int aeo_cutcell_divergence(aeo_grid *g, int precision, void *div, const void *u, const void *v,
+                                      const void *w, const double *ax, const double *ay, const double *az,
+                                      const double *vol_eff) {
+    CC_CHECK();
+    if (!div || !u || !v || !w || !ax || !ay || !az || !vol_eff) return fail(AEO_ERR_ARG, "null argument");
+    if (precision == AEO_F64)
+        return cutcell_call<double>(g, 0, (double *)div, nullptr, nullptr, (const double *)u, (const double *)v,
+                                    (const double *)w, nullptr, nullptr, 0, 0, 0, 0, ax, ay, az, vol_eff, 0, 0);
+    return cutcell_call<float>(g, 0, (float *)div, nullptr, nullptr, (const float *)u, (const float *)v,
+                               (const float *)w, nullptr, nullptr, 0, 0, 0, 0, ax, ay, az, vol_eff, 0, 0);
+}
+extern "C" int aeo_cutcell_gradient(aeo_grid *g, int precision, void *gx, void *gy, void *gz, const double *p,
+                                    const double *ax, const double *ay, const double *az) {
+    CC_CHECK();
+    if (!gx || !gy || !gz || !p || !ax || !ay || !az) return fail(AEO_ERR_ARG, "null argument");
+    if (precision == AEO_F64)
+        return cutcell_call<double>(g, 1, (double *)gx, (double *)gy, (double *)gz, nullptr, nullptr, nullptr,
+                                    nullptr, p, 0, 0, 0, 0, ax, ay, az, nullptr, 0, 0);
+    return cutcell_call<float>(g, 1, (float *)gx, (float *)gy, (float *)gz, nullptr, nullptr, nullptr, nullptr, p, 0,
+                               0, 0, 0, ax, ay, az, nullptr, 0, 0);
+}
+extern "C" int aeo_cutcell_laplacian(aeo_grid *g, int precision, void *out, const void *f, const double *ax,
+                                     const double *ay, const double *az, const double *vol_eff, int aperture_power) {
+    CC_CHECK();
+    if (!out || !f || !ax || !ay || !az || !vol_eff) return fail(AEO_ERR_ARG, "null argument");
+    if (aperture_power != 1 && aperture_power != 2) return fail(AEO_ERR_ARG, "aperture_power must be 1 or 2");
+    if (precision == AEO_F64)
+        return cutcell_call<double>(g, 2, (double *)out, nullptr, nullptr, (const double *)f, nullptr, nullptr,
+                                    nullptr, nullptr, 0, 0, 0, 0, ax, ay, az, vol_eff, aperture_power == 2, 0);
+    return cutcell_call<float>(g, 2, (float *)out, nullptr, nullptr, (const float *)f, nullptr, nullptr, nullptr,
+                               nullptr, 0, 0, 0, 0, ax, ay, az, vol_eff, aperture_power == 2, 0);
+}
+extern "C" int aeo_cutcell_project_source(aeo_grid *g, int precision, aeo_poisson *solver, const void *u,
+                                          const void *v, const void *w, const double *ax, const double *ay,
+                                          const double *az, const double *vol_eff) {
+    CC_CHECK();
+    if (!solver || !u || !v || !w || !ax || !ay || !az || !vol_eff) return fail(AEO_ERR_ARG, "null argument");
+    if (solver->Nx != g->Nx || solver->Ny != g->Ny || solver->Nz != g->Nz)
+        return fail(AEO_ERR_ARG, "solver and grid sizes differ");
+    if (precision == AEO_F64)
+        return cutcell_call<double>(g, 0, nullptr, nullptr, nullptr, (const double *)u, (const double *)v,
+                                    (const double *)w, solver->src, nullptr, 0, 0, 0, 0, ax, ay, az, vol_eff, 1, 0);
+    return cutcell_call<float>(g, 0, nullptr, nullptr, nullptr, (const float *)u, (const float *)v, (const float *)w,
+                               solver->src, nullptr, 0, 0, 0, 0, ax, ay, az, vol_eff, 1, 0);
+}
+extern "C" int aeo_cutcell_project_correct(aeo_grid *g, int precision, void *u, void *v, void *w, const double *phi,
+                                           const double *ax, const double *ay, const double *az, int bc_x,
+                                           int bc_y, int bc_z, double scale) {
+    CC_CHECK();
+    if (!u || !v || !w || !phi || !ax || !ay || !az) return fail(AEO_ERR_ARG, "null argument");
+    for (int bc : {bc_x, bc_y, bc_z})
+        if (bc != AEO_PERIODIC && bc != AEO_NOFLUX) return fail(AEO_ERR_ARG, "φ boundary: periodic or Neumann");
+    if (precision == AEO_F64)
+        return cutcell_call<double>(g, 1, (double *)u, (double *)v, (double *)w, nullptr, nullptr, nullptr, nullptr,
+                                    phi, 1, bc_x, bc_y, bc_z, ax, ay, az, nullptr, 1, scale);
+    return cutcell_call<float>(g, 1, (float *)u, (float *)v, (float *)w, nullptr, nullptr, nullptr, nullptr, phi, 1,
+                               bc_x, bc_y, bc_z, ax, ay, az, nullptr, 1, scale);
 }
 
 // ---------------------------------------------------------------------------
@@ -880,28 +961,42 @@ extern "C" int aeo_step_euler(aeo_stepper *h, void *u, void *v, void *w, void *t
 // ---------------------------------------------------------------------------
 // diagnostics and profiling
 // ---------------------------------------------------------------------------
-static int run_max(aeo_grid *g, int precision, const void *u, const void *v, const void *w, int what, double *out) {
+static int run_diag(aeo_grid *g, int precision, const void *u, const void *v, const void *w, int what, double *out) {
     NEED_CTX();
     if (!g || !u || !v || !w || !out) return fail(AEO_ERR_ARG, "null argument");
-    unsigned long long *d = nullptr;
-    HIPCHK(hipMalloc(&d, sizeof(unsigned long long)));
-    HIPCHK(hipMemsetAsync(d, 0, sizeof(unsigned long long), g_ctx->stream));
-    if (precision == AEO_F64) launch_max<double>(g_ctx->stream, d, (const double *)u, (const double *)v, (const double *)w, g->m64, what);
-    else launch_max<float>(g_ctx->stream, d, (const float *)u, (const float *)v, (const float *)w, g->m64, what);
-    unsigned long long bits = 0;
-    HIPCHK(hipMemcpyAsync(&bits, d, sizeof bits, hipMemcpyDeviceToHost, g_ctx->stream));
+    if (precision != AEO_F64 && precision != AEO_F32) return fail(AEO_ERR_ARG, "bad precision %d", precision);
+    constexpr int NB = 512;
+    double *d = nullptr;
+    HIPCHK(hipMalloc(&d, sizeof(double) * NB));
+    if (precision == AEO_F64)
+        launch_diag<double>(g_ctx->stream, d, NB, (const double *)u, (const double *)v, (const double *)w, g->m64, what);
+    else
+        launch_diag<float>(g_ctx->stream, d, NB, (const float *)u, (const float *)v, (const float *)w, g->m64, what);
+    double h[NB];
+    HIPCHK(hipMemcpyAsync(h, d, sizeof h, hipMemcpyDeviceToHost, g_ctx->stream));
     HIPCHK(hipStreamSynchronize(g_ctx->stream));
     HIPCHK(hipFree(d));
-    memcpy(out, &bits, sizeof(double));
+    double r = 0.0;
+    for (int b = 0; b < NB; b++) {
+        if (what == 2) r += h[b];
+        else if (h[b] > r || h[b] != h[b]) r = h[b];
+    }
+    *out = r;
     return AEO_OK;
 }
 
 extern "C" int aeo_max_divergence(aeo_grid *g, int precision, const void *u, const void *v, const void *w, double *out) {
-    return run_max(g, precision, u, v, w, 0, out);
+    return run_diag(g, precision, u, v, w, 0, out);
 }
 extern "C" int aeo_max_advective_rate(aeo_grid *g, int precision, const void *u, const void *v, const void *w,
                                       double *out) {
-    return run_max(g, precision, u, v, w, 1, out);
+    return run_diag(g, precision, u, v, w, 1, out);
+}
+extern "C" int aeo_kinetic_energy(aeo_grid *g, int precision, const void *u, const void *v, const void *w,
+                                  double *out) {
+    int rc = run_diag(g, precision, u, v, w, 2, out);
+    if (rc == AEO_OK) *out = 0.5 * *out * (g->stretched ? g->dx * g->dy : g->dx * g->dy * g->dz);
+    return rc;
 }
 
 extern "C" int aeo_profile_enable(int on) {

@@ -1,0 +1,141 @@
+// Cut-cell operators on a regular grid (src/CutCell.jl:602-781): the
+// aperture-weighted divergence D, the aperture-scaled face gradient G, the
+// Poisson operator L = (1/vol_eff) Σ a^q (f_nbr − f)/Δ² (q = 2 is exactly
+// D·G), and the two ends of the cut-cell projection: the source
+// vol_eff·(D u*) of the symmetric operator vol_eff·L (face coefficients a²,
+// the multigrid's form) and the correction u ← u* − G φ.
+//
+// Geometry arrays (ax, ay, az, vol_eff) are storage-layout FP64 on the device.
+// The small-cell redistribution of cutcell_divergence! (:628-648) needs the
+// merge map, which the geometry builder here does not produce: D is the
+// strict adjoint form (redistribute = false).
+#include <hip/hip_runtime.h>
+
+#include "kernels.h"
+
+namespace aeo {
+
+template <typename T>
+__global__ void cc_div_kernel(T *__restrict__ out, double *__restrict__ buf, const T *__restrict__ u,
+                              const T *__restrict__ v, const T *__restrict__ w, const double *__restrict__ ax,
+                              const double *__restrict__ ay, const double *__restrict__ az,
+                              const double *__restrict__ ve, Metrics<double> m, int weighted) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y, k = blockIdx.z;
+    if (i >= m.Nx) return;
+    const int H = m.H, sx = m.sx, sy = m.sy;
+    const size_t c = lin(i + H, j + H, k + H, sx, sy);
+    const size_t sz = (size_t)sx * sy;
+    const double vc = ve[c];
+    double r = 0.0;
+    if (vc > 0) {
+        const double fx = (ax[c + 1] * (double)u[c + 1] - ax[c] * (double)u[c]) * m.idx;
+        const double fy = (ay[c + sx] * (double)v[c + sx] - ay[c] * (double)v[c]) * m.idy;
+        const double fz = (az[c + sz] * (double)w[c + sz] - az[c] * (double)w[c]) * m.idz;
+        r = weighted ? (fx + fy) + fz : ((fx + fy) + fz) / vc;
+    }
+    if (out) out[c] = (T)r;
+    if (buf) buf[(size_t)i + (size_t)m.Nx * (j + (size_t)m.Ny * k)] = r;
+}
+
+// storage-layout field or interior-sized φ (with the boundary maps)
+struct PhiView {
+    const double *p;
+    int interior;  // 1: p is (Nx, Ny, Nz), neighbours across the domain boundary by bc
+    int bcx, bcy, bcz;
+};
+__device__ __forceinline__ int phimap(int a, int n, int bc) {
+    if (a < 0) return bc == BC_PERIODIC ? a + n : -a - 1;
+    if (a >= n) return bc == BC_PERIODIC ? a - n : 2 * n - 1 - a;
+    return a;
+}
+__device__ __forceinline__ double phi_at(const PhiView &P, const Metrics<double> &m, int i, int j, int k) {
+    // (i, j, k) interior-relative
+    if (P.interior) {
+        i = phimap(i, m.Nx, P.bcx);
+        j = phimap(j, m.Ny, P.bcy);
+        k = phimap(k, m.Nz, P.bcz);
+        return P.p[(size_t)i + (size_t)m.Nx * (j + (size_t)m.Ny * k)];
+    }
+    return P.p[lin(i + m.H, j + m.H, k + m.H, m.sx, m.sy)];
+}
+
+// G φ at the interior faces (face i is the west face of cell i), or
+// u ← u − s·Gφ when correct
+template <typename T>
+__global__ void cc_grad_kernel(T *__restrict__ gx, T *__restrict__ gy, T *__restrict__ gz, PhiView P,
+                               const double *__restrict__ ax, const double *__restrict__ ay,
+                               const double *__restrict__ az, Metrics<double> m, int correct, double s) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y, k = blockIdx.z;
+    if (i >= m.Nx) return;
+    const size_t c = lin(i + m.H, j + m.H, k + m.H, m.sx, m.sy);
+    const double pc = phi_at(P, m, i, j, k);
+    const double a = ax[c], b = ay[c], d = az[c];
+    const double ggx = a > 0 ? a * (pc - phi_at(P, m, i - 1, j, k)) * m.idx : 0.0;
+    const double ggy = b > 0 ? b * (pc - phi_at(P, m, i, j - 1, k)) * m.idy : 0.0;
+    const double ggz = d > 0 ? d * (pc - phi_at(P, m, i, j, k - 1)) * m.idz : 0.0;
+    if (correct) {
+        gx[c] = (T)((double)gx[c] - s * ggx);
+        gy[c] = (T)((double)gy[c] - s * ggy);
+        gz[c] = (T)((double)gz[c] - s * ggz);
+    } else {
+        gx[c] = (T)ggx;
+        gy[c] = (T)ggy;
+        gz[c] = (T)ggz;
+    }
+}
+
+// cutcell_laplacian! (:735-781)
+template <typename T>
+__global__ void cc_lap_kernel(T *__restrict__ out, const T *__restrict__ f, const double *__restrict__ ax,
+                              const double *__restrict__ ay, const double *__restrict__ az,
+                              const double *__restrict__ ve, Metrics<double> m, int sq) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y, k = blockIdx.z;
+    if (i >= m.Nx) return;
+    const int sx = m.sx;
+    const size_t sz = (size_t)m.sx * m.sy;
+    const size_t c = lin(i + m.H, j + m.H, k + m.H, m.sx, m.sy);
+    const double vc = ve[c];
+    if (!(vc > 0)) {
+        out[c] = T(0);
+        return;
+    }
+    const double fc = (double)f[c];
+    double axp = ax[c + 1], axm = ax[c], ayp = ay[c + sx], aym = ay[c], azp = az[c + sz], azm = az[c];
+    if (sq) {
+        axp *= axp; axm *= axm; ayp *= ayp; aym *= aym; azp *= azp; azm *= azm;
+    }
+    double s = (axp * ((double)f[c + 1] - fc) + axm * ((double)f[c - 1] - fc)) * m.idx2;
+    s += (ayp * ((double)f[c + sx] - fc) + aym * ((double)f[c - sx] - fc)) * m.idy2;
+    s += (azp * ((double)f[c + sz] - fc) + azm * ((double)f[c - sz] - fc)) * m.idz2;
+    out[c] = (T)(s / vc);
+}
+
+template <typename T>
+int launch_cutcell(hipStream_t st, int op, const Metrics<double> &m, T *a, T *b, T *c, const T *f0, const T *f1,
+                   const T *f2, double *buf, const double *phi, int phi_interior, int bcx, int bcy, int bcz,
+                   const double *ax, const double *ay, const double *az, const double *ve, int flag, double s) {
+    dim3 grid((m.Nx + 127) / 128, m.Ny, m.Nz), blk(128);
+    switch (op) {
+    case 0:  // divergence (a = out storage, or buf interior; flag: weighted by vol_eff)
+        hipLaunchKernelGGL(cc_div_kernel<T>, grid, blk, 0, st, a, buf, f0, f1, f2, ax, ay, az, ve, m, flag);
+        return 0;
+    case 1: {  // gradient (flag 0) or correction (flag 1) of (a, b, c) by φ
+        PhiView P{phi, phi_interior, bcx, bcy, bcz};
+        hipLaunchKernelGGL(cc_grad_kernel<T>, grid, blk, 0, st, a, b, c, P, ax, ay, az, m, flag, s);
+        return 0;
+    }
+    case 2:  // Laplacian of f0 into a (flag: aperture power 2)
+        hipLaunchKernelGGL(cc_lap_kernel<T>, grid, blk, 0, st, a, f0, ax, ay, az, ve, m, flag);
+        return 0;
+    }
+    return -1;
+}
+template int launch_cutcell<double>(hipStream_t, int, const Metrics<double> &, double *, double *, double *,
+                                    const double *, const double *, const double *, double *, const double *, int,
+                                    int, int, int, const double *, const double *, const double *, const double *,
+                                    int, double);
+template int launch_cutcell<float>(hipStream_t, int, const Metrics<double> &, float *, float *, float *, const float *,
+                                   const float *, const float *, double *, const double *, int, int, int, int,
+                                   const double *, const double *, const double *, const double *, int, double);
+
+}  // namespace aeo

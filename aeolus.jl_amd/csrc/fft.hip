@@ -38,6 +38,8 @@ struct SpectralPlan {
            *gauge = nullptr;
     double measure_sum = 0, gauge_sum = 0, norm = 0;
     int mode = -1;  // 0 fft, 1 column
+    int neu_x = 0, neu_y = 0;  // Neumann lateral axes: REDFT10 / REDFT01 (DCT-II / DCT-III)
+    int nkx = 0;               // spectral columns along x: Nx/2+1 (periodic) or Nx (Neumann)
 };
 
 static int ilog2(int n) {
@@ -294,6 +296,97 @@ __global__ void thomas_kernel(double2 *__restrict__ hat, double *__restrict__ cp
     }
 }
 
+// ---- DCT-II (REDFT10) and DCT-III (REDFT01) through one complex FFT --------
+// REDFT10: X_k = 2 Σ_n x_n cos(πk(2n+1)/(2N)); with v_n = x_{2n}, v_{N-1-n} =
+// x_{2n+1} and V = DFT_N(v), X_k = 2 Re(e^{-iπk/(2N)} V_k) (Makhoul, IEEE
+// TASSP 28 (1980) 27). REDFT01: x_n = X_0 + 2 Σ_{k≥1} X_k cos(πk(2n+1)/(2N));
+// with W_k = (X_k − i X_{N−k}) e^{iπk/(2N)} (X_N = 0), v = N·IDFT_N(W) and
+// x_{2n} = v_n, x_{2n+1} = v_{N-1-n}. REDFT01 ∘ REDFT10 = 2N.
+__device__ __forceinline__ int dct_pos(int m, int N) { return m < N / 2 ? 2 * m : 2 * (N - 1 - m) + 1; }
+
+// rows along x of N reals: forward src → hat (X_k, 0), k < N; inverse Re(hat) → dst · scale
+__global__ void dct_rows_kernel(double2 *__restrict__ hat, const double *__restrict__ src, double *__restrict__ dst,
+                                int N, int lg, int rpb, int nrows, int LD, const double2 *__restrict__ tw, int dir,
+                                double scale) {
+    extern __shared__ double2 sm[];
+    const int row0 = blockIdx.x * rpb;
+    for (int t = threadIdx.x; t < rpb * N; t += blockDim.x) {
+        const int r = t / N, m = t - r * N, row = row0 + r;
+        double2 z = make_double2(0, 0);
+        if (row < nrows) {
+            if (dir > 0) {
+                z.x = src[(size_t)row * N + dct_pos(m, N)];
+            } else {
+                const double xk = hat[(size_t)row * LD + m].x;
+                const double xn = m ? hat[(size_t)row * LD + (N - m)].x : 0.0;
+                double sn, cs;
+                sincospi((double)m / (2.0 * N), &sn, &cs);
+                z = make_double2(xk * cs + xn * sn, xk * sn - xn * cs);
+            }
+        }
+        sm[bitrev(m, lg) * rpb + r] = z;
+    }
+    __syncthreads();
+    lds_fft(sm, N, lg, rpb, rpb, tw, dir > 0 ? -1 : +1);
+    for (int t = threadIdx.x; t < rpb * N; t += blockDim.x) {
+        const int r = t / N, m = t - r * N, row = row0 + r;
+        if (row >= nrows) continue;
+        const double2 v = sm[m * rpb + r];
+        if (dir > 0) {
+            double sn, cs;
+            sincospi((double)m / (2.0 * N), &sn, &cs);
+            hat[(size_t)row * LD + m] = make_double2(2.0 * (cs * v.x + sn * v.y), 0.0);
+        } else {
+            dst[(size_t)row * N + dct_pos(m, N)] = v.x * scale;
+        }
+    }
+}
+
+// complex columns along a strided axis (element m at base + m*es + b), in
+// place: the transform of the real and the imaginary parts
+template <int TB>
+__global__ void dct_cols_kernel(double2 *__restrict__ d, int L, int lg, long es, long os, int ncols, int dir,
+                                const double2 *__restrict__ tw) {
+    extern __shared__ double2 sm[];
+    const int g = blockIdx.x, o = blockIdx.y;
+    const long base = (long)o * os + (long)g * TB;
+    for (int t = threadIdx.x; t < L * TB; t += blockDim.x) {
+        const int m = t / TB, b = t - m * TB;
+        double2 z = make_double2(0, 0);
+        if (g * TB + b < ncols) {
+            if (dir > 0) {
+                z = d[base + (long)dct_pos(m, L) * es + b];
+            } else {
+                const double2 zk = d[base + (long)m * es + b];
+                const double2 zn = m ? d[base + (long)(L - m) * es + b] : make_double2(0, 0);
+                const double2 a = make_double2(zk.x + zn.y, zk.y - zn.x);  // Z_k − i Z_{L−k}
+                double sn, cs;
+                sincospi((double)m / (2.0 * L), &sn, &cs);
+                z = make_double2(a.x * cs - a.y * sn, a.x * sn + a.y * cs);
+            }
+        }
+        sm[bitrev(m, lg) * TB + b] = z;
+    }
+    __syncthreads();
+    lds_fft(sm, L, lg, TB, TB, tw, dir > 0 ? -1 : +1);
+    for (int t = threadIdx.x; t < L * TB; t += blockDim.x) {
+        const int m = t / TB, b = t - m * TB;
+        if (g * TB + b >= ncols) continue;
+        if (dir > 0) {
+            // split V into the transforms of the real and imaginary parts
+            const double2 v = sm[m * TB + b];
+            const double2 w = sm[((L - m) & (L - 1)) * TB + b];
+            const double2 va = make_double2(0.5 * (v.x + w.x), 0.5 * (v.y - w.y));  // (V + conj W)/2
+            const double2 vb = make_double2(0.5 * (v.y + w.y), -0.5 * (v.x - w.x)); // (V − conj W)/(2i)
+            double sn, cs;
+            sincospi((double)m / (2.0 * L), &sn, &cs);
+            d[base + (long)m * es + b] = make_double2(2.0 * (cs * va.x + sn * va.y), 2.0 * (cs * vb.x + sn * vb.y));
+        } else {
+            d[base + (long)dct_pos(m, L) * es + b] = sm[m * TB + b];
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------
@@ -360,6 +453,7 @@ int spectral_fft_setup(SpectralPlan *p, double dx, double dy, double dz) {
     for (int m = 0; m < Ny; m++) { double s = sin(M_PI * m / Ny); ky[m] = 4 / (dy * dy) * (s * s); }
     for (int m = 0; m < Nz; m++) { double s = sin(M_PI * m / Nz); kz[m] = 4 / (dz * dz) * (s * s); }
     p->nrm = 1 / (double)((long long)Nx * Ny * Nz);
+    p->nkx = Nx / 2 + 1;
     if (upload(&p->kx2, kx) || upload(&p->ky2, ky) || upload(&p->kz2, kz)) return -1;
     p->mode = 0;
     return 0;
@@ -368,14 +462,35 @@ int spectral_fft_setup(SpectralPlan *p, double dx, double dy, double dz) {
 // ColumnTridiagonalSolver constructor (periodic x-y), src/PressureSolvers.jl:626-666
 int spectral_column_setup(SpectralPlan *p, double dx, double dy, const double *lower, const double *upper,
                           const double *hweight, const double *measure, const double *inv_coupling,
-                          const double *gauge) {
+                          const double *gauge, int neu_x, int neu_y) {
     int Nx = p->Nx, Ny = p->Ny, Nz = p->Nz;
-    std::vector<double> lx(Nx / 2 + 1), ly(Ny);
-    // _r2r_eigen: half-complex index p <-> frequency min(p, N-p)
-    for (int q = 0; q <= Nx / 2; q++) { double s = sin(M_PI * (double)q / Nx); lx[q] = 4 / (dx * dx) * (s * s); }
+    p->neu_x = neu_x;
+    p->neu_y = neu_y;
+    p->nkx = neu_x ? Nx : Nx / 2 + 1;
+    if (neu_x && p->LD < Nx) {
+        // the REDFT10 spectrum along x has Nx real coefficients per row
+        hipFree(p->hat);
+        hipFree(p->cp);
+        p->LD = (Nx + 15) / 16 * 16;
+        size_t nh = (size_t)p->LD * Ny * Nz;
+        if (hipMalloc(&p->hat, sizeof(double2) * nh) != hipSuccess) return -1;
+        if (hipMalloc(&p->cp, sizeof(double) * nh) != hipSuccess) return -1;
+        if (hipMemset(p->hat, 0, sizeof(double2) * nh) != hipSuccess) return -1;
+    }
+    std::vector<double> lx(p->nkx), ly(Ny);
+    // _r2r_eigen (src/PressureSolvers.jl:618-624): R2HC index q <-> frequency
+    // min(q, N-q), 4/Δ² sin²(πf/N); REDFT10 index q: 4/Δ² sin²(πq/(2N))
+    for (int q = 0; q < p->nkx; q++) {
+        double s = neu_x ? sin(M_PI * (double)q / (2.0 * Nx)) : sin(M_PI * (double)q / Nx);
+        lx[q] = 4 / (dx * dx) * (s * s);
+    }
     for (int q = 0; q < Ny; q++) {
-        int f = q < Ny - q ? q : Ny - q;
-        double s = sin(M_PI * (double)f / Ny);
+        double s;
+        if (neu_y) s = sin(M_PI * (double)q / (2.0 * Ny));
+        else {
+            int f = q < Ny - q ? q : Ny - q;
+            s = sin(M_PI * (double)f / Ny);
+        }
         ly[q] = 4 / (dy * dy) * (s * s);
     }
     std::vector<double> lo(lower, lower + Nz), up(upper, upper + Nz), hw(hweight, hweight + Nz),
@@ -385,7 +500,8 @@ int spectral_column_setup(SpectralPlan *p, double dx, double dy, const double *l
     for (int k = 0; k < Nz; k++) gsum += ga[k];
     p->measure_sum = msum;
     p->gauge_sum = gsum;
-    p->norm = 1.0 / (double)((long long)Nx * Ny);
+    // norm = 1/(s_x N_x s_y N_y), s = 2 on a REDFT10 axis (:660-664)
+    p->norm = 1.0 / (double)((long long)(neu_x ? 2 : 1) * Nx * (neu_y ? 2 : 1) * Ny);
     if (upload(&p->lx, lx) || upload(&p->ly, ly) || upload(&p->lower, lo) || upload(&p->upper, up) ||
         upload(&p->hweight, hw) || upload(&p->measure, ms) || upload(&p->icoup, ic) || upload(&p->gauge, ga))
         return -1;
@@ -419,7 +535,7 @@ static void cols(SpectralPlan *p, hipStream_t st, int axis, int sign, int mode) 
     long es = axis == 1 ? (long)p->LD : (long)p->LD * p->Ny;
     long os = axis == 1 ? (long)p->LD * p->Ny : (long)p->LD;
     int nouter = axis == 1 ? p->Nz : p->Ny;
-    int ncols = p->Nx / 2 + 1;
+    int ncols = p->mode == 1 ? p->nkx : p->Nx / 2 + 1;
     EigArgs ea{p->kx2, p->ky2, p->kz2, p->nrm, ncols};
     if (L <= 256) {
         constexpr int TB = 16;
@@ -444,16 +560,50 @@ int spectral_fft_solve(SpectralPlan *p, hipStream_t st, double *phi, const doubl
     return 0;
 }
 
+// REDFT10 (dir +1) or REDFT01 (dir −1) along x of every row
+static void xdct(SpectralPlan *p, hipStream_t st, const double *src, double *dst, int dir, double scale) {
+    int N = p->Nx, lg = ilog2(N);
+    int tpr = N / 2 > 0 ? N / 2 : 1;
+    int rpb = tpr >= 256 ? 1 : 256 / tpr;
+    int nrows = p->Ny * p->Nz;
+    int bt = tpr >= 256 ? tpr : 256;
+    size_t lds = sizeof(double2) * (size_t)rpb * N;
+    hipLaunchKernelGGL(dct_rows_kernel, dim3((nrows + rpb - 1) / rpb), dim3(bt), lds, st, p->hat, src, dst, N, lg, rpb,
+                       nrows, p->LD, p->tw[lg], dir, scale);
+}
+// the y transform of the column solver: complex DFT (periodic) or the DCT pair (Neumann)
+static void ytrans(SpectralPlan *p, hipStream_t st, int dir) {
+    if (!p->neu_y) {
+        cols(p, st, 1, dir > 0 ? -1 : +1, 0);
+        return;
+    }
+    int L = p->Ny, lg = ilog2(L);
+    long es = (long)p->LD, os = (long)p->LD * p->Ny;
+    if (L <= 256) {
+        constexpr int TB = 16;
+        size_t lds = sizeof(double2) * (size_t)L * TB;
+        hipLaunchKernelGGL(dct_cols_kernel<TB>, dim3(p->LD / TB, p->Nz), dim3(256), lds, st, p->hat, L, lg, es, os,
+                           p->nkx, dir, p->tw[lg]);
+    } else {
+        constexpr int TB = 8;
+        size_t lds = sizeof(double2) * (size_t)L * TB;
+        hipLaunchKernelGGL(dct_cols_kernel<TB>, dim3(p->LD / TB, p->Nz), dim3(256), lds, st, p->hat, L, lg, es, os,
+                           p->nkx, dir, p->tw[lg]);
+    }
+}
+
 int spectral_column_solve(SpectralPlan *p, hipStream_t st, double *phi, const double *src) {
     if (p->mode != 1) return -1;
-    xfwd(p, st, src);
-    cols(p, st, 1, -1, 0);
-    int nkx = p->Nx / 2 + 1;
+    if (p->neu_x) xdct(p, st, src, nullptr, +1, 1.0);
+    else xfwd(p, st, src);
+    ytrans(p, st, +1);
+    int nkx = p->nkx;
     hipLaunchKernelGGL(thomas_kernel, dim3((nkx + 63) / 64, p->Ny), dim3(64), 0, st, p->hat, p->cp, nkx, p->Ny, p->Nz,
                        p->LD, p->lx, p->ly, p->lower, p->upper, p->hweight, p->measure, p->icoup, p->gauge,
                        p->measure_sum, p->gauge_sum);
-    cols(p, st, 1, +1, 0);
-    xinv(p, st, phi, p->norm);
+    ytrans(p, st, -1);
+    if (p->neu_x) xdct(p, st, nullptr, phi, -1, p->norm);
+    else xinv(p, st, phi, p->norm);
     return 0;
 }
 

@@ -37,8 +37,14 @@ template <typename T>
 void launch_shell(hipStream_t st, T *out, const T *x, const T *snap, int Nx, int Ny, int Nz, int H, double ca,
                   double cb);
 template <typename T>
-void launch_max(hipStream_t st, unsigned long long *out, const T *u, const T *v, const T *w, const Metrics<double> &md,
-                int what);
+void launch_diag(hipStream_t st, double *part, int nblocks, const T *u, const T *v, const T *w,
+                 const Metrics<double> &md, int what);
+
+// ---- cut-cell operators (cutcell.hip): op 0 divergence, 1 gradient/correction, 2 Laplacian ----
+template <typename T>
+int launch_cutcell(hipStream_t st, int op, const Metrics<double> &m, T *a, T *b, T *c, const T *f0, const T *f1,
+                   const T *f2, double *buf, const double *phi, int phi_interior, int bcx, int bcy, int bcz,
+                   const double *ax, const double *ay, const double *az, const double *ve, int flag, double s);
 
 // ---- spectral / column solvers (fft.hip) ----
 struct SpectralPlan;  // opaque: twiddle tables and work arrays on the device
@@ -50,7 +56,7 @@ int spectral_fft_solve(SpectralPlan *p, hipStream_t st, double *phi, const doubl
 // column solver: periodic x-y transform + FP64 Thomas per wavenumber + mean mode
 int spectral_column_setup(SpectralPlan *p, double dx, double dy, const double *lower, const double *upper,
                           const double *hweight, const double *measure, const double *inv_coupling,
-                          const double *gauge);
+                          const double *gauge, int neu_x, int neu_y);
 int spectral_column_solve(SpectralPlan *p, hipStream_t st, double *phi, const double *src);
 
 // ---- multigrid and MG-preconditioned CG (mg.hip) ----

@@ -548,46 +548,66 @@ template void launch_shell<float>(hipStream_t, float *, const float *, const flo
 // 172-182). Non-negative doubles order as their bit patterns, so the maximum
 // is an atomicMax on the bits (NaN-propagating, as the reference's max).
 // ---------------------------------------------------------------------------
+// Diagnostics over the interior cells, one partial per block (grid-stride
+// over the flattened interior, a tree in LDS): what = 0 max |∇·u|
+// (max_divergence, src/Simulations.jl:1045-1050); 1 max Σ|u_i|/Δx_i
+// (_max_advective_rate, src/TimeIntegration.jl:164-182); 2 Σ of the kinetic
+// energy density (u² + v²)·Δz_c + w²·Δz_f (kinetic_energy,
+// src/Simulations.jl:992-1015; uniform grids Δz_c = Δz_f = Δz).
 template <typename T>
-__global__ void max_kernel(unsigned long long *__restrict__ out, const T *__restrict__ u, const T *__restrict__ v,
-                           const T *__restrict__ w, Metrics<double> md, int what) {
+__global__ void __launch_bounds__(256) diag_kernel(double *__restrict__ part, const T *__restrict__ u,
+                                                   const T *__restrict__ v, const T *__restrict__ w,
+                                                   Metrics<double> md, int what) {
+    __shared__ double sh[256];
     const int H = md.H, sx = md.sx, sy = md.sy;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y, k = blockIdx.z;
-    double r = 0.0;
-    if (i < md.Nx) {
+    const long n = (long)md.Nx * md.Ny * md.Nz;
+    double acc = 0.0;
+    for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x) {
+        const int i = (int)(e % md.Nx);
+        const long r = e / md.Nx;
+        const int j = (int)(r % md.Ny), k = (int)(r / md.Ny);
         const int K = k + H;
         const size_t c = lin(i + H, j + H, K, sx, sy);
+        double val;
         if (what == 0) {
-            double du = (double)u[c + 1] - (double)u[c];
-            double dv = (double)v[c + sx] - (double)v[c];
-            double dw = (double)w[c + (size_t)sx * sy] - (double)w[c];
-            r = fabs((du * md.idx + dv * md.idy) + dw * md.m_idz(K));
+            const double du = (double)u[c + 1] - (double)u[c];
+            const double dv = (double)v[c + sx] - (double)v[c];
+            const double dw = (double)w[c + (size_t)sx * sy] - (double)w[c];
+            val = fabs((du * md.idx + dv * md.idy) + dw * md.m_idz(K));
+        } else if (what == 1) {
+            const double au = fmax(fabs((double)u[c]), fabs((double)u[c + 1]));
+            const double av = fmax(fabs((double)v[c]), fabs((double)v[c + sx]));
+            const double aw = fmax(fabs((double)w[c]), fabs((double)w[c + (size_t)sx * sy]));
+            val = (au * md.idx + av * md.idy) + aw * md.m_idz(K);
         } else {
-            double au = fmax(fabs((double)u[c]), fabs((double)u[c + 1]));
-            double av = fmax(fabs((double)v[c]), fabs((double)v[c + sx]));
-            double aw = fmax(fabs((double)w[c]), fabs((double)w[c + (size_t)sx * sy]));
-            r = (au * md.idx + av * md.idy) + aw * md.m_idz(K);
+            const double uu = (double)u[c], vv = (double)v[c], ww = (double)w[c];
+            if (md.stretched) val = (uu * uu + vv * vv) * (1.0 / md.idzc[K]) + ww * ww * (1.0 / md.idzf[K]);
+            else val = (uu * uu + vv * vv) + ww * ww;
         }
+        // a NaN propagates through the max (range_max, src/Backends.jl:1060-1092)
+        if (what == 2) acc += val;
+        else acc = (val > acc || val != val) ? val : acc;
     }
-    unsigned long long bits = __double_as_longlong(r);
-    // wave-level max, then one atomic per wave
-    for (int off = 32; off > 0; off >>= 1) {
-        unsigned long long o = __shfl_down(bits, off);
-        bits = o > bits ? o : bits;
+    sh[threadIdx.x] = acc;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) {
+            const double a = sh[threadIdx.x], b = sh[threadIdx.x + s];
+            sh[threadIdx.x] = what == 2 ? a + b : ((b > a || b != b) ? b : a);
+        }
+        __syncthreads();
     }
-    if ((threadIdx.x & 63) == 0) atomicMax(out, bits);
+    if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
 }
 
 template <typename T>
-void launch_max(hipStream_t st, unsigned long long *out, const T *u, const T *v, const T *w, const Metrics<double> &md,
-                int what) {
-    dim3 grid((md.Nx + 255) / 256, md.Ny, md.Nz);
-    hipLaunchKernelGGL(max_kernel<T>, grid, dim3(256), 0, st, out, u, v, w, md, what);
+void launch_diag(hipStream_t st, double *part, int nblocks, const T *u, const T *v, const T *w,
+                 const Metrics<double> &md, int what) {
+    hipLaunchKernelGGL(diag_kernel<T>, dim3(nblocks), dim3(256), 0, st, part, u, v, w, md, what);
 }
-template void launch_max<double>(hipStream_t, unsigned long long *, const double *, const double *, const double *,
+template void launch_diag<double>(hipStream_t, double *, int, const double *, const double *, const double *,
+                                  const Metrics<double> &, int);
+template void launch_diag<float>(hipStream_t, double *, int, const float *, const float *, const float *,
                                  const Metrics<double> &, int);
-template void launch_max<float>(hipStream_t, unsigned long long *, const float *, const float *, const float *,
-                                const Metrics<double> &, int);
 
 }  // namespace aeo

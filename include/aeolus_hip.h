@@ -107,7 +107,7 @@ int aeo_poisson_fft_create(int Nx, int Ny, int Nz, double dx, double dy, double 
 /* ColumnTridiagonalSolver(T, (Nx,Ny,Nz), (dx,dy); lower, upper, hweight,
  * measure, inv_coupling, gauge, bc_x, bc_y), src/PressureSolvers.jl:626-666.
  * Vectors are host arrays of length Nz (inv_coupling Nz+1). Lateral BCs:
- * AEO_PERIODIC only in this build. */
+ * AEO_PERIODIC (R2HC) or AEO_NOFLUX (REDFT10, the DCT-II). */
 int aeo_poisson_column_create(int Nx, int Ny, int Nz, double dx, double dy, int bc_x, int bc_y,
                               const double *lower, const double *upper, const double *hweight,
                               const double *measure, const double *inv_coupling, const double *gauge,
@@ -174,6 +174,37 @@ int aeo_step_euler(aeo_stepper *h, void *u, void *v, void *w, void *theta, void 
 int aeo_max_divergence(aeo_grid *g, int precision, const void *u, const void *v, const void *w, double *out);
 /* _max_advective_rate (src/TimeIntegration.jl:164-182): dt = C / rate */
 int aeo_max_advective_rate(aeo_grid *g, int precision, const void *u, const void *v, const void *w, double *out);
+/* kinetic_energy (src/Simulations.jl:992-1015): ½ Σ (u² + v² + w²) dV with u,
+ * v, w as stored on their faces (per-level volumes on a stretched grid) */
+int aeo_kinetic_energy(aeo_grid *g, int precision, const void *u, const void *v, const void *w, double *out);
+
+/* ---- cut-cell operators on a regular grid (src/CutCell.jl:602-781) --------
+ * Geometry arrays ax, ay, az (face apertures) and vol_eff (effective volume
+ * fractions, 0 in solid cells) are FP64 storage-layout device arrays, as
+ * CutCellGeometry holds them (:177-195). */
+/* cutcell_divergence!(div, u, v, w, geom; redistribute = false) (:602-626):
+ * div's interior; solid cells 0 */
+int aeo_cutcell_divergence(aeo_grid *g, int precision, void *div, const void *u, const void *v, const void *w,
+                           const double *ax, const double *ay, const double *az, const double *vol_eff);
+/* cutcell_gradient_x!/_y!/_z! (:666-700): a·(p⁺ − p⁻)/Δ on the interior
+ * faces (face i = west face of cell i); p FP64 storage layout, halo filled */
+int aeo_cutcell_gradient(aeo_grid *g, int precision, void *gx, void *gy, void *gz, const double *p,
+                         const double *ax, const double *ay, const double *az);
+/* cutcell_laplacian!(out, f, geom; aperture_power) (:735-781); power 2 is D·G */
+int aeo_cutcell_laplacian(aeo_grid *g, int precision, void *out, const void *f, const double *ax, const double *ay,
+                          const double *az, const double *vol_eff, int aperture_power);
+/* The cut-cell projection (the fractional step of :580-594 with D and G):
+ * the source vol_eff·(D u*) of the symmetric operator vol_eff·D·G (face
+ * coefficients a², an MG-preconditioned CG's form) into solver's source
+ * buffer; then, with φ from aeo_poisson_solve_source_buffer (interior-sized
+ * FP64, boundary neighbours by bc: periodic or Neumann),
+ * u ← u − scale·G φ on the interior faces. */
+int aeo_cutcell_project_source(aeo_grid *g, int precision, aeo_poisson *solver, const void *u, const void *v,
+                               const void *w, const double *ax, const double *ay, const double *az,
+                               const double *vol_eff);
+int aeo_cutcell_project_correct(aeo_grid *g, int precision, void *u, void *v, void *w, const double *phi,
+                                const double *ax, const double *ay, const double *az, int bc_x, int bc_y, int bc_z,
+                                double scale);
 
 /* ---- the learned preconditioner ------------------------------------------
  * No reference counterpart: the reference has no network (its nearest hook

@@ -103,7 +103,8 @@ def test_unet_training_reduces_loss(hip):
 
 
 def test_fcg_vcycle_mode_matches_mgpcg(hip):
-    # without the network the flexible CG is MG-PCG with a flexible update
+    # without the network the flexible CG is MG-PCG with a flexible update: the
+    # same solution, at least as few iterations
     from aeolus_hip import MGPreconditionedCG, solve_interior, solver_stats
     from aeolus_hip import workloads as W
     from aeolus_hip.learned import FlexibleCG
@@ -117,7 +118,62 @@ def test_fcg_vcycle_mode_matches_mgpcg(hip):
     it_f, r_f = solver_stats(f)
     print(f"MG-PCG {it_mg} its ({r_mg:.2e}); flexible CG, V-cycle {it_f} its ({r_f:.2e})")
     assert f.last_status == "converged" and r_f <= 1e-8
-    assert abs(it_f - it_mg) <= 3
+    # the flexible update re-orthogonalises each direction, which the FP32
+    # V-cycle (not exactly symmetric in rounding) rewards: never more iterations
+    assert it_f <= it_mg + 2
     fluid = w["mask"] > 0
     d = x_f[fluid] - x_mg[fluid]
     assert np.abs(d - d.mean()).max() < 1e-6 * np.abs(x_mg[fluid]).max()
+
+
+def test_trained_network_gain_on_held_out_seeds(hip):
+    # trained on the solver's own pairs, the network approximates the solution
+    # on layouts it has not seen far better than before training; the flexible
+    # CG with it is stated against the same CG without it
+    from aeolus_hip import MGPreconditionedCG, solve_interior, solver_stats
+    from aeolus_hip import workloads as W
+    from aeolus_hip.fields import DeviceArray
+    from aeolus_hip.learned import FlexibleCG, UNet, train
+    N = (32, 32, 16)
+    sp = (2.0,) * 3
+
+    def pair(seed):
+        w = W.cutcell_workload(N, 2.0, n_buildings=4, seed=seed, height=(4.0, 20.0))
+        mg = MGPreconditionedCG(N, sp, coeff=w["coeff"], tol=1e-10, maxit=300)
+        x = solve_interior(mg, w["source"])
+        fluid = w["mask"] > 0
+        x[~fluid] = 0.0
+        x[fluid] -= x[fluid].mean()
+        return w, x
+
+    def sol_err(net, w, x):
+        fluid = w["mask"] > 0
+        y = net(w["source"], w["mask"])[fluid]
+        y -= y.mean()
+        return np.linalg.norm(y - x[fluid]) / np.linalg.norm(x[fluid])
+
+    samples = []
+    for s in range(16):
+        w, x = pair(500 + s)
+        samples.append((DeviceArray.from_numpy(w["source"]), DeviceArray.from_numpy(x),
+                        DeviceArray.from_numpy(w["mask"])))
+    net = UNet(N, in_channels=2, seed=3)
+    held = [pair(900 + s) for s in range(3)]
+    e0 = [sol_err(net, w, x) for w, x in held]
+    losses = train(net, samples, epochs=60, lr=2e-3)
+    e1 = [sol_err(net, w, x) for w, x in held]
+    for (w, x), a, b_ in zip(held, e0, e1):
+        b = w["source"]
+        mg = MGPreconditionedCG(N, sp, coeff=w["coeff"], tol=1e-8, maxit=300, vcycle="f32", smoother="jacobi")
+        dmask = DeviceArray.from_numpy(w["mask"])
+        its = {}
+        for name, kw in (("vcycle", dict(mode="vcycle", guess=False)),
+                         ("net guess + vcycle", dict(mode="vcycle", guess=True)),
+                         ("net + vcycle", dict(mode="net+vcycle", guess=True))):
+            f = FlexibleCG(mg, net, dmask, tol=1e-8, maxit=300, **kw)
+            solve_interior(f, b)
+            its[name] = solver_stats(f)[0]
+            assert f.last_status == "converged"
+        print(f"held-out: solution error untrained {a:.3f}, trained {b_:.3f}; FCG iterations {its}")
+    print(f"training loss {losses[0]:.3e} -> {losses[-1]:.3e}")
+    assert max(e1) < 0.7 and all(t < u for t, u in zip(e1, e0))

@@ -205,3 +205,96 @@ def test_single_vcycle_parity(hip, orc, coeff):
     assert cyc == 1
     assert rel_err(x, x_ref) < 1e-12
     assert abs(r - rel.value) < 1e-10 * rel.value
+
+
+# ---------------------------------------------------------------------------
+# cut-cell operators on the GPU (src/CutCell.jl:602-781)
+# ---------------------------------------------------------------------------
+def _cc_setup(hip, seed=3, N=(24, 20, 16), H=2):
+    from aeolus_hip.cutcell import CutCellGeometry
+    g = hip.Grid(N, (2.0 * N[0], 2.0 * N[1], 2.0 * N[2]), halo=H)
+    rng = np.random.default_rng(seed)
+    h = np.zeros(N[:2])
+    for _ in range(5):
+        i0, j0 = rng.integers(0, N[0] - 6), rng.integers(0, N[1] - 6)
+        h[i0:i0 + rng.integers(2, 6), j0:j0 + rng.integers(2, 6)] = rng.uniform(3.0, 20.0)
+    return g, CutCellGeometry(g, h)
+
+
+def _np_div(u, v, w, G, g):
+    # the loops of cutcell_divergence! (redistribute = false), numpy
+    H, (Nx, Ny, Nz) = g.H, (g.Nx, g.Ny, g.Nz)
+    I, J, K = slice(H, H + Nx), slice(H, H + Ny), slice(H, H + Nz)
+    Ip, Jp, Kp = slice(H + 1, H + Nx + 1), slice(H + 1, H + Ny + 1), slice(H + 1, H + Nz + 1)
+    ax, ay, az, ve = G["ax"], G["ay"], G["az"], G["vol_eff"]
+    fx = (ax[Ip, J, K] * u[Ip, J, K] - ax[I, J, K] * u[I, J, K]) * (1 / g.dx)
+    fy = (ay[I, Jp, K] * v[I, Jp, K] - ay[I, J, K] * v[I, J, K]) * (1 / g.dy)
+    fz = (az[I, J, Kp] * w[I, J, Kp] - az[I, J, K] * w[I, J, K]) * (1 / g.dz)
+    vc = ve[I, J, K]
+    return np.where(vc > 0, (fx + fy + fz) / np.where(vc > 0, vc, 1), 0.0)
+
+
+def test_cutcell_divergence_gradient_laplacian(hip):
+    from aeolus_hip.cutcell import divergence, gradient, laplacian
+    from aeolus_hip.fields import DeviceArray
+    g, geom = _cc_setup(hip)
+    G = geom.host
+    rng = np.random.default_rng(4)
+    st = g.storage
+    u, v, w, p = (np.asfortranarray(rng.standard_normal(st)) for _ in range(4))
+    du, dv, dw, dp = (DeviceArray.from_numpy(a) for a in (u, v, w, p))
+    H, (Nx, Ny, Nz) = g.H, (g.Nx, g.Ny, g.Nz)
+    I, J, K = slice(H, H + Nx), slice(H, H + Ny), slice(H, H + Nz)
+    div = DeviceArray(st)
+    divergence(div, du, dv, dw, geom)
+    ref = _np_div(u, v, w, G, g)
+    assert rel_err(div.download()[I, J, K], ref) < 1e-14
+    gx, gy, gz = DeviceArray(st), DeviceArray(st), DeviceArray(st)
+    gradient(gx, gy, gz, dp, geom)
+    ax = G["ax"][I, J, K]
+    refx = np.where(ax > 0, ax * (p[I, J, K] - p[H - 1:H + Nx - 1, J, K]) * (1 / g.dx), 0.0)
+    assert rel_err(gx.download()[I, J, K], refx) < 1e-14
+    # L = D·G exactly for aperture power 2 (p's halo and the gradient's east/north/top faces filled)
+    for a in (gx, gy, gz):
+        a.zero()
+    hip.fill_halos(g, dp, 0, ("periodic", "periodic", "noflux"))
+    gradient(gx, gy, gz, dp, geom)
+    for a, loc in ((gx, 1), (gy, 2), (gz, 3)):
+        hip.fill_halos(g, a, loc, ("periodic", "periodic", "noflux"))
+    lap = DeviceArray(st)
+    laplacian(lap, dp, geom, 2)
+    divergence(div, gx, gy, gz, geom)
+    L1, L2 = lap.download()[I, J, K], div.download()[I, J, K]
+    # the top boundary face: the mirrored halo gives G = 0 there in both
+    assert rel_err(L1[:, :, :-1], L2[:, :, :-1]) < 1e-12
+
+
+def test_cutcell_projection_is_divergence_free(hip):
+    from aeolus_hip.cutcell import divergence, project
+    from aeolus_hip.fields import DeviceArray
+    g, geom = _cc_setup(hip, seed=8, N=(32, 32, 16))
+    rng = np.random.default_rng(9)
+    st = g.storage
+    u, v, w = (np.asfortranarray(rng.standard_normal(st)) for _ in range(3))
+    H = g.H
+    # no flux through the domain top (the Neumann boundary of the pressure)
+    w[:, :, H + g.Nz:] = 0.0
+    du, dv, dw = (DeviceArray.from_numpy(a) for a in (u, v, w))
+    for a, loc in ((du, 1), (dv, 2), (dw, 3)):
+        hip.fill_halos(g, a, loc, ("periodic", "periodic", "noflux"))
+    w_top = dw.download()
+    w_top[:, :, H + g.Nz] = 0.0
+    dw.upload(w_top)
+    div = DeviceArray(st)
+    divergence(div, du, dv, dw, geom)
+    I = (slice(H, H + g.Nx), slice(H, H + g.Ny), slice(H, H + g.Nz))
+    d0 = np.abs(div.download()[I]).max()
+    mg = hip.MGPreconditionedCG((g.Nx, g.Ny, g.Nz), (g.dx, g.dy, g.dz), coeff=geom.coefficients(), tol=1e-13,
+                                maxit=500)
+    project(du, dv, dw, geom, mg)
+    for a, loc in ((du, 1), (dv, 2)):
+        hip.fill_halos(g, a, loc, ("periodic", "periodic", "noflux"))
+    divergence(div, du, dv, dw, geom)
+    d1 = np.abs(div.download()[I]).max()
+    print(f"cut-cell projection: max |D u| {d0:.3e} -> {d1:.3e}")
+    assert d1 < 1e-9 * d0

@@ -133,6 +133,20 @@ def test_column_solver_parity(hip, orc, size, stretched):
     assert it == 0 and np.isnan(res)  # a direct solver: (0, NaN)
 
 
+@pytest.mark.parametrize("bcx,bcy", [("neumann", "periodic"), ("periodic", "neumann"), ("neumann", "neumann")])
+@pytest.mark.parametrize("size", [(16, 8, 12), (64, 32, 40), (512, 16, 6)])
+def test_column_solver_neumann_lateral_parity(hip, orc, size, bcx, bcy):
+    # REDFT10 / REDFT01 (DCT-II / III) on the Neumann lateral axes
+    extent = (100.0, 60.0, 80.0)
+    g, og = grids(hip, orc, size, extent, H=1, stretched=True)
+    src = np.asfortranarray(np.random.default_rng(size[0]).normal(size=size))
+    ref = orc.stretched_column_solver(og, bcx, bcy).solve(src)
+    dzc, dzf = g.interior_dz()
+    s = hip.ColumnTridiagonalSolver.stretched(size, (g.dx, g.dy), dzc, dzf, bc_x=bcx, bc_y=bcy)
+    got = hip.solve_interior(s, src)
+    assert rel_err(got, ref) <= TOL64
+
+
 def test_density_weighted_column_parity(hip, orc):
     Nx, Ny, Nz = 32, 16, 20
     dz = 50.0
@@ -227,3 +241,30 @@ def test_dcbl_full_size_properties(hip):
     w = sim.w.download()
     assert np.all(w[:, :, g.H] == 0)
     assert np.isfinite(st["p"][I]).all()
+
+
+@pytest.mark.parametrize("stretched", [False, True])
+def test_diagnostics_parity(hip, orc, stretched):
+    # max |∇·u|, the advective rate against the oracle; the kinetic energy
+    # against its definition in numpy (u, v, w as stored, per-level volumes)
+    g, og = grids(hip, orc, (40, 24, 20), (4.0, 2.0, 2.0), 3, stretched)
+    bcs = ("periodic", "periodic", "wall")
+    u, v, w, _ = _fields(orc, og, 300, bcs)
+    sim = hip.BoussinesqSimulation(g, scheme="centered", nu=0.0, theta0=0.0, bcs=bcs)
+    sim.set(u, v, w, np.zeros(og.storage))
+    import ctypes
+    du, dv, dw = (up(hip, a) for a in (u, v, w))
+    out = ctypes.c_double()
+    L = hip._lib
+    L.check(L.init().aeo_max_divergence(g.handle, 0, L.dptr(du), L.dptr(dv), L.dptr(dw), ctypes.byref(out)))
+    assert abs(out.value - orc.max_divergence(og, u, v, w)) <= 1e-14 * abs(out.value)
+    L.check(L.init().aeo_max_advective_rate(g.handle, 0, L.dptr(du), L.dptr(dv), L.dptr(dw),
+                                            ctypes.byref(out)))
+    ref_rate = orc.lib().orc_max_adv_rate(orc.ptr(u), orc.ptr(v), orc.ptr(w), ctypes.byref(og.c))
+    assert abs(out.value - ref_rate) <= 1e-14 * abs(ref_rate)
+    ke = sim.kinetic_energy()
+    I = og.interior
+    dzc, dzf = g.interior_dz()
+    dzf_c = dzf[:-1] if dzf.size == g.Nz + 1 else dzf
+    ref = 0.5 * ((u[I] ** 2 + v[I] ** 2) * dzc[None, None, :] + w[I] ** 2 * dzf_c[None, None, :]).sum() * g.dx * g.dy
+    assert abs(ke - ref) <= 1e-12 * ref

@@ -497,3 +497,36 @@ def test_weno_faces_match_exact_rational_golden(orc):
             worst = max(worst, e)
             assert e < 1e-13, (k, v, g, float.fromhex(c[k]))
     print(f"worst WENO face error vs exact rational: {worst:.2e} (relative to the stencil's max)")
+
+
+@pytest.mark.parametrize("bcx,bcy", [("neumann", "periodic"), ("periodic", "neumann"), ("neumann", "neumann")])
+def test_column_solver_neumann_lateral_inverts_laplacian(orc, bcx, bcy):
+    # the REDFT10 path: the 7-point Laplacian with mirror (Neumann) lateral
+    # neighbours and the stretched z operator, applied in numpy to the
+    # solution, returns the source (minus its measure-weighted mean)
+    from aeolus_hip.fields import faces_geometric
+    N = (16, 12, 10)
+    zf = faces_geometric(N[2], 0.0, 50.0, 1.08)
+    g = orc.Grid(N, (32.0, 36.0, 50.0), halo=1, zf=zf)
+    dzc, dzf = g.interior_dz()
+    src = np.random.default_rng(5).normal(size=N)
+    src -= (src * dzc).sum() / (dzc.sum() * N[0] * N[1])
+    phi = orc.stretched_column_solver(g, bcx, bcy).solve(np.asfortranarray(src))
+
+    def nb(a, axis, bc, shift):
+        if bc == "periodic":
+            return np.roll(a, shift, axis=axis)
+        pad = [(0, 0)] * 3
+        pad[axis] = (1, 1)
+        e = np.pad(a, pad, mode="edge")
+        sl = [slice(None)] * 3
+        sl[axis] = slice(1 - shift, 1 - shift + a.shape[axis])
+        return e[tuple(sl)]
+    lap = (nb(phi, 0, bcx, 1) - 2 * phi + nb(phi, 0, bcx, -1)) / g.dx ** 2
+    lap += (nb(phi, 1, bcy, 1) - 2 * phi + nb(phi, 1, bcy, -1)) / g.dy ** 2
+    up_ = np.zeros_like(phi)
+    dn = np.zeros_like(phi)
+    up_[:, :, :-1] = (phi[:, :, 1:] - phi[:, :, :-1]) / dzf[1:-1]
+    dn[:, :, 1:] = (phi[:, :, 1:] - phi[:, :, :-1]) / dzf[1:-1]
+    lap += (up_ - dn) / dzc
+    assert np.abs(lap - src).max() < 1e-10 * np.abs(src).max()
