@@ -36,6 +36,17 @@ class StepParams(ctypes.Structure):
     ]
 
 
+class MWParams(ctypes.Structure):
+    """aeo_mw_params (include/aeolus_hip.h)."""
+    _fields_ = [
+        ("nu", ctypes.c_double), ("g_accel", ctypes.c_double), ("flux_bot", ctypes.c_double),
+        ("flux_top", ctypes.c_double), ("buoyancy", ctypes.c_int),
+        ("bc_x", ctypes.c_int), ("bc_y", ctypes.c_int), ("bc_z", ctypes.c_int),
+        ("tbc_x", ctypes.c_int), ("tbc_y", ctypes.c_int), ("tbc_z", ctypes.c_int),
+        ("precision", ctypes.c_int),
+    ]
+
+
 _vp = ctypes.c_void_p
 _pvp = ctypes.POINTER(ctypes.c_void_p)
 _i = ctypes.c_int
@@ -81,6 +92,10 @@ SIGNATURES = {
     "aeo_max_divergence": [_vp, _i, _vp, _vp, _vp, _pd],
     "aeo_max_advective_rate": [_vp, _i, _vp, _vp, _vp, _pd],
     "aeo_kinetic_energy": [_vp, _i, _vp, _vp, _vp, _pd],
+    "aeo_mw_stepper_create": [_vp, ctypes.POINTER(MWParams), _pd, _pd, _pd, _vp, _pvp],
+    "aeo_mw_stepper_destroy": [_vp],
+    "aeo_mw_step": [_vp, _vp, _vp, _vp, _vp, _vp, _d],
+    "aeo_weighted_divergence": [_vp, _i, _vp, _vp, _vp, _vp, _vp, _vp, _i],
     "aeo_cutcell_divergence": [_vp, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
     "aeo_cutcell_gradient": [_vp, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
     "aeo_cutcell_laplacian": [_vp, _i, _vp, _vp, _vp, _vp, _vp, _vp, _i],

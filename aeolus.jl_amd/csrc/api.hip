@@ -959,6 +959,139 @@ extern "C" int aeo_step_euler(aeo_stepper *h, void *u, void *v, void *w, void *t
 }
 
 // ---------------------------------------------------------------------------
+// the sound-proof (mass-weighted) core, regular grids
+// ---------------------------------------------------------------------------
+struct aeo_mw_stepper {
+    aeo_grid *g;
+    aeo_mw_params sp;
+    aeo_poisson *solver;
+    size_t n, esz;
+    void *us, *vs, *ws, *tn;
+    double *Wc = nullptr, *Wf = nullptr, *thf = nullptr;  // device, Nz + 2H
+};
+
+extern "C" int aeo_mw_stepper_create(aeo_grid *g, const aeo_mw_params *sp, const double *Wc, const double *Wf,
+                                     const double *thetabar_f, aeo_poisson *solver, aeo_mw_stepper **out) {
+    NEED_CTX();
+    if (!g || !sp || !Wc || !Wf || !thetabar_f || !solver || !out) return fail(AEO_ERR_ARG, "null argument");
+    if (g->stretched) return fail(AEO_ERR_UNSUPPORTED, "the sound-proof core runs on regular grids");
+    if (g->H < 1) return fail(AEO_ERR_ARG, "the sound-proof core needs a halo of at least 1");
+    if (sp->precision != AEO_F64 && sp->precision != AEO_F32) return fail(AEO_ERR_ARG, "bad precision");
+    if (solver->Nx != g->Nx || solver->Ny != g->Ny || solver->Nz != g->Nz)
+        return fail(AEO_ERR_ARG, "solver and grid sizes differ");
+    int bcs[6] = {sp->bc_x, sp->bc_y, sp->bc_z, sp->tbc_x, sp->tbc_y, sp->tbc_z};
+    for (int q = 0; q < 6; q++)
+        if (!check_bc(bcs[q])) return fail(AEO_ERR_ARG, "bad boundary kind");
+    if (sp->bc_x == AEO_WALL || sp->bc_y == AEO_WALL) return fail(AEO_ERR_UNSUPPORTED, "walls are z boundaries");
+    if ((sp->bc_x == AEO_PERIODIC) != solver->periodic_x || (sp->bc_y == AEO_PERIODIC) != solver->periodic_y ||
+        (sp->bc_z == AEO_PERIODIC) != solver->periodic_z)
+        return fail(AEO_ERR_ARG, "the pressure solver's axis kinds do not match the velocity boundary conditions");
+    const int nz = g->Nz + 2 * g->H;
+    for (int k = 0; k < nz; k++)
+        if (!(Wc[k] > 0) || !(Wf[k] > 0) || !(thetabar_f[k] > 0))
+            return fail(AEO_ERR_ARG, "reference profiles must be strictly positive");
+    auto h = std::make_unique<aeo_mw_stepper>();
+    h->g = g; h->sp = *sp; h->solver = solver;
+    h->n = (size_t)(g->Nx + 2 * g->H) * (g->Ny + 2 * g->H) * nz;
+    h->esz = sp->precision == AEO_F64 ? 8 : 4;
+    void **bufs[4] = {&h->us, &h->vs, &h->ws, &h->tn};
+    for (int q = 0; q < 4; q++) {
+        HIPCHK(hipMalloc(bufs[q], h->n * h->esz));
+        HIPCHK(hipMemsetAsync(*bufs[q], 0, h->n * h->esz, g_ctx->stream));
+    }
+    // the face profile has Nz + 2H + 1 entries addressed (k + 1 at the top level); pad one
+    for (double **d : {&h->Wc, &h->Wf, &h->thf}) HIPCHK(hipMalloc(d, sizeof(double) * (nz + 1)));
+    std::vector<double> wc(Wc, Wc + nz), wf(Wf, Wf + nz), tf(thetabar_f, thetabar_f + nz);
+    wc.push_back(wc.back()); wf.push_back(wf.back()); tf.push_back(tf.back());
+    HIPCHK(hipMemcpy(h->Wc, wc.data(), sizeof(double) * (nz + 1), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(h->Wf, wf.data(), sizeof(double) * (nz + 1), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(h->thf, tf.data(), sizeof(double) * (nz + 1), hipMemcpyHostToDevice));
+    *out = h.release();
+    return AEO_OK;
+}
+
+extern "C" int aeo_mw_stepper_destroy(aeo_mw_stepper *h) {
+    if (!h) return AEO_OK;
+    if (g_ctx) hipStreamSynchronize(g_ctx->stream);
+    for (void *b : {h->us, h->vs, h->ws, h->tn}) hipFree(b);
+    for (double *b : {h->Wc, h->Wf, h->thf}) hipFree(b);
+    delete h;
+    return AEO_OK;
+}
+
+template <typename T>
+static int mw_step(aeo_mw_stepper *h, T *u, T *v, T *w, T *th, T *p, double dt) {
+    aeo_grid *g = h->g;
+    const aeo_mw_params &sp = h->sp;
+    hipStream_t st = g_ctx->stream;
+    const int Nx = g->Nx, Ny = g->Ny, Nz = g->Nz, H = g->H;
+    T *us = (T *)h->us, *vs = (T *)h->vs, *ws = (T *)h->ws, *tn = (T *)h->tn;
+    {
+        Timed t("halo");
+        launch_halo<T>(st, u, Nx, Ny, Nz, H, LOC_FX, sp.bc_x, sp.bc_y, sp.bc_z);
+        launch_halo<T>(st, v, Nx, Ny, Nz, H, LOC_FY, sp.bc_x, sp.bc_y, sp.bc_z);
+        launch_halo<T>(st, w, Nx, Ny, Nz, H, LOC_FZ, sp.bc_x, sp.bc_y, sp.bc_z);
+        launch_halo<T>(st, th, Nx, Ny, Nz, H, LOC_CC, sp.tbc_x, sp.tbc_y, sp.tbc_z);
+    }
+    MWPhys ph{sp.nu, dt, sp.g_accel, sp.flux_bot * (1 / g->dz), sp.flux_top * (1 / g->dz), sp.buoyancy};
+    {
+        Timed t("tendency");
+        launch_mw_tendency<T>(st, metrics<T>(g), us, vs, ws, tn, u, v, w, th, h->Wc, h->Wf, h->thf, ph);
+    }
+    {
+        Timed t("halo");
+        launch_halo<T>(st, us, Nx, Ny, Nz, H, LOC_FX, sp.bc_x, sp.bc_y, sp.bc_z);
+        launch_halo<T>(st, vs, Nx, Ny, Nz, H, LOC_FY, sp.bc_x, sp.bc_y, sp.bc_z);
+        launch_halo<T>(st, ws, Nx, Ny, Nz, H, LOC_FZ, sp.bc_x, sp.bc_y, sp.bc_z);
+    }
+    {
+        Timed t("divergence");
+        launch_wdiv<T>(st, g->m64, h->solver->src, (T *)nullptr, us, vs, ws, h->Wc, h->Wf, 0);
+    }
+    int rc = poisson_run(h->solver, h->solver->phi);
+    if (rc) return rc;
+    {
+        Timed t("correct");
+        int bx = sp.bc_x == AEO_PERIODIC ? BC_PERIODIC : BC_NOFLUX;
+        int by = sp.bc_y == AEO_PERIODIC ? BC_PERIODIC : BC_NOFLUX;
+        int bz = sp.bc_z == AEO_PERIODIC ? BC_PERIODIC : BC_NOFLUX;
+        launch_correct<T>(st, u, v, w, us, vs, ws, u, v, w, h->solver->phi, p, g->m64, bx, by, bz, 0.0, 1.0, 0, dt);
+        launch_shell<T>(st, u, us, u, Nx, Ny, Nz, H, 0.0, 1.0);
+        launch_shell<T>(st, v, vs, v, Nx, Ny, Nz, H, 0.0, 1.0);
+        launch_shell<T>(st, w, ws, w, Nx, Ny, Nz, H, 0.0, 1.0);
+        dim3 grid((Nx + 255) / 256, Ny, Nz);
+        hipLaunchKernelGGL(copy_interior_kernel<T>, grid, dim3(256), 0, st, th, tn, Nx, Ny, Nz, H);
+    }
+    HIPCHK(hipGetLastError());
+    return AEO_OK;
+}
+
+extern "C" int aeo_mw_step(aeo_mw_stepper *h, void *u, void *v, void *w, void *theta, void *p, double dt) {
+    NEED_CTX();
+    if (!h || !u || !v || !w || !theta) return fail(AEO_ERR_ARG, "null argument");
+    if (!(dt > 0) || !std::isfinite(dt)) return fail(AEO_ERR_ARG, "dt must be positive and finite");
+    return h->sp.precision == AEO_F64
+               ? mw_step<double>(h, (double *)u, (double *)v, (double *)w, (double *)theta, (double *)p, dt)
+               : mw_step<float>(h, (float *)u, (float *)v, (float *)w, (float *)theta, (float *)p, dt);
+}
+
+extern "C" int aeo_weighted_divergence(aeo_grid *g, int precision, void *out, const void *u, const void *v,
+                                       const void *w, const double *Wc, const double *Wf, int normalize) {
+    NEED_CTX();
+    if (!g || !out || !u || !v || !w || !Wc || !Wf) return fail(AEO_ERR_ARG, "null argument");
+    if (g->stretched) return fail(AEO_ERR_UNSUPPORTED, "weighted_divergence!: regular grids");
+    if (precision == AEO_F64)
+        launch_wdiv<double>(g_ctx->stream, g->m64, nullptr, (double *)out, (const double *)u, (const double *)v,
+                            (const double *)w, Wc, Wf, normalize);
+    else if (precision == AEO_F32)
+        launch_wdiv<float>(g_ctx->stream, g->m64, nullptr, (float *)out, (const float *)u, (const float *)v,
+                           (const float *)w, Wc, Wf, normalize);
+    else return fail(AEO_ERR_ARG, "bad precision %d", precision);
+    HIPCHK(hipGetLastError());
+    return AEO_OK;
+}
+
+// ---------------------------------------------------------------------------
 // diagnostics and profiling
 // ---------------------------------------------------------------------------
 static int run_diag(aeo_grid *g, int precision, const void *u, const void *v, const void *w, int what, double *out) {

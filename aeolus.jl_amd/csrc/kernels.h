@@ -40,6 +40,19 @@ template <typename T>
 void launch_diag(hipStream_t st, double *part, int nblocks, const T *u, const T *v, const T *w,
                  const Metrics<double> &md, int what);
 
+// ---- the sound-proof core (anelastic.hip) ----
+struct MWPhys {
+    double nu, dt, g_accel, fb, ft;  // fb, ft: surface fluxes times 1/Δz
+    int buoyancy;
+};
+template <typename T>
+void launch_mw_tendency(hipStream_t st, const Metrics<T> &m, T *us, T *vs, T *ws, T *tn, const T *u, const T *v,
+                        const T *w, const T *th, const double *Wc, const double *Wf, const double *thf,
+                        const MWPhys &ph);
+template <typename T>
+void launch_wdiv(hipStream_t st, const Metrics<double> &m, double *buf, T *out, const T *u, const T *v, const T *w,
+                 const double *Wc, const double *Wf, int normalize);
+
 // ---- cut-cell operators (cutcell.hip): op 0 divergence, 1 gradient/correction, 2 Laplacian ----
 template <typename T>
 int launch_cutcell(hipStream_t st, int op, const Metrics<double> &m, T *a, T *b, T *c, const T *f0, const T *f1,

@@ -169,6 +169,32 @@ int aeo_step_rk3(aeo_stepper *h, void *u, void *v, void *w, void *theta, void *p
 /* step!(sim, dt) (src/Simulations.jl:474-487): one projected forward-Euler step */
 int aeo_step_euler(aeo_stepper *h, void *u, void *v, void *w, void *theta, void *p, double dt);
 
+/* ---- the sound-proof (mass-weighted) core, regular grids ------------------
+ * _mass_weighted_step! (src/Anelastic.jl:1791-1876) with the anelastic
+ * weighting W = ρ̄: weighted momentum and scalar advection (centred, the
+ * reference's forms :1170-1311), ν∇², the anelastic buoyancy
+ * g(θ_face − θ̄_f)/θ̄_f (:1328-1360), the surface heat flux, the predictor,
+ * _project_mass_weighted! (:1756-1778: ∇·(W∇φ) = ∇·(W u*) with `solver`, the
+ * density-weighted column solver, u ← u* − ∇φ), θ += Δt tθ, p = φ/Δt.
+ * Wc, Wf, thetabar_f: host arrays of the halo-padded ReferenceState
+ * profiles ρ̄c_pad, ρ̄f_pad, θ̄f_pad (Nz + 2H entries, storage-level index). */
+typedef struct {
+    double nu, g_accel, flux_bot, flux_top;
+    int buoyancy;
+    int bc_x, bc_y, bc_z;    /* velocity BCs */
+    int tbc_x, tbc_y, tbc_z; /* θ BCs */
+    int precision;
+} aeo_mw_params;
+typedef struct aeo_mw_stepper aeo_mw_stepper;
+int aeo_mw_stepper_create(aeo_grid *g, const aeo_mw_params *p, const double *Wc, const double *Wf,
+                          const double *thetabar_f, aeo_poisson *solver, aeo_mw_stepper **h);
+int aeo_mw_stepper_destroy(aeo_mw_stepper *h);
+int aeo_mw_step(aeo_mw_stepper *h, void *u, void *v, void *w, void *theta, void *p, double dt);
+/* weighted_divergence!(out, u, v, w, Wc, Wf; normalize) (src/Anelastic.jl:1082-1110):
+ * out's interior; Wc, Wf device arrays (Nz + 2H + 1 entries) */
+int aeo_weighted_divergence(aeo_grid *g, int precision, void *out, const void *u, const void *v, const void *w,
+                            const double *Wc, const double *Wf, int normalize);
+
 /* ---- diagnostics (halos of the inputs filled) ------------------------------ */
 /* max_divergence (src/Simulations.jl:1045-1050) */
 int aeo_max_divergence(aeo_grid *g, int precision, const void *u, const void *v, const void *w, double *out);

@@ -1736,3 +1736,164 @@ void orc_cutcell_heightmap(const orc_grid *g, const double *hfull, int nsub, dou
     for (size_t c = 0; c < n; c++) vol_eff[c] = solid[c] ? 0.0 : (vol[c] < vol_min ? vol_min : vol[c]);
     free(solid);
 }
+
+/* ---------------------------------------------------------------------------
+ * The sound-proof (mass-weighted) core: one projected forward-Euler step,
+ * _mass_weighted_step! (src/Anelastic.jl:1791-1876), anelastic weighting
+ * (W = ρ̄), regular grid, in its pass form: weighted momentum and scalar
+ * advection (:1170-1310), ν∇², anelastic buoyancy (:1328-1360), surface
+ * flux, predictor, _project_mass_weighted! (:1756-1778) with
+ * weighted_divergence! (:1082-1130), θ += Δt tθ, p = φ/Δt.
+ * Wc, Wf, thf: the halo-padded profiles (storage-level index, Nz + 2H).
+ * ------------------------------------------------------------------------ */
+void orc_mw_step(void *h, double *u, double *v, double *w, double *th, double *p, double dt, const double *Wc,
+                 const double *Wf, const double *thf, double g_accel, int buoyancy) {
+    orc_stepper *s = (orc_stepper *)h;
+    const orc_grid *g = &s->g;
+    const orc_step_params *sp = &s->sp;
+    GRID_DIMS(g);
+    int H = g->H;
+    const double idx = 1.0 / g->dx, idy = 1.0 / g->dy, idz = 1.0 / g->dz;
+    orc_fill_halos(u, g, ORC_FX, sp->bcx, sp->bcy, sp->bcz);
+    orc_fill_halos(v, g, ORC_FY, sp->bcx, sp->bcy, sp->bcz);
+    orc_fill_halos(w, g, ORC_FZ, sp->bcx, sp->bcy, sp->bcz);
+    orc_fill_halos(th, g, ORC_CC, sp->tbcx, sp->tbcy, sp->tbcz);
+#define U(i, j, k) u[ID(i, j, k)]
+#define V(i, j, k) v[ID(i, j, k)]
+#define W(i, j, k) w[ID(i, j, k)]
+#define F(i, j, k) th[ID(i, j, k)]
+#pragma omp parallel for schedule(static)
+    for (int k = H + 1; k <= H + g->Nz; k++)
+        for (int j = H + 1; j <= H + g->Ny; j++)
+            for (int i = H + 1; i <= H + g->Nx; i++) {
+                size_t c = ID(i, j, k);
+                /* Wc, Wf are 0-based arrays of the 1-based storage level k */
+                double iwc = 1 / Wc[k - 1], wfb = Wf[k - 1], wft = Wf[k];
+                /* _wmom_u (:1192-1215) */
+                double uce = 0.5 * (U(i, j, k) + U(i + 1, j, k)), ucw = 0.5 * (U(i - 1, j, k) + U(i, j, k));
+                double d_uu_dx = (uce * uce - ucw * ucw) * idx;
+                double vun = 0.5 * (V(i - 1, j + 1, k) + V(i, j + 1, k)), uvn = 0.5 * (U(i, j, k) + U(i, j + 1, k));
+                double vus = 0.5 * (V(i - 1, j, k) + V(i, j, k)), uvs = 0.5 * (U(i, j - 1, k) + U(i, j, k));
+                double d_vu_dy = (vun * uvn - vus * uvs) * idy;
+                double wut = 0.5 * (W(i - 1, j, k + 1) + W(i, j, k + 1)), uwt = 0.5 * (U(i, j, k) + U(i, j, k + 1));
+                double wub = 0.5 * (W(i - 1, j, k) + W(i, j, k)), uwb = 0.5 * (U(i, j, k - 1) + U(i, j, k));
+                double d_wu_dz = (wft * wut * uwt - wfb * wub * uwb) * idz * iwc;
+                double tu = 0.0 - ((d_uu_dx + d_vu_dy) + d_wu_dz);
+                /* _wmom_v (:1221-1243) */
+                double uve = 0.5 * (U(i + 1, j - 1, k) + U(i + 1, j, k)), vue = 0.5 * (V(i, j, k) + V(i + 1, j, k));
+                double uvw = 0.5 * (U(i, j - 1, k) + U(i, j, k)), vuw = 0.5 * (V(i - 1, j, k) + V(i, j, k));
+                double d_uv_dx = (uve * vue - uvw * vuw) * idx;
+                double vcn = 0.5 * (V(i, j, k) + V(i, j + 1, k)), vcs = 0.5 * (V(i, j - 1, k) + V(i, j, k));
+                double d_vv_dy = (vcn * vcn - vcs * vcs) * idy;
+                double wvt = 0.5 * (W(i, j - 1, k + 1) + W(i, j, k + 1)), vwt = 0.5 * (V(i, j, k) + V(i, j, k + 1));
+                double wvb = 0.5 * (W(i, j - 1, k) + W(i, j, k)), vwb = 0.5 * (V(i, j, k - 1) + V(i, j, k));
+                double d_wv_dz = (wft * wvt * vwt - wfb * wvb * vwb) * idz * iwc;
+                double tv = 0.0 - ((d_uv_dx + d_vv_dy) + d_wv_dz);
+                /* _wmom_w (:1249-1271) */
+                double iwf = 1 / Wf[k - 1], wcb = Wc[k - 2], wct = Wc[k - 1];
+                double uwe = 0.5 * (U(i + 1, j, k - 1) + U(i + 1, j, k)), wue = 0.5 * (W(i, j, k) + W(i + 1, j, k));
+                double uww = 0.5 * (U(i, j, k - 1) + U(i, j, k)), wuw = 0.5 * (W(i - 1, j, k) + W(i, j, k));
+                double d_uw_dx = (uwe * wue - uww * wuw) * idx;
+                double vwn = 0.5 * (V(i, j + 1, k - 1) + V(i, j + 1, k)), wvn = 0.5 * (W(i, j, k) + W(i, j + 1, k));
+                double vws = 0.5 * (V(i, j, k - 1) + V(i, j, k)), wvs = 0.5 * (W(i, j - 1, k) + W(i, j, k));
+                double d_vw_dy = (vwn * wvn - vws * wvs) * idy;
+                double wtop = 0.5 * (W(i, j, k) + W(i, j, k + 1)), wbot = 0.5 * (W(i, j, k - 1) + W(i, j, k));
+                double d_ww_dz = (wct * wtop * wtop - wcb * wbot * wbot) * idz * iwf;
+                double tw = 0.0 - ((d_uw_dx + d_vw_dy) + d_ww_dz);
+                /* _wscal (:1297-1311) */
+                double ufe = U(i + 1, j, k) * 0.5 * (F(i, j, k) + F(i + 1, j, k));
+                double ufw = U(i, j, k) * 0.5 * (F(i - 1, j, k) + F(i, j, k));
+                double vfn = V(i, j + 1, k) * 0.5 * (F(i, j, k) + F(i, j + 1, k));
+                double vfs = V(i, j, k) * 0.5 * (F(i, j - 1, k) + F(i, j, k));
+                double wft_ = wft * W(i, j, k + 1) * 0.5 * (F(i, j, k) + F(i, j, k + 1));
+                double wfb_ = wfb * W(i, j, k) * 0.5 * (F(i, j, k - 1) + F(i, j, k));
+                double tt = 0.0 - (((ufe - ufw) * idx + (vfn - vfs) * idy) + (wft_ - wfb_) * idz * iwc);
+                if (sp->nu != 0) {
+                    tu = tu + sp->nu * lap_at(g, sx, sy, i, j, k, u, ORC_FX);
+                    tv = tv + sp->nu * lap_at(g, sx, sy, i, j, k, v, ORC_FY);
+                    tw = tw + sp->nu * lap_at(g, sx, sy, i, j, k, w, ORC_FZ);
+                    tt = tt + sp->nu * lap_at(g, sx, sy, i, j, k, th, ORC_CC);
+                }
+                if (buoyancy) {
+                    /* _anelastic_buoyancy (:1352-1360) */
+                    double tb = thf[k - 1];
+                    double coef = g_accel / tb;
+                    double tface = 0.5 * (F(i, j, k - 1) + F(i, j, k));
+                    tw += coef * (tface - tb);
+                }
+                s->tu[c] = tu; s->tv[c] = tv; s->tw[c] = tw; s->tt[c] = tt;
+            }
+#undef U
+#undef V
+#undef W
+#undef F
+    if (sp->flux_bot != 0 || sp->flux_top != 0) {
+        int kb = H + 1, kt = H + g->Nz;
+        double fb = sp->flux_bot * idz, ft = sp->flux_top * idz;
+        for (int j = H + 1; j <= H + g->Ny; j++)
+            for (int i = H + 1; i <= H + g->Nx; i++) {
+                s->tt[ID(i, j, kb)] += fb;
+                s->tt[ID(i, j, kt)] -= ft;
+            }
+    }
+    /* 7. predictor */
+#pragma omp parallel for schedule(static)
+    for (int k = H + 1; k <= H + g->Nz; k++)
+        for (int j = H + 1; j <= H + g->Ny; j++)
+            for (int i = H + 1; i <= H + g->Nx; i++) {
+                size_t c = ID(i, j, k);
+                u[c] = u[c] + dt * s->tu[c];
+                v[c] = v[c] + dt * s->tv[c];
+                w[c] = w[c] + dt * s->tw[c];
+            }
+    /* 8. weighted divergence (normalize = false) of the predicted velocity */
+    orc_fill_halos(u, g, ORC_FX, sp->bcx, sp->bcy, sp->bcz);
+    orc_fill_halos(v, g, ORC_FY, sp->bcx, sp->bcy, sp->bcz);
+    orc_fill_halos(w, g, ORC_FZ, sp->bcx, sp->bcy, sp->bcz);
+    for (int k = H + 1; k <= H + g->Nz; k++)
+        for (int j = H + 1; j <= H + g->Ny; j++)
+            for (int i = H + 1; i <= H + g->Nx; i++) {
+                double wc = Wc[k - 1], wfb = Wf[k - 1], wft = Wf[k];
+                double d = wc * ((u[ID(i + 1, j, k)] - u[ID(i, j, k)]) * idx + (v[ID(i, j + 1, k)] - v[ID(i, j, k)]) * idy) +
+                           (wft * w[ID(i, j, k + 1)] - wfb * w[ID(i, j, k)]) * idz;
+                s->buf[(size_t)(i - H - 1) + (size_t)g->Nx * ((size_t)(j - H - 1) + (size_t)g->Ny * (k - H - 1))] = d;
+            }
+    /* 9. ∇·(W∇φ) = ∇·(W u*) */
+    orc_column_solve(s->solver, s->pbuf, s->buf);
+    for (int k = 1; k <= g->Nz; k++)
+        for (int j = 1; j <= g->Ny; j++)
+            for (int i = 1; i <= g->Nx; i++)
+                s->phi[ID(i + H, j + H, k + H)] =
+                    s->pbuf[(size_t)(i - 1) + (size_t)g->Nx * ((size_t)(j - 1) + (size_t)g->Ny * (k - 1))];
+    orc_fill_halos(s->phi, g, ORC_CC, sp->bcx, sp->bcy, sp->bcz);
+    /* 10. u = u* − ∇φ; 11. θ += Δt tθ; 12. p = φ/Δt */
+#pragma omp parallel for schedule(static)
+    for (int k = H + 1; k <= H + g->Nz; k++)
+        for (int j = H + 1; j <= H + g->Ny; j++)
+            for (int i = H + 1; i <= H + g->Nx; i++) {
+                size_t c = ID(i, j, k);
+                double pc = s->phi[c];
+                u[c] = u[c] + -1.0 * ((pc - s->phi[ID(i - 1, j, k)]) * idx);
+                v[c] = v[c] + -1.0 * ((pc - s->phi[ID(i, j - 1, k)]) * idy);
+                w[c] = w[c] + -1.0 * ((pc - s->phi[ID(i, j, k - 1)]) * idz);
+                th[c] = th[c] + dt * s->tt[c];
+                p[c] = pc / dt;
+            }
+}
+
+/* weighted_divergence! (src/Anelastic.jl:1082-1110) over the interior into out (storage) */
+void orc_weighted_divergence(double *out, const double *u, const double *v, const double *w, const orc_grid *g,
+                             const double *Wc, const double *Wf, int normalize) {
+    GRID_DIMS(g);
+    int H = g->H;
+    const double idx = 1.0 / g->dx, idy = 1.0 / g->dy, idz = 1.0 / g->dz;
+    for (int k = H + 1; k <= H + g->Nz; k++)
+        for (int j = H + 1; j <= H + g->Ny; j++)
+            for (int i = H + 1; i <= H + g->Nx; i++) {
+                double wc = Wc[k - 1], wfb = Wf[k - 1], wft = Wf[k];
+                double scale = normalize ? 1 / wc : 1.0;
+                double d = wc * ((u[ID(i + 1, j, k)] - u[ID(i, j, k)]) * idx + (v[ID(i, j + 1, k)] - v[ID(i, j, k)]) * idy) +
+                           (wft * w[ID(i, j, k + 1)] - wfb * w[ID(i, j, k)]) * idz;
+                out[ID(i, j, k)] = scale * d;
+            }
+}

@@ -89,6 +89,8 @@ def lib():
             "orc_mgpcg_solve": (i, [vp, pd, pd, d, i, P(i), pd, pd]),
             "orc_mg_apply": (None, [vp, pd, pd]),
             "orc_cutcell_heightmap": (None, [pg, pd, i, d, pd, pd, pd, pd, pd]),
+            "orc_mw_step": (None, [vp, pd, pd, pd, pd, pd, d, pd, pd, pd, d, i]),
+            "orc_weighted_divergence": (None, [pd, pd, pd, pd, pg, pd, pd, i]),
             "orc_unet_num_params": (ctypes.c_longlong, [i]),
             "orc_unet_forward": (None, [P(ctypes.c_float), i, i, i, i, pd, pd, i, pd]),
             "orc_unet_gradients": (None, [P(ctypes.c_float), i, i, i, i, pd, pd, pd, i, pd, pd]),
@@ -330,3 +332,50 @@ def unet_gradients(params, src, target, mask=None, bf16=True):
     lib().orc_unet_gradients(_fptr(params), nx, ny, nz, inc, ptr(src), m, ptr(target), int(bf16), ptr(grad),
                              ctypes.byref(loss))
     return loss.value, grad
+
+
+# ---------------------------------------------------------------------------
+# the sound-proof core (orc_mw_step)
+# ---------------------------------------------------------------------------
+class MWStepper:
+    """_mass_weighted_step! of the anelastic core with the density-weighted
+    column solver; Wc, Wf, thf the padded reference profiles."""
+
+    def __init__(self, g, Wc, Wf, thf, rho, rho_f, nu=0.0, g_accel=9.81, flux_bot=0.0, flux_top=0.0, buoyancy=True,
+                 bcs=("periodic", "periodic", "wall"), theta_bcs=("periodic", "periodic", "noflux")):
+        Nz, dz = g.Nz, g.dz
+        lower = np.array([rho_f[k] / dz ** 2 if k > 0 else 0.0 for k in range(Nz)])
+        upper = np.array([rho_f[k + 1] / dz ** 2 if k < Nz - 1 else 0.0 for k in range(Nz)])
+        lat = ["periodic" if b == "periodic" else "noflux" for b in bcs[:2]]
+        self.solver = column_solver((g.Nx, g.Ny, Nz), (g.dx, g.dy), lower, upper, np.asarray(rho, dtype=np.float64),
+                                    np.full(Nz, dz ** 2), 1.0 / np.asarray(rho_f, dtype=np.float64), np.ones(Nz),
+                                    lat[0], lat[1])
+        sp = OrcStep()
+        sp.mom_scheme = sp.scal_scheme = CENTERED
+        sp.weno_p = 2
+        sp.nu, sp.theta0, sp.g_accel, sp.flux_bot, sp.flux_top = nu, 0.0, g_accel, flux_bot, flux_top
+        sp.bcx, sp.bcy, sp.bcz = (BCS[b] for b in bcs)
+        sp.tbcx, sp.tbcy, sp.tbcz = (BCS[b] for b in theta_bcs)
+        sp.solver = 1
+        self._sp = sp
+        self.g = g
+        self.h = lib().orc_stepper_create(ctypes.byref(g.c), ctypes.byref(sp), self.solver.h)
+        self.prof = [np.ascontiguousarray(a, dtype=np.float64) for a in (Wc, Wf, thf)]
+        self.g_accel, self.buoyancy = g_accel, int(buoyancy)
+
+    def step(self, u, v, w, th, p, dt):
+        lib().orc_mw_step(self.h, ptr(u), ptr(v), ptr(w), ptr(th), ptr(p), dt, *[ptr(a) for a in self.prof],
+                          self.g_accel, self.buoyancy)
+
+    def __del__(self):
+        try:
+            lib().orc_stepper_destroy(self.h)
+        except Exception:
+            pass
+
+
+def weighted_divergence(g, u, v, w, Wc, Wf, normalize=False):
+    out = g.zeros()
+    lib().orc_weighted_divergence(ptr(out), ptr(u), ptr(v), ptr(w), ctypes.byref(g.c),
+                                  ptr(np.ascontiguousarray(Wc)), ptr(np.ascontiguousarray(Wf)), int(normalize))
+    return out

@@ -1,0 +1,69 @@
+"""The sound-proof core oracle (CPU): a base state at rest stays at rest, the
+post-step mass-flux divergence vanishes (test/anelastic.jl:112-129), and the
+isothermal reference profiles are the closed forms."""
+import numpy as np
+
+from helpers import rel_err
+
+
+def _setup(orc, N=(16, 12, 20), H=1, dz=100.0):
+    import sys
+    from aeolus_hip.anelastic import isothermal_reference
+    g = orc.Grid(N, (N[0] * 200.0, N[1] * 200.0, N[2] * dz), halo=H)
+    ref = isothermal_reference(N[2], dz, H)
+    return g, ref
+
+
+def _state(g, ref, seed, amp=1.0):
+    rng = np.random.default_rng(seed)
+    H = g.H
+    st = g.storage
+    th = np.zeros(st)
+    th[:, :, H:H + g.Nz] = ref.theta[None, None, :]
+    u, v, w = (np.zeros(st) for _ in range(3))
+    I = g.interior
+    if amp:
+        th[I] += 0.5 * amp * rng.standard_normal(th[I].shape)
+        u[I] += amp * rng.standard_normal(u[I].shape)
+        v[I] += amp * rng.standard_normal(v[I].shape)
+    return [np.asfortranarray(a) for a in (u, v, w, th)]
+
+
+def test_isothermal_reference_closed_forms():
+    from aeolus_hip.anelastic import isothermal_reference
+    ref = isothermal_reference(40, 250.0, 2)
+    zc = (np.arange(40) + 0.5) * 250.0
+    Hs = 287.0 * 288.0 / 9.81
+    assert rel_err(ref.rho, 1e5 / (287.0 * 288.0) * np.exp(-zc / Hs)) < 1e-14
+    # the log-mean face densities make the discrete hydrostatic balance exact:
+    # (p̄[k-1] − p̄[k])/Δz = g ρ̄_f[k]
+    dp = (ref.p[:-1] - ref.p[1:]) / 250.0
+    assert rel_err(dp, 9.81 * ref.rho_f[1:40]) < 1e-12
+    assert ref.rho_f[0] == ref.rho[0] and ref.rho_f[40] == ref.rho[39]
+    assert len(ref.rho_c_pad) == 44 and ref.rho_c_pad[2] == ref.rho[0]
+
+
+def test_base_state_at_rest_stays_at_rest(orc):
+    g, ref = _setup(orc)
+    u, v, w, th = _state(g, ref, 0, amp=0.0)
+    th0 = th.copy()
+    st = orc.MWStepper(g, ref.rho_c_pad, ref.rho_f_pad, ref.theta_f_pad, ref.rho, ref.rho_f)
+    p = g.zeros()
+    for _ in range(3):
+        st.step(u, v, w, th, p, 5.0)
+    I = g.interior
+    assert np.abs(u[I]).max() == 0 and np.abs(v[I]).max() == 0 and np.abs(w[I]).max() < 1e-12
+    assert np.array_equal(th[I], th0[I])
+
+
+def test_post_step_mass_flux_divergence_vanishes(orc):
+    g, ref = _setup(orc)
+    u, v, w, th = _state(g, ref, 1)
+    st = orc.MWStepper(g, ref.rho_c_pad, ref.rho_f_pad, ref.theta_f_pad, ref.rho, ref.rho_f, nu=1.0)
+    p = g.zeros()
+    st.step(u, v, w, th, p, 2.0)
+    for a, loc in ((u, orc.FX), (v, orc.FY), (w, orc.FZ)):
+        orc.fill_halos(g, a, loc, ("periodic", "periodic", "wall"))
+    d = orc.weighted_divergence(g, u, v, w, ref.rho_c_pad, np.append(ref.rho_f_pad, ref.rho_f_pad[-1]))
+    scale = ref.rho.max() * np.abs(u[g.interior]).max() / g.dx
+    assert np.abs(d[g.interior]).max() < 1e-10 * scale

@@ -268,3 +268,35 @@ def test_diagnostics_parity(hip, orc, stretched):
     dzf_c = dzf[:-1] if dzf.size == g.Nz + 1 else dzf
     ref = 0.5 * ((u[I] ** 2 + v[I] ** 2) * dzc[None, None, :] + w[I] ** 2 * dzf_c[None, None, :]).sum() * g.dx * g.dy
     assert abs(ke - ref) <= 1e-12 * ref
+
+
+@pytest.mark.parametrize("nu", [0.0, 1.0])
+def test_anelastic_step_parity_fp64(hip, orc, nu):
+    # _mass_weighted_step! (anelastic weighting) against the oracle, two steps
+    from aeolus_hip.anelastic import AnelasticSimulation, isothermal_reference
+    N, H, dz = (16, 12, 20), 1, 100.0
+    extent = (N[0] * 200.0, N[1] * 200.0, N[2] * dz)
+    g, og = hip.Grid(N, extent, halo=H), orc.Grid(N, extent, halo=H)
+    ref = isothermal_reference(N[2], dz, H)
+    rng = np.random.default_rng(21)
+    st = og.storage
+    th = np.zeros(st)
+    th[:, :, H:H + N[2]] = ref.theta[None, None, :]
+    I = og.interior
+    th[I] += 0.5 * rng.standard_normal(th[I].shape)
+    u, v, w = (np.zeros(st) for _ in range(3))
+    u[I] += rng.standard_normal(u[I].shape)
+    v[I] += rng.standard_normal(v[I].shape)
+    u, v, w, th = (np.asfortranarray(a) for a in (u, v, w, th))
+    sim = AnelasticSimulation(g, ref, nu=nu, flux_bot=0.05)
+    sim.set(u, v, w, th)
+    ost = orc.MWStepper(og, ref.rho_c_pad, ref.rho_f_pad, ref.theta_f_pad, ref.rho, ref.rho_f, nu=nu, flux_bot=0.05)
+    p = og.zeros()
+    for _ in range(2):
+        sim.step(3.0)
+        ost.step(u, v, w, th, p, 3.0)
+    got = sim.state()
+    for n, ref_a in (("u", u), ("v", v), ("w", w), ("theta", th), ("p", p)):
+        assert rel_err(got[n][I], ref_a[I]) <= TOL64, n
+    d = sim.mass_flux_divergence()
+    assert np.abs(d).max() < 1e-10 * ref.rho.max() * np.abs(got["u"][I]).max() / g.dx
