@@ -92,6 +92,12 @@ SIGNATURES = {
     "aeo_max_divergence": [_vp, _i, _vp, _vp, _vp, _pd],
     "aeo_max_advective_rate": [_vp, _i, _vp, _vp, _vp, _pd],
     "aeo_kinetic_energy": [_vp, _i, _vp, _vp, _vp, _pd],
+    "aeo_terrain_create": [_vp, _pd, _d, _pd, _pd, _pd, _pd, _i, _i, _pvp],
+    "aeo_terrain_destroy": [_vp],
+    "aeo_terrain_divergence": [_vp, _vp, _vp, _vp, _vp],
+    "aeo_terrain_laplacian": [_vp, _vp, _vp],
+    "aeo_terrain_apply": [_vp, _vp, _vp],
+    "aeo_poisson_terrain_create": [_vp, _vp, _d, _i, _pvp],
     "aeo_mw_stepper_create": [_vp, ctypes.POINTER(MWParams), _pd, _pd, _pd, _vp, _pvp],
     "aeo_mw_stepper_destroy": [_vp],
     "aeo_mw_step": [_vp, _vp, _vp, _vp, _vp, _vp, _d],

@@ -1,0 +1,143 @@
+"""Terrain-following (Gal-Chen) grids: the metric operators on the GPU and
+the terrain Poisson solve (src/StretchedGrids.jl:608-1343).
+
+    z(x, y, σ) = h(x, y) + σ (H − h)/H,   J = (H − h)/H
+
+The terrain plane is extended to the storage by periodic wrap (an (Nx, Ny)
+matrix, _terrain_matrix :716-740); the metrics are evaluated in the kernels
+from the plane and its centred slopes.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib as L
+from .fields import DeviceArray
+
+
+def _pd(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def storage_plane(h, N, H):
+    """An (Nx, Ny) terrain extended to the (Nx+2H, Ny+2H) storage plane by periodic wrap."""
+    Nx, Ny = N[0], N[1]
+    h = np.asarray(h, dtype=np.float64)
+    if h.shape == (Nx + 2 * H, Ny + 2 * H):
+        return np.asfortranarray(h)
+    ii = (np.arange(Nx + 2 * H) - H) % Nx
+    jj = (np.arange(Ny + 2 * H) - H) % Ny
+    return np.asfortranarray(h[np.ix_(ii, jj)])
+
+
+def uniform_sigma(Nz, dz, H):
+    """σ levels of a uniform base column: (sigc, sigf, dzc, dzf) per storage level."""
+    k = np.arange(Nz + 2 * H) - H
+    sigf = k * dz
+    sigc = (k + 0.5) * dz
+    return sigc, sigf, np.full(Nz + 2 * H, dz), np.full(Nz + 2 * H, dz)
+
+
+def gaussian_hills(N, spacing, n=3, seed=0, height=(100.0, 400.0), half_width=(500.0, 2000.0)):
+    """n seeded Gaussian hills on an (Nx, Ny) periodic plane: h = Σ A exp(−r²/(2 w²)), r the periodic distance."""
+    rng = np.random.default_rng(seed)
+    Nx, Ny = N[0], N[1]
+    Lx, Ly = Nx * spacing[0], Ny * spacing[1]
+    x = (np.arange(Nx) + 0.5) * spacing[0]
+    y = (np.arange(Ny) + 0.5) * spacing[1]
+    h = np.zeros((Nx, Ny))
+    for _ in range(n):
+        A = rng.uniform(*height)
+        w = rng.uniform(*half_width)
+        x0, y0 = rng.uniform(0, Lx), rng.uniform(0, Ly)
+        dx = (x[:, None] - x0 + Lx / 2) % Lx - Lx / 2
+        dy = (y[None, :] - y0 + Ly / 2) % Ly - Ly / 2
+        h += A * np.exp(-(dx * dx + dy * dy) / (2 * w * w))
+    return h
+
+
+class TerrainFollowingGrid:
+    """terrain_following_grid(mesh, h) (src/StretchedGrids.jl:764-835) over a
+    regular hip.Grid; lid = the domain top Lz."""
+
+    def __init__(self, grid, h, lid=None, bcs=("periodic", "periodic")):
+        if grid.zf is not None:
+            raise ValueError("the terrain grid's base must be regular")
+        self.grid = grid
+        N = (grid.Nx, grid.Ny, grid.Nz)
+        self.lid = float(grid.Nz * grid.dz if lid is None else lid)
+        self.plane = storage_plane(h, N, grid.H)
+        self.sigc, self.sigf, self.dzc, self.dzf = uniform_sigma(grid.Nz, grid.dz, grid.H)
+        bk = {"periodic": L.PERIODIC, "neumann": L.NOFLUX, "noflux": L.NOFLUX}
+        t = ctypes.c_void_p()
+        L.check(L.init().aeo_terrain_create(grid.handle, _pd(self.plane), self.lid, _pd(self.sigc), _pd(self.sigf),
+                                            _pd(self.dzc), _pd(self.dzf), bk[bcs[0]], bk[bcs[1]], ctypes.byref(t)))
+        self._h = t.value
+        self.bcs = bcs
+
+    @property
+    def handle(self):
+        return ctypes.c_void_p(self._h)
+
+    def divergence(self, out, u, v, w):
+        L.check(L.init().aeo_terrain_divergence(self.handle, L.dptr(out), L.dptr(u), L.dptr(v), L.dptr(w)))
+        return out
+
+    def laplacian(self, out, f):
+        L.check(L.init().aeo_terrain_laplacian(self.handle, L.dptr(out), L.dptr(f)))
+        return out
+
+    def apply(self, out, x):
+        """out = J·∇²x on interior-sized vectors."""
+        L.check(L.init().aeo_terrain_apply(self.handle, L.dptr(out), L.dptr(x)))
+        return out
+
+    def mg_coefficients(self):
+        """Face coefficients of the 7-point part of J∇² (J_x, J_y, J·G^σσ) for the
+        multigrid preconditioner, from the reference's metric formulas."""
+        g, H, P = self.grid, self.grid.H, self.plane
+        Hl = self.lid
+        Nx, Ny, Nz = g.Nx, g.Ny, g.Nz
+        hxf = (P[H:H + Nx + 1, H:H + Ny] + P[H - 1:H + Nx, H:H + Ny]) / 2
+        hyf = (P[H:H + Nx, H:H + Ny + 1] + P[H:H + Nx, H - 1:H + Ny]) / 2
+        cx = np.repeat(((Hl - hxf) / Hl)[:, :, None], Nz, axis=2)
+        cy = np.repeat(((Hl - hyf) / Hl)[:, :, None], Nz, axis=2)
+        sx_, sy_ = P.shape
+        hx = np.empty_like(P)
+        hy = np.empty_like(P)
+        hx[1:-1, :] = (P[2:, :] - P[:-2, :]) / (2 * g.dx)
+        hx[0, :] = (P[1, :] - P[0, :]) / g.dx
+        hx[-1, :] = (P[-1, :] - P[-2, :]) / g.dx
+        hy[:, 1:-1] = (P[:, 2:] - P[:, :-2]) / (2 * g.dy)
+        hy[:, 0] = (P[:, 1] - P[:, 0]) / g.dy
+        hy[:, -1] = (P[:, -1] - P[:, -2]) / g.dy
+        hh = P[H:H + Nx, H:H + Ny][:, :, None]
+        J = (Hl - hh) / Hl
+        sf = self.sigf[H:H + Nz + 1][None, None, :]
+        sxz = -(Hl - sf) * hx[H:H + Nx, H:H + Ny][:, :, None] / (Hl - hh)
+        syz = -(Hl - sf) * hy[H:H + Nx, H:H + Ny][:, :, None] / (Hl - hh)
+        cz = J * ((sxz * sxz + syz * syz) + 1 / (J * J))
+        return tuple(np.asfortranarray(c) for c in (cx, cy, cz))
+
+    def poisson_solver(self, tol=1e-8, maxit=300, vcycle="f32", smoother="rbgs"):
+        """FlexibleCG on J∇² preconditioned by a V-cycle of its 7-point part."""
+        from .pressure import MGPreconditionedCG, PressureSolver
+        g = self.grid
+        base = MGPreconditionedCG((g.Nx, g.Ny, g.Nz), (g.dx, g.dy, g.dz),
+                                  bcs=(self.bcs[0], self.bcs[1], "neumann"), coeff=self.mg_coefficients(),
+                                  tol=tol, maxit=maxit, vcycle=vcycle, smoother=smoother)
+        h = ctypes.c_void_p()
+        L.check(L.init().aeo_poisson_terrain_create(self.handle, base.handle, float(tol), int(maxit),
+                                                    ctypes.byref(h)))
+        s = PressureSolver(h.value, (g.Nx, g.Ny, g.Nz))
+        s._keep = (base, self)
+        s.kind = "terrain-fcg"
+        s.axis_bcs = base.axis_bcs
+        return s
+
+    def __del__(self):
+        try:
+            if self._h and L._lib is not None and L._initialised:
+                L._lib.aeo_terrain_destroy(ctypes.c_void_p(self._h))
+        except Exception:
+            pass

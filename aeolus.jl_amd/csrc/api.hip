@@ -704,6 +704,109 @@ extern "C" int aeo_cutcell_project_correct(aeo_grid *g, int precision, void *u, 
 }
 
 // ---------------------------------------------------------------------------
+// terrain-following (Gal-Chen) operators and their Poisson solve
+// ---------------------------------------------------------------------------
+struct aeo_terrain {
+    aeo_grid *g;
+    TFView v;
+    double *d[7] = {};
+};
+
+extern "C" int aeo_terrain_create(aeo_grid *g, const double *h_plane, double lid, const double *sigc,
+                                  const double *sigf, const double *dzc, const double *dzf, int bc_x, int bc_y,
+                                  aeo_terrain **out) {
+    NEED_CTX();
+    if (!g || !h_plane || !sigc || !sigf || !dzc || !dzf || !out) return fail(AEO_ERR_ARG, "null argument");
+    if (g->H < 1) return fail(AEO_ERR_ARG, "the terrain operators need a halo of at least 1");
+    for (int bc : {bc_x, bc_y})
+        if (bc != AEO_PERIODIC && bc != AEO_NOFLUX) return fail(AEO_ERR_ARG, "lateral maps: periodic or Neumann");
+    const int sx = g->Nx + 2 * g->H, sy = g->Ny + 2 * g->H, sz = g->Nz + 2 * g->H;
+    double hmax = -1e300;
+    for (int q = 0; q < sx * sy; q++) hmax = h_plane[q] > hmax ? h_plane[q] : hmax;
+    // terrain_following_grid (src/StretchedGrids.jl:781-783)
+    if (!(hmax < lid)) return fail(AEO_ERR_ARG, "terrain height max = %g reaches or exceeds the model lid H = %g", hmax, lid);
+    // _slopes (src/StretchedGrids.jl:742-762): centred, one-sided at the outermost ghost ring
+    std::vector<double> hx(sx * sy), hy(sx * sy);
+    auto H2 = [&](int i, int j) { return h_plane[i + sx * j]; };
+    for (int j = 0; j < sy; j++)
+        for (int i = 0; i < sx; i++) {
+            hx[i + sx * j] = i == 0 ? (H2(1, j) - H2(0, j)) / g->dx
+                             : i == sx - 1 ? (H2(sx - 1, j) - H2(sx - 2, j)) / g->dx
+                                           : (H2(i + 1, j) - H2(i - 1, j)) / (2 * g->dx);
+            hy[i + sx * j] = j == 0 ? (H2(i, 1) - H2(i, 0)) / g->dy
+                             : j == sy - 1 ? (H2(i, sy - 1) - H2(i, sy - 2)) / g->dy
+                                           : (H2(i, j + 1) - H2(i, j - 1)) / (2 * g->dy);
+        }
+    auto t = std::make_unique<aeo_terrain>();
+    t->g = g;
+    const double *src[7] = {h_plane, hx.data(), hy.data(), sigc, sigf, dzc, dzf};
+    const size_t len[7] = {(size_t)sx * sy, (size_t)sx * sy, (size_t)sx * sy, (size_t)sz, (size_t)sz, (size_t)sz,
+                           (size_t)sz};
+    for (int q = 0; q < 7; q++) {
+        HIPCHK(hipMalloc(&t->d[q], sizeof(double) * len[q]));
+        HIPCHK(hipMemcpy(t->d[q], src[q], sizeof(double) * len[q], hipMemcpyHostToDevice));
+    }
+    t->v = TFView{t->d[0], t->d[1], t->d[2], t->d[3], t->d[4], t->d[5], t->d[6], lid, g->dx, g->dy,
+                  g->Nx, g->Ny, g->Nz, g->H, sx, sy,
+                  bc_x == AEO_PERIODIC ? BC_PERIODIC : BC_NOFLUX, bc_y == AEO_PERIODIC ? BC_PERIODIC : BC_NOFLUX};
+    *out = t.release();
+    return AEO_OK;
+}
+extern "C" int aeo_terrain_destroy(aeo_terrain *t) {
+    if (!t) return AEO_OK;
+    if (g_ctx) hipStreamSynchronize(g_ctx->stream);
+    for (double *p : t->d) hipFree(p);
+    delete t;
+    return AEO_OK;
+}
+extern "C" int aeo_terrain_divergence(aeo_terrain *t, double *out, const double *u, const double *v, const double *w) {
+    NEED_CTX();
+    if (!t || !out || !u || !v || !w) return fail(AEO_ERR_ARG, "null argument");
+    launch_tf(g_ctx->stream, 0, t->v, out, u, v, w);
+    HIPCHK(hipGetLastError());
+    return AEO_OK;
+}
+extern "C" int aeo_terrain_laplacian(aeo_terrain *t, double *out, const double *f) {
+    NEED_CTX();
+    if (!t || !out || !f) return fail(AEO_ERR_ARG, "null argument");
+    launch_tf(g_ctx->stream, 1, t->v, out, f, nullptr, nullptr);
+    HIPCHK(hipGetLastError());
+    return AEO_OK;
+}
+static void terrain_apply_cb(void *ctx, hipStream_t st, double *out, const double *x) {
+    launch_tf(st, 2, ((aeo_terrain *)ctx)->v, out, x, nullptr, nullptr);
+}
+extern "C" int aeo_terrain_apply(aeo_terrain *t, double *out, const double *x) {
+    NEED_CTX();
+    if (!t || !out || !x) return fail(AEO_ERR_ARG, "null argument");
+    terrain_apply_cb(t, g_ctx->stream, out, x);
+    HIPCHK(hipGetLastError());
+    return AEO_OK;
+}
+extern "C" int aeo_poisson_terrain_create(aeo_terrain *t, aeo_poisson *base, double tol, int maxit,
+                                          aeo_poisson **out) {
+    NEED_CTX();
+    if (!t || !base || !out) return fail(AEO_ERR_ARG, "null argument");
+    if (!base->mg || base->kind != 2) return fail(AEO_ERR_ARG, "the preconditioner must be an MG-preconditioned CG");
+    if (base->Nx != t->g->Nx || base->Ny != t->g->Ny || base->Nz != t->g->Nz)
+        return fail(AEO_ERR_ARG, "solver and terrain sizes differ");
+    if (!(tol >= 0) || maxit < 0) return fail(AEO_ERR_ARG, "bad tolerance or cap");
+    auto h = std::make_unique<aeo_poisson>();
+    h->kind = 4; h->Nx = base->Nx; h->Ny = base->Ny; h->Nz = base->Nz;
+    h->periodic_x = base->periodic_x; h->periodic_y = base->periodic_y; h->periodic_z = false;
+    h->base = base; h->mode = 2; h->use_guess = 0; h->tol = tol; h->maxit = maxit;
+    FCGOp op;
+    op.apply = terrain_apply_cb;
+    op.ctx = t;
+    h->fcg = fcg_create(base->Nx * base->Ny * base->Nz, op);
+    if (!h->fcg) return fail(AEO_ERR_HIP, "allocation failed");
+    int rc = poisson_alloc(h.get());
+    if (rc) return rc;
+    *out = h.release();
+    return AEO_OK;
+}
+
+// ---------------------------------------------------------------------------
 // the learned preconditioner
 // ---------------------------------------------------------------------------
 extern "C" int aeo_unet_create(int nx, int ny, int nz, int in_channels, unsigned long long seed, aeo_unet **out) {
@@ -783,7 +886,7 @@ extern "C" int aeo_poisson_fcg_create(aeo_poisson *base, aeo_unet *net, const do
     h->tol = tol; h->maxit = maxit;
     if (net && (unet_dims_match(net->N, base->Nx, base->Ny, base->Nz) == 0))
         return fail(AEO_ERR_ARG, "flexible CG: the network's grid differs from the solver's");
-    h->fcg = fcg_create(base->Nx * base->Ny * base->Nz);
+    h->fcg = fcg_create(base->Nx * base->Ny * base->Nz, FCGOp());
     if (!h->fcg) return fail(AEO_ERR_HIP, "flexible CG: allocation failed");
     int rc = poisson_alloc(h.get());
     if (rc) return rc;

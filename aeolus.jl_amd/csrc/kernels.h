@@ -53,6 +53,18 @@ template <typename T>
 void launch_wdiv(hipStream_t st, const Metrics<double> &m, double *buf, T *out, const T *u, const T *v, const T *w,
                  const double *Wc, const double *Wf, int normalize);
 
+// ---- terrain-following operators (terrain.hip): op 0 divergence, 1 Laplacian, 2 J·∇² on vectors ----
+struct TFView {
+    const double *h, *hx, *hy;   // storage planes (sx, sy): terrain and its centred slopes
+    const double *sigc, *sigf;   // σ of the cell centres and faces, per storage level
+    const double *dzc, *dzf;     // Δσ_c, Δσ_f per storage level
+    double Hl, dx, dy;           // the lid, horizontal spacings
+    int Nx, Ny, Nz, H, sx, sy;
+    int bcx, bcy;                // lateral maps of interior-sized vectors
+};
+void launch_tf(hipStream_t st, int op, const TFView &t, double *out, const double *a, const double *b,
+               const double *c);
+
 // ---- cut-cell operators (cutcell.hip): op 0 divergence, 1 gradient/correction, 2 Laplacian ----
 template <typename T>
 int launch_cutcell(hipStream_t st, int op, const Metrics<double> &m, T *a, T *b, T *c, const T *f0, const T *f1,
@@ -101,7 +113,12 @@ int unet_gradients(UNet *N, hipStream_t st, const double *src, const double *mas
 int unet_train_step(UNet *N, hipStream_t st, const double *src, const double *mask, const double *target, float lr,
                     float *loss);
 struct FCGState;
-FCGState *fcg_create(int n);
+// an operator on interior-sized FP64 vectors: apply(ctx, stream, out, x)
+struct FCGOp {
+    void (*apply)(void *ctx, hipStream_t st, double *out, const double *x) = nullptr;
+    void *ctx = nullptr;
+};
+FCGState *fcg_create(int n, FCGOp op = FCGOp());
 void fcg_destroy(FCGState *F);
 int fcg_solve(FCGState *F, MGPlan *P, UNet *N, const double *mask, int mode, int use_guess, hipStream_t st,
               double *xout, const double *bin, double tol, int maxit, int *iters_out, double *resid_out,

@@ -846,13 +846,15 @@ __global__ void fcg_update_kernel(double *__restrict__ x, double *__restrict__ r
 
 struct FCGState {
     int n = 0;
+    FCGOp op;  // the operator A (default: the multigrid plan's fine operator)
     double *b = nullptr, *x = nullptr, *r = nullptr, *z = nullptr, *p = nullptr, *w = nullptr, *t = nullptr;
     double *part = nullptr, *sc = nullptr;
 };
 
-FCGState *fcg_create(int n) {
+FCGState *fcg_create(int n, FCGOp op) {
     FCGState *F = new FCGState();
     F->n = n;
+    F->op = op;
     bool ok = true;
     for (double **q : {&F->b, &F->x, &F->r, &F->z, &F->p, &F->w, &F->t})
         if (hipMalloc(q, sizeof(double) * n) != hipSuccess) ok = false;
@@ -887,6 +889,11 @@ static void fcg_project(FCGState *F, hipStream_t st, double *y) {
                        s[2] / F->n, F->n);
 }
 // z = M(r), mean-free
+static void fcg_apply(FCGState *F, MGPlan *P, hipStream_t st, double *out, const double *x) {
+    if (F->op.apply) F->op.apply(F->op.ctx, st, out, x);
+    else mg_apply(P, st, out, x);
+}
+
 static int fcg_precond(FCGState *F, MGPlan *P, UNet *N, const double *mask, int mode, hipStream_t st) {
     if (mode == 2) {
         mg_precond_apply(P, st, F->z, F->r);
@@ -894,7 +901,7 @@ static int fcg_precond(FCGState *F, MGPlan *P, UNet *N, const double *mask, int 
         int rc = unet_forward(N, st, F->r, mask, F->z);
         if (rc) return rc;
         if (mode == 1) {
-            mg_apply(P, st, F->t, F->z);  // t = A z1
+            fcg_apply(F, P, st, F->t, F->z);  // t = A z1
             hipLaunchKernelGGL(fcg_axpby_kernel, dim3(nblk(F->n)), dim3(UB), 0, st, F->t, (const double *)F->r, 1.0,
                                -1.0, 0.0, F->n);  // t = r − A z1
             mg_precond_apply(P, st, F->t, F->t);
@@ -929,7 +936,7 @@ int fcg_solve(FCGState *F, MGPlan *P, UNet *N, const double *mask, int mode, int
         int rc = unet_forward(N, st, F->b, mask, F->x);
         if (rc) return -1;
         fcg_project(F, st, F->x);
-        mg_apply(P, st, F->w, F->x);
+        fcg_apply(F, P, st, F->w, F->x);
         hipMemcpyAsync(F->r, F->b, sizeof(double) * n, hipMemcpyDeviceToDevice, st);
         hipLaunchKernelGGL(fcg_axpby_kernel, dim3(nb), dim3(UB), 0, st, F->r, (const double *)F->w, -1.0, 1.0, 0.0, n);
     } else {
@@ -944,7 +951,7 @@ int fcg_solve(FCGState *F, MGPlan *P, UNet *N, const double *mask, int mode, int
         hipMemcpyAsync(F->p, F->z, sizeof(double) * n, hipMemcpyDeviceToDevice, st);
     }
     while (status != 0 && iters < maxit) {
-        mg_apply(P, st, F->w, F->p);
+        fcg_apply(F, P, st, F->w, F->p);
         fcg_sums(F, st, F->p, F->w, F->p, F->r, s);  // p·w, p·r
         const double pw = s[0], pr = s[1];
         if (!(fabs(pw) > 1e-300) || !std::isfinite(pw)) { status = 2; break; }

@@ -232,6 +232,31 @@ int aeo_cutcell_project_correct(aeo_grid *g, int precision, void *u, void *v, vo
                                 const double *ax, const double *ay, const double *az, int bc_x, int bc_y, int bc_z,
                                 double scale);
 
+/* ---- terrain-following (Gal-Chen) operators, src/StretchedGrids.jl:608-1343 -
+ * z = h + σ(H − h)/H over the regular base grid g. h_plane: the terrain on
+ * the (Nx+2H) x (Ny+2H) storage plane (x fastest); lid: H (> max h);
+ * sigc, sigf, dzc, dzf: σ of the cell centres and faces and Δσ_c, Δσ_f per
+ * storage level (host, Nz+2H each). The metrics (J, J at faces, σ_x, σ_y at
+ * faces, terrain_following_grid :777-835) are evaluated on the fly from the
+ * plane and its centred slopes (_slopes, :742-762). bc_x, bc_y: the lateral
+ * maps of interior-sized vectors (periodic or Neumann); σ is mirrored. */
+typedef struct aeo_terrain aeo_terrain;
+int aeo_terrain_create(aeo_grid *g, const double *h_plane, double lid, const double *sigc, const double *sigf,
+                       const double *dzc, const double *dzf, int bc_x, int bc_y, aeo_terrain **t);
+int aeo_terrain_destroy(aeo_terrain *t);
+/* _b_tf_divergence! with _tf_Ω (:1180-1205): out's interior, FP64 storage fields with filled halos */
+int aeo_terrain_divergence(aeo_terrain *t, double *out, const double *u, const double *v, const double *w);
+/* _laplacian!(out, f, ::TerrainFollowingGrid) for a cell-centred f (:1285-1343) */
+int aeo_terrain_laplacian(aeo_terrain *t, double *out, const double *f);
+/* out = J·∇²x on interior-sized FP64 vectors (the symmetric operator of the terrain Poisson problem) */
+int aeo_terrain_apply(aeo_terrain *t, double *out, const double *x);
+/* The terrain Poisson solve J∇²φ = b (no reference counterpart: the
+ * reference has no solver for this operator): flexible CG on the full
+ * metric operator, preconditioned by one V-cycle of `base`, an
+ * MG-preconditioned CG on its diagonal-metric 7-point part (face
+ * coefficients J_x, J_y, J·G^σσ). Stats as for MGPreconditionedCG. */
+int aeo_poisson_terrain_create(aeo_terrain *t, aeo_poisson *base, double tol, int maxit, aeo_poisson **h);
+
 /* ---- the learned preconditioner ------------------------------------------
  * No reference counterpart: the reference has no network (its nearest hook
  * is src/ML.jl:40-56). A 3-level 3-D U-Net on an nx x ny x nz grid (each

@@ -1897,3 +1897,167 @@ void orc_weighted_divergence(double *out, const double *u, const double *v, cons
                 out[ID(i, j, k)] = scale * d;
             }
 }
+
+/* ---------------------------------------------------------------------------
+ * Terrain-following (Gal-Chen) metrics and operators, src/StretchedGrids.jl:
+ * terrain_following_grid's metric arrays (:777-835), _tf_Ω and
+ * _b_tf_divergence! (:1180-1205), the Laplace–Beltrami operator
+ * _b_tf_lap_fx!/_fy!/_fs!/_div! (:1285-1327) for cell-centred fields.
+ * Storage arrays 1-based in the loops (ID), the column vectors dzc, dzf,
+ * sigc, sigf 0-based arrays of the 1-based storage level.
+ * hmat: terrain on the (sx, sy) storage plane; Hl the model lid.
+ * ------------------------------------------------------------------------ */
+typedef struct {
+    int sx, sy, sz;
+    double *J, *Jx, *Jy, *Jzf, *sxf, *syf, *sxz, *syz; /* dσdx_xf, dσdy_yf, dσdx_zf, dσdy_zf */
+} orc_tf;
+
+void *orc_tf_create(const orc_grid *g, const double *hmat, double Hl, const double *sigc, const double *sigf) {
+    GRID_DIMS(g);
+    orc_tf *t = (orc_tf *)calloc(1, sizeof(orc_tf));
+    t->sx = sx; t->sy = sy; t->sz = sz;
+    size_t n = (size_t)sx * sy * sz;
+    double **arr[] = {&t->J, &t->Jx, &t->Jy, &t->Jzf, &t->sxf, &t->syf, &t->sxz, &t->syz};
+    for (int a = 0; a < 8; a++) *arr[a] = (double *)calloc(n, sizeof(double));
+    double *hx = (double *)malloc(sizeof(double) * sx * sy), *hy = (double *)malloc(sizeof(double) * sx * sy);
+#define HM(i, j) hmat[(size_t)((i) - 1) + (size_t)sx * ((j) - 1)]
+    /* _slopes: centred, one-sided at the outermost ghost ring */
+    for (int j = 1; j <= sy; j++)
+        for (int i = 1; i <= sx; i++) {
+            double a, b;
+            if (i == 1) a = (HM(2, j) - HM(1, j)) / g->dx;
+            else if (i == sx) a = (HM(sx, j) - HM(sx - 1, j)) / g->dx;
+            else a = (HM(i + 1, j) - HM(i - 1, j)) / (2 * g->dx);
+            if (j == 1) b = (HM(i, 2) - HM(i, 1)) / g->dy;
+            else if (j == sy) b = (HM(i, sy) - HM(i, sy - 1)) / g->dy;
+            else b = (HM(i, j + 1) - HM(i, j - 1)) / (2 * g->dy);
+            hx[(size_t)(i - 1) + (size_t)sx * (j - 1)] = a;
+            hy[(size_t)(i - 1) + (size_t)sx * (j - 1)] = b;
+        }
+    for (int j = 1; j <= sy; j++)
+        for (int i = 1; i <= sx; i++) {
+            double hij = HM(i, j);
+            double Jij = (Hl - hij) / Hl;
+            int im = i - 1 < 1 ? 1 : i - 1, jm = j - 1 < 1 ? 1 : j - 1;
+            double h_xf = (HM(i, j) + HM(im, j)) / 2, h_yf = (HM(i, j) + HM(i, jm)) / 2;
+            double hx_xf = (HM(i, j) - HM(im, j)) / g->dx, hy_yf = (HM(i, j) - HM(i, jm)) / g->dy;
+            double Jx = (Hl - h_xf) / Hl, Jy = (Hl - h_yf) / Hl;
+            double hxc = hx[(size_t)(i - 1) + (size_t)sx * (j - 1)], hyc = hy[(size_t)(i - 1) + (size_t)sx * (j - 1)];
+            for (int k = 1; k <= sz; k++) {
+                size_t c = ID(i, j, k);
+                t->J[c] = Jij; t->Jx[c] = Jx; t->Jy[c] = Jy; t->Jzf[c] = Jij;
+                t->sxf[c] = -(Hl - sigc[k - 1]) * hx_xf / (Hl - h_xf);
+                t->syf[c] = -(Hl - sigc[k - 1]) * hy_yf / (Hl - h_yf);
+                t->sxz[c] = -(Hl - sigf[k - 1]) * hxc / (Hl - hij);
+                t->syz[c] = -(Hl - sigf[k - 1]) * hyc / (Hl - hij);
+            }
+        }
+#undef HM
+    free(hx); free(hy);
+    return t;
+}
+void orc_tf_destroy(void *h) {
+    orc_tf *t = (orc_tf *)h;
+    if (!t) return;
+    free(t->J); free(t->Jx); free(t->Jy); free(t->Jzf); free(t->sxf); free(t->syf); free(t->sxz); free(t->syz);
+    free(t);
+}
+/* the metric arrays (for the tests): which 0..7 = J, Jx, Jy, Jzf, dσdx_xf, dσdy_yf, dσdx_zf, dσdy_zf */
+const double *orc_tf_metric(void *h, int which) {
+    orc_tf *t = (orc_tf *)h;
+    const double *a[] = {t->J, t->Jx, t->Jy, t->Jzf, t->sxf, t->syf, t->sxz, t->syz};
+    return a[which];
+}
+
+static double tf_omega(const orc_tf *t, int sx, int sy, const double *u, const double *v, const double *w,
+                       const double *dzc, int i, int j, int k) {
+    double wlo = dzc[k - 1] / (dzc[k - 2] + dzc[k - 1]);
+    double whi = dzc[k - 2] / (dzc[k - 2] + dzc[k - 1]);
+    double ucm = (u[ID(i, j, k - 1)] + u[ID(i + 1, j, k - 1)]) / 2;
+    double uck = (u[ID(i, j, k)] + u[ID(i + 1, j, k)]) / 2;
+    double vcm = (v[ID(i, j, k - 1)] + v[ID(i, j + 1, k - 1)]) / 2;
+    double vck = (v[ID(i, j, k)] + v[ID(i, j + 1, k)]) / 2;
+    double ub = wlo * ucm + whi * uck;
+    double vb = wlo * vcm + whi * vck;
+    size_t c = ID(i, j, k);
+    return w[c] + t->Jzf[c] * (t->sxz[c] * ub + t->syz[c] * vb);
+}
+
+void orc_tf_divergence(void *h, const orc_grid *g, double *out, const double *u, const double *v, const double *w,
+                       const double *dzc) {
+    orc_tf *t = (orc_tf *)h;
+    GRID_DIMS(g);
+    int H = g->H;
+    double idx = 1 / g->dx, idy = 1 / g->dy;
+    for (int k = H + 1; k <= H + g->Nz; k++)
+        for (int j = H + 1; j <= H + g->Ny; j++)
+            for (int i = H + 1; i <= H + g->Nx; i++) {
+                double Wb = tf_omega(t, sx, sy, u, v, w, dzc, i, j, k);
+                double Wt = tf_omega(t, sx, sy, u, v, w, dzc, i, j, k + 1);
+                double hxf = (t->Jx[ID(i + 1, j, k)] * u[ID(i + 1, j, k)] - t->Jx[ID(i, j, k)] * u[ID(i, j, k)]) * idx;
+                double hyf = (t->Jy[ID(i, j + 1, k)] * v[ID(i, j + 1, k)] - t->Jy[ID(i, j, k)] * v[ID(i, j, k)]) * idy;
+                double vzf = (Wt - Wb) / dzc[k - 1];
+                out[ID(i, j, k)] = ((hxf + hyf) + vzf) / t->J[ID(i, j, k)];
+            }
+}
+
+static void dz_weights(double hm, double hp, double *wm, double *wc, double *wp) {
+    double s = hm + hp;
+    *wm = -hp / (hm * s);
+    *wc = (hp - hm) / (hm * hp);
+    *wp = hm / (hp * s);
+}
+static double dsig_c(int sx, int sy, const double *d, int i, int j, int k, const double *dzf) {
+    double wm, wc, wp;
+    dz_weights(dzf[k - 1], dzf[k], &wm, &wc, &wp);
+    return (wm * d[ID(i, j, k - 1)] + wc * d[ID(i, j, k)]) + wp * d[ID(i, j, k + 1)];
+}
+
+/* the cell-centred Laplace–Beltrami operator (its three face fluxes, then their divergence) */
+void orc_tf_laplacian(void *h, const orc_grid *g, double *out, const double *d, const double *dzc, const double *dzf) {
+    orc_tf *t = (orc_tf *)h;
+    GRID_DIMS(g);
+    int H = g->H;
+    size_t n = (size_t)sx * sy * sz;
+    double *Fx = (double *)calloc(n, sizeof(double)), *Fy = (double *)calloc(n, sizeof(double)),
+           *Fs = (double *)calloc(n, sizeof(double));
+    double dx = g->dx, dy = g->dy;
+    int i1 = H + 1, i2 = H + g->Nx, j1 = H + 1, j2 = H + g->Ny, k1 = H + 1, k2 = H + g->Nz;
+    for (int k = k1; k <= k2; k++)
+        for (int j = j1; j <= j2; j++)
+            for (int i = i1; i <= i2 + 1; i++) {
+                double dfds = (dsig_c(sx, sy, d, i - 1, j, k, dzf) + dsig_c(sx, sy, d, i, j, k, dzf)) / 2;
+                size_t c = ID(i, j, k);
+                Fx[c] = t->Jx[c] * ((d[c] - d[ID(i - 1, j, k)]) / dx + t->sxf[c] * dfds);
+            }
+    for (int k = k1; k <= k2; k++)
+        for (int j = j1; j <= j2 + 1; j++)
+            for (int i = i1; i <= i2; i++) {
+                double dfds = (dsig_c(sx, sy, d, i, j - 1, k, dzf) + dsig_c(sx, sy, d, i, j, k, dzf)) / 2;
+                size_t c = ID(i, j, k);
+                Fy[c] = t->Jy[c] * ((d[c] - d[ID(i, j - 1, k)]) / dy + t->syf[c] * dfds);
+            }
+    for (int k = k1; k <= k2 + 1; k++)
+        for (int j = j1; j <= j2; j++)
+            for (int i = i1; i <= i2; i++) {
+                double wlo = dzc[k - 1] / (dzc[k - 2] + dzc[k - 1]);
+                double whi = dzc[k - 2] / (dzc[k - 2] + dzc[k - 1]);
+                double dfdx = wlo * (d[ID(i + 1, j, k - 1)] - d[ID(i - 1, j, k - 1)]) / (2 * dx) +
+                              whi * (d[ID(i + 1, j, k)] - d[ID(i - 1, j, k)]) / (2 * dx);
+                double dfdy = wlo * (d[ID(i, j + 1, k - 1)] - d[ID(i, j - 1, k - 1)]) / (2 * dy) +
+                              whi * (d[ID(i, j + 1, k)] - d[ID(i, j - 1, k)]) / (2 * dy);
+                double dfds = (d[ID(i, j, k)] - d[ID(i, j, k - 1)]) / dzf[k - 1];
+                size_t c = ID(i, j, k);
+                double sxx = t->sxz[c], syy = t->syz[c], Jf = t->Jzf[c];
+                double Gss = (sxx * sxx + syy * syy) + 1 / (Jf * Jf);
+                Fs[c] = Jf * ((sxx * dfdx + syy * dfdy) + Gss * dfds);
+            }
+    for (int k = k1; k <= k2; k++)
+        for (int j = j1; j <= j2; j++)
+            for (int i = i1; i <= i2; i++) {
+                size_t c = ID(i, j, k);
+                out[c] = (((Fx[ID(i + 1, j, k)] - Fx[c]) / dx + (Fy[ID(i, j + 1, k)] - Fy[c]) / dy) +
+                          (Fs[ID(i, j, k + 1)] - Fs[c]) / dzc[k - 1]) / t->J[c];
+            }
+    free(Fx); free(Fy); free(Fs);
+}

@@ -89,6 +89,11 @@ def lib():
             "orc_mgpcg_solve": (i, [vp, pd, pd, d, i, P(i), pd, pd]),
             "orc_mg_apply": (None, [vp, pd, pd]),
             "orc_cutcell_heightmap": (None, [pg, pd, i, d, pd, pd, pd, pd, pd]),
+            "orc_tf_create": (vp, [pg, pd, d, pd, pd]),
+            "orc_tf_destroy": (None, [vp]),
+            "orc_tf_metric": (pd, [vp, i]),
+            "orc_tf_divergence": (None, [vp, pg, pd, pd, pd, pd, pd]),
+            "orc_tf_laplacian": (None, [vp, pg, pd, pd, pd, pd]),
             "orc_mw_step": (None, [vp, pd, pd, pd, pd, pd, d, pd, pd, pd, d, i]),
             "orc_weighted_divergence": (None, [pd, pd, pd, pd, pg, pd, pd, i]),
             "orc_unet_num_params": (ctypes.c_longlong, [i]),
@@ -379,3 +384,37 @@ def weighted_divergence(g, u, v, w, Wc, Wf, normalize=False):
     lib().orc_weighted_divergence(ptr(out), ptr(u), ptr(v), ptr(w), ctypes.byref(g.c),
                                   ptr(np.ascontiguousarray(Wc)), ptr(np.ascontiguousarray(Wf)), int(normalize))
     return out
+
+
+class Terrain:
+    """terrain_following_grid's metrics and the Gal-Chen operators (unet-free CPU restatement)."""
+
+    def __init__(self, g, h_plane, lid, sigc, sigf):
+        self.g = g
+        self.keep = [np.asfortranarray(h_plane, dtype=np.float64), np.ascontiguousarray(sigc, dtype=np.float64),
+                     np.ascontiguousarray(sigf, dtype=np.float64)]
+        self.h = lib().orc_tf_create(ctypes.byref(g.c), ptr(self.keep[0]), lid, ptr(self.keep[1]), ptr(self.keep[2]))
+
+    def metric(self, which):
+        n = int(np.prod(self.g.storage))
+        a = np.ctypeslib.as_array(lib().orc_tf_metric(self.h, which), shape=(n,))
+        return a.reshape(self.g.storage, order="F").copy()
+
+    def divergence(self, u, v, w, dzc):
+        out = self.g.zeros()
+        lib().orc_tf_divergence(self.h, ctypes.byref(self.g.c), ptr(out), ptr(u), ptr(v), ptr(w),
+                                ptr(np.ascontiguousarray(dzc, dtype=np.float64)))
+        return out
+
+    def laplacian(self, f, dzc, dzf):
+        out = self.g.zeros()
+        lib().orc_tf_laplacian(self.h, ctypes.byref(self.g.c), ptr(out), ptr(f),
+                               ptr(np.ascontiguousarray(dzc, dtype=np.float64)),
+                               ptr(np.ascontiguousarray(dzf, dtype=np.float64)))
+        return out
+
+    def __del__(self):
+        try:
+            lib().orc_tf_destroy(self.h)
+        except Exception:
+            pass

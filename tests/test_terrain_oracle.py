@@ -1,0 +1,66 @@
+"""The terrain-following operator oracle (CPU): over flat terrain it reduces
+to the regular operators (src/StretchedGrids.jl:642-646 states identity to
+round-off), its metrics are the Gal-Chen closed forms, and constants are in
+the Laplacian's null space."""
+import numpy as np
+
+from helpers import rand_storage, rel_err
+
+
+def _grid(orc, N=(12, 10, 8), H=2, dz=50.0):
+    g = orc.Grid(N, (N[0] * 100.0, N[1] * 80.0, N[2] * dz), halo=H)
+    k = np.arange(N[2] + 2 * H) - H
+    sigf = k * dz
+    sigc = (k + 0.5) * dz
+    return g, sigc, sigf, np.full(N[2] + 2 * H, dz), np.full(N[2] + 2 * H, dz)
+
+
+def _hill_plane(g, A=120.0):
+    H = g.H
+    sx, sy = g.storage[0], g.storage[1]
+    x = (np.arange(sx) - H + 0.5) * g.dx
+    y = (np.arange(sy) - H + 0.5) * g.dy
+    X, Y = np.meshgrid(x, y, indexing="ij")
+    return A * np.exp(-((X - 600) ** 2 + (Y - 400) ** 2) / (2 * 250.0 ** 2))
+
+
+def test_flat_terrain_reduces_to_regular_operators(orc):
+    g, sigc, sigf, dzc, dzf = _grid(orc)
+    t = orc.Terrain(g, np.zeros(g.storage[:2]), g.Nz * g.dz, sigc, sigf)
+    f = rand_storage(g.storage, 3)
+    lap = t.laplacian(f, dzc, dzf)
+    ref = g.zeros()
+    orc.lib().orc_laplacian(orc.ptr(ref), orc.ptr(f), __import__("ctypes").byref(g.c), orc.CC)
+    I = g.interior
+    assert rel_err(lap[I], ref[I]) < 1e-12
+    u, v, w = (rand_storage(g.storage, s) for s in (4, 5, 6))
+    div = t.divergence(u, v, w, dzc)
+    d_ref = orc.divergence_interior(g, u, v, w)
+    assert rel_err(div[I], d_ref) < 1e-12
+
+
+def test_metrics_are_gal_chen_closed_forms(orc):
+    g, sigc, sigf, dzc, dzf = _grid(orc)
+    lid = g.Nz * g.dz
+    hp = _hill_plane(g)
+    t = orc.Terrain(g, hp, lid, sigc, sigf)
+    J = t.metric(0)
+    assert rel_err(J[:, :, 0], (lid - hp) / lid) < 1e-15
+    # σ_x at σ-faces: −(H − σ_f) h_x/(H − h), h_x the centred slope
+    sxz = t.metric(6)
+    hx = np.zeros_like(hp)
+    hx[1:-1] = (hp[2:] - hp[:-2]) / (2 * g.dx)
+    k = 5
+    ref = -(lid - sigf[k]) * hx / (lid - hp)
+    assert rel_err(sxz[1:-1, :, k], ref[1:-1]) < 1e-14
+    # σ_x vanishes at the lid
+    ktop = np.argmin(np.abs(sigf - lid))
+    assert np.abs(sxz[:, :, ktop]).max() < 1e-12 * np.abs(sxz).max()
+
+
+def test_constants_in_laplacian_null_space(orc):
+    g, sigc, sigf, dzc, dzf = _grid(orc)
+    t = orc.Terrain(g, _hill_plane(g), g.Nz * g.dz, sigc, sigf)
+    f = np.full(g.storage, 3.7, order="F")
+    lap = t.laplacian(f, dzc, dzf)
+    assert np.abs(lap[g.interior]).max() < 1e-12
