@@ -101,6 +101,7 @@ SIGNATURES = {
     "aeo_mw_stepper_create": [_vp, ctypes.POINTER(MWParams), _pd, _pd, _pd, _vp, _pvp],
     "aeo_mw_stepper_destroy": [_vp],
     "aeo_mw_step": [_vp, _vp, _vp, _vp, _vp, _vp, _d],
+    "aeo_mw_step_rk3": [_vp, _vp, _vp, _vp, _vp, _vp, _d],
     "aeo_weighted_divergence": [_vp, _i, _vp, _vp, _vp, _vp, _vp, _vp, _i],
     "aeo_cutcell_divergence": [_vp, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
     "aeo_cutcell_gradient": [_vp, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp],

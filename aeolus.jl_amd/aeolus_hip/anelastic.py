@@ -153,6 +153,11 @@ class AnelasticSimulation:
         L.check(L.init().aeo_mw_step(ctypes.c_void_p(self._h), L.dptr(self.u), L.dptr(self.v), L.dptr(self.w),
                                      L.dptr(self.theta), L.dptr(self.p), float(dt)))
 
+    def step_rk3(self, dt):
+        """step_rk3! of the sound-proof core (src/Anelastic.jl:1905-1945)."""
+        L.check(L.init().aeo_mw_step_rk3(ctypes.c_void_p(self._h), L.dptr(self.u), L.dptr(self.v), L.dptr(self.w),
+                                         L.dptr(self.theta), L.dptr(self.p), float(dt)))
+
     def mass_flux_divergence(self):
         """∇·(ρ̄u) over the interior (mass_flux_divergence!, src/Anelastic.jl:1118-1123)."""
         g = self.grid

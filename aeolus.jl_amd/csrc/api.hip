@@ -1070,8 +1070,15 @@ struct aeo_mw_stepper {
     aeo_poisson *solver;
     size_t n, esz;
     void *us, *vs, *ws, *tn;
+    void *snap[4] = {};  // the step-start state of step_rk3 (allocated on first use)
     double *Wc = nullptr, *Wf = nullptr, *thf = nullptr;  // device, Nz + 2H
 };
+
+template <typename T>
+__global__ void combine_kernel(T *__restrict__ f, const T *__restrict__ a, T ca, T cb, size_t n) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        f[i] = ca * a[i] + cb * f[i];
+}
 
 extern "C" int aeo_mw_stepper_create(aeo_grid *g, const aeo_mw_params *sp, const double *Wc, const double *Wf,
                                      const double *thetabar_f, aeo_poisson *solver, aeo_mw_stepper **out) {
@@ -1117,6 +1124,8 @@ extern "C" int aeo_mw_stepper_destroy(aeo_mw_stepper *h) {
     if (!h) return AEO_OK;
     if (g_ctx) hipStreamSynchronize(g_ctx->stream);
     for (void *b : {h->us, h->vs, h->ws, h->tn}) hipFree(b);
+    for (void *b : h->snap)
+        if (b) hipFree(b);
     for (double *b : {h->Wc, h->Wf, h->thf}) hipFree(b);
     delete h;
     return AEO_OK;
@@ -1176,6 +1185,39 @@ extern "C" int aeo_mw_step(aeo_mw_stepper *h, void *u, void *v, void *w, void *t
     return h->sp.precision == AEO_F64
                ? mw_step<double>(h, (double *)u, (double *)v, (double *)w, (double *)theta, (double *)p, dt)
                : mw_step<float>(h, (float *)u, (float *)v, (float *)w, (float *)theta, (float *)p, dt);
+}
+
+// step_rk3! of the sound-proof core (src/Anelastic.jl:1905-1945): three
+// mass-weighted Euler steps with the whole storage combined in between
+template <typename T>
+static int mw_rk3(aeo_mw_stepper *h, T *u, T *v, T *w, T *th, T *p, double dt) {
+    hipStream_t st = g_ctx->stream;
+    for (auto &b : h->snap)
+        if (!b) HIPCHK(hipMalloc(&b, h->n * sizeof(T)));
+    T *f[4] = {u, v, w, th};
+    for (int q = 0; q < 4; q++) HIPCHK(hipMemcpyAsync(h->snap[q], f[q], h->n * sizeof(T), hipMemcpyDeviceToDevice, st));
+    int rc;
+    if ((rc = mw_step<T>(h, u, v, w, th, p, dt))) return rc;
+    if ((rc = mw_step<T>(h, u, v, w, th, p, dt))) return rc;
+    for (int q = 0; q < 4; q++)
+        hipLaunchKernelGGL(combine_kernel<T>, dim3(1024), dim3(256), 0, st, f[q], (const T *)h->snap[q], T(0.75),
+                           T(0.25), h->n);
+    if ((rc = mw_step<T>(h, u, v, w, th, p, dt))) return rc;
+    const double third = 1.0 / 3;
+    for (int q = 0; q < 4; q++)
+        hipLaunchKernelGGL(combine_kernel<T>, dim3(1024), dim3(256), 0, st, f[q], (const T *)h->snap[q], T(third),
+                           T(2 * third), h->n);
+    HIPCHK(hipGetLastError());
+    return AEO_OK;
+}
+
+extern "C" int aeo_mw_step_rk3(aeo_mw_stepper *h, void *u, void *v, void *w, void *theta, void *p, double dt) {
+    NEED_CTX();
+    if (!h || !u || !v || !w || !theta) return fail(AEO_ERR_ARG, "null argument");
+    if (!(dt > 0) || !std::isfinite(dt)) return fail(AEO_ERR_ARG, "dt must be positive and finite");
+    return h->sp.precision == AEO_F64
+               ? mw_rk3<double>(h, (double *)u, (double *)v, (double *)w, (double *)theta, (double *)p, dt)
+               : mw_rk3<float>(h, (float *)u, (float *)v, (float *)w, (float *)theta, (float *)p, dt);
 }
 
 extern "C" int aeo_weighted_divergence(aeo_grid *g, int precision, void *out, const void *u, const void *v,

@@ -190,6 +190,10 @@ int aeo_mw_stepper_create(aeo_grid *g, const aeo_mw_params *p, const double *Wc,
                           const double *thetabar_f, aeo_poisson *solver, aeo_mw_stepper **h);
 int aeo_mw_stepper_destroy(aeo_mw_stepper *h);
 int aeo_mw_step(aeo_mw_stepper *h, void *u, void *v, void *w, void *theta, void *p, double dt);
+/* step_rk3! of the sound-proof core (src/Anelastic.jl:1905-1945): three
+ * aeo_mw_step stages; the whole storage of u, v, w, θ combined as
+ * ¾·start + ¼·f after the second and ⅓·start + ⅔·f after the third */
+int aeo_mw_step_rk3(aeo_mw_stepper *h, void *u, void *v, void *w, void *theta, void *p, double dt);
 /* weighted_divergence!(out, u, v, w, Wc, Wf; normalize) (src/Anelastic.jl:1082-1110):
  * out's interior; Wc, Wf device arrays (Nz + 2H + 1 entries) */
 int aeo_weighted_divergence(aeo_grid *g, int precision, void *out, const void *u, const void *v, const void *w,

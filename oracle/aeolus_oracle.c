@@ -2061,3 +2061,25 @@ void orc_tf_laplacian(void *h, const orc_grid *g, double *out, const double *d, 
             }
     free(Fx); free(Fy); free(Fs);
 }
+
+/* step_rk3! of the sound-proof core (src/Anelastic.jl:1905-1945): three
+ * mass-weighted Euler steps, the whole storage combined as
+ * 0.75·start + 0.25·f, then (1/3)·start + (2/3)·f */
+void orc_mw_step_rk3(void *h, double *u, double *v, double *w, double *th, double *p, double dt, const double *Wc,
+                     const double *Wf, const double *thf, double g_accel, int buoyancy) {
+    orc_stepper *s = (orc_stepper *)h;
+    size_t n = s->n;
+    memcpy(s->u0, u, sizeof(double) * n);
+    memcpy(s->v0, v, sizeof(double) * n);
+    memcpy(s->w0, w, sizeof(double) * n);
+    memcpy(s->t0, th, sizeof(double) * n);
+    double *f[4] = {u, v, w, th}, *a[4] = {s->u0, s->v0, s->w0, s->t0};
+    orc_mw_step(h, u, v, w, th, p, dt, Wc, Wf, thf, g_accel, buoyancy);
+    orc_mw_step(h, u, v, w, th, p, dt, Wc, Wf, thf, g_accel, buoyancy);
+    for (int q = 0; q < 4; q++)
+        for (size_t c = 0; c < n; c++) f[q][c] = 0.75 * a[q][c] + 0.25 * f[q][c];
+    orc_mw_step(h, u, v, w, th, p, dt, Wc, Wf, thf, g_accel, buoyancy);
+    double third = 1.0 / 3, two3 = 2 * third;
+    for (int q = 0; q < 4; q++)
+        for (size_t c = 0; c < n; c++) f[q][c] = third * a[q][c] + two3 * f[q][c];
+}

@@ -95,6 +95,7 @@ def lib():
             "orc_tf_divergence": (None, [vp, pg, pd, pd, pd, pd, pd]),
             "orc_tf_laplacian": (None, [vp, pg, pd, pd, pd, pd]),
             "orc_mw_step": (None, [vp, pd, pd, pd, pd, pd, d, pd, pd, pd, d, i]),
+            "orc_mw_step_rk3": (None, [vp, pd, pd, pd, pd, pd, d, pd, pd, pd, d, i]),
             "orc_weighted_divergence": (None, [pd, pd, pd, pd, pg, pd, pd, i]),
             "orc_unet_num_params": (ctypes.c_longlong, [i]),
             "orc_unet_forward": (None, [P(ctypes.c_float), i, i, i, i, pd, pd, i, pd]),
@@ -371,6 +372,10 @@ class MWStepper:
     def step(self, u, v, w, th, p, dt):
         lib().orc_mw_step(self.h, ptr(u), ptr(v), ptr(w), ptr(th), ptr(p), dt, *[ptr(a) for a in self.prof],
                           self.g_accel, self.buoyancy)
+
+    def step_rk3(self, u, v, w, th, p, dt):
+        lib().orc_mw_step_rk3(self.h, ptr(u), ptr(v), ptr(w), ptr(th), ptr(p), dt, *[ptr(a) for a in self.prof],
+                              self.g_accel, self.buoyancy)
 
     def __del__(self):
         try:

@@ -270,8 +270,9 @@ def test_diagnostics_parity(hip, orc, stretched):
     assert abs(ke - ref) <= 1e-12 * ref
 
 
+@pytest.mark.parametrize("rk3", [False, True])
 @pytest.mark.parametrize("nu", [0.0, 1.0])
-def test_anelastic_step_parity_fp64(hip, orc, nu):
+def test_anelastic_step_parity_fp64(hip, orc, nu, rk3):
     # _mass_weighted_step! (anelastic weighting) against the oracle, two steps
     from aeolus_hip.anelastic import AnelasticSimulation, isothermal_reference
     N, H, dz = (16, 12, 20), 1, 100.0
@@ -293,8 +294,12 @@ def test_anelastic_step_parity_fp64(hip, orc, nu):
     ost = orc.MWStepper(og, ref.rho_c_pad, ref.rho_f_pad, ref.theta_f_pad, ref.rho, ref.rho_f, nu=nu, flux_bot=0.05)
     p = og.zeros()
     for _ in range(2):
-        sim.step(3.0)
-        ost.step(u, v, w, th, p, 3.0)
+        if rk3:
+            sim.step_rk3(3.0)
+            ost.step_rk3(u, v, w, th, p, 3.0)
+        else:
+            sim.step(3.0)
+            ost.step(u, v, w, th, p, 3.0)
     got = sim.state()
     for n, ref_a in (("u", u), ("v", v), ("w", w), ("theta", th), ("p", p)):
         assert rel_err(got[n][I], ref_a[I]) <= TOL64, n
