@@ -15,6 +15,38 @@ def _bc(b):
     return BC_NAMES[b] if isinstance(b, str) else b
 
 
+def damping_coordinates(grid):
+    """The heights of the Nz interior levels at cell centres (u, v) and at z
+    faces (w): damping_coordinates (src/FiniteVolume.jl:695, the RegularGrid
+    znodes of src/Grids.jl:156-170; src/StretchedGrids.jl:372-378, 602)."""
+    k = np.arange(1, grid.Nz + 1, dtype=np.float64)
+    if grid.zf is None:
+        return grid.z0 + (k - 0.5) * grid.dz, grid.z0 + (k - 1) * grid.dz
+    zf = np.asarray(grid.zf, dtype=np.float64)
+    return (zf[:-1] + zf[1:]) / 2, zf[:-1].copy()
+
+
+def sponge_levels(target, Nz):
+    """sponge_target(u, v, w) (src/Simulations.jl:319-352): a number or an
+    Nz-level vector per component, absent components zero; returned as
+    Nz-vectors (None for zero)."""
+    target = {} if target is None else dict(target)
+    for key in target:
+        if key not in ("u", "v", "w"):
+            raise ValueError(f"sponge_target: unknown component {key!r} (use u, v, w)")
+    out = []
+    for key in ("u", "v", "w"):
+        t = target.get(key, 0)
+        if np.isscalar(t):
+            out.append(None if t == 0 else np.full(Nz, float(t)))
+        else:
+            t = np.ascontiguousarray(t, dtype=np.float64)
+            if t.shape != (Nz,):
+                raise ValueError(f"damping target: a level vector needs Nz = {Nz} values, got {t.size}")
+            out.append(t)
+    return out
+
+
 class BoussinesqSimulation:
     """State u, v, w, theta, p on the device and one stepper.
 
@@ -26,7 +58,7 @@ class BoussinesqSimulation:
 
     def __init__(self, grid: Grid, scheme="centered", weno_p=2, nu=0.0, theta0=0.0, g_accel=9.81,
                  flux_bot=0.0, flux_top=0.0, bcs=("periodic",) * 3, theta_bcs=None, precision=L.F64,
-                 solver=None):
+                 solver=None, f_coriolis=0.0, sponge_alpha_max=0.0, sponge_z_start=None, sponge_target=None):
         self.grid = grid
         self.precision = precision
         self.bcs = tuple(bcs)
@@ -51,6 +83,17 @@ class BoussinesqSimulation:
         L.check(L.init().aeo_stepper_create(grid.handle, ctypes.byref(self.params), solver.handle, ctypes.byref(h)))
         self._h = h.value
         self.t = 0.0
+        if f_coriolis:
+            L.check(L.init().aeo_stepper_set_coriolis(ctypes.c_void_p(self._h), float(f_coriolis)))
+        z_top = grid.z0 + grid.Lz
+        z_start = z_top if sponge_z_start is None else float(sponge_z_start)
+        if sponge_alpha_max != 0 and z_start < z_top:
+            zc, zf = damping_coordinates(grid)
+            tg = sponge_levels(sponge_target, grid.Nz)
+            self._sponge = [zc, zf, *tg]
+            pd = lambda a: None if a is None else a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+            L.check(L.init().aeo_stepper_set_sponge(ctypes.c_void_p(self._h), float(sponge_alpha_max), z_start, z_top,
+                                                    *[pd(a) for a in self._sponge]))
 
     # ---- state ----
     def set(self, u=None, v=None, w=None, theta=None):

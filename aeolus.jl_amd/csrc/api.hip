@@ -906,6 +906,8 @@ struct aeo_stepper {
     void *us, *vs, *ws;  // predictor u*
     void *ua, *va, *wa;  // the stage state
     void *ta, *tb;       // θ stage buffers
+    double fcor = 0;     // f-plane Coriolis parameter
+    void *spg = nullptr; // sponge rate and target vectors (state precision), 5 × (Nz + 2H)
 };
 
 extern "C" int aeo_stepper_create(aeo_grid *g, const aeo_step_params *sp, aeo_poisson *solver, aeo_stepper **out) {
@@ -948,7 +950,64 @@ extern "C" int aeo_stepper_destroy(aeo_stepper *h) {
     if (g_ctx) hipStreamSynchronize(g_ctx->stream);
     void *bufs[8] = {h->us, h->vs, h->ws, h->ua, h->va, h->wa, h->ta, h->tb};
     for (auto b : bufs) hipFree(b);
+    if (h->spg) hipFree(h->spg);
     delete h;
+    return AEO_OK;
+}
+
+// f_coriolis of BoussinesqSimulation (src/Simulations.jl:269-283): the f-plane
+// term of _euler_projection! step 6 (:602-605)
+extern "C" int aeo_stepper_set_coriolis(aeo_stepper *h, double f) {
+    NEED_CTX();
+    if (!h) return fail(AEO_ERR_ARG, "null argument");
+    if (!std::isfinite(f)) return fail(AEO_ERR_ARG, "the Coriolis parameter must be finite");
+    h->fcor = f;
+    return AEO_OK;
+}
+
+// sponge_α_max, sponge_z_start, sponge_target of BoussinesqSimulation
+// (src/Simulations.jl:272-274, 334-352) and add_velocity_sponge!
+// (:375-381): the sin² rate of sponge_profile / damping_rate
+// (src/FiniteVolume.jl:524-566) sampled at the levels' heights, evaluated
+// here once with the reference's arithmetic.
+extern "C" int aeo_stepper_set_sponge(aeo_stepper *h, double alpha_max, double z_start, double z_top,
+                                      const double *zc, const double *zf, const double *target_u,
+                                      const double *target_v, const double *target_w) {
+    NEED_CTX();
+    if (!h) return fail(AEO_ERR_ARG, "null argument");
+    if (h->spg) {
+        HIPCHK(hipStreamSynchronize(g_ctx->stream));
+        hipFree(h->spg);
+        h->spg = nullptr;
+    }
+    // has_sponge = α_max ≠ 0 && z_start < z_top; otherwise the sponge is off
+    if (alpha_max == 0 || !(z_start < z_top)) return AEO_OK;
+    if (!zc || !zf) return fail(AEO_ERR_ARG, "the sponge needs the level heights zc and zf");
+    if (!std::isfinite(alpha_max) || !std::isfinite(z_start) || !std::isfinite(z_top))
+        return fail(AEO_ERR_ARG, "sponge parameters must be finite");
+    const int Nz = h->g->Nz, H = h->g->H, sz = Nz + 2 * H;
+    const double inv_rise = 1 / (z_top - z_start), half_pi = 0.5 * M_PI;
+    auto rate = [&](double xi) {
+        if (!(xi >= z_start)) return 0.0;
+        const double sn = std::sin(half_pi * (xi - z_start) * inv_rise);
+        return alpha_max * sn * sn;
+    };
+    std::vector<double> v(5 * (size_t)sz, 0.0);
+    for (int k = 0; k < Nz; k++) {
+        v[H + k] = rate(zc[k]);
+        v[sz + H + k] = rate(zf[k]);
+        v[2 * sz + H + k] = target_u ? target_u[k] : 0.0;
+        v[3 * sz + H + k] = target_v ? target_v[k] : 0.0;
+        v[4 * sz + H + k] = target_w ? target_w[k] : 0.0;
+    }
+    if (h->sp.precision == AEO_F64) {
+        HIPCHK(hipMalloc(&h->spg, v.size() * sizeof(double)));
+        HIPCHK(hipMemcpy(h->spg, v.data(), v.size() * sizeof(double), hipMemcpyHostToDevice));
+    } else {
+        std::vector<float> f(v.begin(), v.end());
+        HIPCHK(hipMalloc(&h->spg, f.size() * sizeof(float)));
+        HIPCHK(hipMemcpy(h->spg, f.data(), f.size() * sizeof(float), hipMemcpyHostToDevice));
+    }
     return AEO_OK;
 }
 
@@ -984,7 +1043,8 @@ static int substage(aeo_stepper *h, int stage, T *u, T *v, T *w, T *th, T *p, do
         launch_halo<T>(st, wi, Nx, Ny, Nz, H, LOC_FZ, sp.bc_x, sp.bc_y, sp.bc_z);
         launch_halo<T>(st, ti, Nx, Ny, Nz, H, LOC_CC, sp.tbc_x, sp.tbc_y, sp.tbc_z);
     }
-    StepPhys ph{sp.nu, dt, sp.theta0, sp.g_accel, sp.flux_bot, sp.flux_top, g->idzc[H], g->idzc[H + Nz - 1]};
+    StepPhys ph{sp.nu,       dt, sp.theta0, sp.g_accel, sp.flux_bot, sp.flux_top, g->idzc[H], g->idzc[H + Nz - 1],
+                h->fcor, h->spg};
     {
         Timed t("tendency");
         if (launch_tendency_step<T>(st, metrics<T>(g), sp.mom_scheme, sp.weno_p, ui, vi, wi, ti, us, vs, ws, to, th,

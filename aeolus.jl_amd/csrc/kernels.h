@@ -8,6 +8,11 @@ namespace aeo {
 struct StepPhys {
     double nu, dt, theta0, g_accel, flux_bot, flux_top;
     double idz_bot, idz_top;  // 1/Δz_c of the bottom and top interior levels
+    double fcor;              // f-plane Coriolis parameter (0: off)
+    // Rayleigh sponge: device vectors of the state's type, 5 × (Nz + 2H),
+    // storage-level index: rate at centres, rate at z faces, targets of u, v, w
+    // (nullptr: off)
+    const void *sponge;
 };
 
 // The RK3 stage combination c(x, snap) = a·snap + b·x of step_rk3!

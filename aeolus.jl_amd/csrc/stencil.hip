@@ -196,9 +196,10 @@ struct MarchArgs {
     // MODE_STEP: predictor u* (uo, vo, wo) and the advanced scalar so
     T *uo, *vo, *wo, *so;
     const T *s0;  // step-start scalar for the stage combination
-    T nu, dt, theta0, coef, fb, ft;
+    T nu, dt, theta0, coef, fb, ft, fcor;
     T ca, cb;
-    int combine, has_visc, has_buoy, has_sflux;
+    const T *spg;  // sponge vectors (rate_c, rate_f, target u, v, w), sz = Nz + 2H each
+    int combine, has_visc, has_buoy, has_sflux, has_cor;
     int kc;  // levels per chunk (<= KCM)
 };
 
@@ -329,11 +330,31 @@ __global__ void __launch_bounds__(MW *BY) adv_march_kernel(MarchArgs<T> a) {
                         const T tf = m.m_zface(k, ldo(a.in.s, co - b.z), ldo(a.in.s, co));
                         t += a.coef * (tf - a.theta0);
                     }
+                    // f-plane Coriolis, _b_coriolis! (src/FiniteVolume.jl:900-911)
+                    if (F == F_U && a.has_cor) {
+                        const T *vp = a.in.v;
+                        const T vu = T(0.25) * (((ldo(vp, co - ES) + ldo(vp, co)) + ldo(vp, co - ES + b.y)) +
+                                                ldo(vp, co + b.y));
+                        t += a.fcor * vu;
+                    }
+                    if (F == F_V && a.has_cor) {
+                        const T *up = a.in.u;
+                        const T uv = T(0.25) * (((ldo(up, co - b.y) + ldo(up, co + ES - b.y)) + ldo(up, co)) +
+                                                ldo(up, co + ES));
+                        t -= a.fcor * uv;
+                    }
                     if (F == F_S && a.has_sflux) {
                         if (k == H) t += a.fb;
                         if (k == H + m.Nz - 1) t -= a.ft;
                     }
                     const T x = ldo(fp, co);
+                    // Rayleigh sponge −λ(z)(f − f_target), _b_damping! (src/FiniteVolume.jl:639-647)
+                    if (F != F_S && a.spg) {
+                        const int sz = m.Nz + 2 * H;
+                        const T lam = a.spg[(F == F_W ? sz : 0) + k];
+                        const T tg = a.spg[(2 + F - F_U) * sz + k];
+                        t -= lam * (x - tg);
+                    }
                     if (F == F_U) sto(a.uo, co, x + a.dt * t);
                     else if (F == F_V) sto(a.vo, co, x + a.dt * t);
                     else if (F == F_W) sto(a.wo, co, x + a.dt * t);
@@ -411,6 +432,9 @@ int launch_tendency_step(hipStream_t st, const Metrics<T> &m, int scheme, int p,
     // F_bot · 1/Δz_c at the bottom level, F_top · 1/Δz_c at the top (src/FiniteVolume.jl:431-439)
     a.fb = T(ph.flux_bot * ph.idz_bot);
     a.ft = T(ph.flux_top * ph.idz_top);
+    a.fcor = T(ph.fcor);
+    a.has_cor = ph.fcor != 0;
+    a.spg = (const T *)ph.sponge;
     stage_coeffs(stage, &ph, &a.ca, &a.cb, &a.combine);
     a.kc = m.Nz < KCM ? m.Nz : KCM;
     return launch_march_mode<MODE_STEP, T>(st, a, scheme, p);

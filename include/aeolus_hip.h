@@ -166,6 +166,21 @@ int aeo_stepper_destroy(aeo_stepper *h);
 /* step_rk3!(sim, dt): u, v, w, theta, p are the state arrays (device, storage
  * layout), updated in place */
 int aeo_step_rk3(aeo_stepper *h, void *u, void *v, void *w, void *theta, void *p, double dt);
+/* f_coriolis of BoussinesqSimulation (src/Simulations.jl:269-283): the
+ * f-plane term tu += f·v̄, tv −= f·ū of _b_coriolis! (src/FiniteVolume.jl:900-911),
+ * added after buoyancy (_euler_projection! step 6, :602-605); 0 turns it off */
+int aeo_stepper_set_coriolis(aeo_stepper *h, double f);
+/* sponge_α_max, sponge_z_start, sponge_target of BoussinesqSimulation and
+ * add_velocity_sponge! (src/Simulations.jl:272-274, 334-381): the Rayleigh
+ * sponge −λ(z)(f − f_target) on u, v, w with λ = α_max sin²(½π(z − z_start)/(z_top − z_start))
+ * for z ≥ z_start (sponge_profile, damping_rate, src/FiniteVolume.jl:524-566).
+ * zc, zf: the Nz interior level heights of centres (u, v) and z faces (w),
+ * damping_coordinates (src/FiniteVolume.jl:695, src/StretchedGrids.jl:602);
+ * target_u/v/w: Nz-level vectors, NULL for zero. α_max = 0 or z_start ≥ z_top
+ * turns the sponge off. */
+int aeo_stepper_set_sponge(aeo_stepper *h, double alpha_max, double z_start, double z_top, const double *zc,
+                           const double *zf, const double *target_u, const double *target_v,
+                           const double *target_w);
 /* step!(sim, dt) (src/Simulations.jl:474-487): one projected forward-Euler step */
 int aeo_step_euler(aeo_stepper *h, void *u, void *v, void *w, void *theta, void *p, double dt);
 

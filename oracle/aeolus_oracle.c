@@ -917,7 +917,42 @@ typedef struct {
     orc_solver *solver;
     size_t n, nint;
     double *tu, *tv, *tw, *tt, *u0, *v0, *w0, *t0, *phi, *buf, *pbuf;
+    double fcor;                                 /* f_coriolis */
+    double sp_alpha, sp_start, sp_top;           /* sponge_α_max, sponge_z_start, z0 + Lz */
+    double *sp_zc, *sp_zf, *sp_tu, *sp_tv, *sp_tw; /* Nz each (NULL: no sponge) */
 } orc_stepper;
+
+/* f_coriolis and the velocity sponge of BoussinesqSimulation
+ * (src/Simulations.jl:269-283, 334-381); zc, zf: damping_coordinates of
+ * centres and z faces; targets NULL for zero */
+void orc_stepper_set_forcing(void *h, double fcor, double alpha_max, double z_start, double z_top, const double *zc,
+                             const double *zf, const double *tu, const double *tv, const double *tw) {
+    orc_stepper *s = (orc_stepper *)h;
+    int Nz = s->g.Nz;
+    s->fcor = fcor;
+    double **a[5] = {&s->sp_zc, &s->sp_zf, &s->sp_tu, &s->sp_tv, &s->sp_tw};
+    const double *src[5] = {zc, zf, tu, tv, tw};
+    for (int q = 0; q < 5; q++) {
+        free(*a[q]);
+        *a[q] = NULL;
+    }
+    s->sp_alpha = alpha_max;
+    s->sp_start = z_start;
+    s->sp_top = z_top;
+    if (!(alpha_max != 0 && z_start < z_top)) return;
+    for (int q = 0; q < 5; q++) {
+        *a[q] = (double *)calloc(Nz, sizeof(double));
+        if (src[q]) memcpy(*a[q], src[q], sizeof(double) * Nz);
+    }
+}
+
+/* damping_rate of the sin² sponge profile (src/FiniteVolume.jl:562-567) */
+static double sponge_rate(const orc_stepper *s, double xi) {
+    if (!(xi >= s->sp_start)) return 0.0;
+    double half_pi = 0.5 * M_PI;
+    double sn = sin(half_pi * (xi - s->sp_start) * (1 / (s->sp_top - s->sp_start)));
+    return s->sp_alpha * sn * sn;
+}
 
 void *orc_stepper_create(const orc_grid *g, const orc_step_params *sp, void *solver) {
     orc_stepper *s = (orc_stepper *)calloc(1, sizeof(orc_stepper));
@@ -939,6 +974,7 @@ void orc_stepper_destroy(void *h) {
     if (!s) return;
     free(s->tu); free(s->tv); free(s->tw); free(s->tt); free(s->u0); free(s->v0); free(s->w0);
     free(s->t0); free(s->phi); free(s->buf); free(s->pbuf);
+    free(s->sp_zc); free(s->sp_zf); free(s->sp_tu); free(s->sp_tv); free(s->sp_tw);
     free(s);
 }
 
@@ -998,6 +1034,13 @@ static void euler_projection(orc_stepper *s, double *u, double *v, double *w, do
                     double tf = m_zface(g, k, th[ID(i, j, k - 1)], th[ID(i, j, k)]);
                     s->tw[c] += coef * (tf - sp->theta0);
                 }
+                /* 6. f-plane Coriolis, _b_coriolis!, src/FiniteVolume.jl:900-911 */
+                if (s->fcor != 0) {
+                    double v_at_u = 0.25 * (((v[ID(i - 1, j, k)] + v[c]) + v[ID(i - 1, j + 1, k)]) + v[ID(i, j + 1, k)]);
+                    s->tu[c] += s->fcor * v_at_u;
+                    double u_at_v = 0.25 * (((u[ID(i, j - 1, k)] + u[ID(i + 1, j - 1, k)]) + u[c]) + u[ID(i + 1, j, k)]);
+                    s->tv[c] -= s->fcor * u_at_v;
+                }
             }
     /* 6a. surface flux, _b_surface_flux!, src/FiniteVolume.jl:417-439 */
     if (sp->flux_bot != 0 || sp->flux_top != 0) {
@@ -1008,6 +1051,25 @@ static void euler_projection(orc_stepper *s, double *u, double *v, double *w, do
                 s->tt[ID(i, j, kb)] += fb;
                 s->tt[ID(i, j, kt)] -= ft;
             }
+    }
+    /* 6b. Rayleigh sponge, add_velocity_sponge! / _b_damping!
+     * (src/Simulations.jl:375-381, src/FiniteVolume.jl:639-647) */
+    if (s->sp_zc) {
+#pragma omp parallel for schedule(static)
+        for (int k = H + 1; k <= H + g->Nz; k++) {
+            int kl = k - H - 1;
+            double lc = sponge_rate(s, s->sp_zc[kl]), lf = sponge_rate(s, s->sp_zf[kl]);
+            int inc = s->sp_zc[kl] >= s->sp_start, inf = s->sp_zf[kl] >= s->sp_start;
+            for (int j = H + 1; j <= H + g->Ny; j++)
+                for (int i = H + 1; i <= H + g->Nx; i++) {
+                    size_t c = ID(i, j, k);
+                    if (inc) {
+                        s->tu[c] -= lc * (u[c] - s->sp_tu[kl]);
+                        s->tv[c] -= lc * (v[c] - s->sp_tv[kl]);
+                    }
+                    if (inf) s->tw[c] -= lf * (w[c] - s->sp_tw[kl]);
+                }
+        }
     }
     /* 7. predictor u* = u + Δt F over the interior */
 #pragma omp parallel for schedule(static)

@@ -80,6 +80,7 @@ def lib():
             "orc_solver_destroy": (None, [vp]),
             "orc_stepper_create": (vp, [pg, P(OrcStep), vp]),
             "orc_stepper_destroy": (None, [vp]),
+            "orc_stepper_set_forcing": (None, [vp, d, d, d, d, pd, pd, pd, pd, pd]),
             "orc_step_rk3": (None, [vp, pd, pd, pd, pd, pd, d]),
             "orc_step_euler": (None, [vp, pd, pd, pd, pd, pd, d]),
             "orc_mg_create": (vp, [i, i, i, d, d, d, i, i, i, pd, pd, pd, pd, i, i, i, i, d]),
@@ -288,6 +289,13 @@ class Stepper:
         self.g = g
         self.sp = sp
         self.h = lib().orc_stepper_create(ctypes.byref(g.c), ctypes.byref(sp), solver.h)
+
+    def set_forcing(self, fcor=0.0, alpha_max=0.0, z_start=0.0, z_top=0.0, zc=None, zf=None, targets=(None,) * 3):
+        """f-plane Coriolis and the velocity sponge (zc, zf: the Nz level heights)."""
+        keep = [None if a is None else np.ascontiguousarray(a, dtype=np.float64) for a in (zc, zf, *targets)]
+        self._forcing = keep
+        lib().orc_stepper_set_forcing(self.h, fcor, alpha_max, z_start, z_top,
+                                      *[None if a is None else ptr(a) for a in keep])
 
     def rk3(self, u, v, w, th, p, dt):
         lib().orc_step_rk3(self.h, ptr(u), ptr(v), ptr(w), ptr(th), ptr(p), dt)

@@ -162,7 +162,7 @@ def test_density_weighted_column_parity(hip, orc):
     assert rel_err(hip.solve_interior(s, src), o.solve(src)) <= TOL64
 
 
-def _run_step(hip, orc, kind, scheme, steps, prec=0, N=None, euler=False):
+def _run_step(hip, orc, kind, scheme, steps, prec=0, N=None, euler=False, forcing=None):
     from aeolus_hip import workloads as W
     if kind == "tgv":
         N = N or 32
@@ -182,9 +182,19 @@ def _run_step(hip, orc, kind, scheme, steps, prec=0, N=None, euler=False):
             a[g.interior] += 0.5 * rng.uniform(-1, 1, size=a[g.interior].shape)
         phys = dict(W.DCBL_PHYS)
         dt = 0.5
-    sim = hip.BoussinesqSimulation(g, scheme=scheme, precision=prec, **phys)
+    fk = {}
+    if forcing:
+        # f-plane Coriolis and a sponge over the top 40 % toward (u, v) level targets
+        from aeolus_hip.simulation import damping_coordinates
+        tu = np.linspace(1.0, 3.0, g.Nz)
+        fk = dict(f_coriolis=forcing, sponge_alpha_max=0.02, sponge_z_start=0.6 * g.Lz,
+                  sponge_target=dict(u=tu, v=-0.5))
+    sim = hip.BoussinesqSimulation(g, scheme=scheme, precision=prec, **phys, **fk)
     sim.set(*state)
     ost = orc.Stepper(og, scheme=scheme, **{k: v for k, v in phys.items()})
+    if forcing:
+        zc, zf = damping_coordinates(g)
+        ost.set_forcing(forcing, 0.02, 0.6 * g.Lz, g.z0 + g.Lz, zc, zf, (tu, np.full(g.Nz, -0.5), None))
     u, v, w, th = (np.asfortranarray(a.copy()) for a in state)
     p = og.zeros()
     for _ in range(steps):
@@ -201,6 +211,14 @@ def _run_step(hip, orc, kind, scheme, steps, prec=0, N=None, euler=False):
                                          ("dcbl", "weno_z5"), ("dcbl", "centered")])
 def test_rk3_step_parity_fp64(hip, orc, kind, scheme):
     got, ref, og = _run_step(hip, orc, kind, scheme, steps=2)
+    for n in ("u", "v", "w", "theta", "p"):
+        assert rel_err(got[n], ref[n]) <= TOL64, n
+
+
+@pytest.mark.parametrize("kind", ["tgv", "dcbl"])
+def test_rk3_step_coriolis_sponge_parity_fp64(hip, orc, kind):
+    # f-plane Coriolis and the Rayleigh sponge toward level-vector targets
+    got, ref, og = _run_step(hip, orc, kind, "weno_z5", steps=2, forcing=0.3 if kind == "tgv" else 1e-3)
     for n in ("u", "v", "w", "theta", "p"):
         assert rel_err(got[n], ref[n]) <= TOL64, n
 
