@@ -13,11 +13,13 @@ from .fields import DeviceArray, Grid, faces_geometric, faces_uniform, fill_halo
 from . import learned, workloads
 from .pressure import (ColumnTridiagonalSolver, FFTPoissonSolver, MGPreconditionedCG, MultigridPoissonSolver, solve,
                        solve_interior, solve_source_buffer, solver_for, solver_stats, source_buffer, stretched_operator)
-from .simulation import BoussinesqSimulation
+from .simulation import BoussinesqSimulation, damping_coordinates, sponge_levels
+from .state_io import checkpoint, read_checkpoint, restart
 
 __all__ = [
     "AeolusError", "init", "sync", "advection_scheme", "advect_weno_z", "momentum_advect_weno_z", "required_halo",
     "DeviceArray", "Grid", "faces_geometric", "faces_uniform", "fill_halos", "ColumnTridiagonalSolver",
     "FFTPoissonSolver", "MGPreconditionedCG", "MultigridPoissonSolver", "stretched_operator", "solve", "solve_interior", "solve_source_buffer", "solver_for", "solver_stats",
-    "source_buffer", "BoussinesqSimulation",
+    "source_buffer", "BoussinesqSimulation", "damping_coordinates", "sponge_levels", "checkpoint",
+    "read_checkpoint", "restart",
 ]

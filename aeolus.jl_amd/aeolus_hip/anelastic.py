@@ -140,6 +140,14 @@ class AnelasticSimulation:
         # device copies of the padded weights for the diagnostic divergence (one spare entry)
         self._wc = DeviceArray.from_numpy(np.append(ref.rho_c_pad, ref.rho_c_pad[-1]))
         self._wf = DeviceArray.from_numpy(np.append(ref.rho_f_pad, ref.rho_f_pad[-1]))
+        self.t = 0.0
+        self._parameters = dict(nu=float(nu), g_accel=float(ref.g), flux_bot=float(flux_bot),
+                                flux_top=float(flux_top), buoyancy=bool(buoyancy), bcs=list(bcs),
+                                theta_bcs=list(theta_bcs))
+
+    def parameters(self):
+        """The run's scalar parameters, compared on restart."""
+        return dict(self._parameters)
 
     def set(self, u, v, w, theta):
         for d, a in ((self.u, u), (self.v, v), (self.w, w), (self.theta, theta)):
@@ -152,11 +160,13 @@ class AnelasticSimulation:
     def step(self, dt):
         L.check(L.init().aeo_mw_step(ctypes.c_void_p(self._h), L.dptr(self.u), L.dptr(self.v), L.dptr(self.w),
                                      L.dptr(self.theta), L.dptr(self.p), float(dt)))
+        self.t += dt
 
     def step_rk3(self, dt):
         """step_rk3! of the sound-proof core (src/Anelastic.jl:1905-1945)."""
         L.check(L.init().aeo_mw_step_rk3(ctypes.c_void_p(self._h), L.dptr(self.u), L.dptr(self.v), L.dptr(self.w),
                                          L.dptr(self.theta), L.dptr(self.p), float(dt)))
+        self.t += dt
 
     def mass_flux_divergence(self):
         """∇·(ρ̄u) over the interior (mass_flux_divergence!, src/Anelastic.jl:1118-1123)."""

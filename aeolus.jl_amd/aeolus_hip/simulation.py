@@ -83,6 +83,13 @@ class BoussinesqSimulation:
         L.check(L.init().aeo_stepper_create(grid.handle, ctypes.byref(self.params), solver.handle, ctypes.byref(h)))
         self._h = h.value
         self.t = 0.0
+        self._parameters = dict(scheme=scheme, weno_p=int(weno_p), nu=float(nu), theta0=float(theta0),
+                                g_accel=float(g_accel), flux_bot=float(flux_bot), flux_top=float(flux_top),
+                                f_coriolis=float(f_coriolis), sponge_alpha_max=float(sponge_alpha_max),
+                                sponge_z_start=None if sponge_z_start is None else float(sponge_z_start),
+                                sponge_target=None if sponge_target is None else
+                                {k: np.asarray(v, dtype=np.float64).tolist() for k, v in dict(sponge_target).items()},
+                                bcs=list(self.bcs), theta_bcs=list(self.theta_bcs))
         if f_coriolis:
             L.check(L.init().aeo_stepper_set_coriolis(ctypes.c_void_p(self._h), float(f_coriolis)))
         z_top = grid.z0 + grid.Lz
@@ -94,6 +101,10 @@ class BoussinesqSimulation:
             pd = lambda a: None if a is None else a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
             L.check(L.init().aeo_stepper_set_sponge(ctypes.c_void_p(self._h), float(sponge_alpha_max), z_start, z_top,
                                                     *[pd(a) for a in self._sponge]))
+
+    def parameters(self):
+        """The run's scalar parameters (_parameters, src/io/checkpoint.jl), compared on restart."""
+        return dict(self._parameters)
 
     # ---- state ----
     def set(self, u=None, v=None, w=None, theta=None):
