@@ -15,8 +15,10 @@
 //                   each face flux evaluated by both of its cells with the
 //                   same arithmetic
 //   tf_apply        J·∇² on interior-sized vectors with the boundary maps
-//                   (periodic x, y; mirror σ), the symmetric operator of the
-//                   terrain Poisson solve
+//                   (periodic x, y; mirror σ) and zero σ-flux through the
+//                   walls: the operator of the terrain Poisson solve (not
+//                   symmetric: the metric cross terms are evaluated at
+//                   different points, hence flexible CG)
 #include <hip/hip_runtime.h>
 
 #include "kernels.h"
@@ -129,10 +131,15 @@ __device__ __forceinline__ double tf_Fs(const FieldRef &f, const TFView &t, int 
     const double Gss = (sxx * sxx + syy * syy) + 1 / (Jf * Jf);
     return Jf * ((sxx * dfdx + syy * dfdy) + Gss * dfds);
 }
-__device__ __forceinline__ double tf_lap_at(const FieldRef &f, const TFView &t, int i, int j, int k) {
+// wall0: the σ-flux through the bottom and top walls is zero (the no-flux
+// Poisson operator: Σ Δσ_c·J∇²f then telescopes to zero, so the range is
+// orthogonal to the constants and a zero-mean source is compatible)
+__device__ __forceinline__ double tf_lap_at(const FieldRef &f, const TFView &t, int i, int j, int k, bool wall0) {
+    const double ft = (wall0 && k + 1 == t.H + t.Nz) ? 0.0 : tf_Fs(f, t, i, j, k + 1);
+    const double fb = (wall0 && k == t.H) ? 0.0 : tf_Fs(f, t, i, j, k);
     return (((tf_Fx(f, t, i + 1, j, k) - tf_Fx(f, t, i, j, k)) / t.dx +
              (tf_Fy(f, t, i, j + 1, k) - tf_Fy(f, t, i, j, k)) / t.dy) +
-            (tf_Fs(f, t, i, j, k + 1) - tf_Fs(f, t, i, j, k)) / t.dzc[k]) /
+            (ft - fb) / t.dzc[k]) /
            tf_J(t, i, j);
 }
 
@@ -142,15 +149,15 @@ __global__ void tf_lap_kernel(double *__restrict__ out, const double *__restrict
     if (i0 >= t.Nx) return;
     const int i = i0 + t.H, j = j0 + t.H, k = k0 + t.H;
     FieldRef F{f, 0};
-    out[(size_t)i + (size_t)t.sx * (j + (size_t)t.sy * k)] = tf_lap_at(F, t, i, j, k);
+    out[(size_t)i + (size_t)t.sx * (j + (size_t)t.sy * k)] = tf_lap_at(F, t, i, j, k, false);
 }
-// out (interior-sized) = J·∇²x (x interior-sized, boundary maps)
+// out (interior-sized) = J·∇²x (x interior-sized, boundary maps, no-flux walls)
 __global__ void tf_apply_kernel(double *__restrict__ out, const double *__restrict__ x, TFView t) {
     const int i0 = blockIdx.x * blockDim.x + threadIdx.x, j0 = blockIdx.y, k0 = blockIdx.z;
     if (i0 >= t.Nx) return;
     const int i = i0 + t.H, j = j0 + t.H, k = k0 + t.H;
     FieldRef F{x, 1};
-    out[(size_t)i0 + (size_t)t.Nx * (j0 + (size_t)t.Ny * k0)] = tf_J(t, i, j) * tf_lap_at(F, t, i, j, k);
+    out[(size_t)i0 + (size_t)t.Nx * (j0 + (size_t)t.Ny * k0)] = tf_J(t, i, j) * tf_lap_at(F, t, i, j, k, true);
 }
 
 void launch_tf(hipStream_t st, int op, const TFView &t, double *out, const double *a, const double *b,

@@ -2075,8 +2075,11 @@ static double dsig_c(int sx, int sy, const double *d, int i, int j, int k, const
     return (wm * d[ID(i, j, k - 1)] + wc * d[ID(i, j, k)]) + wp * d[ID(i, j, k + 1)];
 }
 
-/* the cell-centred Laplace–Beltrami operator (its three face fluxes, then their divergence) */
-void orc_tf_laplacian(void *h, const orc_grid *g, double *out, const double *d, const double *dzc, const double *dzf) {
+/* the cell-centred Laplace–Beltrami operator (its three face fluxes, then
+ * their divergence); wall0: the σ-flux of the bottom and top walls is zero
+ * (the no-flux form of the terrain Poisson operator) */
+void orc_tf_laplacian_ex(void *h, const orc_grid *g, double *out, const double *d, const double *dzc,
+                         const double *dzf, int wall0) {
     orc_tf *t = (orc_tf *)h;
     GRID_DIMS(g);
     int H = g->H;
@@ -2113,6 +2116,7 @@ void orc_tf_laplacian(void *h, const orc_grid *g, double *out, const double *d, 
                 double sxx = t->sxz[c], syy = t->syz[c], Jf = t->Jzf[c];
                 double Gss = (sxx * sxx + syy * syy) + 1 / (Jf * Jf);
                 Fs[c] = Jf * ((sxx * dfdx + syy * dfdy) + Gss * dfds);
+                if (wall0 && (k == k1 || k == k2 + 1)) Fs[c] = 0.0;
             }
     for (int k = k1; k <= k2; k++)
         for (int j = j1; j <= j2; j++)
@@ -2122,6 +2126,10 @@ void orc_tf_laplacian(void *h, const orc_grid *g, double *out, const double *d, 
                           (Fs[ID(i, j, k + 1)] - Fs[c]) / dzc[k - 1]) / t->J[c];
             }
     free(Fx); free(Fy); free(Fs);
+}
+
+void orc_tf_laplacian(void *h, const orc_grid *g, double *out, const double *d, const double *dzc, const double *dzf) {
+    orc_tf_laplacian_ex(h, g, out, d, dzc, dzf, 0);
 }
 
 /* step_rk3! of the sound-proof core (src/Anelastic.jl:1905-1945): three

@@ -95,6 +95,7 @@ def lib():
             "orc_tf_metric": (pd, [vp, i]),
             "orc_tf_divergence": (None, [vp, pg, pd, pd, pd, pd, pd]),
             "orc_tf_laplacian": (None, [vp, pg, pd, pd, pd, pd]),
+            "orc_tf_laplacian_ex": (None, [vp, pg, pd, pd, pd, pd, i]),
             "orc_mw_step": (None, [vp, pd, pd, pd, pd, pd, d, pd, pd, pd, d, i]),
             "orc_mw_step_rk3": (None, [vp, pd, pd, pd, pd, pd, d, pd, pd, pd, d, i]),
             "orc_weighted_divergence": (None, [pd, pd, pd, pd, pg, pd, pd, i]),
@@ -419,11 +420,12 @@ class Terrain:
                                 ptr(np.ascontiguousarray(dzc, dtype=np.float64)))
         return out
 
-    def laplacian(self, f, dzc, dzf):
+    def laplacian(self, f, dzc, dzf, wall0=False):
+        """∇²f (storage, halos filled); wall0: zero σ-flux through the walls."""
         out = self.g.zeros()
-        lib().orc_tf_laplacian(self.h, ctypes.byref(self.g.c), ptr(out), ptr(f),
-                               ptr(np.ascontiguousarray(dzc, dtype=np.float64)),
-                               ptr(np.ascontiguousarray(dzf, dtype=np.float64)))
+        lib().orc_tf_laplacian_ex(self.h, ctypes.byref(self.g.c), ptr(out), ptr(f),
+                                  ptr(np.ascontiguousarray(dzc, dtype=np.float64)),
+                                  ptr(np.ascontiguousarray(dzf, dtype=np.float64)), int(wall0))
         return out
 
     def __del__(self):

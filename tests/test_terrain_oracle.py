@@ -64,3 +64,33 @@ def test_constants_in_laplacian_null_space(orc):
     f = np.full(g.storage, 3.7, order="F")
     lap = t.laplacian(f, dzc, dzf)
     assert np.abs(lap[g.interior]).max() < 1e-12
+
+
+def test_poisson_form_is_compatible(orc):
+    # with zero σ-flux through the walls, Σ Δσ_c·J∇²f telescopes to zero for
+    # every f: the range is orthogonal to the constants (a zero-mean source is
+    # solvable); the reference's storage form (halo-mirrored wall flux) is not
+    from aeolus_hip.terrain import storage_plane, uniform_sigma
+    N, H, dz = (12, 10, 8), 2, 50.0
+    g = orc.Grid(N, (N[0] * 100.0, N[1] * 100.0, N[2] * dz), halo=H)
+    x = (np.arange(N[0]) + 0.5) * 100.0
+    y = (np.arange(N[1]) + 0.5) * 100.0
+    X, Y = np.meshgrid(x, y, indexing="ij")
+    h = 150.0 * np.exp(-((X - 600) ** 2 + (Y - 500) ** 2) / (2 * 250.0 ** 2))
+    sigc, sigf, dzc, dzf = uniform_sigma(N[2], dz, H)
+    t = orc.Terrain(g, storage_plane(h, N, H), N[2] * dz, sigc, sigf)
+    J = t.metric(0)[g.interior]
+    w = dzc[H:H + N[2]][None, None, :]
+    sums = []
+    for seed in range(3):
+        f = g.zeros()
+        f[g.interior] = np.random.default_rng(seed).standard_normal(N)
+        orc.fill_halos(g, f, orc.CC, ("periodic", "periodic", "noflux"))
+        a0 = J * t.laplacian(f, dzc, dzf, wall0=True)[g.interior]
+        a1 = J * t.laplacian(f, dzc, dzf)[g.interior]
+        scale = np.abs(a0).sum() * dz
+        assert abs((w * a0).sum()) < 1e-13 * scale
+        sums.append(abs((w * a1).sum()) / scale)
+        # away from the walls the two forms agree exactly
+        assert np.array_equal(a0[:, :, 1:-1], a1[:, :, 1:-1])
+    assert max(sums) > 1e-6
