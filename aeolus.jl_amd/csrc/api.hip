@@ -413,7 +413,7 @@ extern "C" int aeo_poisson_fft_create(int Nx, int Ny, int Nz, double dx, double 
     h->sp = spectral_create(Nx, Ny, Nz, g_ctx->stream, err, sizeof err);
     if (!h->sp) return fail(AEO_ERR_UNSUPPORTED, "%s", err);
     int rc = spectral_fft_setup(h->sp, dx, dy, dz);
-    if (rc == -2) { spectral_destroy(h->sp); return fail(AEO_ERR_UNSUPPORTED, "FFT solver: Nz (%d) must be a power of two <= 512", Nz); }
+    if (rc == -2) { spectral_destroy(h->sp); return fail(AEO_ERR_UNSUPPORTED, "FFT solver: Nz (%d) must be a product of 2, 3 and 5 in 2..512", Nz); }
     if (rc) { spectral_destroy(h->sp); return fail(AEO_ERR_HIP, "FFT solver setup failed"); }
     rc = poisson_alloc(h.get());
     if (rc) return rc;

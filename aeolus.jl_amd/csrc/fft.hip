@@ -24,11 +24,20 @@
 
 namespace aeo {
 
+// One transform length: L = 2^a 3^b 5^c, its radices in stage order (DIT:
+// first stage first) and the table tw[m] = exp(−2πi m/L), m < L.
+constexpr int FFT_MAXS = 16;
+struct FFTLen {
+    int L, ns;
+    int r[FFT_MAXS];
+    const double2 *tw;
+};
+
 struct SpectralPlan {
     int Nx, Ny, Nz, LD;
     double2 *hat = nullptr;
     double *cp = nullptr;  // Thomas scratch c'_k per column
-    double2 *tw[12] = {};  // tw[l][m] = exp(-2 pi i m / 2^l), m < 2^l
+    std::vector<std::pair<int, double2 *>> tables;  // twiddle tables by length
     // FFT solver: squared modified wavenumbers and 1/(Nx Ny Nz)
     double *kx2 = nullptr, *ky2 = nullptr, *kz2 = nullptr;
     double nrm = 0;
@@ -42,57 +51,155 @@ struct SpectralPlan {
     int nkx = 0;               // spectral columns along x: Nx/2+1 (periodic) or Nx (Neumann)
 };
 
-static int ilog2(int n) {
-    int l = 0;
-    while ((1 << l) < n) l++;
-    return (1 << l) == n ? l : -1;
+// the radices of L (4s first, then 2, 3, 5), or ns = 0 when L has another prime factor
+static FFTLen fft_factor(int L) {
+    FFTLen f{};
+    f.L = L;
+    int n = L;
+    while (n % 4 == 0 && f.ns < FFT_MAXS) { f.r[f.ns++] = 4; n /= 4; }
+    while (n % 2 == 0 && f.ns < FFT_MAXS) { f.r[f.ns++] = 2; n /= 2; }
+    while (n % 3 == 0 && f.ns < FFT_MAXS) { f.r[f.ns++] = 3; n /= 3; }
+    while (n % 5 == 0 && f.ns < FFT_MAXS) { f.r[f.ns++] = 5; n /= 5; }
+    if (n != 1 || L < 1) f.ns = 0;
+    if (L == 1) { f.ns = 0; }
+    return f;
 }
-
-__device__ __forceinline__ int bitrev(int m, int lg) { return (int)(__brev((unsigned)m) >> (32 - lg)); }
+static bool fft_ok(int L) { return L == 1 || fft_factor(L).ns > 0; }
 
 __device__ __forceinline__ double2 cmul(double2 a, double2 b) {
     return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
 
-// In-LDS radix-2 DIT FFT of `nb` interleaved sequences of length L stored
-// s[m * ld + b] in bit-reversed order; sign -1 forward, +1 inverse.
-__device__ void lds_fft(double2 *s, int L, int lg, int nb, int ld, const double2 *__restrict__ tw, int sign) {
-    const int nbf = (L >> 1) * nb;
-    for (int len = 2, st = L >> 1; len <= L; len <<= 1, st >>= 1) {
-        const int half = len >> 1;
-        for (int t = threadIdx.x; t < nbf; t += blockDim.x) {
-            const int b = t % nb, bf = t / nb;
-            const int q = bf & (half - 1), g = bf >> (__ffs(half) - 1);
-            const int i0 = g * len + q, i1 = i0 + half;
-            double2 w = tw[q * st];
-            if (sign > 0) w.y = -w.y;
-            double2 x0 = s[i0 * ld + b], x1 = s[i1 * ld + b];
-            double2 y = cmul(x1, w);
-            s[i0 * ld + b] = make_double2(x0.x + y.x, x0.y + y.y);
-            s[i1 * ld + b] = make_double2(x0.x - y.x, x0.y - y.y);
-        }
-        __syncthreads();
+// position of input element x in the digit-reversed order of the DIT stages
+// (the output position of element x after the DIF stages): with M_0 = 1,
+// M_{i+1} = M_i r_i, the digits of x taken least significant first in the
+// radices r_{s−1}, …, r_0 weigh M_{s−1}, …, M_0 (bit reversal when every r = 2)
+__device__ __forceinline__ int fft_pos(int x, const FFTLen &f) {
+    int p = 0, M = f.L;
+    for (int i = f.ns - 1; i >= 0; i--) {
+        const int r = f.r[i];
+        M /= r;
+        const int q = x / r;
+        p += (x - q * r) * M;
+        x = q;
     }
-    (void)lg;
+    return p;
+}
+
+// the radix-r DFT of a[0..r) in place; sign −1 forward, +1 inverse
+__device__ __forceinline__ void small_dft(double2 *a, int r, int sign) {
+    const double sg = (double)sign;
+    if (r == 2) {
+        const double2 a0 = a[0], a1 = a[1];
+        a[0] = make_double2(a0.x + a1.x, a0.y + a1.y);
+        a[1] = make_double2(a0.x - a1.x, a0.y - a1.y);
+    } else if (r == 4) {
+        const double2 s02 = make_double2(a[0].x + a[2].x, a[0].y + a[2].y);
+        const double2 d02 = make_double2(a[0].x - a[2].x, a[0].y - a[2].y);
+        const double2 s13 = make_double2(a[1].x + a[3].x, a[1].y + a[3].y);
+        const double2 d13 = make_double2(a[1].x - a[3].x, a[1].y - a[3].y);
+        // sign·i·d13
+        const double2 j13 = make_double2(-sg * d13.y, sg * d13.x);
+        a[0] = make_double2(s02.x + s13.x, s02.y + s13.y);
+        a[2] = make_double2(s02.x - s13.x, s02.y - s13.y);
+        a[1] = make_double2(d02.x + j13.x, d02.y + j13.y);
+        a[3] = make_double2(d02.x - j13.x, d02.y - j13.y);
+    } else if (r == 3) {
+        // y1,2 = a0 − (a1 + a2)/2 ± sign·i·(√3/2)(a1 − a2)
+        const double s3 = 0.86602540378443864676;
+        const double2 t = make_double2(a[1].x + a[2].x, a[1].y + a[2].y);
+        const double2 d = make_double2(a[1].x - a[2].x, a[1].y - a[2].y);
+        const double2 m = make_double2(a[0].x - 0.5 * t.x, a[0].y - 0.5 * t.y);
+        const double2 j = make_double2(-sg * s3 * d.y, sg * s3 * d.x);
+        a[0] = make_double2(a[0].x + t.x, a[0].y + t.y);
+        a[1] = make_double2(m.x + j.x, m.y + j.y);
+        a[2] = make_double2(m.x - j.x, m.y - j.y);
+    } else {  // r == 5
+        const double c1 = 0.30901699437494742410, c2 = -0.80901699437494742410;
+        const double s1 = 0.95105651629515357212, s2 = 0.58778525229247312917;
+        const double2 b1 = make_double2(a[1].x + a[4].x, a[1].y + a[4].y);
+        const double2 b2 = make_double2(a[2].x + a[3].x, a[2].y + a[3].y);
+        const double2 d1 = make_double2(a[1].x - a[4].x, a[1].y - a[4].y);
+        const double2 d2 = make_double2(a[2].x - a[3].x, a[2].y - a[3].y);
+        const double2 a0 = a[0];
+        const double2 m1 = make_double2(a0.x + c1 * b1.x + c2 * b2.x, a0.y + c1 * b1.y + c2 * b2.y);
+        const double2 m2 = make_double2(a0.x + c2 * b1.x + c1 * b2.x, a0.y + c2 * b1.y + c1 * b2.y);
+        // e1 = s1 d1 + s2 d2, e2 = s2 d1 − s1 d2; y = m ± sign·i·e
+        const double2 e1 = make_double2(s1 * d1.x + s2 * d2.x, s1 * d1.y + s2 * d2.y);
+        const double2 e2 = make_double2(s2 * d1.x - s1 * d2.x, s2 * d1.y - s1 * d2.y);
+        const double2 j1 = make_double2(-sg * e1.y, sg * e1.x), j2 = make_double2(-sg * e2.y, sg * e2.x);
+        a[0] = make_double2(a0.x + b1.x + b2.x, a0.y + b1.y + b2.y);
+        a[1] = make_double2(m1.x + j1.x, m1.y + j1.y);
+        a[4] = make_double2(m1.x - j1.x, m1.y - j1.y);
+        a[2] = make_double2(m2.x + j2.x, m2.y + j2.y);
+        a[3] = make_double2(m2.x - j2.x, m2.y - j2.y);
+    }
+}
+
+// One stage of `nb` interleaved sequences s[m * ld + b]: radix r over blocks of
+// m = M_i; dit: twiddle W_len^{qj} before the small DFT, else (DIF) after it
+__device__ __forceinline__ void fft_stage(double2 *s, const FFTLen &f, int i, int m, int nb, int ld, int sign,
+                                          bool dit) {
+    const int r = f.r[i], len = m * r, step = f.L / len;
+    const int nbf = (f.L / r) * nb;
+    for (int t = threadIdx.x; t < nbf; t += blockDim.x) {
+        const int b = t % nb, bf = t / nb;
+        const int g = bf / m, q = bf - g * m;
+        const int base = g * len + q;
+        double2 a[5];
+        for (int j = 0; j < r; j++) a[j] = s[(base + j * m) * ld + b];
+        if (dit)
+            for (int j = 1; j < r; j++) {
+                double2 w = f.tw[q * j * step];
+                if (sign > 0) w.y = -w.y;
+                a[j] = cmul(a[j], w);
+            }
+        small_dft(a, r, sign);
+        if (!dit)
+            for (int j = 1; j < r; j++) {
+                double2 w = f.tw[q * j * step];
+                if (sign > 0) w.y = -w.y;
+                a[j] = cmul(a[j], w);
+            }
+        for (int j = 0; j < r; j++) s[(base + j * m) * ld + b] = a[j];
+    }
+    __syncthreads();
+}
+
+// In-LDS mixed-radix FFT, DIT: input in fft_pos order, output natural
+__device__ void lds_fft(double2 *s, const FFTLen &f, int nb, int ld, int sign) {
+    int m = 1;
+    for (int i = 0; i < f.ns; i++) {
+        fft_stage(s, f, i, m, nb, ld, sign, true);
+        m *= f.r[i];
+    }
+}
+// DIF: input natural, output in fft_pos order
+__device__ void lds_fft_dif(double2 *s, const FFTLen &f, int nb, int ld, int sign) {
+    int m = f.L;
+    for (int i = f.ns - 1; i >= 0; i--) {
+        m /= f.r[i];
+        fft_stage(s, f, i, m, nb, ld, sign, false);
+    }
 }
 
 // ---- r2c along x: rows of N = 2L reals -> N/2+1 complex ---------------------
 // z[m] = x[2m] + i x[2m+1]; Z = DFT_L(z); X[k] = E[k] + W^k O[k] with
 // E = (Z[k] + conj Z[L-k]) / 2, O = (Z[k] - conj Z[L-k]) / 2i, W = e^{-2 pi i/N}.
-__global__ void r2c_rows_kernel(double2 *__restrict__ hat, const double *__restrict__ src, int L, int lg, int rpb,
-                                int nrows, int LD, const double2 *__restrict__ twL, const double2 *__restrict__ twN) {
+__global__ void r2c_rows_kernel(double2 *__restrict__ hat, const double *__restrict__ src, FFTLen f, int rpb,
+                                int nrows, int LD, const double2 *__restrict__ twN) {
     extern __shared__ double2 sm[];
     const int row0 = blockIdx.x * rpb;
-    const int N = 2 * L;
+    const int L = f.L, N = 2 * L;
     for (int t = threadIdx.x; t < rpb * L; t += blockDim.x) {
         int r = t / L, m = t - r * L;
         int row = row0 + r;
         double2 z = make_double2(0, 0);
         if (row < nrows) z = reinterpret_cast<const double2 *>(src + (size_t)row * N)[m];
-        sm[bitrev(m, lg) * rpb + r] = z;
+        sm[fft_pos(m, f) * rpb + r] = z;
     }
     __syncthreads();
-    lds_fft(sm, L, lg, rpb, rpb, twL, -1);
+    lds_fft(sm, f, rpb, rpb, -1);
     for (int t = threadIdx.x; t < rpb * (L + 1); t += blockDim.x) {
         int r = t / (L + 1), k = t - r * (L + 1);
         int row = row0 + r;
@@ -110,12 +217,11 @@ __global__ void r2c_rows_kernel(double2 *__restrict__ hat, const double *__restr
 }
 
 // ---- c2r along x: N/2+1 Hermitian-half complex -> N reals, times scale ------
-__global__ void c2r_rows_kernel(double *__restrict__ dst, const double2 *__restrict__ hat, int L, int lg, int rpb,
-                                int nrows, int LD, const double2 *__restrict__ twL, const double2 *__restrict__ twN,
-                                double scale) {
+__global__ void c2r_rows_kernel(double *__restrict__ dst, const double2 *__restrict__ hat, FFTLen f, int rpb,
+                                int nrows, int LD, const double2 *__restrict__ twN, double scale) {
     extern __shared__ double2 sm[];
     const int row0 = blockIdx.x * rpb;
-    const int N = 2 * L;
+    const int L = f.L, N = 2 * L;
     for (int t = threadIdx.x; t < rpb * L; t += blockDim.x) {
         int r = t / L, k = t - r * L;
         int row = row0 + r;
@@ -132,10 +238,10 @@ __global__ void c2r_rows_kernel(double *__restrict__ dst, const double2 *__restr
             double2 wd = cmul(w, d);
             z = make_double2(s.x - wd.y, s.y + wd.x);  // s + i·wd
         }
-        sm[bitrev(k, lg) * rpb + r] = z;
+        sm[fft_pos(k, f) * rpb + r] = z;
     }
     __syncthreads();
-    lds_fft(sm, L, lg, rpb, rpb, twL, +1);
+    lds_fft(sm, f, rpb, rpb, +1);
     for (int t = threadIdx.x; t < rpb * L; t += blockDim.x) {
         int r = t / L, m = t - r * L;
         int row = row0 + r;
@@ -156,19 +262,20 @@ struct EigArgs {
 };
 
 template <int TB>
-__global__ void c2c_cols_kernel(double2 *__restrict__ d, int L, int lg, long es, long os, int ncols, int sign, int mode,
-                                const double2 *__restrict__ tw, EigArgs ea) {
+__global__ void c2c_cols_kernel(double2 *__restrict__ d, FFTLen f, long es, long os, int ncols, int sign, int mode,
+                                EigArgs ea) {
     extern __shared__ double2 sm[];
+    const int L = f.L;
     const int g = blockIdx.x, o = blockIdx.y;
     const long base = (long)o * os + (long)g * TB;
     for (int t = threadIdx.x; t < L * TB; t += blockDim.x) {
         int m = t / TB, b = t - m * TB;
         double2 z = make_double2(0, 0);
         if (g * TB + b < ncols) z = d[base + (long)m * es + b];
-        sm[bitrev(m, lg) * TB + b] = z;
+        sm[fft_pos(m, f) * TB + b] = z;
     }
     __syncthreads();
-    lds_fft(sm, L, lg, TB, TB, tw, sign);
+    lds_fft(sm, f, TB, TB, sign);
     if (mode == 1) {
         // here the axis is z (m = kz), o = ky, column g*TB + b = kx
         for (int t = threadIdx.x; t < L * TB; t += blockDim.x) {
@@ -182,18 +289,13 @@ __global__ void c2c_cols_kernel(double2 *__restrict__ d, int L, int lg, long es,
             sm[m * TB + b] = z;
         }
         __syncthreads();
-        // back to bit-reversed order for the inverse: each pair swapped once
+        // the inverse by decimation in frequency: natural order in, fft_pos order out
+        lds_fft_dif(sm, f, TB, TB, +1);
         for (int t = threadIdx.x; t < L * TB; t += blockDim.x) {
             int m = t / TB, b = t - m * TB;
-            int r = bitrev(m, lg);
-            if (m < r) {
-                double2 x = sm[m * TB + b];
-                sm[m * TB + b] = sm[r * TB + b];
-                sm[r * TB + b] = x;
-            }
+            if (g * TB + b < ncols) d[base + (long)m * es + b] = sm[fft_pos(m, f) * TB + b];
         }
-        __syncthreads();
-        lds_fft(sm, L, lg, TB, TB, tw, +1);
+        return;
     }
     for (int t = threadIdx.x; t < L * TB; t += blockDim.x) {
         int m = t / TB, b = t - m * TB;
@@ -306,9 +408,9 @@ __device__ __forceinline__ int dct_pos(int m, int N) { return m < N / 2 ? 2 * m 
 
 // rows along x of N reals: forward src → hat (X_k, 0), k < N; inverse Re(hat) → dst · scale
 __global__ void dct_rows_kernel(double2 *__restrict__ hat, const double *__restrict__ src, double *__restrict__ dst,
-                                int N, int lg, int rpb, int nrows, int LD, const double2 *__restrict__ tw, int dir,
-                                double scale) {
+                                FFTLen f, int rpb, int nrows, int LD, int dir, double scale) {
     extern __shared__ double2 sm[];
+    const int N = f.L;
     const int row0 = blockIdx.x * rpb;
     for (int t = threadIdx.x; t < rpb * N; t += blockDim.x) {
         const int r = t / N, m = t - r * N, row = row0 + r;
@@ -324,10 +426,10 @@ __global__ void dct_rows_kernel(double2 *__restrict__ hat, const double *__restr
                 z = make_double2(xk * cs + xn * sn, xk * sn - xn * cs);
             }
         }
-        sm[bitrev(m, lg) * rpb + r] = z;
+        sm[fft_pos(m, f) * rpb + r] = z;
     }
     __syncthreads();
-    lds_fft(sm, N, lg, rpb, rpb, tw, dir > 0 ? -1 : +1);
+    lds_fft(sm, f, rpb, rpb, dir > 0 ? -1 : +1);
     for (int t = threadIdx.x; t < rpb * N; t += blockDim.x) {
         const int r = t / N, m = t - r * N, row = row0 + r;
         if (row >= nrows) continue;
@@ -345,9 +447,9 @@ __global__ void dct_rows_kernel(double2 *__restrict__ hat, const double *__restr
 // complex columns along a strided axis (element m at base + m*es + b), in
 // place: the transform of the real and the imaginary parts
 template <int TB>
-__global__ void dct_cols_kernel(double2 *__restrict__ d, int L, int lg, long es, long os, int ncols, int dir,
-                                const double2 *__restrict__ tw) {
+__global__ void dct_cols_kernel(double2 *__restrict__ d, FFTLen f, long es, long os, int ncols, int dir) {
     extern __shared__ double2 sm[];
+    const int L = f.L;
     const int g = blockIdx.x, o = blockIdx.y;
     const long base = (long)o * os + (long)g * TB;
     for (int t = threadIdx.x; t < L * TB; t += blockDim.x) {
@@ -365,17 +467,17 @@ __global__ void dct_cols_kernel(double2 *__restrict__ d, int L, int lg, long es,
                 z = make_double2(a.x * cs - a.y * sn, a.x * sn + a.y * cs);
             }
         }
-        sm[bitrev(m, lg) * TB + b] = z;
+        sm[fft_pos(m, f) * TB + b] = z;
     }
     __syncthreads();
-    lds_fft(sm, L, lg, TB, TB, tw, dir > 0 ? -1 : +1);
+    lds_fft(sm, f, TB, TB, dir > 0 ? -1 : +1);
     for (int t = threadIdx.x; t < L * TB; t += blockDim.x) {
         const int m = t / TB, b = t - m * TB;
         if (g * TB + b >= ncols) continue;
         if (dir > 0) {
             // split V into the transforms of the real and imaginary parts
             const double2 v = sm[m * TB + b];
-            const double2 w = sm[((L - m) & (L - 1)) * TB + b];
+            const double2 w = sm[(m ? L - m : 0) * TB + b];
             const double2 va = make_double2(0.5 * (v.x + w.x), 0.5 * (v.y - w.y));  // (V + conj W)/2
             const double2 vb = make_double2(0.5 * (v.y + w.y), -0.5 * (v.x - w.x)); // (V − conj W)/(2i)
             double sn, cs;
@@ -407,9 +509,30 @@ static hipError_t upload_table(double2 **dst, int n) {
     return hipMemcpy(*dst, h.data(), sizeof(double2) * n, hipMemcpyHostToDevice);
 }
 
+// the table of length L (created at plan set-up, never during a solve)
+static const double2 *table(const SpectralPlan *p, int L) {
+    for (auto &t : p->tables)
+        if (t.first == L) return t.second;
+    return nullptr;
+}
+static hipError_t ensure_table(SpectralPlan *p, int L) {
+    if (L < 1 || table(p, L)) return hipSuccess;
+    double2 *d = nullptr;
+    hipError_t e = upload_table(&d, L);
+    if (e == hipSuccess) p->tables.emplace_back(L, d);
+    return e;
+}
+static FFTLen fftlen(const SpectralPlan *p, int L) {
+    FFTLen f = fft_factor(L);
+    f.tw = table(p, L);
+    return f;
+}
+
 SpectralPlan *spectral_create(int Nx, int Ny, int Nz, hipStream_t st, char *err, size_t errlen) {
-    if (ilog2(Nx) < 2 || ilog2(Ny) < 1 || Nx > 2048 || Ny > 512) {
-        snprintf(err, errlen, "spectral solver: Nx (%d) and Ny (%d) must be powers of two (Nx in 4..2048, Ny in 2..512)",
+    if (Nx % 2 || !fft_ok(Nx / 2) || !fft_ok(Ny) || Nx < 4 || Ny < 2 || Nx > 2048 || Ny > 512) {
+        snprintf(err, errlen,
+                 "spectral solver: Nx (%d) must be even with Nx/2 a product of 2, 3 and 5, and Ny (%d) a product "
+                 "of 2, 3 and 5 (Nx in 4..2048, Ny in 2..512)",
                  Nx, Ny);
         return nullptr;
     }
@@ -420,11 +543,8 @@ SpectralPlan *spectral_create(int Nx, int Ny, int Nz, hipStream_t st, char *err,
     HCHK(hipMalloc(&p->hat, sizeof(double2) * nh));
     HCHK(hipMemsetAsync(p->hat, 0, sizeof(double2) * nh, st));
     HCHK(hipMalloc(&p->cp, sizeof(double) * nh));
-    int need[4] = {Nx, Nx / 2, Ny, ilog2(Nz) >= 1 ? Nz : 0};
-    for (int q = 0; q < 4; q++) {
-        int l = need[q] > 0 ? ilog2(need[q]) : -1;
-        if (l >= 0 && !p->tw[l]) HCHK(upload_table(&p->tw[l], need[q]));
-    }
+    const int need[4] = {Nx, Nx / 2, Ny, fft_ok(Nz) ? Nz : 0};
+    for (int q = 0; q < 4; q++) HCHK(ensure_table(p, need[q]));
     HCHK(hipStreamSynchronize(st));
     return p;
 }
@@ -432,7 +552,7 @@ SpectralPlan *spectral_create(int Nx, int Ny, int Nz, hipStream_t st, char *err,
 void spectral_destroy(SpectralPlan *p) {
     if (!p) return;
     hipFree(p->hat); hipFree(p->cp);
-    for (auto &t : p->tw) if (t) hipFree(t);
+    for (auto &t : p->tables) hipFree(t.second);
     hipFree(p->kx2); hipFree(p->ky2); hipFree(p->kz2);
     hipFree(p->lx); hipFree(p->ly);
     hipFree(p->lower); hipFree(p->upper); hipFree(p->hweight); hipFree(p->measure); hipFree(p->icoup); hipFree(p->gauge);
@@ -446,7 +566,7 @@ static int upload(double **dst, const std::vector<double> &h) {
 
 // FFTPoissonSolver constructor, src/PressureSolvers.jl:121-138
 int spectral_fft_setup(SpectralPlan *p, double dx, double dy, double dz) {
-    if (ilog2(p->Nz) < 1 || p->Nz > 512) return -2;
+    if (!fft_ok(p->Nz) || p->Nz < 2 || p->Nz > 512) return -2;
     int Nx = p->Nx, Ny = p->Ny, Nz = p->Nz;
     std::vector<double> kx(Nx / 2 + 1), ky(Ny), kz(Nz);
     for (int m = 0; m <= Nx / 2; m++) { double s = sin(M_PI * m / Nx); kx[m] = 4 / (dx * dx) * (s * s); }
@@ -510,28 +630,29 @@ int spectral_column_setup(SpectralPlan *p, double dx, double dy, const double *l
 }
 
 static void xfwd(SpectralPlan *p, hipStream_t st, const double *src) {
-    int L = p->Nx / 2, lg = ilog2(L);
+    int L = p->Nx / 2;
     int tpr = L / 2 > 0 ? L / 2 : 1;
     int rpb = tpr >= 256 ? 1 : 256 / tpr;
     int nrows = p->Ny * p->Nz;
     int bt = tpr >= 256 ? tpr : 256;
     size_t lds = sizeof(double2) * (size_t)rpb * L;
-    hipLaunchKernelGGL(r2c_rows_kernel, dim3((nrows + rpb - 1) / rpb), dim3(bt), lds, st, p->hat, src, L, lg, rpb,
-                       nrows, p->LD, p->tw[lg], p->tw[lg + 1]);
+    hipLaunchKernelGGL(r2c_rows_kernel, dim3((nrows + rpb - 1) / rpb), dim3(bt), lds, st, p->hat, src, fftlen(p, L),
+                       rpb, nrows, p->LD, table(p, 2 * L));
 }
 static void xinv(SpectralPlan *p, hipStream_t st, double *dst, double scale) {
-    int L = p->Nx / 2, lg = ilog2(L);
+    int L = p->Nx / 2;
     int tpr = L / 2 > 0 ? L / 2 : 1;
     int rpb = tpr >= 256 ? 1 : 256 / tpr;
     int nrows = p->Ny * p->Nz;
     int bt = tpr >= 256 ? tpr : 256;
     size_t lds = sizeof(double2) * (size_t)rpb * L;
-    hipLaunchKernelGGL(c2r_rows_kernel, dim3((nrows + rpb - 1) / rpb), dim3(bt), lds, st, dst, p->hat, L, lg, rpb,
-                       nrows, p->LD, p->tw[lg], p->tw[lg + 1], scale);
+    hipLaunchKernelGGL(c2r_rows_kernel, dim3((nrows + rpb - 1) / rpb), dim3(bt), lds, st, dst, p->hat, fftlen(p, L),
+                       rpb, nrows, p->LD, table(p, 2 * L), scale);
 }
 // a complex DFT along y (axis = 1) or z (axis = 2) of every kx column group
 static void cols(SpectralPlan *p, hipStream_t st, int axis, int sign, int mode) {
-    int L = axis == 1 ? p->Ny : p->Nz, lg = ilog2(L);
+    int L = axis == 1 ? p->Ny : p->Nz;
+    const FFTLen f = fftlen(p, L);
     long es = axis == 1 ? (long)p->LD : (long)p->LD * p->Ny;
     long os = axis == 1 ? (long)p->LD * p->Ny : (long)p->LD;
     int nouter = axis == 1 ? p->Nz : p->Ny;
@@ -540,13 +661,13 @@ static void cols(SpectralPlan *p, hipStream_t st, int axis, int sign, int mode) 
     if (L <= 256) {
         constexpr int TB = 16;
         size_t lds = sizeof(double2) * (size_t)L * TB;
-        hipLaunchKernelGGL(c2c_cols_kernel<TB>, dim3(p->LD / TB, nouter), dim3(256), lds, st, p->hat, L, lg, es, os,
-                           ncols, sign, mode, p->tw[lg], ea);
+        hipLaunchKernelGGL(c2c_cols_kernel<TB>, dim3(p->LD / TB, nouter), dim3(256), lds, st, p->hat, f, es, os,
+                           ncols, sign, mode, ea);
     } else {
         constexpr int TB = 8;
         size_t lds = sizeof(double2) * (size_t)L * TB;
-        hipLaunchKernelGGL(c2c_cols_kernel<TB>, dim3(p->LD / TB, nouter), dim3(256), lds, st, p->hat, L, lg, es, os,
-                           ncols, sign, mode, p->tw[lg], ea);
+        hipLaunchKernelGGL(c2c_cols_kernel<TB>, dim3(p->LD / TB, nouter), dim3(256), lds, st, p->hat, f, es, os,
+                           ncols, sign, mode, ea);
     }
 }
 
@@ -562,14 +683,14 @@ int spectral_fft_solve(SpectralPlan *p, hipStream_t st, double *phi, const doubl
 
 // REDFT10 (dir +1) or REDFT01 (dir −1) along x of every row
 static void xdct(SpectralPlan *p, hipStream_t st, const double *src, double *dst, int dir, double scale) {
-    int N = p->Nx, lg = ilog2(N);
+    int N = p->Nx;
     int tpr = N / 2 > 0 ? N / 2 : 1;
     int rpb = tpr >= 256 ? 1 : 256 / tpr;
     int nrows = p->Ny * p->Nz;
     int bt = tpr >= 256 ? tpr : 256;
     size_t lds = sizeof(double2) * (size_t)rpb * N;
-    hipLaunchKernelGGL(dct_rows_kernel, dim3((nrows + rpb - 1) / rpb), dim3(bt), lds, st, p->hat, src, dst, N, lg, rpb,
-                       nrows, p->LD, p->tw[lg], dir, scale);
+    hipLaunchKernelGGL(dct_rows_kernel, dim3((nrows + rpb - 1) / rpb), dim3(bt), lds, st, p->hat, src, dst,
+                       fftlen(p, N), rpb, nrows, p->LD, dir, scale);
 }
 // the y transform of the column solver: complex DFT (periodic) or the DCT pair (Neumann)
 static void ytrans(SpectralPlan *p, hipStream_t st, int dir) {
@@ -577,18 +698,19 @@ static void ytrans(SpectralPlan *p, hipStream_t st, int dir) {
         cols(p, st, 1, dir > 0 ? -1 : +1, 0);
         return;
     }
-    int L = p->Ny, lg = ilog2(L);
+    int L = p->Ny;
+    const FFTLen f = fftlen(p, L);
     long es = (long)p->LD, os = (long)p->LD * p->Ny;
     if (L <= 256) {
         constexpr int TB = 16;
         size_t lds = sizeof(double2) * (size_t)L * TB;
-        hipLaunchKernelGGL(dct_cols_kernel<TB>, dim3(p->LD / TB, p->Nz), dim3(256), lds, st, p->hat, L, lg, es, os,
-                           p->nkx, dir, p->tw[lg]);
+        hipLaunchKernelGGL(dct_cols_kernel<TB>, dim3(p->LD / TB, p->Nz), dim3(256), lds, st, p->hat, f, es, os,
+                           p->nkx, dir);
     } else {
         constexpr int TB = 8;
         size_t lds = sizeof(double2) * (size_t)L * TB;
-        hipLaunchKernelGGL(dct_cols_kernel<TB>, dim3(p->LD / TB, p->Nz), dim3(256), lds, st, p->hat, L, lg, es, os,
-                           p->nkx, dir, p->tw[lg]);
+        hipLaunchKernelGGL(dct_cols_kernel<TB>, dim3(p->LD / TB, p->Nz), dim3(256), lds, st, p->hat, f, es, os,
+                           p->nkx, dir);
     }
 }
 

@@ -102,6 +102,11 @@ int aeo_advect_momentum(aeo_grid *g, int precision, int scheme, int p, void *tu,
 
 /* ---- pressure solvers: AbstractPressureSolver (src/PressureSolvers.jl:19-80) */
 typedef struct aeo_poisson aeo_poisson;
+/* Sizes of the spectral solvers (FFTW takes any length; these hand-written
+ * LDS transforms take mixed radices 2, 3, 4, 5): Nx even with Nx/2 a
+ * product of 2, 3 and 5, Nx ≤ 2048; Ny a product of 2, 3 and 5, ≤ 512; for
+ * the triply periodic solver Nz likewise, ≤ 512. Other sizes are refused
+ * with AEO_ERR_ARG. */
 /* FFTPoissonSolver(mesh), src/PressureSolvers.jl:103-174 (triply periodic) */
 int aeo_poisson_fft_create(int Nx, int Ny, int Nz, double dx, double dy, double dz, aeo_poisson **h);
 /* ColumnTridiagonalSolver(T, (Nx,Ny,Nz), (dx,dy); lower, upper, hweight,

@@ -110,8 +110,9 @@ def test_advect_fp32_error_stated(hip, orc, scheme):
     assert err < 1e-5
 
 
-@pytest.mark.parametrize("size", [(16, 8, 32), (64, 64, 64), (128, 32, 16)])
+@pytest.mark.parametrize("size", [(16, 8, 32), (64, 64, 64), (128, 32, 16), (24, 20, 30), (96, 48, 80), (40, 6, 25)])
 def test_fft_solver_parity(hip, orc, size):
+    # powers of two and mixed radices 2, 3, 4, 5
     extent = (1.0, 2.0, 3.0)
     g, og = grids(hip, orc, size, extent, H=1)
     src = np.asfortranarray(np.random.default_rng(size[0]).normal(size=size))
@@ -121,7 +122,7 @@ def test_fft_solver_parity(hip, orc, size):
 
 
 @pytest.mark.parametrize("stretched", [False, True])
-@pytest.mark.parametrize("size", [(16, 8, 12), (64, 32, 40), (256, 64, 9)])
+@pytest.mark.parametrize("size", [(16, 8, 12), (64, 32, 40), (256, 64, 9), (48, 36, 16), (96, 40, 12)])
 def test_column_solver_parity(hip, orc, size, stretched):
     extent = (100.0, 60.0, 80.0)
     g, og = grids(hip, orc, size, extent, H=1, stretched=stretched)
@@ -134,7 +135,7 @@ def test_column_solver_parity(hip, orc, size, stretched):
 
 
 @pytest.mark.parametrize("bcx,bcy", [("neumann", "periodic"), ("periodic", "neumann"), ("neumann", "neumann")])
-@pytest.mark.parametrize("size", [(16, 8, 12), (64, 32, 40), (512, 16, 6)])
+@pytest.mark.parametrize("size", [(16, 8, 12), (64, 32, 40), (512, 16, 6), (40, 24, 10), (60, 90, 6)])
 def test_column_solver_neumann_lateral_parity(hip, orc, size, bcx, bcy):
     # REDFT10 / REDFT01 (DCT-II / III) on the Neumann lateral axes
     extent = (100.0, 60.0, 80.0)
@@ -207,10 +208,11 @@ def _run_step(hip, orc, kind, scheme, steps, prec=0, N=None, euler=False, forcin
     return sim.state(), dict(u=u, v=v, w=w, theta=th, p=p), og
 
 
-@pytest.mark.parametrize("kind,scheme", [("tgv", "weno5"), ("tgv", "centered"), ("tgv", "weno_z5"),
-                                         ("dcbl", "weno_z5"), ("dcbl", "centered")])
-def test_rk3_step_parity_fp64(hip, orc, kind, scheme):
-    got, ref, og = _run_step(hip, orc, kind, scheme, steps=2)
+@pytest.mark.parametrize("kind,scheme,N", [("tgv", "weno5", None), ("tgv", "centered", None), ("tgv", "weno_z5", None),
+                                           ("dcbl", "weno_z5", None), ("dcbl", "centered", None),
+                                           ("dcbl", "weno_z5", (48, 20)), ("tgv", "weno_z5", 24)])
+def test_rk3_step_parity_fp64(hip, orc, kind, scheme, N):
+    got, ref, og = _run_step(hip, orc, kind, scheme, steps=2, N=N)
     for n in ("u", "v", "w", "theta", "p"):
         assert rel_err(got[n], ref[n]) <= TOL64, n
 
