@@ -315,6 +315,11 @@ def test_anelastic_step_parity_fp64(hip, orc, nu, rk3):
     sim.set(u, v, w, th)
     ost = orc.MWStepper(og, ref.rho_c_pad, ref.rho_f_pad, ref.theta_f_pad, ref.rho, ref.rho_f, nu=nu, flux_bot=0.05)
     p = og.zeros()
+    if rk3:
+        # start from a divergence-free state: RK3 ends on ⅓·u₀ + ⅔·u⁽³⁾, so
+        # ∇·(ρ̄u) vanishes afterwards only if it does at the start
+        sim.step(1e-9)
+        ost.step(u, v, w, th, p, 1e-9)
     for _ in range(2):
         if rk3:
             sim.step_rk3(3.0)
