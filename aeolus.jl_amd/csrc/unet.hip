@@ -580,8 +580,9 @@ UNet *unet_create(int nx, int ny, int nz, int inc, uint64_t seed, char *err, siz
         snprintf(err, errlen, "in_channels must be 1 (source) or 2 (source and mask)");
         return nullptr;
     }
-    if ((size_t)nx * ny * nz * 96 >= ((size_t)1 << 31) * 2) {
-        snprintf(err, errlen, "grid too large for the U-Net's activation buffers");
+    // voxel indices are int (channel offsets are size_t products): nvox < 2^30
+    if ((size_t)nx * ny * nz >= ((size_t)1 << 30)) {
+        snprintf(err, errlen, "grid too large for the U-Net (at most 2^30 voxels)");
         return nullptr;
     }
     UNet *N = new UNet();
