@@ -86,91 +86,114 @@ __device__ __forceinline__ int fft_pos(int x, const FFTLen &f) {
     return p;
 }
 
-// the radix-r DFT of a[0..r) in place; sign −1 forward, +1 inverse
-__device__ __forceinline__ void small_dft(double2 *a, int r, int sign) {
-    const double sg = (double)sign;
-    if (r == 2) {
-        const double2 a0 = a[0], a1 = a[1];
-        a[0] = make_double2(a0.x + a1.x, a0.y + a1.y);
-        a[1] = make_double2(a0.x - a1.x, a0.y - a1.y);
-    } else if (r == 4) {
-        const double2 s02 = make_double2(a[0].x + a[2].x, a[0].y + a[2].y);
-        const double2 d02 = make_double2(a[0].x - a[2].x, a[0].y - a[2].y);
-        const double2 s13 = make_double2(a[1].x + a[3].x, a[1].y + a[3].y);
-        const double2 d13 = make_double2(a[1].x - a[3].x, a[1].y - a[3].y);
-        // sign·i·d13
-        const double2 j13 = make_double2(-sg * d13.y, sg * d13.x);
-        a[0] = make_double2(s02.x + s13.x, s02.y + s13.y);
-        a[2] = make_double2(s02.x - s13.x, s02.y - s13.y);
-        a[1] = make_double2(d02.x + j13.x, d02.y + j13.y);
-        a[3] = make_double2(d02.x - j13.x, d02.y - j13.y);
-    } else if (r == 3) {
-        // y1,2 = a0 − (a1 + a2)/2 ± sign·i·(√3/2)(a1 − a2)
-        const double s3 = 0.86602540378443864676;
-        const double2 t = make_double2(a[1].x + a[2].x, a[1].y + a[2].y);
-        const double2 d = make_double2(a[1].x - a[2].x, a[1].y - a[2].y);
-        const double2 m = make_double2(a[0].x - 0.5 * t.x, a[0].y - 0.5 * t.y);
-        const double2 j = make_double2(-sg * s3 * d.y, sg * s3 * d.x);
-        a[0] = make_double2(a[0].x + t.x, a[0].y + t.y);
-        a[1] = make_double2(m.x + j.x, m.y + j.y);
-        a[2] = make_double2(m.x - j.x, m.y - j.y);
-    } else {  // r == 5
-        const double c1 = 0.30901699437494742410, c2 = -0.80901699437494742410;
-        const double s1 = 0.95105651629515357212, s2 = 0.58778525229247312917;
-        const double2 b1 = make_double2(a[1].x + a[4].x, a[1].y + a[4].y);
-        const double2 b2 = make_double2(a[2].x + a[3].x, a[2].y + a[3].y);
-        const double2 d1 = make_double2(a[1].x - a[4].x, a[1].y - a[4].y);
-        const double2 d2 = make_double2(a[2].x - a[3].x, a[2].y - a[3].y);
-        const double2 a0 = a[0];
-        const double2 m1 = make_double2(a0.x + c1 * b1.x + c2 * b2.x, a0.y + c1 * b1.y + c2 * b2.y);
-        const double2 m2 = make_double2(a0.x + c2 * b1.x + c1 * b2.x, a0.y + c2 * b1.y + c1 * b2.y);
-        // e1 = s1 d1 + s2 d2, e2 = s2 d1 − s1 d2; y = m ± sign·i·e
-        const double2 e1 = make_double2(s1 * d1.x + s2 * d2.x, s1 * d1.y + s2 * d2.y);
-        const double2 e2 = make_double2(s2 * d1.x - s1 * d2.x, s2 * d1.y - s1 * d2.y);
-        const double2 j1 = make_double2(-sg * e1.y, sg * e1.x), j2 = make_double2(-sg * e2.y, sg * e2.x);
-        a[0] = make_double2(a0.x + b1.x + b2.x, a0.y + b1.y + b2.y);
-        a[1] = make_double2(m1.x + j1.x, m1.y + j1.y);
-        a[4] = make_double2(m1.x - j1.x, m1.y - j1.y);
-        a[2] = make_double2(m2.x + j2.x, m2.y + j2.y);
-        a[3] = make_double2(m2.x - j2.x, m2.y - j2.y);
-    }
+// the radix-R DFT of a[0..R) in place; sign −1 forward, +1 inverse
+template <int R>
+__device__ __forceinline__ void small_dft(double2 *a, double sg);
+template <>
+__device__ __forceinline__ void small_dft<2>(double2 *a, double) {
+    const double2 a0 = a[0], a1 = a[1];
+    a[0] = make_double2(a0.x + a1.x, a0.y + a1.y);
+    a[1] = make_double2(a0.x - a1.x, a0.y - a1.y);
+}
+template <>
+__device__ __forceinline__ void small_dft<4>(double2 *a, double sg) {
+    const double2 s02 = make_double2(a[0].x + a[2].x, a[0].y + a[2].y);
+    const double2 d02 = make_double2(a[0].x - a[2].x, a[0].y - a[2].y);
+    const double2 s13 = make_double2(a[1].x + a[3].x, a[1].y + a[3].y);
+    const double2 d13 = make_double2(a[1].x - a[3].x, a[1].y - a[3].y);
+    const double2 j13 = make_double2(-sg * d13.y, sg * d13.x);  // sign·i·d13
+    a[0] = make_double2(s02.x + s13.x, s02.y + s13.y);
+    a[2] = make_double2(s02.x - s13.x, s02.y - s13.y);
+    a[1] = make_double2(d02.x + j13.x, d02.y + j13.y);
+    a[3] = make_double2(d02.x - j13.x, d02.y - j13.y);
+}
+template <>
+__device__ __forceinline__ void small_dft<3>(double2 *a, double sg) {
+    // y1,2 = a0 − (a1 + a2)/2 ± sign·i·(√3/2)(a1 − a2)
+    const double s3 = 0.86602540378443864676;
+    const double2 t = make_double2(a[1].x + a[2].x, a[1].y + a[2].y);
+    const double2 d = make_double2(a[1].x - a[2].x, a[1].y - a[2].y);
+    const double2 m = make_double2(a[0].x - 0.5 * t.x, a[0].y - 0.5 * t.y);
+    const double2 j = make_double2(-sg * s3 * d.y, sg * s3 * d.x);
+    a[0] = make_double2(a[0].x + t.x, a[0].y + t.y);
+    a[1] = make_double2(m.x + j.x, m.y + j.y);
+    a[2] = make_double2(m.x - j.x, m.y - j.y);
+}
+template <>
+__device__ __forceinline__ void small_dft<5>(double2 *a, double sg) {
+    const double c1 = 0.30901699437494742410, c2 = -0.80901699437494742410;
+    const double s1 = 0.95105651629515357212, s2 = 0.58778525229247312917;
+    const double2 b1 = make_double2(a[1].x + a[4].x, a[1].y + a[4].y);
+    const double2 b2 = make_double2(a[2].x + a[3].x, a[2].y + a[3].y);
+    const double2 d1 = make_double2(a[1].x - a[4].x, a[1].y - a[4].y);
+    const double2 d2 = make_double2(a[2].x - a[3].x, a[2].y - a[3].y);
+    const double2 a0 = a[0];
+    const double2 m1 = make_double2(a0.x + c1 * b1.x + c2 * b2.x, a0.y + c1 * b1.y + c2 * b2.y);
+    const double2 m2 = make_double2(a0.x + c2 * b1.x + c1 * b2.x, a0.y + c2 * b1.y + c1 * b2.y);
+    // e1 = s1 d1 + s2 d2, e2 = s2 d1 − s1 d2; y = m ± sign·i·e
+    const double2 e1 = make_double2(s1 * d1.x + s2 * d2.x, s1 * d1.y + s2 * d2.y);
+    const double2 e2 = make_double2(s2 * d1.x - s1 * d2.x, s2 * d1.y - s1 * d2.y);
+    const double2 j1 = make_double2(-sg * e1.y, sg * e1.x), j2 = make_double2(-sg * e2.y, sg * e2.x);
+    a[0] = make_double2(a0.x + b1.x + b2.x, a0.y + b1.y + b2.y);
+    a[1] = make_double2(m1.x + j1.x, m1.y + j1.y);
+    a[4] = make_double2(m1.x - j1.x, m1.y - j1.y);
+    a[2] = make_double2(m2.x + j2.x, m2.y + j2.y);
+    a[3] = make_double2(m2.x - j2.x, m2.y - j2.y);
 }
 
-// One stage of `nb` interleaved sequences s[m * ld + b]: radix r over blocks of
-// m = M_i; dit: twiddle W_len^{qj} before the small DFT, else (DIF) after it
-__device__ __forceinline__ void fft_stage(double2 *s, const FFTLen &f, int i, int m, int nb, int ld, int sign,
-                                          bool dit) {
-    const int r = f.r[i], len = m * r, step = f.L / len;
-    const int nbf = (f.L / r) * nb;
+// One radix-R stage of `nb` interleaved sequences s[m * ld + b] over blocks
+// of m = M_i; DIT: twiddle W_len^{qj} before the small DFT, DIF: after it.
+// R is a template argument so the butterfly stays in registers.
+template <int R, bool DIT>
+__device__ __forceinline__ void fft_stage_r(double2 *s, const double2 *__restrict__ tw, int L, int m, int nb, int ld,
+                                            int sign) {
+    const int len = m * R, step = L / len;
+    const int nbf = (L / R) * nb;
+    const double sg = (double)sign;
     for (int t = threadIdx.x; t < nbf; t += blockDim.x) {
-        const int b = t % nb, bf = t / nb;
+        const int bf = t / nb, b = t - bf * nb;
         const int g = bf / m, q = bf - g * m;
         const int base = g * len + q;
-        double2 a[5];
-        for (int j = 0; j < r; j++) a[j] = s[(base + j * m) * ld + b];
-        if (dit)
-            for (int j = 1; j < r; j++) {
-                double2 w = f.tw[q * j * step];
-                if (sign > 0) w.y = -w.y;
+        double2 a[R];
+#pragma unroll
+        for (int j = 0; j < R; j++) a[j] = s[(base + j * m) * ld + b];
+        if (DIT) {
+#pragma unroll
+            for (int j = 1; j < R; j++) {
+                double2 w = tw[q * j * step];
+                w.y *= -sg;  // conj for the inverse
                 a[j] = cmul(a[j], w);
             }
-        small_dft(a, r, sign);
-        if (!dit)
-            for (int j = 1; j < r; j++) {
-                double2 w = f.tw[q * j * step];
-                if (sign > 0) w.y = -w.y;
+        }
+        small_dft<R>(a, sg);
+        if (!DIT) {
+#pragma unroll
+            for (int j = 1; j < R; j++) {
+                double2 w = tw[q * j * step];
+                w.y *= -sg;
                 a[j] = cmul(a[j], w);
             }
-        for (int j = 0; j < r; j++) s[(base + j * m) * ld + b] = a[j];
+        }
+#pragma unroll
+        for (int j = 0; j < R; j++) s[(base + j * m) * ld + b] = a[j];
     }
     __syncthreads();
+}
+template <bool DIT>
+__device__ __forceinline__ void fft_stage(double2 *s, const FFTLen &f, int i, int m, int nb, int ld, int sign) {
+    switch (f.r[i]) {
+    case 4: fft_stage_r<4, DIT>(s, f.tw, f.L, m, nb, ld, sign); break;
+    case 2: fft_stage_r<2, DIT>(s, f.tw, f.L, m, nb, ld, sign); break;
+    case 3: fft_stage_r<3, DIT>(s, f.tw, f.L, m, nb, ld, sign); break;
+    default: fft_stage_r<5, DIT>(s, f.tw, f.L, m, nb, ld, sign); break;
+    }
 }
 
 // In-LDS mixed-radix FFT, DIT: input in fft_pos order, output natural
 __device__ void lds_fft(double2 *s, const FFTLen &f, int nb, int ld, int sign) {
     int m = 1;
     for (int i = 0; i < f.ns; i++) {
-        fft_stage(s, f, i, m, nb, ld, sign, true);
+        fft_stage<true>(s, f, i, m, nb, ld, sign);
         m *= f.r[i];
     }
 }
@@ -179,7 +202,7 @@ __device__ void lds_fft_dif(double2 *s, const FFTLen &f, int nb, int ld, int sig
     int m = f.L;
     for (int i = f.ns - 1; i >= 0; i--) {
         m /= f.r[i];
-        fft_stage(s, f, i, m, nb, ld, sign, false);
+        fft_stage<false>(s, f, i, m, nb, ld, sign);
     }
 }
 

@@ -168,7 +168,10 @@ def poisson_legs(reps=3, weights=None, log=print):
         net.set_params(np.load(weights, allow_pickle=False))
         mask = hip.DeviceArray.from_numpy(w["mask"])
         run("fcg_net_guess_vcycle", FlexibleCG(mg32, net, mask, mode="vcycle", guess=True, tol=1e-8, maxit=500))
-        run("fcg_net_vcycle", FlexibleCG(mg32, net, mask, mode="net+vcycle", guess=True, tol=1e-8, maxit=500))
+        # the network as the preconditioner of every iteration: it is trained on
+        # sources, not on the residuals FCG hands it, and does not converge
+        # here; capped so the leg reports its status without holding the GPU
+        run("fcg_net_vcycle", FlexibleCG(mg32, net, mask, mode="net+vcycle", guess=True, tol=1e-8, maxit=60))
         run("fcg_vcycle", FlexibleCG(mg32, None, None, mode="vcycle", guess=False, tol=1e-8, maxit=500))
     return out
 
