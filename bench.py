@@ -176,6 +176,14 @@ def poisson_legs(reps=3, weights=None, log=print):
     return out
 
 
+def learned_ms(legs):
+    """The fastest converged leg that uses the network (None without one)."""
+    if not legs:
+        return None
+    ms = [v["ms"] for k, v in legs.items() if k.startswith("fcg_net") and v["status"] == "converged"]
+    return min(ms) if ms else None
+
+
 def cpu_baseline(threads):
     """The CPU oracle (FP64, OpenMP) on a bounded sample of the same workload:
     the full 256x256x128 grid, one warm-up step and two timed RK3 steps."""
@@ -276,8 +284,7 @@ def main():
         "poisson_ms": {
             "direct_column_256x256x128": pms,
             "mgpcg": None if legs is None else legs["mgpcg_f32_jacobi"]["ms"],
-            "learned_fcg": None if legs is None or "fcg_net_vcycle" not in legs else min(
-                legs["fcg_net_vcycle"]["ms"], legs["fcg_net_guess_vcycle"]["ms"]),
+            "learned_fcg": learned_ms(legs),
             "workload": "cut-cell 512x512x192, dx 2 m, 64 seeded buildings, source = div of a seeded k^-5/3 "
                         "velocity; to ||r|| <= 1e-8 ||b||",
             "legs": legs,
