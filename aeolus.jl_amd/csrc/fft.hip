@@ -31,6 +31,7 @@ struct FFTLen {
     int L, ns;
     int r[FFT_MAXS];
     const double2 *tw;
+    const int *perm;  // perm[x] = fft_pos(x), precomputed per length
 };
 
 struct SpectralPlan {
@@ -38,6 +39,7 @@ struct SpectralPlan {
     double2 *hat = nullptr;
     double *cp = nullptr;  // Thomas scratch c'_k per column
     std::vector<std::pair<int, double2 *>> tables;  // twiddle tables by length
+    std::vector<std::pair<int, int *>> perms;       // digit-reversal tables by length
     // FFT solver: squared modified wavenumbers and 1/(Nx Ny Nz)
     double *kx2 = nullptr, *ky2 = nullptr, *kz2 = nullptr;
     double nrm = 0;
@@ -147,11 +149,12 @@ __device__ __forceinline__ void small_dft<5>(double2 *a, double sg) {
 template <int R, bool DIT>
 __device__ __forceinline__ void fft_stage_r(double2 *s, const double2 *__restrict__ tw, int L, int m, int nb, int ld,
                                             int sign) {
+    // nb divides blockDim.x: thread t always works on sequence b = t % nb
     const int len = m * R, step = L / len;
-    const int nbf = (L / R) * nb;
+    const int nbf = L / R, bstep = blockDim.x / nb;
     const double sg = (double)sign;
-    for (int t = threadIdx.x; t < nbf; t += blockDim.x) {
-        const int bf = t / nb, b = t - bf * nb;
+    const int b = threadIdx.x % nb;
+    for (int bf = threadIdx.x / nb; bf < nbf; bf += bstep) {
         const int g = bf / m, q = bf - g * m;
         const int base = g * len + q;
         double2 a[R];
@@ -219,7 +222,7 @@ __global__ void r2c_rows_kernel(double2 *__restrict__ hat, const double *__restr
         int row = row0 + r;
         double2 z = make_double2(0, 0);
         if (row < nrows) z = reinterpret_cast<const double2 *>(src + (size_t)row * N)[m];
-        sm[fft_pos(m, f) * rpb + r] = z;
+        sm[f.perm[m] * rpb + r] = z;
     }
     __syncthreads();
     lds_fft(sm, f, rpb, rpb, -1);
@@ -261,7 +264,7 @@ __global__ void c2r_rows_kernel(double *__restrict__ dst, const double2 *__restr
             double2 wd = cmul(w, d);
             z = make_double2(s.x - wd.y, s.y + wd.x);  // s + i·wd
         }
-        sm[fft_pos(k, f) * rpb + r] = z;
+        sm[f.perm[k] * rpb + r] = z;
     }
     __syncthreads();
     lds_fft(sm, f, rpb, rpb, +1);
@@ -295,7 +298,7 @@ __global__ void c2c_cols_kernel(double2 *__restrict__ d, FFTLen f, long es, long
         int m = t / TB, b = t - m * TB;
         double2 z = make_double2(0, 0);
         if (g * TB + b < ncols) z = d[base + (long)m * es + b];
-        sm[fft_pos(m, f) * TB + b] = z;
+        sm[f.perm[m] * TB + b] = z;
     }
     __syncthreads();
     lds_fft(sm, f, TB, TB, sign);
@@ -316,7 +319,7 @@ __global__ void c2c_cols_kernel(double2 *__restrict__ d, FFTLen f, long es, long
         lds_fft_dif(sm, f, TB, TB, +1);
         for (int t = threadIdx.x; t < L * TB; t += blockDim.x) {
             int m = t / TB, b = t - m * TB;
-            if (g * TB + b < ncols) d[base + (long)m * es + b] = sm[fft_pos(m, f) * TB + b];
+            if (g * TB + b < ncols) d[base + (long)m * es + b] = sm[f.perm[m] * TB + b];
         }
         return;
     }
@@ -449,7 +452,7 @@ __global__ void dct_rows_kernel(double2 *__restrict__ hat, const double *__restr
                 z = make_double2(xk * cs + xn * sn, xk * sn - xn * cs);
             }
         }
-        sm[fft_pos(m, f) * rpb + r] = z;
+        sm[f.perm[m] * rpb + r] = z;
     }
     __syncthreads();
     lds_fft(sm, f, rpb, rpb, dir > 0 ? -1 : +1);
@@ -490,7 +493,7 @@ __global__ void dct_cols_kernel(double2 *__restrict__ d, FFTLen f, long es, long
                 z = make_double2(a.x * cs - a.y * sn, a.x * sn + a.y * cs);
             }
         }
-        sm[fft_pos(m, f) * TB + b] = z;
+        sm[f.perm[m] * TB + b] = z;
     }
     __syncthreads();
     lds_fft(sm, f, TB, TB, dir > 0 ? -1 : +1);
@@ -538,16 +541,39 @@ static const double2 *table(const SpectralPlan *p, int L) {
         if (t.first == L) return t.second;
     return nullptr;
 }
+static const int *perm_table(const SpectralPlan *p, int L) {
+    for (auto &t : p->perms)
+        if (t.first == L) return t.second;
+    return nullptr;
+}
+// the host form of fft_pos
+static int fft_pos_host(int x, const FFTLen &f) {
+    int pos = 0, M = f.L;
+    for (int i = f.ns - 1; i >= 0; i--) {
+        M /= f.r[i];
+        pos += (x % f.r[i]) * M;
+        x /= f.r[i];
+    }
+    return pos;
+}
 static hipError_t ensure_table(SpectralPlan *p, int L) {
     if (L < 1 || table(p, L)) return hipSuccess;
     double2 *d = nullptr;
     hipError_t e = upload_table(&d, L);
-    if (e == hipSuccess) p->tables.emplace_back(L, d);
-    return e;
+    if (e != hipSuccess) return e;
+    p->tables.emplace_back(L, d);
+    const FFTLen f = fft_factor(L);
+    std::vector<int> h(L);
+    for (int x = 0; x < L; x++) h[x] = f.ns ? fft_pos_host(x, f) : x;
+    int *dp = nullptr;
+    if ((e = hipMalloc(&dp, sizeof(int) * L)) != hipSuccess) return e;
+    p->perms.emplace_back(L, dp);
+    return hipMemcpy(dp, h.data(), sizeof(int) * L, hipMemcpyHostToDevice);
 }
 static FFTLen fftlen(const SpectralPlan *p, int L) {
     FFTLen f = fft_factor(L);
     f.tw = table(p, L);
+    f.perm = perm_table(p, L);
     return f;
 }
 
@@ -576,6 +602,7 @@ void spectral_destroy(SpectralPlan *p) {
     if (!p) return;
     hipFree(p->hat); hipFree(p->cp);
     for (auto &t : p->tables) hipFree(t.second);
+    for (auto &t : p->perms) hipFree(t.second);
     hipFree(p->kx2); hipFree(p->ky2); hipFree(p->kz2);
     hipFree(p->lx); hipFree(p->ly);
     hipFree(p->lower); hipFree(p->upper); hipFree(p->hweight); hipFree(p->measure); hipFree(p->icoup); hipFree(p->gauge);
@@ -652,22 +679,29 @@ int spectral_column_setup(SpectralPlan *p, double dx, double dy, const double *l
     return 0;
 }
 
+// rows kernels: tpr threads per row (the butterflies of a radix-4 stage), rpb
+// rows per block, bt = rpb·tpr threads (a multiple of rpb: fft_stage_r
+// keeps each thread on one row)
+static void row_shape(int L, int *tpr, int *rpb, int *bt) {
+    *tpr = L / 4 > 0 ? L / 4 : 1;
+    *rpb = *tpr >= 256 ? 1 : 256 / *tpr;
+    *bt = *rpb * *tpr;
+}
+
 static void xfwd(SpectralPlan *p, hipStream_t st, const double *src) {
     int L = p->Nx / 2;
-    int tpr = L / 2 > 0 ? L / 2 : 1;
-    int rpb = tpr >= 256 ? 1 : 256 / tpr;
+    int tpr, rpb, bt;
+    row_shape(L, &tpr, &rpb, &bt);
     int nrows = p->Ny * p->Nz;
-    int bt = tpr >= 256 ? tpr : 256;
     size_t lds = sizeof(double2) * (size_t)rpb * L;
     hipLaunchKernelGGL(r2c_rows_kernel, dim3((nrows + rpb - 1) / rpb), dim3(bt), lds, st, p->hat, src, fftlen(p, L),
                        rpb, nrows, p->LD, table(p, 2 * L));
 }
 static void xinv(SpectralPlan *p, hipStream_t st, double *dst, double scale) {
     int L = p->Nx / 2;
-    int tpr = L / 2 > 0 ? L / 2 : 1;
-    int rpb = tpr >= 256 ? 1 : 256 / tpr;
+    int tpr, rpb, bt;
+    row_shape(L, &tpr, &rpb, &bt);
     int nrows = p->Ny * p->Nz;
-    int bt = tpr >= 256 ? tpr : 256;
     size_t lds = sizeof(double2) * (size_t)rpb * L;
     hipLaunchKernelGGL(c2r_rows_kernel, dim3((nrows + rpb - 1) / rpb), dim3(bt), lds, st, dst, p->hat, fftlen(p, L),
                        rpb, nrows, p->LD, table(p, 2 * L), scale);
@@ -707,10 +741,9 @@ int spectral_fft_solve(SpectralPlan *p, hipStream_t st, double *phi, const doubl
 // REDFT10 (dir +1) or REDFT01 (dir −1) along x of every row
 static void xdct(SpectralPlan *p, hipStream_t st, const double *src, double *dst, int dir, double scale) {
     int N = p->Nx;
-    int tpr = N / 2 > 0 ? N / 2 : 1;
-    int rpb = tpr >= 256 ? 1 : 256 / tpr;
+    int tpr, rpb, bt;
+    row_shape(N, &tpr, &rpb, &bt);
     int nrows = p->Ny * p->Nz;
-    int bt = tpr >= 256 ? tpr : 256;
     size_t lds = sizeof(double2) * (size_t)rpb * N;
     hipLaunchKernelGGL(dct_rows_kernel, dim3((nrows + rpb - 1) / rpb), dim3(bt), lds, st, p->hat, src, dst,
                        fftlen(p, N), rpb, nrows, p->LD, dir, scale);
