@@ -334,7 +334,7 @@ __global__ void c2c_cols_kernel(double2 *__restrict__ d, FFTLen f, long es, long
 // reference's constructor (:648-660) is evaluated on the fly, the same
 // operations in the same order, with c'_k kept in `cp` for the back
 // substitution. The (0, 0) column is the mean mode (_mean_mode!, :780-805).
-__global__ void thomas_kernel(double2 *__restrict__ hat, double *__restrict__ cp, int nkx, int Ny, int Nz, int LD,
+__global__ void __launch_bounds__(64) thomas_kernel(double2 *__restrict__ hat, double *__restrict__ cp, int nkx, int Ny, int Nz, int LD,
                               const double *__restrict__ lx, const double *__restrict__ ly,
                               const double *__restrict__ lower, const double *__restrict__ upper,
                               const double *__restrict__ hweight, const double *__restrict__ measure,
@@ -366,17 +366,23 @@ __global__ void thomas_kernel(double2 *__restrict__ hat, double *__restrict__ cp
         return;
     }
     const double lam = lx[kx] + ly[ky];
-    // the sweeps take TB levels at a time: the loads of a batch are issued
-    // together (the recurrence itself is sequential; with one column per
-    // thread there are few waves, so memory latency is hidden by batching)
-    constexpr int TB = 8;
+    // The sweeps take TB levels at a time and are software-pipelined: the
+    // loads of the next batch are issued before the current one is worked
+    // (the recurrence itself is sequential; with one column per thread there
+    // are few waves, so memory latency is hidden by the loads in flight).
+    constexpr int TB = 16;
     double cprev = 0.0;
     double2 dprev = make_double2(0, 0);
-    for (int kb = 0; kb < Nz; kb += TB) {
-        double2 v[TB];
+    double2 cur[TB], nxt[TB];
 #pragma unroll
-        for (int q = 0; q < TB; q++)
-            if (kb + q < Nz) v[q] = hat[c0 + plane * (kb + q)];
+    for (int q = 0; q < TB; q++)
+        if (q < Nz) cur[q] = hat[c0 + plane * q];
+    for (int kb = 0; kb < Nz; kb += TB) {
+        if (kb + TB < Nz) {
+#pragma unroll
+            for (int q = 0; q < TB; q++)
+                if (kb + TB + q < Nz) nxt[q] = hat[c0 + plane * (kb + TB + q)];
+        }
 #pragma unroll
         for (int q = 0; q < TB; q++) {
             const int k = kb + q;
@@ -387,40 +393,56 @@ __global__ void thomas_kernel(double2 *__restrict__ hat, double *__restrict__ cp
                 cprev = upper[k] / den;
                 cp[c0 + plane * k] = cprev;
                 if (k == 0) {
-                    v[q].x = v[q].x * inv_den;
-                    v[q].y = v[q].y * inv_den;
+                    cur[q].x = cur[q].x * inv_den;
+                    cur[q].y = cur[q].y * inv_den;
                 } else {
                     const double a = lower[k];
-                    v[q].x = (v[q].x - a * dprev.x) * inv_den;
-                    v[q].y = (v[q].y - a * dprev.y) * inv_den;
+                    cur[q].x = (cur[q].x - a * dprev.x) * inv_den;
+                    cur[q].y = (cur[q].y - a * dprev.y) * inv_den;
                 }
-                dprev = v[q];
+                dprev = cur[q];
             }
         }
 #pragma unroll
-        for (int q = 0; q < TB; q++)
-            if (kb + q < Nz) hat[c0 + plane * (kb + q)] = v[q];
+        for (int q = 0; q < TB; q++) {
+            if (kb + q < Nz) hat[c0 + plane * (kb + q)] = cur[q];
+            cur[q] = nxt[q];
+        }
     }
-    double2 nxt = dprev;
+    // back substitution, levels Nz−2 … 0 in batches of TB, pipelined likewise
+    double2 xn = dprev;
+    double cc[TB], cn[TB];
+    {
+        const int kt = Nz - 2;
+#pragma unroll
+        for (int q = 0; q < TB; q++)
+            if (kt - q >= 0) {
+                cur[q] = hat[c0 + plane * (kt - q)];
+                cc[q] = cp[c0 + plane * (kt - q)];
+            }
+    }
     for (int kt = Nz - 2; kt >= 0; kt -= TB) {
-        double2 v[TB];
-        double c[TB];
+        if (kt - TB >= 0) {
+#pragma unroll
+            for (int q = 0; q < TB; q++)
+                if (kt - TB - q >= 0) {
+                    nxt[q] = hat[c0 + plane * (kt - TB - q)];
+                    cn[q] = cp[c0 + plane * (kt - TB - q)];
+                }
+        }
 #pragma unroll
         for (int q = 0; q < TB; q++)
             if (kt - q >= 0) {
-                v[q] = hat[c0 + plane * (kt - q)];
-                c[q] = cp[c0 + plane * (kt - q)];
+                cur[q].x -= cc[q] * xn.x;
+                cur[q].y -= cc[q] * xn.y;
+                xn = cur[q];
             }
 #pragma unroll
-        for (int q = 0; q < TB; q++)
-            if (kt - q >= 0) {
-                v[q].x -= c[q] * nxt.x;
-                v[q].y -= c[q] * nxt.y;
-                nxt = v[q];
-            }
-#pragma unroll
-        for (int q = 0; q < TB; q++)
-            if (kt - q >= 0) hat[c0 + plane * (kt - q)] = v[q];
+        for (int q = 0; q < TB; q++) {
+            if (kt - q >= 0) hat[c0 + plane * (kt - q)] = cur[q];
+            cur[q] = nxt[q];
+            cc[q] = cn[q];
+        }
     }
 }
 

@@ -244,7 +244,6 @@ __global__ void __launch_bounds__(MW *BY) adv_march_kernel(MarchArgs<T> a) {
 
     __shared__ T xb[NF][BY][KCM];  // east boundary x-flux of each row, per level of the chunk
     __shared__ T yn[NF][KCM][MW];  // north y-flux of the top row, per level of the chunk
-    __shared__ T xs[NF][BY][MW];   // west x-flux of the current level
     __shared__ T ys[NF][BY][MW];   // south y-flux of the current level
 
     // batched east-boundary x-fluxes (the west faces of cell ilast + 1): lane l does level k0 + l
@@ -292,12 +291,12 @@ __global__ void __launch_bounds__(MW *BY) adv_march_kernel(MarchArgs<T> a) {
         asm volatile("" : "+v"(co));
         // the faces of this level, one at a time: west and south fluxes to
         // LDS, the top z-flux in a register (the next level's bottom flux)
-        T fzt[NF];
+        T fzt[NF], fxw[NF];
         if (ok) {
 #pragma unroll
             for (int F = F0; F <= F1; F++) {
                 const int q = F - F0;
-                xs[q][ty][lane] = flux_f<0, S, P, T>(F, a.in, m, co, k, b);
+                fxw[q] = flux_f<0, S, P, T>(F, a.in, m, co, k, b);
                 __builtin_amdgcn_sched_barrier(0);
                 ys[q][ty][lane] = flux_f<1, S, P, T>(F, a.in, m, co, k, b);
                 __builtin_amdgcn_sched_barrier(0);
@@ -310,11 +309,15 @@ __global__ void __launch_bounds__(MW *BY) adv_march_kernel(MarchArgs<T> a) {
 #pragma unroll
             for (int F = F0; F <= F1; F++) {
                 const int q = F - F0;
-                const T fxw = xs[q][ty][lane], fys = ys[q][ty][lane];
-                const T fxe = (i == ilast) ? xb[q][ty][k - k0] : xs[q][ty][lane + 1 < MW ? lane + 1 : lane];
+                // the east x-flux is the west x-flux of the next lane (one
+                // row of the tile is one wave); the row's last cell takes the
+                // batched boundary flux
+                const T fys = ys[q][ty][lane];
+                const T fnext = __shfl_down(fxw[q], 1, MW);
+                const T fxe = (i == ilast) ? xb[q][ty][k - k0] : fnext;
                 const T fyn = (j < jlast) ? ys[q][ty + 1 < BY ? ty + 1 : ty][lane] : yn[q][k - k0][lane];
                 const T idzF = (F == F_W) ? m.m_idzf(k) : m.m_idz(k);
-                const T adv = ((fxe - fxw) * m.idx + (fyn - fys) * m.idy) + (fzt[q] - fzb[q]) * idzF;
+                const T adv = ((fxe - fxw[q]) * m.idx + (fyn - fys) * m.idy) + (fzt[q] - fzb[q]) * idzF;
                 if constexpr (MODE == MODE_SCALAR) {
                     sto(a.ts, co, ldo(a.ts, co) - adv);
                 } else if constexpr (MODE == MODE_MOMENTUM) {
