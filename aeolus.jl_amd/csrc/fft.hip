@@ -346,23 +346,56 @@ __global__ void __launch_bounds__(64) thomas_kernel(double2 *__restrict__ hat, d
     const size_t plane = (size_t)LD * Ny;
     const size_t c0 = (size_t)kx + (size_t)LD * ky;
     if (kx == 0 && ky == 0) {
-        // the mean mode: remove the m-weighted mean, integrate the flux up, gauge
+        // the mean mode: remove the m-weighted mean, integrate the flux up,
+        // gauge. One thread walks the column three times; the loads go in
+        // batches of MB levels (one memory round trip per batch, not per
+        // level); every sum keeps the level order.
+        constexpr int MB = 16;
         double mb = 0.0;
-        for (int k = 0; k < Nz; k++) mb += hat[c0 + plane * k].x * measure[k];
-        mb /= measure_sum;
-        double F = 0.0, ph = 0.0;
-        double prev = hat[c0].x - mb;
-        cp[c0] = 0.0;  // the column, in the scratch
-        for (int k = 1; k < Nz; k++) {
-            F += prev * measure[k - 1];
-            ph += F * icoup[k];
-            prev = hat[c0 + plane * k].x - mb;
-            cp[c0 + plane * k] = ph;
+        for (int kb = 0; kb < Nz; kb += MB) {
+            double h[MB];
+#pragma unroll
+            for (int q = 0; q < MB; q++)
+                if (kb + q < Nz) h[q] = hat[c0 + plane * (kb + q)].x;
+#pragma unroll
+            for (int q = 0; q < MB; q++)
+                if (kb + q < Nz) mb += h[q] * measure[kb + q];
         }
-        double mphi = 0.0;
-        for (int k = 0; k < Nz; k++) mphi += cp[c0 + plane * k] * gauge[k];
-        double sh = mphi / gauge_sum;
-        for (int k = 0; k < Nz; k++) hat[c0 + plane * k] = make_double2(cp[c0 + plane * k] - sh, 0.0);
+        mb /= measure_sum;
+        // φ_0 = 0, φ_k = φ_{k−1} + icoup_k Σ_{j<k} (b_j − mb) m_j; mphi = Σ φ_k g_k
+        double F = 0.0, ph = 0.0, mphi = 0.0, prev = 0.0;
+        for (int kb = 0; kb < Nz; kb += MB) {
+            double h[MB], o[MB];
+#pragma unroll
+            for (int q = 0; q < MB; q++)
+                if (kb + q < Nz) h[q] = hat[c0 + plane * (kb + q)].x;
+#pragma unroll
+            for (int q = 0; q < MB; q++) {
+                const int k = kb + q;
+                if (k < Nz) {
+                    if (k > 0) {
+                        F += prev * measure[k - 1];
+                        ph += F * icoup[k];
+                    }
+                    prev = h[q] - mb;
+                    o[q] = ph;
+                    mphi += ph * gauge[k];
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < MB; q++)
+                if (kb + q < Nz) cp[c0 + plane * (kb + q)] = o[q];
+        }
+        const double sh = mphi / gauge_sum;
+        for (int kb = 0; kb < Nz; kb += MB) {
+            double o[MB];
+#pragma unroll
+            for (int q = 0; q < MB; q++)
+                if (kb + q < Nz) o[q] = cp[c0 + plane * (kb + q)];
+#pragma unroll
+            for (int q = 0; q < MB; q++)
+                if (kb + q < Nz) hat[c0 + plane * (kb + q)] = make_double2(o[q] - sh, 0.0);
+        }
         return;
     }
     const double lam = lx[kx] + ly[ky];
