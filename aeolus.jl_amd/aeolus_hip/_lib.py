@@ -78,6 +78,7 @@ SIGNATURES = {
     "aeo_poisson_mgpcg_create": [_i, _i, _i, _d, _d, _d, _i, _i, _i, _pd, _pd, _pd, _pd, _i, _i, _i, _d, _i, _d,
                                  _i, _pvp],
     "aeo_poisson_mg_create": [_i, _i, _i, _d, _d, _d, _i, _i, _i, _pd, _pd, _pd, _pd, _i, _i, _d, _i, _pvp],
+    "aeo_poisson_mg_set_cycle": [_vp, _i, _i],
     "aeo_poisson_levels": [_vp, _pi],
     "aeo_poisson_apply": [_vp, _vp, _vp],
     "aeo_poisson_source_buffer": [_vp, _pvp],

@@ -176,7 +176,9 @@ class MultigridPoissonSolver(MGPreconditionedCG):
     kind = "mg"
 
     def __init__(self, size, spacing, bcs=("periodic", "periodic", "neumann"), coeff=None, rhs_scale=None, nu1=2,
-                 nu2=2, tol=1e-10, maxcycles=50):
+                 nu2=2, tol=1e-10, maxcycles=50, cycle="V", fmg=False):
+        if cycle not in ("V", "W"):
+            raise ValueError("Multigrid: cycle must be :V or :W")
         nx, ny, nz = size
         pcx, pcy, pcz, keep = _coeffs(coeff, size)
         rs = None if rhs_scale is None else np.ascontiguousarray(np.asarray(rhs_scale, dtype=np.float64))
@@ -187,6 +189,7 @@ class MultigridPoissonSolver(MGPreconditionedCG):
         self._keep = (keep, rs)
         self.axis_bcs = tuple(bcs)
         PressureSolver.__init__(self, h.value, (nx, ny, nz))
+        L.check(L.init().aeo_poisson_mg_set_cycle(ctypes.c_void_p(h.value), int(cycle == "W"), int(bool(fmg))))
 
 
 def stretched_operator(grid):

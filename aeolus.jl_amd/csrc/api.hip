@@ -556,6 +556,15 @@ extern "C" int aeo_poisson_mg_create(int nx, int ny, int nz, double dx, double d
                             AEO_F64, tol, maxcycles, out);
 }
 
+extern "C" int aeo_poisson_mg_set_cycle(aeo_poisson *h, int w_cycle, int fmg) {
+    if (!h) return fail(AEO_ERR_ARG, "null argument");
+    if (!h->mg) return fail(AEO_ERR_UNSUPPORTED, "cycle options apply to multigrid solvers");
+    if ((w_cycle != 0 && w_cycle != 1) || (fmg != 0 && fmg != 1))
+        return fail(AEO_ERR_ARG, "Multigrid: cycle must be :V or :W");
+    mg_set_cycle(h->mg, w_cycle, fmg);
+    return AEO_OK;
+}
+
 extern "C" int aeo_poisson_levels(aeo_poisson *h, int *levels) {
     if (!h || !levels) return fail(AEO_ERR_ARG, "null argument");
     *levels = h->mg ? mg_levels(h->mg) : 1;

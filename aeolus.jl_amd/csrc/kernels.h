@@ -101,6 +101,7 @@ int mg_pcg_solve(MGPlan *P, hipStream_t st, double *xout, const double *bin, dou
 int mg_cycles_solve(MGPlan *P, hipStream_t st, double *xout, const double *bin, double tol, int maxcycles,
                     int *cycles_out, double *rel_out);
 void mg_apply(MGPlan *P, hipStream_t st, double *out, const double *x);
+void mg_set_cycle(MGPlan *P, int wcycle, int fmg);
 // z = one V-cycle of the preconditioner on r (its mean removed), z mean-free (FP64)
 void mg_precond_apply(MGPlan *P, hipStream_t st, double *z, const double *r);
 

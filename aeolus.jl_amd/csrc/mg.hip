@@ -429,6 +429,7 @@ struct MGPlan {
     int nlev = 0;
     MGLevelH L[24];
     int bcx, bcy, bcz, nu1, nu2, smoother, prec;  // prec 0 double, 1 float
+    int wcycle = 0, fmg = 0;  // cycle = :W (two coarse recursions); the nested-iteration start
     double omega;
     bool coeff = false;
     // FP64 fine operator for the CG
@@ -535,7 +536,7 @@ static void vcycle(MGPlan *P, hipStream_t st, int l) {
     hipLaunchKernelGGL(finalize_kernel, dim3(1), dim3(RB), 0, st, P->scal + 6, P->part, nbc, 1.0 / nc);
     hipLaunchKernelGGL(shift_kernel<T>, dim3(flat_blocks(nc)), dim3(RB), 0, st, (T *)c.b, nc, P->scal + 6);
     hipLaunchKernelGGL(zero_kernel<T>, dim3(flat_blocks(nc)), dim3(RB), 0, st, (T *)c.phi, nc);
-    vcycle<T>(P, st, l + 1);
+    for (int rec = 0; rec < (P->wcycle ? 2 : 1); rec++) vcycle<T>(P, st, l + 1);
     // prolong and correct (with the sum of the corrected φ), project
     L = devlevel<T>(P, l);
     C = devlevel<T>(P, l + 1);
@@ -782,6 +783,44 @@ int mg_pcg_solve(MGPlan *P, hipStream_t st, double *xout, const double *bin, dou
 }
 
 // plain multigrid cycles (mg_solve!, src/Multigrid.jl:801-845) with FP64 levels
+// cycle = :V / :W and fmg of MultigridPoissonSolver (src/Multigrid.jl:559-570)
+void mg_set_cycle(MGPlan *P, int wcycle, int fmg) {
+    P->wcycle = wcycle;
+    P->fmg = fmg;
+}
+
+// _fmg_start! (src/Multigrid.jl:743-760), FP64 levels: the right-hand side
+// restricted to every level (mean projected), the coarsest level solved,
+// then at each finer level φ = prolongation of the coarse φ and one cycle
+static void fmg_start(MGPlan *P, hipStream_t st) {
+    for (int l = 0; l < P->nlev - 1; l++) {
+        MGLevelH &h = P->L[l], &c = P->L[l + 1];
+        const int n = h.nx * h.ny * h.nz, nc = c.nx * c.ny * c.nz;
+        hipMemcpyAsync(h.r, h.b, sizeof(double) * n, hipMemcpyDeviceToDevice, st);
+        LevelD<double> L = devlevel<double>(P, l), C = devlevel<double>(P, l + 1);
+        const int nbc = nblocks3(c.nx, c.ny, c.nz);
+        hipLaunchKernelGGL(restrict_kernel<double>, grid3(c.nx, c.ny, c.nz), dim3(RB), 0, st, C, L,
+                           (const double *)nullptr, P->part);
+        hipLaunchKernelGGL(finalize_kernel, dim3(1), dim3(RB), 0, st, P->scal + 6, P->part, nbc, 1.0 / nc);
+        hipLaunchKernelGGL(shift_kernel<double>, dim3(flat_blocks(nc)), dim3(RB), 0, st, (double *)c.b, nc,
+                           P->scal + 6);
+    }
+    {
+        MGLevelH &c = P->L[P->nlev - 1];
+        const int nc = c.nx * c.ny * c.nz;
+        hipLaunchKernelGGL(zero_kernel<double>, dim3(flat_blocks(nc)), dim3(RB), 0, st, (double *)c.phi, nc);
+        vcycle<double>(P, st, P->nlev - 1);  // the coarsest level: _solve_coarsest!
+    }
+    for (int l = P->nlev - 2; l >= 0; l--) {
+        MGLevelH &h = P->L[l];
+        const int n = h.nx * h.ny * h.nz;
+        hipLaunchKernelGGL(zero_kernel<double>, dim3(flat_blocks(n)), dim3(RB), 0, st, (double *)h.phi, n);
+        LevelD<double> L = devlevel<double>(P, l), C = devlevel<double>(P, l + 1);
+        hipLaunchKernelGGL(prolong_kernel<double>, grid3(h.nx, h.ny, h.nz), dim3(RB), 0, st, L, C, P->part);
+        vcycle<double>(P, st, l);
+    }
+}
+
 int mg_cycles_solve(MGPlan *P, hipStream_t st, double *xout, const double *bin, double tol, int maxcycles,
                     int *cycles_out, double *rel_out) {
     if (P->prec != 0) return -1;
@@ -805,6 +844,7 @@ int mg_cycles_solve(MGPlan *P, hipStream_t st, double *xout, const double *bin, 
         *rel_out = 0;
         return 0;
     }
+    if (P->fmg) fmg_start(P, st);
     double rel = 1;
     int ncyc = 0;
     const int nb3 = nblocks3(h.nx, h.ny, h.nz);

@@ -137,6 +137,12 @@ int aeo_poisson_mgpcg_create(int nx, int ny, int nz, double dx, double dy, doubl
 int aeo_poisson_mg_create(int nx, int ny, int nz, double dx, double dy, double dz, int bc_x, int bc_y, int bc_z,
                           const double *cx, const double *cy, const double *cz, const double *rhs_scale, int nu1,
                           int nu2, double tol, int maxcycles, aeo_poisson **h);
+/* cycle and fmg of MultigridPoissonSolver (src/Multigrid.jl:559-570,
+ * 706-760): w_cycle 1 makes every cycle recurse twice on the coarser level
+ * (:W; 0 is :V), in the multigrid solve and in MG-PCG's preconditioner; fmg 1
+ * starts aeo_poisson_mg_create's cycles from the full-multigrid
+ * (nested-iteration) guess. Multigrid-based solvers only. */
+int aeo_poisson_mg_set_cycle(aeo_poisson *h, int w_cycle, int fmg);
 /* mg_level_count (src/Multigrid.jl:628): the hierarchy depth (1 for direct solvers) */
 int aeo_poisson_levels(aeo_poisson *h, int *levels);
 /* _apply_fine! (src/Multigrid.jl:911-917): out = A x on interior-sized FP64

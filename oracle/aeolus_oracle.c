@@ -1160,6 +1160,7 @@ typedef struct {
     int nlev;
     mg_level L[32];
     int bcx, bcy, bcz, singular, nu1, nu2, smoother;
+    int wcycle, fmg; /* cycle = :W (two coarse recursions), fmg = the nested-iteration start */
     double omega;
     double *rhs_scale;
     /* pcg workspace (interior-sized) */
@@ -1486,11 +1487,38 @@ static void mg_vcycle(orc_mg *s, int lev) {
     mg_restrict(C, L, m);
     if (s->singular) mg_project_mean(C, C->b);
     memset(C->phi, 0, sizeof(double) * (size_t)(C->nx + 2) * (C->ny + 2) * (C->nz + 2));
-    mg_vcycle(s, lev + 1);
+    for (int rec = 0; rec < (s->wcycle ? 2 : 1); rec++) mg_vcycle(s, lev + 1);
     mg_prolong_add(s, L, C);
     if (s->singular) mg_project_mean(L, L->phi);
     mg_smooth(s, L, s->nu2);
     if (s->singular) mg_project_mean(L, L->phi);
+}
+
+/* cycle = :V / :W and fmg of MultigridPoissonSolver (src/Multigrid.jl:559-570) */
+void orc_mg_set_cycle(void *h, int wcycle, int fmg) {
+    orc_mg *s = (orc_mg *)h;
+    s->wcycle = wcycle;
+    s->fmg = fmg;
+}
+
+/* _fmg_start!, src/Multigrid.jl:743-760: the right-hand side restricted to
+ * every level, the coarsest solved, then prolongation and one cycle per level */
+static void mg_fmg_start(orc_mg *s) {
+    for (int lev = 0; lev < s->nlev - 1; lev++) {
+        mg_level *L = &s->L[lev], *C = &s->L[lev + 1];
+        memcpy(L->r, L->b, sizeof(double) * (size_t)(L->nx + 2) * (L->ny + 2) * (L->nz + 2));
+        mg_restrict(C, L, 0.0);
+        if (s->singular) mg_project_mean(C, C->b);
+    }
+    mg_level *Lc = &s->L[s->nlev - 1];
+    memset(Lc->phi, 0, sizeof(double) * (size_t)(Lc->nx + 2) * (Lc->ny + 2) * (Lc->nz + 2));
+    mg_vcycle(s, s->nlev - 1); /* the coarsest level: _solve_coarsest! */
+    for (int lev = s->nlev - 2; lev >= 0; lev--) {
+        mg_level *L = &s->L[lev];
+        memset(L->phi, 0, sizeof(double) * (size_t)(L->nx + 2) * (L->ny + 2) * (L->nz + 2));
+        mg_prolong_add(s, L, &s->L[lev + 1]);
+        mg_vcycle(s, lev);
+    }
 }
 
 /* _load_rhs!, src/Multigrid.jl:764-775: copy in with its canonical sum, then the shift */
@@ -1533,6 +1561,7 @@ int orc_mg_solve(void *h, double *x, const double *rhs_in, double tol, int maxcy
     memset(L->phi, 0, sizeof(double) * (size_t)(nx + 2) * (ny + 2) * (nz + 2));
     double bn = sqrt(mg_isum_sq(L, L->b));
     if (bn == 0) { memset(x, 0, sizeof(double) * ni); *rel_out = 0; return 0; }
+    if (s->fmg) mg_fmg_start(s);
     double rel = 1.0;
     int ncyc = 0;
     for (int cyc = 1; cyc <= maxcycles; cyc++) {

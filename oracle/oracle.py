@@ -86,6 +86,7 @@ def lib():
             "orc_mg_create": (vp, [i, i, i, d, d, d, i, i, i, pd, pd, pd, pd, i, i, i, i, d]),
             "orc_mg_destroy": (None, [vp]),
             "orc_mg_levels": (i, [vp]),
+            "orc_mg_set_cycle": (None, [vp, i, i]),
             "orc_mg_solve": (i, [vp, pd, pd, d, i, pd]),
             "orc_mgpcg_solve": (i, [vp, pd, pd, d, i, P(i), pd, pd]),
             "orc_mg_apply": (None, [vp, pd, pd]),

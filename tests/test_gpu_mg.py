@@ -158,6 +158,32 @@ def test_mg_cycles_parity(hip, orc):
     assert rel_err(x, x_ref) < 1e-8
 
 
+@pytest.mark.parametrize("cycle,fmg", [("V", True), ("W", False), ("W", True)])
+@pytest.mark.parametrize("cut", [False, True])
+def test_mg_w_cycle_fmg_parity(hip, orc, cycle, fmg, cut):
+    # cycle = :W and the FMG start of MultigridPoissonSolver against the oracle:
+    # the cycle count (reductions differ in order: ±1) and the solution
+    import ctypes
+    N = (32, 32, 16)
+    spacing = (1000 / 32, 1000 / 32, 200 / 16)
+    cs = cutcell_coeffs(N) if cut else None
+    src = cutcell_source(N, 304) if cut else np.asfortranarray(np.random.default_rng(304).normal(size=N))
+    L = orc.lib()
+    cp = [None] * 3 if cs is None else [orc.ptr(np.asfortranarray(c)) for c in cs]
+    h = L.orc_mg_create(*N, *spacing, 0, 0, 1, *cp, None, 2, 2, 4, 0, 1.0)
+    L.orc_mg_set_cycle(h, int(cycle == "W"), int(fmg))
+    x_ref = np.zeros(N, order="F")
+    rel = ctypes.c_double()
+    cyc_ref = L.orc_mg_solve(h, orc.ptr(x_ref), orc.ptr(src), 1e-10, 200, ctypes.byref(rel))
+    L.orc_mg_destroy(h)
+    s = hip.MultigridPoissonSolver(N, spacing, coeff=cs, tol=1e-10, maxcycles=200, cycle=cycle, fmg=fmg)
+    x = hip.solve_interior(s, src)
+    cyc, r = hip.solver_stats(s)
+    print(f"{'cut-cell' if cut else 'constant'} {cycle}{' + FMG' if fmg else ''}: {cyc} cycles (oracle {cyc_ref})")
+    assert abs(cyc - cyc_ref) <= 1 and r < 1e-10
+    assert rel_err(x, x_ref) < 1e-8
+
+
 def test_mgpcg_operator_identity(hip):
     # the returned x satisfies A x = b − mean(b) to the tolerance, recomputed in FP64
     N = (64, 64, 32)

@@ -409,6 +409,38 @@ def test_mg_manufactured_and_levels(orc):
     L.orc_mg_destroy(h)
 
 
+def test_mg_w_cycle_and_fmg(orc):
+    # cycle = :W and fmg = true (src/Multigrid.jl:559-570, 706-760): every
+    # variant converges to the direct column solver's solution; a W-cycle
+    # contracts more per cycle than a V-cycle (fewer cycles); the FMG start
+    # leaves a first-cycle residual far below a zero start's
+    import ctypes
+    N = (32, 32, 16)
+    g = orc.Grid(N, (32.0, 32.0, 16.0), halo=1)
+    src = np.asfortranarray(np.random.default_rng(5).normal(size=N))
+    src -= src.mean()
+    x_dir = orc.stretched_column_solver(g).solve(src)
+    L = orc.lib()
+    res = {}
+    for w in (0, 1):
+        for f in (0, 1):
+            h = _orc_mg(orc, N, (1.0, 1.0, 1.0), ("periodic", "periodic", "neumann"))
+            L.orc_mg_set_cycle(h, w, f)
+            out = np.zeros(N, order="F")
+            rel = ctypes.c_double()
+            cyc = L.orc_mg_solve(h, orc.ptr(out), orc.ptr(src), 1e-11, 100, ctypes.byref(rel))
+            one = np.zeros(N, order="F")
+            rel1 = ctypes.c_double()
+            L.orc_mg_solve(h, orc.ptr(one), orc.ptr(src), 1e-30, 1, ctypes.byref(rel1))
+            L.orc_mg_destroy(h)
+            assert rel.value < 1e-11
+            assert np.abs(out - x_dir).max() < 1e-8 * np.abs(x_dir).max()
+            res[(w, f)] = (cyc, rel1.value)
+    assert res[(1, 0)][0] < res[(0, 0)][0]
+    assert res[(0, 1)][1] < 0.1 * res[(0, 0)][1] and res[(1, 1)][1] < 0.1 * res[(1, 0)][1]
+    assert res[(0, 1)][0] < res[(0, 0)][0]
+
+
 def test_mgpcg_matches_column_solver(orc):
     # MG-CG and the direct column solver invert the same operator (test/multigrid.jl:23-31)
     import ctypes
