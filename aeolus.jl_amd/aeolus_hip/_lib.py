@@ -107,6 +107,7 @@ SIGNATURES = {
     "aeo_mw_step_rk3": [_vp, _vp, _vp, _vp, _vp, _vp, _d],
     "aeo_weighted_divergence": [_vp, _i, _vp, _vp, _vp, _vp, _vp, _vp, _i],
     "aeo_cutcell_divergence": [_vp, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
+    "aeo_cutcell_redistribute": [_vp, _i, _vp, _vp, _vp, _i],
     "aeo_cutcell_gradient": [_vp, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
     "aeo_cutcell_laplacian": [_vp, _i, _vp, _vp, _vp, _vp, _vp, _vp, _i],
     "aeo_cutcell_project_source": [_vp, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp],

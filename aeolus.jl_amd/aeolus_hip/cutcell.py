@@ -4,14 +4,16 @@ cut_cell_geometry_from_heightmap (src/CutCell.jl:372-396) through
 cut_cell_geometry's sub-cell sampling (:213-312) and _finalize_geometry's
 consistency sweeps and small-cell floor (:398-480), vectorised in numpy:
 the level set is φ(x, y, z) = z − h(column), φ > 0 fluid. The small-cell
-merge map and the surface normals are not built (the projection's operator
-needs the volumes and apertures only).
+merge partners are built (small_cell_merges) for the divergence's
+redistribution; the surface normals are not (no operator here reads them).
 
 The pressure operator of the projection is D·G (aperture power 2,
 cutcell_laplacian!, :735-781); in the multigrid's face-coefficient form the
 symmetric operator vol_eff·L has c = a² on every face (0 on blocked faces)
 and the source vol_eff·(D u) is the aperture-weighted face-flux balance.
 """
+import ctypes
+
 import numpy as np
 
 
@@ -113,6 +115,39 @@ def heightmap_geometry(N, spacing, H, height, nsub=4, vol_min=1e-3):
     return F(vol), F(vol_eff), F(ax), F(ay), F(az)
 
 
+def small_cell_merges(vol, vol_eff, ax, ay, az, N, H, vol_min=1e-3):
+    """The small cells (not solid, vol < vol_min) of the interior and their
+    merge partners, as an int32 array of (cell, target) 0-based storage
+    linear indices (Fortran order), in the interior loop order of
+    _redistribute_small_cells! (src/CutCell.jl:628-648). The partner is the
+    neighbour of largest vol through an open face, faces taken −x, +x, −y,
+    +y, −z, +z, the first strictly largest kept (src/CutCell.jl:466-488);
+    small cells without one are left out."""
+    sx, sy, sz = vol.shape
+    small = (vol_eff > 0) & (vol < vol_min)
+    Nx, Ny, Nz = N
+    inner = np.zeros_like(small)
+    inner[H:H + Nx, H:H + Ny, H:H + Nz] = True
+    cells = np.argwhere(small & inner)
+    # the loop order of `for k, j, i`: k slowest
+    cells = cells[np.lexsort((cells[:, 0], cells[:, 1], cells[:, 2]))]
+    pairs = []
+    for i, j, k in cells:
+        cand = ((ax[i, j, k] if i > 0 else 0.0, i - 1, j, k), (ax[i + 1, j, k] if i < sx - 1 else 0.0, i + 1, j, k),
+                (ay[i, j, k] if j > 0 else 0.0, i, j - 1, k), (ay[i, j + 1, k] if j < sy - 1 else 0.0, i, j + 1, k),
+                (az[i, j, k] if k > 0 else 0.0, i, j, k - 1), (az[i, j, k + 1] if k < sz - 1 else 0.0, i, j, k + 1))
+        best, bestv = -1, 0.0
+        for ap, i2, j2, k2 in cand:
+            if not ap > 0 or not (0 <= i2 < sx and 0 <= j2 < sy and 0 <= k2 < sz):
+                continue
+            v2 = vol[i2, j2, k2]
+            if v2 > bestv:
+                bestv, best = v2, i2 + sx * (j2 + sy * k2)
+        if best >= 0:
+            pairs.append((i + sx * (j + sy * k), best))
+    return np.asarray(pairs, dtype=np.int32).reshape(-1, 2)
+
+
 def mg_coefficients(ax, ay, az, N, H):
     """Face coefficients a² of the D·G operator on the interior faces:
     cx (Nx+1, Ny, Nz), cy (Nx, Ny+1, Nz), cz (Nx, Ny, Nz+1)."""
@@ -157,6 +192,23 @@ class CutCellGeometry:
         self.grid = grid
         self.host = dict(vol=vol, vol_eff=ve, ax=ax, ay=ay, az=az)
         self.ax, self.ay, self.az, self.vol_eff = (DeviceArray.from_numpy(a) for a in (ax, ay, az, ve))
+        self.merges = small_cell_merges(vol, ve, ax, ay, az, N, grid.H, vol_min)
+        self._merges_dev = None
+        if len(self.merges):
+            from . import _lib as L
+            p = ctypes.c_void_p()
+            nbytes = self.merges.nbytes
+            L.check(L.init().aeo_malloc(ctypes.byref(p), nbytes))
+            L.check(L.init().aeo_memcpy_h2d(p, self.merges.ctypes.data_as(ctypes.c_void_p), nbytes))
+            self._merges_dev = p.value
+
+    def __del__(self):
+        try:
+            from . import _lib as L
+            if self._merges_dev and L._lib is not None and L._initialised:
+                L._lib.aeo_free(ctypes.c_void_p(self._merges_dev))
+        except Exception:
+            pass
 
     def coefficients(self):
         """Face coefficients a² of vol_eff·D·G for the multigrid solvers."""
@@ -172,11 +224,16 @@ class CutCellGeometry:
         return L.dptr(self.ax), L.dptr(self.ay), L.dptr(self.az)
 
 
-def divergence(div, u, v, w, geom):
-    """cutcell_divergence!(div, u, v, w, geom; redistribute = false) (src/CutCell.jl:602-626)."""
+def divergence(div, u, v, w, geom, redistribute=True):
+    """cutcell_divergence!(div, u, v, w, geom; redistribute) (src/CutCell.jl:602-648):
+    with redistribute, each small cell and its merge partner take the
+    cluster's vol_eff-weighted mean divergence."""
     from . import _lib as L
     L.check(L.init().aeo_cutcell_divergence(geom.grid.handle, div.precision, L.dptr(div), L.dptr(u), L.dptr(v),
                                             L.dptr(w), *geom._geo(), L.dptr(geom.vol_eff)))
+    if redistribute and geom._merges_dev:
+        L.check(L.init().aeo_cutcell_redistribute(geom.grid.handle, div.precision, L.dptr(div), L.dptr(geom.vol_eff),
+                                                  ctypes.c_void_p(geom._merges_dev), len(geom.merges)))
     return div
 
 

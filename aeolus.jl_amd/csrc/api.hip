@@ -664,6 +664,21 @@ extern "C" int aeo_cutcell_divergence(aeo_grid *g, int precision, void *div, con
     return cutcell_call<float>(g, 0, (float *)div, nullptr, nullptr, (const float *)u, (const float *)v,
                                (const float *)w, nullptr, nullptr, 0, 0, 0, 0, ax, ay, az, vol_eff, 0, 0);
 }
+extern "C" int aeo_cutcell_redistribute(aeo_grid *g, int precision, void *div, const double *vol_eff,
+                                        const int *pairs, int npairs) {
+    NEED_CTX();
+    if (!g || !div || !vol_eff || (npairs > 0 && !pairs)) return fail(AEO_ERR_ARG, "null argument");
+    if (npairs < 0) return fail(AEO_ERR_ARG, "npairs must be >= 0");
+    if (precision != AEO_F64 && precision != AEO_F32) return fail(AEO_ERR_ARG, "bad precision");
+    if (npairs == 0) return AEO_OK;
+    const long long nstore = (long long)(g->Nx + 2 * g->H) * (g->Ny + 2 * g->H) * (g->Nz + 2 * g->H);
+    if (precision == AEO_F64)
+        launch_redistribute<double>(g_ctx->stream, (double *)div, vol_eff, pairs, npairs, nstore);
+    else launch_redistribute<float>(g_ctx->stream, (float *)div, vol_eff, pairs, npairs, nstore);
+    HIPCHK(hipGetLastError());
+    return AEO_OK;
+}
+
 extern "C" int aeo_cutcell_gradient(aeo_grid *g, int precision, void *gx, void *gy, void *gz, const double *p,
                                     const double *ax, const double *ay, const double *az) {
     CC_CHECK();

@@ -6,9 +6,11 @@
 // the multigrid's form) and the correction u ← u* − G φ.
 //
 // Geometry arrays (ax, ay, az, vol_eff) are storage-layout FP64 on the device.
-// The small-cell redistribution of cutcell_divergence! (:628-648) needs the
-// merge map, which the geometry builder here does not produce: D is the
-// strict adjoint form (redistribute = false).
+// The small-cell redistribution of cutcell_divergence! (:628-648) is a
+// separate pass over the (cell, partner) list the geometry builder makes: it
+// is sequential (a partner shared by several small cells accumulates them
+// one at a time, in the interior loop order) and small cells are rare, so
+// one work item walks the list.
 #include <hip/hip_runtime.h>
 
 #include "kernels.h"
@@ -109,6 +111,29 @@ __global__ void cc_lap_kernel(T *__restrict__ out, const T *__restrict__ f, cons
     s += (azp * ((double)f[c + sz] - fc) + azm * ((double)f[c - sz] - fc)) * m.idz2;
     out[c] = (T)(s / vc);
 }
+
+// _redistribute_small_cells! (src/CutCell.jl:628-648)
+template <typename T>
+__global__ void cc_redistribute_kernel(T *__restrict__ div, const double *__restrict__ ve,
+                                       const int *__restrict__ pairs, int npairs, long long nstore) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    for (int q = 0; q < npairs; q++) {
+        const int s = pairs[2 * q], t = pairs[2 * q + 1];
+        if (s < 0 || t < 0 || s >= nstore || t >= nstore) continue;  // outside the storage: skipped
+        const double vs = ve[s], vt = ve[t], vtot = vs + vt;
+        if (!(vtot > 0)) continue;
+        const double dcl = (vs * (double)div[s] + vt * (double)div[t]) / vtot;
+        div[s] = (T)dcl;
+        div[t] = (T)dcl;
+    }
+}
+
+template <typename T>
+void launch_redistribute(hipStream_t st, T *div, const double *ve, const int *pairs, int npairs, long long nstore) {
+    hipLaunchKernelGGL(cc_redistribute_kernel<T>, dim3(1), dim3(64), 0, st, div, ve, pairs, npairs, nstore);
+}
+template void launch_redistribute<double>(hipStream_t, double *, const double *, const int *, int, long long);
+template void launch_redistribute<float>(hipStream_t, float *, const double *, const int *, int, long long);
 
 template <typename T>
 int launch_cutcell(hipStream_t st, int op, const Metrics<double> &m, T *a, T *b, T *c, const T *f0, const T *f1,

@@ -72,6 +72,8 @@ void launch_tf(hipStream_t st, int op, const TFView &t, double *out, const doubl
 
 // ---- cut-cell operators (cutcell.hip): op 0 divergence, 1 gradient/correction, 2 Laplacian ----
 template <typename T>
+void launch_redistribute(hipStream_t st, T *div, const double *ve, const int *pairs, int npairs, long long nstore);
+template <typename T>
 int launch_cutcell(hipStream_t st, int op, const Metrics<double> &m, T *a, T *b, T *c, const T *f0, const T *f1,
                    const T *f2, double *buf, const double *phi, int phi_interior, int bcx, int bcy, int bcz,
                    const double *ax, const double *ay, const double *az, const double *ve, int flag, double s);

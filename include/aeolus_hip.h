@@ -242,6 +242,15 @@ int aeo_kinetic_energy(aeo_grid *g, int precision, const void *u, const void *v,
  * div's interior; solid cells 0 */
 int aeo_cutcell_divergence(aeo_grid *g, int precision, void *div, const void *u, const void *v, const void *w,
                            const double *ax, const double *ay, const double *az, const double *vol_eff);
+/* _redistribute_small_cells! (src/CutCell.jl:628-648), the second half of
+ * cutcell_divergence!(…; redistribute = true): for each (cell, partner)
+ * pair in order, both take (V_s d_s + V_t d_t)/(V_s + V_t), V = vol_eff.
+ * pairs: device int32 array of 2·npairs 0-based storage linear indices, the
+ * small cells (not solid, vol < vol_min) of the interior in the loop order,
+ * each with its merge partner (src/CutCell.jl:466-488). Pairs outside the
+ * grid's storage are skipped. */
+int aeo_cutcell_redistribute(aeo_grid *g, int precision, void *div, const double *vol_eff, const int *pairs,
+                             int npairs);
 /* cutcell_gradient_x!/_y!/_z! (:666-700): a·(p⁺ − p⁻)/Δ on the interior
  * faces (face i = west face of cell i); p FP64 storage layout, halo filled */
 int aeo_cutcell_gradient(aeo_grid *g, int precision, void *gx, void *gy, void *gz, const double *p,

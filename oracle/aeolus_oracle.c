@@ -2182,3 +2182,75 @@ void orc_mw_step_rk3(void *h, double *u, double *v, double *w, double *th, doubl
     for (int q = 0; q < 4; q++)
         for (size_t c = 0; c < n; c++) f[q][c] = third * a[q][c] + two3 * f[q][c];
 }
+
+/* The merge partner of each small cell (0 < vol < vol_min, not solid): the
+ * neighbour of largest vol through an open face, the faces taken in the
+ * order −x, +x, −y, +y, −z, +z, the first strictly largest kept
+ * (_finalize_geometry, src/CutCell.jl:466-488). target: 1-based storage
+ * linear index, 0 for none (and for every cell that is not small). */
+void orc_cutcell_merge_targets(const orc_grid *g, const double *vol, const double *vol_eff, const double *ax,
+                               const double *ay, const double *az, double vol_min, long long *target) {
+    GRID_DIMS(g);
+    int n1 = sx, n2 = sy, n3 = sz;
+    for (int k = 1; k <= n3; k++)
+        for (int j = 1; j <= n2; j++)
+            for (int i = 1; i <= n1; i++) {
+                size_t c = ID(i, j, k);
+                target[c] = 0;
+                if (!(vol_eff[c] > 0) || !(vol[c] < vol_min)) continue;
+                double ap[6] = {i > 1 ? ax[c] : 0.0, i < n1 ? ax[ID(i + 1, j, k)] : 0.0,
+                                j > 1 ? ay[c] : 0.0, j < n2 ? ay[ID(i, j + 1, k)] : 0.0,
+                                k > 1 ? az[c] : 0.0, k < n3 ? az[ID(i, j, k + 1)] : 0.0};
+                int nb[6][3] = {{i - 1, j, k}, {i + 1, j, k}, {i, j - 1, k}, {i, j + 1, k}, {i, j, k - 1}, {i, j, k + 1}};
+                long long best = 0;
+                double bestv = 0.0;
+                for (int q = 0; q < 6; q++) {
+                    if (!(ap[q] > 0)) continue;
+                    int i2 = nb[q][0], j2 = nb[q][1], k2 = nb[q][2];
+                    if (i2 < 1 || i2 > n1 || j2 < 1 || j2 > n2 || k2 < 1 || k2 > n3) continue;
+                    double v2 = vol[ID(i2, j2, k2)];
+                    if (v2 > bestv) {
+                        bestv = v2;
+                        best = (long long)ID(i2, j2, k2) + 1;
+                    }
+                }
+                target[c] = best;
+            }
+}
+
+/* cutcell_divergence!(div, u, v, w, geom; redistribute) (src/CutCell.jl:602-648):
+ * the aperture-weighted divergence over the interior, zero in solid cells;
+ * with redistribute, each small cell and its merge target (target, 1-based,
+ * NULL for none) take the cluster's vol_eff-weighted mean, cells taken in
+ * the interior loop order */
+void orc_cutcell_divergence(const orc_grid *g, double *div, const double *u, const double *v, const double *w,
+                            const double *ax, const double *ay, const double *az, const double *vol_eff,
+                            const long long *target, int redistribute) {
+    GRID_DIMS(g);
+    int H = g->H;
+    double idx = 1 / g->dx, idy = 1 / g->dy, idz = 1 / g->dz;
+    for (int k = H + 1; k <= H + g->Nz; k++)
+        for (int j = H + 1; j <= H + g->Ny; j++)
+            for (int i = H + 1; i <= H + g->Nx; i++) {
+                size_t c = ID(i, j, k);
+                double vc = vol_eff[c];
+                if (!(vc > 0)) { div[c] = 0; continue; }
+                double fx = (ax[ID(i + 1, j, k)] * u[ID(i + 1, j, k)] - ax[c] * u[c]) * idx;
+                double fy = (ay[ID(i, j + 1, k)] * v[ID(i, j + 1, k)] - ay[c] * v[c]) * idy;
+                double fz = (az[ID(i, j, k + 1)] * w[ID(i, j, k + 1)] - az[c] * w[c]) * idz;
+                div[c] = ((fx + fy) + fz) / vc;
+            }
+    if (!redistribute || !target) return;
+    for (int k = H + 1; k <= H + g->Nz; k++)
+        for (int j = H + 1; j <= H + g->Ny; j++)
+            for (int i = H + 1; i <= H + g->Nx; i++) {
+                size_t c = ID(i, j, k);
+                long long t = target[c];
+                if (t == 0) continue;
+                double vs = vol_eff[c], vt = vol_eff[t - 1], vtot = vs + vt;
+                if (!(vtot > 0)) continue;
+                double dcl = (vs * div[c] + vt * div[t - 1]) / vtot;
+                div[c] = dcl;
+                div[t - 1] = dcl;
+            }
+}

@@ -91,6 +91,8 @@ def lib():
             "orc_mgpcg_solve": (i, [vp, pd, pd, d, i, P(i), pd, pd]),
             "orc_mg_apply": (None, [vp, pd, pd]),
             "orc_cutcell_heightmap": (None, [pg, pd, i, d, pd, pd, pd, pd, pd]),
+            "orc_cutcell_merge_targets": (None, [pg, pd, pd, pd, pd, pd, d, P(ctypes.c_longlong)]),
+            "orc_cutcell_divergence": (None, [pg, pd, pd, pd, pd, pd, pd, pd, pd, P(ctypes.c_longlong), i]),
             "orc_tf_create": (vp, [pg, pd, d, pd, pd]),
             "orc_tf_destroy": (None, [vp]),
             "orc_tf_metric": (pd, [vp, i]),
@@ -398,6 +400,22 @@ def weighted_divergence(g, u, v, w, Wc, Wf, normalize=False):
     out = g.zeros()
     lib().orc_weighted_divergence(ptr(out), ptr(u), ptr(v), ptr(w), ctypes.byref(g.c),
                                   ptr(np.ascontiguousarray(Wc)), ptr(np.ascontiguousarray(Wf)), int(normalize))
+    return out
+
+
+def cutcell_merge_targets(g, vol, vol_eff, ax, ay, az, vol_min):
+    """1-based storage linear merge targets (0: none) of the small cells."""
+    t = np.zeros(g.storage, dtype=np.int64, order="F")
+    lib().orc_cutcell_merge_targets(ctypes.byref(g.c), ptr(vol), ptr(vol_eff), ptr(ax), ptr(ay), ptr(az), vol_min,
+                                    t.ctypes.data_as(ctypes.POINTER(ctypes.c_longlong)))
+    return t
+
+
+def cutcell_divergence(g, u, v, w, ax, ay, az, vol_eff, target=None, redistribute=True):
+    out = g.zeros()
+    tp = None if target is None else target.ctypes.data_as(ctypes.POINTER(ctypes.c_longlong))
+    lib().orc_cutcell_divergence(ctypes.byref(g.c), ptr(out), ptr(u), ptr(v), ptr(w), ptr(ax), ptr(ay), ptr(az),
+                                 ptr(vol_eff), tp, int(redistribute))
     return out
 
 

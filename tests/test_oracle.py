@@ -500,6 +500,68 @@ def test_cutcell_geometry_matches_oracle(orc):
     assert vol[H + 0, H + 0, H + 2] in (0.0, 0.5, 1.0)
 
 
+def _small_cell_case(orc, seed=3, vol_min=0.3):
+    # roofs at heights that leave a quarter of a cell fluid: with vol_min 0.3
+    # those cells are small (the default vol_min = 1e-3 never is with a
+    # heightmap: a cell's fraction is a multiple of 1/nsub)
+    from aeolus_hip.cutcell import heightmap_geometry
+    N, H = (24, 20, 16), 2
+    g = orc.Grid(N, (48.0, 40.0, 32.0), halo=H)
+    rng = np.random.default_rng(seed)
+    h = np.zeros((N[0], N[1]))
+    for _ in range(6):
+        i0, j0 = rng.integers(0, 18), rng.integers(0, 14)
+        h[i0:i0 + rng.integers(2, 6), j0:j0 + rng.integers(2, 6)] = 2.0 * rng.integers(1, 10) + 1.4
+    vol, ve, ax, ay, az = heightmap_geometry(N, (2.0, 2.0, 2.0), H, h, nsub=4, vol_min=vol_min)
+    return g, N, H, vol, ve, ax, ay, az
+
+
+def test_small_cell_merges_match_oracle(orc):
+    # the product's merge list (host setup) against the oracle's restatement
+    # of _finalize_geometry's partner rule (src/CutCell.jl:466-488)
+    from aeolus_hip.cutcell import small_cell_merges
+    g, N, H, vol, ve, ax, ay, az = _small_cell_case(orc)
+    pairs = small_cell_merges(vol, ve, ax, ay, az, N, H, 0.3)
+    t = orc.cutcell_merge_targets(g, vol, ve, ax, ay, az, 0.3)
+    inner = np.zeros(g.storage, bool)
+    inner[g.interior] = True
+    flat = t.ravel(order="F")
+    want = [(c, flat[c] - 1) for c in np.flatnonzero((inner & (t > 0)).ravel(order="F"))]
+    # the oracle lists cells in Fortran (i fastest) order: the loop order
+    assert len(pairs) > 10
+    assert [tuple(p) for p in pairs] == want
+
+
+def test_redistribution_conserves_and_merges(orc):
+    # cutcell_divergence!(…; redistribute = true): Σ vol_eff·div is kept (the
+    # cluster mean is vol_eff-weighted), each pair ends equal, and cells in no
+    # cluster keep the strict D
+    g, N, H, vol, ve, ax, ay, az = _small_cell_case(orc)
+    t = orc.cutcell_merge_targets(g, vol, ve, ax, ay, az, 0.3)
+    rng = np.random.default_rng(9)
+    u, v, w = (np.asfortranarray(rng.standard_normal(g.storage)) for _ in range(3))
+    d0 = orc.cutcell_divergence(g, u, v, w, ax, ay, az, ve, t, redistribute=False)
+    d1 = orc.cutcell_divergence(g, u, v, w, ax, ay, az, ve, t, redistribute=True)
+    sx, sy, sz = g.storage
+    I = g.interior
+    # partners inside the interior: the conserved sum is over the interior
+    tgt = t[I][t[I] > 0] - 1
+    ti, tj, tk = tgt % sx, (tgt // sx) % sy, tgt // (sx * sy)
+    assert np.all((ti >= H) & (ti < H + N[0]) & (tj >= H) & (tj < H + N[1]) & (tk >= H) & (tk < H + N[2]))
+    s0, s1 = (ve[I] * d0[I]).sum(), (ve[I] * d1[I]).sum()
+    assert abs(s1 - s0) < 1e-12 * np.abs(ve[I] * d0[I]).sum()
+    flat1 = d1.ravel(order="F")
+    last = {}
+    for c in np.flatnonzero(t.ravel(order="F") * np.asarray(np.pad(np.ones(N, bool), H), dtype=np.int64).ravel(order="F")):
+        last[c] = flat1[c]
+    touched = set(last) | set((t.ravel(order="F")[list(last)] - 1).tolist())
+    untouched = np.ones(sx * sy * sz, bool)
+    untouched[list(touched)] = False
+    m = untouched & np.asarray(np.pad(np.ones(N, bool), H)).ravel(order="F")
+    assert np.array_equal(flat1[m], d0.ravel(order="F")[m])
+    assert len(touched) > 10
+
+
 def ctypes_byref(g):
     import ctypes
     return ctypes.byref(g.c)
