@@ -58,7 +58,8 @@ class BoussinesqSimulation:
 
     def __init__(self, grid: Grid, scheme="centered", weno_p=2, nu=0.0, theta0=0.0, g_accel=9.81,
                  flux_bot=0.0, flux_top=0.0, bcs=("periodic",) * 3, theta_bcs=None, precision=L.F64,
-                 solver=None, f_coriolis=0.0, sponge_alpha_max=0.0, sponge_z_start=None, sponge_target=None):
+                 solver=None, f_coriolis=0.0, sponge_alpha_max=0.0, sponge_z_start=None, sponge_target=None,
+                 C_s=0.0, Pr_t=1 / 3):
         self.grid = grid
         self.precision = precision
         self.bcs = tuple(bcs)
@@ -89,7 +90,10 @@ class BoussinesqSimulation:
                                 sponge_z_start=None if sponge_z_start is None else float(sponge_z_start),
                                 sponge_target=None if sponge_target is None else
                                 {k: np.asarray(v, dtype=np.float64).tolist() for k, v in dict(sponge_target).items()},
-                                bcs=list(self.bcs), theta_bcs=list(self.theta_bcs))
+                                bcs=list(self.bcs), theta_bcs=list(self.theta_bcs), C_s=float(C_s),
+                                Pr_t=float(Pr_t))
+        if C_s:
+            L.check(L.init().aeo_stepper_set_sgs(ctypes.c_void_p(self._h), float(C_s), float(Pr_t)))
         if f_coriolis:
             L.check(L.init().aeo_stepper_set_coriolis(ctypes.c_void_p(self._h), float(f_coriolis)))
         z_top = grid.z0 + grid.Lz

@@ -100,6 +100,7 @@ struct aeo_grid {
     double dx, dy, dz;
     int stretched;
     std::vector<double> idzc, idzf, wlo, whi, alo, ahi;  // host, storage levels
+    std::vector<double> dzc, dzf;                         // host, storage levels (Nz+2H, Nz+2H+1)
     double *d64[6] = {};                                 // device copies
     float *d32[6] = {};
     Metrics<double> m64;
@@ -142,6 +143,8 @@ extern "C" int aeo_grid_create(int Nx, int Ny, int Nz, int H, double dx, double 
         g->stretched = 0;
         for (int k = 0; k < n; k++) g->idzc[k] = 1 / dz;
         for (int k = 0; k <= n; k++) g->idzf[k] = 1 / dz;
+        g->dzc.assign(n, dz);
+        g->dzf.assign(n + 1, dz);
     } else {
         g->stretched = 1;
         for (int k = 0; k < Nz; k++)
@@ -158,6 +161,8 @@ extern "C" int aeo_grid_create(int Nx, int Ny, int Nz, int H, double dx, double 
         dzf[0] = dzc[0];
         for (int k = 1; k < n; k++) dzf[k] = zc[k] - zc[k - 1];
         dzf[n] = dzc[n - 1];
+        g->dzc = dzc;
+        g->dzf = dzf;
         // level_metrics, :570-600
         for (int k = 0; k < n; k++) g->idzc[k] = 1 / dzc[k];
         for (int k = 0; k <= n; k++) g->idzf[k] = 1 / dzf[k];
@@ -932,6 +937,10 @@ struct aeo_stepper {
     void *ta, *tb;       // θ stage buffers
     double fcor = 0;     // f-plane Coriolis parameter
     void *spg = nullptr; // sponge rate and target vectors (state precision), 5 × (Nz + 2H)
+    // static Smagorinsky: νt (storage, state precision), per-level (C_s Δ_k)² and
+    // the off-diagonal vertical metric (Nz + 2H each), 1/Pr_t
+    void *nut = nullptr, *sgs_lv = nullptr;
+    double sgs_s = 0;
 };
 
 extern "C" int aeo_stepper_create(aeo_grid *g, const aeo_step_params *sp, aeo_poisson *solver, aeo_stepper **out) {
@@ -975,7 +984,48 @@ extern "C" int aeo_stepper_destroy(aeo_stepper *h) {
     void *bufs[8] = {h->us, h->vs, h->ws, h->ua, h->va, h->wa, h->ta, h->tb};
     for (auto b : bufs) hipFree(b);
     if (h->spg) hipFree(h->spg);
+    if (h->nut) hipFree(h->nut);
+    if (h->sgs_lv) hipFree(h->sgs_lv);
     delete h;
+    return AEO_OK;
+}
+
+// C_s and Pr_t of BoussinesqSimulation (src/Simulations.jl:237-238, 270):
+// the built-in closure StaticSmagorinsky(C_s) of _active_sgs
+// (src/operators/sgs_closures.jl:309-318), with sgs_stress = :laplacian
+extern "C" int aeo_stepper_set_sgs(aeo_stepper *h, double C_s, double Pr_t) {
+    NEED_CTX();
+    if (!h) return fail(AEO_ERR_ARG, "null argument");
+    if (!std::isfinite(C_s) || !std::isfinite(Pr_t) || Pr_t == 0)
+        return fail(AEO_ERR_ARG, "C_s must be finite and Pr_t finite and non-zero");
+    HIPCHK(hipStreamSynchronize(g_ctx->stream));
+    if (h->sgs_lv) { hipFree(h->sgs_lv); h->sgs_lv = nullptr; }
+    if (C_s == 0) return AEO_OK;  // has_sgs = C_s ≠ 0
+    const aeo_grid *g = h->g;
+    const int n = g->Nz + 2 * g->H;
+    std::vector<double> v(2 * (size_t)n, 0.0);
+    for (int k = 0; k < n; k++) {
+        // _filter_width: (Δx Δy Δz)^(1/3) (UniformZ) or (Δx Δy Δz_c[k])^(1/3) (LevelZ); coef = (C_s Δ)^2
+        const double D = g->stretched ? std::pow(g->dx * g->dy * g->dzc[k], 1.0 / 3)
+                                      : std::pow(g->dx * g->dy * g->dz, 1.0 / 3);
+        const double cd = C_s * D;
+        v[k] = cd * cd;
+        // _idz_off: 1/Δz (UniformZ) or 2/(Δz_f[k] + Δz_f[k+1]) (LevelZ)
+        v[n + k] = g->stretched ? 2 / (g->dzf[k] + g->dzf[k + 1]) : 1 / g->dz;
+    }
+    if (!h->nut) {
+        HIPCHK(hipMalloc(&h->nut, h->n * h->esz));
+        HIPCHK(hipMemset(h->nut, 0, h->n * h->esz));
+    }
+    if (h->sp.precision == AEO_F64) {
+        HIPCHK(hipMalloc(&h->sgs_lv, v.size() * sizeof(double)));
+        HIPCHK(hipMemcpy(h->sgs_lv, v.data(), v.size() * sizeof(double), hipMemcpyHostToDevice));
+    } else {
+        std::vector<float> f(v.begin(), v.end());
+        HIPCHK(hipMalloc(&h->sgs_lv, f.size() * sizeof(float)));
+        HIPCHK(hipMemcpy(h->sgs_lv, f.data(), f.size() * sizeof(float), hipMemcpyHostToDevice));
+    }
+    h->sgs_s = 1 / Pr_t;
     return AEO_OK;
 }
 
@@ -1069,11 +1119,31 @@ static int substage(aeo_stepper *h, int stage, T *u, T *v, T *w, T *th, T *p, do
     }
     StepPhys ph{sp.nu,       dt, sp.theta0, sp.g_accel, sp.flux_bot, sp.flux_top, g->idzc[H], g->idzc[H + Nz - 1],
                 h->fcor, h->spg};
+    if (h->sgs_lv) {
+        // the closure's νt from the stage state, then its halo (the velocity BCs at a cell centre)
+        Timed t("sgs");
+        const int n = Nz + 2 * H;
+        launch_smag_nut<T>(st, metrics<T>(g), (T *)h->nut, ui, vi, wi, (const T *)h->sgs_lv,
+                           (const T *)h->sgs_lv + n);
+        int bx = sp.bc_x == AEO_PERIODIC ? BC_PERIODIC : BC_NOFLUX;
+        int by = sp.bc_y == AEO_PERIODIC ? BC_PERIODIC : BC_NOFLUX;
+        int bz = sp.bc_z == AEO_PERIODIC ? BC_PERIODIC : BC_NOFLUX;
+        launch_halo<T>(st, (T *)h->nut, Nx, Ny, Nz, H, LOC_CC, bx, by, bz);
+    }
     {
         Timed t("tendency");
         if (launch_tendency_step<T>(st, metrics<T>(g), sp.mom_scheme, sp.weno_p, ui, vi, wi, ti, us, vs, ws, to, th,
                                     ph, stage))
             return fail(AEO_ERR_ARG, "tendency launch failed");
+    }
+    if (h->sgs_lv) {
+        // u* += Δt·∂_j(νt ∂_j u_i); θ' += b·Δt·(1/Pr_t)∂_j(νt ∂_j θ), b the θ combination's weight
+        Timed t("sgs");
+        double ca_, cb_;
+        int comb_;
+        stage_coeffs<double>(stage, &ph, &ca_, &cb_, &comb_);
+        launch_smag_apply<T>(st, metrics<T>(g), us, vs, ws, to, ui, vi, wi, ti, (const T *)h->nut, dt, cb_,
+                             h->sgs_s);
     }
     {
         Timed t("halo");

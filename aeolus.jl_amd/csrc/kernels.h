@@ -32,6 +32,13 @@ int launch_advect(hipStream_t st, const Metrics<T> &m, int mode, int scheme, int
 template <typename T>
 int launch_tendency_step(hipStream_t st, const Metrics<T> &m, int scheme, int p, const T *u, const T *v, const T *w,
                          const T *s, T *uo, T *vo, T *wo, T *so, const T *s0, const StepPhys &ph, int stage);
+// the static Smagorinsky closure (sgs.hip)
+template <typename T>
+void launch_smag_nut(hipStream_t st, const Metrics<T> &m, T *nut, const T *u, const T *v, const T *w, const T *coef,
+                     const T *idz_off);
+template <typename T>
+void launch_smag_apply(hipStream_t st, const Metrics<T> &m, T *us, T *vs, T *ws, T *so, const T *u, const T *v,
+                       const T *w, const T *th, const T *nut, double dt, double bt, double sc);
 template <typename T>
 void launch_div_into(hipStream_t st, double *buf, const T *u, const T *v, const T *w, const Metrics<double> &md);
 template <typename T>

@@ -386,15 +386,24 @@ __global__ void __launch_bounds__(256) head_back_kernel(const float *__restrict_
     }
     if (threadIdx.x < 18) part[(size_t)blockIdx.x * 18 + threadIdx.x] = sh[threadIdx.x][0];
 }
-// loss (scal[2]) and the head gradients (grad: wh[16], bh)
-__global__ void head_reduce_kernel(const double *__restrict__ part, int nb, int nvox, double *__restrict__ scal,
-                                   float *__restrict__ ghead) {
-    const int q = threadIdx.x;
-    if (q >= 18) return;
+// loss (scal[2]) and the head gradients (grad: wh[16], bh): block q sums
+// partial q over the nb blocks of head_back_kernel, strided over its 256
+// threads, then a fixed-order tree (deterministic)
+__global__ void __launch_bounds__(256) head_reduce_kernel(const double *__restrict__ part, int nb, int nvox,
+                                                          double *__restrict__ scal, float *__restrict__ ghead) {
+    __shared__ double sh[256];
+    const int q = blockIdx.x;
     double s = 0.0;
-    for (int b = 0; b < nb; b++) s += part[(size_t)b * 18 + q];
-    if (q == 0) scal[2] = s / nvox;
-    else ghead[q - 1] = (float)s;  // q-1 = 0..15: wh, 16: bh
+    for (int b = threadIdx.x; b < nb; b += 256) s += part[(size_t)b * 18 + q];
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x != 0) return;
+    if (q == 0) scal[2] = sh[0] / nvox;
+    else ghead[q - 1] = (float)sh[0];  // q-1 = 0..15: wh, 16: bh
 }
 
 // ---------------------------------------------------------------------------
@@ -697,7 +706,7 @@ int unet_backward(UNet *N, hipStream_t st, const double *target) {
     const int nbh = nblk(n0);
     hipLaunchKernelGGL(head_back_kernel, dim3(nbh), dim3(256), 0, st, N->y, target, N->scal, N->a9, N->W + N->headoff,
                        N->g0b16, N->dpart, n0);
-    hipLaunchKernelGGL(head_reduce_kernel, dim3(1), dim3(64), 0, st, N->dpart, nbh, n0, N->scal, G + N->headoff);
+    hipLaunchKernelGGL(head_reduce_kernel, dim3(18), dim3(256), 0, st, N->dpart, nbh, n0, N->scal, G + N->headoff);
     // layer 9 (16 → 16 at full resolution): g9 in g0b16
     auto layer_grads = [&](int l, const bf16 *in, const bf16 *g, Vol gv, int nv) {
         rc |= wgrad_run(st, N, L[l].cinp, L[l].cin, L[l].cout, in, g, gv, G + L[l].woff);

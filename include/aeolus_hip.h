@@ -192,6 +192,13 @@ int aeo_stepper_set_coriolis(aeo_stepper *h, double f);
 int aeo_stepper_set_sponge(aeo_stepper *h, double alpha_max, double z_start, double z_top, const double *zc,
                            const double *zf, const double *target_u, const double *target_v,
                            const double *target_w);
+/* C_s and Pr_t of BoussinesqSimulation (src/Simulations.jl:237-238, 270):
+ * the built-in closure StaticSmagorinsky(C_s) (src/operators/sgs_closures.jl:
+ * 309-318, 403-425) with sgs_stress = :laplacian: νt = (C_s Δ_k)²|S|
+ * (src/FiniteVolume.jl:930-944), ∂_j(νt ∂_j u_i) on u, v, w (:966-1042) and
+ * (1/Pr_t)∂_j(νt ∂_j θ) on θ (:1280-1300), every substage, after ν∇².
+ * C_s = 0 turns the closure off. */
+int aeo_stepper_set_sgs(aeo_stepper *h, double C_s, double Pr_t);
 /* step!(sim, dt) (src/Simulations.jl:474-487): one projected forward-Euler step */
 int aeo_step_euler(aeo_stepper *h, void *u, void *v, void *w, void *theta, void *p, double dt);
 

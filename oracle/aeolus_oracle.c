@@ -918,6 +918,8 @@ typedef struct {
     size_t n, nint;
     double *tu, *tv, *tw, *tt, *u0, *v0, *w0, *t0, *phi, *buf, *pbuf;
     double fcor;                                 /* f_coriolis */
+    double C_s, sgs_s;                           /* Smagorinsky C_s, 1/Pr_t (C_s = 0: off) */
+    double *nut, *sgs_coef, *sgs_ioff;           /* νt (storage), (C_s Δ_k)², off-diagonal 1/Δz (Nz+2H) */
     double sp_alpha, sp_start, sp_top;           /* sponge_α_max, sponge_z_start, z0 + Lz */
     double *sp_zc, *sp_zf, *sp_tu, *sp_tv, *sp_tw; /* Nz each (NULL: no sponge) */
 } orc_stepper;
@@ -943,6 +945,30 @@ void orc_stepper_set_forcing(void *h, double fcor, double alpha_max, double z_st
     for (int q = 0; q < 5; q++) {
         *a[q] = (double *)calloc(Nz, sizeof(double));
         if (src[q]) memcpy(*a[q], src[q], sizeof(double) * Nz);
+    }
+}
+
+/* C_s, Pr_t of BoussinesqSimulation: the built-in StaticSmagorinsky(C_s)
+ * (src/operators/sgs_closures.jl:309-318, 403-425). dzc (Nz+2H), dzf
+ * (Nz+2H+1): the storage-level metrics of a stretched grid, NULL for a
+ * regular one. Filter width _filter_width and the off-diagonal metric
+ * _idz_off as grid_metrics builds them (src/FiniteVolume.jl:145-148,
+ * src/StretchedGrids.jl:576-591). */
+void orc_stepper_set_sgs(void *h, double C_s, double Pr_t, const double *dzc, const double *dzf) {
+    orc_stepper *s = (orc_stepper *)h;
+    const orc_grid *g = &s->g;
+    int n = g->Nz + 2 * g->H;
+    s->C_s = C_s;
+    s->sgs_s = 1 / Pr_t;
+    free(s->sgs_coef); free(s->sgs_ioff);
+    s->sgs_coef = (double *)calloc(n, sizeof(double));
+    s->sgs_ioff = (double *)calloc(n, sizeof(double));
+    if (!s->nut) s->nut = (double *)calloc(s->n, sizeof(double));
+    for (int k = 0; k < n; k++) {
+        double D = dzc ? pow(g->dx * g->dy * dzc[k], 1.0 / 3) : pow(g->dx * g->dy * g->dz, 1.0 / 3);
+        double cd = C_s * D;
+        s->sgs_coef[k] = cd * cd;
+        s->sgs_ioff[k] = dzc ? 2 / (dzf[k] + dzf[k + 1]) : 1 / g->dz;
     }
 }
 
@@ -975,6 +1001,7 @@ void orc_stepper_destroy(void *h) {
     free(s->tu); free(s->tv); free(s->tw); free(s->tt); free(s->u0); free(s->v0); free(s->w0);
     free(s->t0); free(s->phi); free(s->buf); free(s->pbuf);
     free(s->sp_zc); free(s->sp_zf); free(s->sp_tu); free(s->sp_tv); free(s->sp_tw);
+    free(s->nut); free(s->sgs_coef); free(s->sgs_ioff);
     free(s);
 }
 
@@ -982,6 +1009,107 @@ void orc_stepper_destroy(void *h) {
 static inline int interior(const orc_grid *g, int i, int j, int k) {
     int H = g->H;
     return i > H && i <= H + g->Nx && j > H && j <= H + g->Ny && k > H && k <= H + g->Nz;
+}
+
+/* StaticSmagorinsky with sgs_stress = :laplacian: νt = (C_s Δ_k)²|S|
+ * (_b_smagorinsky!, _strain_at, _velocity_gradient_at: src/FiniteVolume.jl:
+ * 930-944, src/Operators.jl:448-496), its halo with the velocity BCs at a
+ * cell centre (a wall mirrors), then _b_eddy_u!/_v!/_w! (src/FiniteVolume.jl:
+ * 966-1042) and _b_eddy_scalar! with s = 1/Pr_t (:1280-1300) */
+static void sgs_terms(orc_stepper *s, const double *u, const double *v, const double *w, const double *th) {
+    const orc_grid *g = &s->g;
+    const orc_step_params *sp = &s->sp;
+    GRID_DIMS(g);
+    int H = g->H;
+    double idx = 1 / g->dx, idy = 1 / g->dy;
+    double *n = s->nut;
+#pragma omp parallel for schedule(static)
+    for (int k = H + 1; k <= H + g->Nz; k++)
+        for (int j = H + 1; j <= H + g->Ny; j++)
+            for (int i = H + 1; i <= H + g->Nx; i++) {
+                double idz = m_idz(g, k), ioff = s->sgs_ioff[k - 1];
+                double du_dx = (u[ID(i + 1, j, k)] - u[ID(i, j, k)]) * idx;
+                double dv_dy = (v[ID(i, j + 1, k)] - v[ID(i, j, k)]) * idy;
+                double dw_dz = (w[ID(i, j, k + 1)] - w[ID(i, j, k)]) * idz;
+                double du_dy = ((u[ID(i, j + 1, k)] + u[ID(i + 1, j + 1, k)]) - (u[ID(i, j - 1, k)] + u[ID(i + 1, j - 1, k)])) * (0.25 * idy);
+                double dv_dx = ((v[ID(i + 1, j, k)] + v[ID(i + 1, j + 1, k)]) - (v[ID(i - 1, j, k)] + v[ID(i - 1, j + 1, k)])) * (0.25 * idx);
+                double du_dz = ((u[ID(i, j, k + 1)] + u[ID(i + 1, j, k + 1)]) - (u[ID(i, j, k - 1)] + u[ID(i + 1, j, k - 1)])) * (0.25 * ioff);
+                double dw_dx = ((w[ID(i + 1, j, k)] + w[ID(i + 1, j, k + 1)]) - (w[ID(i - 1, j, k)] + w[ID(i - 1, j, k + 1)])) * (0.25 * idx);
+                double dv_dz = ((v[ID(i, j, k + 1)] + v[ID(i, j + 1, k + 1)]) - (v[ID(i, j, k - 1)] + v[ID(i, j + 1, k - 1)])) * (0.25 * ioff);
+                double dw_dy = ((w[ID(i, j + 1, k)] + w[ID(i, j + 1, k + 1)]) - (w[ID(i, j - 1, k)] + w[ID(i, j - 1, k + 1)])) * (0.25 * idy);
+                double S12 = 0.5 * (du_dy + dv_dx), S13 = 0.5 * (du_dz + dw_dx), S23 = 0.5 * (dv_dz + dw_dy);
+                double S2 = 2 * ((du_dx * du_dx + dv_dy * dv_dy) + dw_dz * dw_dz) + 4 * ((S12 * S12 + S13 * S13) + S23 * S23);
+                n[ID(i, j, k)] = s->sgs_coef[k - 1] * sqrt(S2);
+            }
+    if (!th) return; /* νt only (orc_stepper_sgs_nut) */
+    int bx = sp->bcx == ORC_PERIODIC ? ORC_PERIODIC : ORC_NOFLUX;
+    int by = sp->bcy == ORC_PERIODIC ? ORC_PERIODIC : ORC_NOFLUX;
+    int bz = sp->bcz == ORC_PERIODIC ? ORC_PERIODIC : ORC_NOFLUX;
+    orc_fill_halos(n, g, ORC_CC, bx, by, bz);
+#define NU(a, b, c) n[ID(a, b, c)]
+#pragma omp parallel for schedule(static)
+    for (int k = H + 1; k <= H + g->Nz; k++)
+        for (int j = H + 1; j <= H + g->Ny; j++)
+            for (int i = H + 1; i <= H + g->Nx; i++) {
+                size_t c = ID(i, j, k);
+                double idzk = m_idz(g, k), idzf_k = m_idzf(g, k), idzf_kp = m_idzf(g, k + 1);
+                {
+                    double fxE = NU(i, j, k) * (u[ID(i + 1, j, k)] - u[c]) * idx;
+                    double fxW = NU(i - 1, j, k) * (u[c] - u[ID(i - 1, j, k)]) * idx;
+                    double eN = 0.25 * (((NU(i - 1, j, k) + NU(i, j, k)) + NU(i - 1, j + 1, k)) + NU(i, j + 1, k));
+                    double eS = 0.25 * (((NU(i - 1, j - 1, k) + NU(i, j - 1, k)) + NU(i - 1, j, k)) + NU(i, j, k));
+                    double fyN = eN * (u[ID(i, j + 1, k)] - u[c]) * idy;
+                    double fyS = eS * (u[c] - u[ID(i, j - 1, k)]) * idy;
+                    double eT = 0.25 * (((NU(i - 1, j, k) + NU(i, j, k)) + NU(i - 1, j, k + 1)) + NU(i, j, k + 1));
+                    double eB = 0.25 * (((NU(i - 1, j, k - 1) + NU(i, j, k - 1)) + NU(i - 1, j, k)) + NU(i, j, k));
+                    double fzT = eT * (u[ID(i, j, k + 1)] - u[c]) * idzf_kp;
+                    double fzB = eB * (u[c] - u[ID(i, j, k - 1)]) * idzf_k;
+                    s->tu[c] += ((fxE - fxW) * idx + (fyN - fyS) * idy) + (fzT - fzB) * idzk;
+                }
+                {
+                    double fyN = NU(i, j, k) * (v[ID(i, j + 1, k)] - v[c]) * idy;
+                    double fyS = NU(i, j - 1, k) * (v[c] - v[ID(i, j - 1, k)]) * idy;
+                    double eE = 0.25 * (((NU(i, j - 1, k) + NU(i + 1, j - 1, k)) + NU(i, j, k)) + NU(i + 1, j, k));
+                    double eW = 0.25 * (((NU(i - 1, j - 1, k) + NU(i, j - 1, k)) + NU(i - 1, j, k)) + NU(i, j, k));
+                    double fxE = eE * (v[ID(i + 1, j, k)] - v[c]) * idx;
+                    double fxW = eW * (v[c] - v[ID(i - 1, j, k)]) * idx;
+                    double eT = 0.25 * (((NU(i, j - 1, k) + NU(i, j, k)) + NU(i, j - 1, k + 1)) + NU(i, j, k + 1));
+                    double eB = 0.25 * (((NU(i, j - 1, k - 1) + NU(i, j, k - 1)) + NU(i, j - 1, k)) + NU(i, j, k));
+                    double fzT = eT * (v[ID(i, j, k + 1)] - v[c]) * idzf_kp;
+                    double fzB = eB * (v[c] - v[ID(i, j, k - 1)]) * idzf_k;
+                    s->tv[c] += ((fxE - fxW) * idx + (fyN - fyS) * idy) + (fzT - fzB) * idzk;
+                }
+                {
+                    double fzT = NU(i, j, k) * (w[ID(i, j, k + 1)] - w[c]) * idzk;
+                    double fzB = NU(i, j, k - 1) * (w[c] - w[ID(i, j, k - 1)]) * m_idz(g, k - 1);
+                    double eE = 0.25 * (((NU(i, j, k - 1) + NU(i + 1, j, k - 1)) + NU(i, j, k)) + NU(i + 1, j, k));
+                    double eW = 0.25 * (((NU(i - 1, j, k - 1) + NU(i, j, k - 1)) + NU(i - 1, j, k)) + NU(i, j, k));
+                    double fxE = eE * (w[ID(i + 1, j, k)] - w[c]) * idx;
+                    double fxW = eW * (w[c] - w[ID(i - 1, j, k)]) * idx;
+                    double eN = 0.25 * (((NU(i, j, k - 1) + NU(i, j + 1, k - 1)) + NU(i, j, k)) + NU(i, j + 1, k));
+                    double eS = 0.25 * (((NU(i, j - 1, k - 1) + NU(i, j, k - 1)) + NU(i, j - 1, k)) + NU(i, j, k));
+                    double fyN = eN * (w[ID(i, j + 1, k)] - w[c]) * idy;
+                    double fyS = eS * (w[c] - w[ID(i, j - 1, k)]) * idy;
+                    s->tw[c] += ((fxE - fxW) * idx + (fyN - fyS) * idy) + (fzT - fzB) * idzf_k;
+                }
+                {
+                    double nE = 0.5 * (NU(i, j, k) + NU(i + 1, j, k)), nW = 0.5 * (NU(i - 1, j, k) + NU(i, j, k));
+                    double nN = 0.5 * (NU(i, j, k) + NU(i, j + 1, k)), nS = 0.5 * (NU(i, j - 1, k) + NU(i, j, k));
+                    double nT = 0.5 * (NU(i, j, k) + NU(i, j, k + 1)), nB = 0.5 * (NU(i, j, k - 1) + NU(i, j, k));
+                    double fxE = nE * (th[ID(i + 1, j, k)] - th[c]) * idx, fxW = nW * (th[c] - th[ID(i - 1, j, k)]) * idx;
+                    double fyN = nN * (th[ID(i, j + 1, k)] - th[c]) * idy, fyS = nS * (th[c] - th[ID(i, j - 1, k)]) * idy;
+                    double fzT = nT * (th[ID(i, j, k + 1)] - th[c]) * idzf_kp, fzB = nB * (th[c] - th[ID(i, j, k - 1)]) * idzf_k;
+                    s->tt[c] += s->sgs_s * (((fxE - fxW) * idx + (fyN - fyS) * idy) + (fzT - fzB) * idzk);
+                }
+            }
+#undef NU
+}
+
+/* the interior νt of the closure for the given velocity (halos as given) */
+void orc_stepper_sgs_nut(void *h, const double *u, const double *v, const double *w, double *out) {
+    orc_stepper *s = (orc_stepper *)h;
+    sgs_terms(s, u, v, w, NULL);
+    memcpy(out, s->nut, sizeof(double) * s->n);
 }
 
 /* One projected forward-Euler substage; stage = 0 (none), 2 (3/4, 1/4), 3 (1/3, 2/3). */
@@ -1028,6 +1156,14 @@ static void euler_projection(orc_stepper *s, double *u, double *v, double *w, do
                     s->tw[c] = s->tw[c] + sp->nu * lap_at(g, sx, sy, i, j, k, w, ORC_FZ);
                     s->tt[c] = s->tt[c] + sp->nu * lap_at(g, sx, sy, i, j, k, th, ORC_CC);
                 }
+            }
+    /* 4. the SGS closure: νt, its halo, the eddy terms (src/Simulations.jl:579-590) */
+    if (s->C_s != 0) sgs_terms(s, u, v, w, th);
+#pragma omp parallel for schedule(static)
+    for (int k = H + 1; k <= H + g->Nz; k++)
+        for (int j = H + 1; j <= H + g->Ny; j++)
+            for (int i = H + 1; i <= H + g->Nx; i++) {
+                size_t c = ID(i, j, k);
                 /* 5. buoyancy, _b_buoyancy!, src/FiniteVolume.jl:838-844 */
                 if (sp->theta0 != 0) {
                     double coef = sp->g_accel / sp->theta0;

@@ -81,6 +81,8 @@ def lib():
             "orc_stepper_create": (vp, [pg, P(OrcStep), vp]),
             "orc_stepper_destroy": (None, [vp]),
             "orc_stepper_set_forcing": (None, [vp, d, d, d, d, pd, pd, pd, pd, pd]),
+            "orc_stepper_set_sgs": (None, [vp, d, d, pd, pd]),
+            "orc_stepper_sgs_nut": (None, [vp, pd, pd, pd, pd]),
             "orc_step_rk3": (None, [vp, pd, pd, pd, pd, pd, d]),
             "orc_step_euler": (None, [vp, pd, pd, pd, pd, pd, d]),
             "orc_mg_create": (vp, [i, i, i, d, d, d, i, i, i, pd, pd, pd, pd, i, i, i, i, d]),
@@ -300,6 +302,17 @@ class Stepper:
         self._forcing = keep
         lib().orc_stepper_set_forcing(self.h, fcor, alpha_max, z_start, z_top,
                                       *[None if a is None else ptr(a) for a in keep])
+
+    def set_sgs(self, C_s, Pr_t=1 / 3):
+        """The built-in StaticSmagorinsky(C_s) closure (C_s = 0: off)."""
+        m = self.g.m
+        self._sgs = None if m is None else (np.ascontiguousarray(m["dzc"]), np.ascontiguousarray(m["dzf"]))
+        lib().orc_stepper_set_sgs(self.h, C_s, Pr_t, *((None, None) if m is None else [ptr(a) for a in self._sgs]))
+
+    def sgs_nut(self, u, v, w):
+        out = self.g.zeros()
+        lib().orc_stepper_sgs_nut(self.h, ptr(u), ptr(v), ptr(w), ptr(out))
+        return out
 
     def rk3(self, u, v, w, th, p, dt):
         lib().orc_step_rk3(self.h, ptr(u), ptr(v), ptr(w), ptr(th), ptr(p), dt)

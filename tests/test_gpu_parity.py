@@ -163,7 +163,7 @@ def test_density_weighted_column_parity(hip, orc):
     assert rel_err(hip.solve_interior(s, src), o.solve(src)) <= TOL64
 
 
-def _run_step(hip, orc, kind, scheme, steps, prec=0, N=None, euler=False, forcing=None):
+def _run_step(hip, orc, kind, scheme, steps, prec=0, N=None, euler=False, forcing=None, C_s=0.0):
     from aeolus_hip import workloads as W
     if kind == "tgv":
         N = N or 32
@@ -190,9 +190,13 @@ def _run_step(hip, orc, kind, scheme, steps, prec=0, N=None, euler=False, forcin
         tu = np.linspace(1.0, 3.0, g.Nz)
         fk = dict(f_coriolis=forcing, sponge_alpha_max=0.02, sponge_z_start=0.6 * g.Lz,
                   sponge_target=dict(u=tu, v=-0.5))
+    if C_s:
+        fk["C_s"] = C_s
     sim = hip.BoussinesqSimulation(g, scheme=scheme, precision=prec, **phys, **fk)
     sim.set(*state)
     ost = orc.Stepper(og, scheme=scheme, **{k: v for k, v in phys.items()})
+    if C_s:
+        ost.set_sgs(C_s)
     if forcing:
         zc, zf = damping_coordinates(g)
         ost.set_forcing(forcing, 0.02, 0.6 * g.Lz, g.z0 + g.Lz, zc, zf, (tu, np.full(g.Nz, -0.5), None))
@@ -221,6 +225,14 @@ def test_rk3_step_parity_fp64(hip, orc, kind, scheme, N):
 def test_rk3_step_coriolis_sponge_parity_fp64(hip, orc, kind):
     # f-plane Coriolis and the Rayleigh sponge toward level-vector targets
     got, ref, og = _run_step(hip, orc, kind, "weno_z5", steps=2, forcing=0.3 if kind == "tgv" else 1e-3)
+    for n in ("u", "v", "w", "theta", "p"):
+        assert rel_err(got[n], ref[n]) <= TOL64, n
+
+
+@pytest.mark.parametrize("kind", ["tgv", "dcbl"])
+def test_rk3_step_smagorinsky_parity_fp64(hip, orc, kind):
+    # the built-in StaticSmagorinsky(C_s) closure (uniform and stretched z)
+    got, ref, og = _run_step(hip, orc, kind, "weno_z5", steps=2, C_s=0.17)
     for n in ("u", "v", "w", "theta", "p"):
         assert rel_err(got[n], ref[n]) <= TOL64, n
 
