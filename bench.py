@@ -176,6 +176,25 @@ def poisson_legs(reps=3, weights=None, log=print):
     return out
 
 
+def read_valu_insts():
+    """VALU wave-instructions per tendency launch from the committed
+    rocprofv3 --pmc SQ summary (SQ_INSTS_VALU per dispatch)."""
+    path = os.path.join(ROOT, "profiles", "step_dcbl_256x256x128_f64_sq.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("void aeo::adv_march_kernel<2, 4, 2, double"):
+            return float(line.split()[-7])
+    return None
+
+
+# one wave64 VALU instruction occupies a 16-lane SIMD for 4 cycles; MI355X:
+# 256 CUs x 4 SIMDs at 2.4 GHz
+VALU_CYCLES_PER_INST = 4
+N_SIMD = 1024
+CLOCK_HZ = 2.4e9
+
+
 def learned_ms(legs):
     """The fastest converged leg that uses the network (None without one)."""
     if not legs:
@@ -254,6 +273,12 @@ def main():
     alg = tendency_bytes_per_launch(8)
     achieved = alg / avg_launch_s / 1e9
     traffic = read_traffic()
+    valu_insts = read_valu_insts()
+    valu = None
+    if valu_insts:
+        floor_s = valu_insts * VALU_CYCLES_PER_INST / (N_SIMD * CLOCK_HZ)
+        valu = {"insts_per_launch": valu_insts, "issue_floor_ms": floor_s * 1e3, "frac": floor_s / avg_launch_s,
+                "note": "FP64 VALU issue: the kernel's actual bound (SQ_INSTS_VALU from profiles/)"}
 
     cpu = None
     if rank == 0 and not args.no_cpu:
@@ -294,7 +319,8 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "adv_march_kernel<MODE_STEP, WENO-Z5, p=2, double> (fused tendency+predictor)",
-                     "avg_launch_ms": avg_launch_s * 1e3, "algorithmic_bytes_per_launch": alg},
+                     "avg_launch_ms": avg_launch_s * 1e3, "algorithmic_bytes_per_launch": alg,
+                     "valu": valu},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
