@@ -71,14 +71,41 @@ def _pad_face(vals, H, Nz):
     return out
 
 
+def _buoyancy_frequency_squared(t_below, t_above, dz, g):
+    """buoyancy_frequency_squared, logarithmic form (src/TurbulenceStability.jl:134-135)."""
+    return g * (math.log(t_above) - math.log(t_below)) / dz
+
+
+def _brunt_vaisala(theta, dz, g):
+    """_brunt_vaisala (src/Anelastic.jl:326-338): N² = g ∂(log θ̄)/∂z,
+    one-sided at the ends, centred over 2Δz inside."""
+    Nz = len(theta)
+    n2 = np.zeros(Nz)
+    if Nz == 1:
+        return n2
+    n2[0] = _buoyancy_frequency_squared(theta[0], theta[1], dz, g)
+    for k in range(1, Nz - 1):
+        n2[k] = _buoyancy_frequency_squared(theta[k - 1], theta[k + 1], 2 * dz, g)
+    n2[Nz - 1] = _buoyancy_frequency_squared(theta[Nz - 2], theta[Nz - 1], dz, g)
+    return n2
+
+
 class ReferenceState:
-    """Cell and face profiles of a base state and their padded twins."""
+    """Cell and face profiles of a base state and their padded twins
+    (_assemble_reference, src/Anelastic.jl:346-405): ρ̄, θ̄, p̄, Exner π̄ =
+    (p̄/p₀)^(R/cₚ), T̄ = θ̄π̄, N², the pseudo-incompressible weight P = ρ̄θ̄;
+    faces (log mean ρ̄, arithmetic θ̄, P_f = ρ̄_f θ̄_f) and padded twins."""
 
     def __init__(self, Nz, dz, H, theta, p, rho, R_d=R_DRY, cp=CP_DRY, p0=P_REF, g=G_STD):
         self.Nz, self.dz, self.H, self.g = Nz, dz, H, g
+        self.R_d, self.cp, self.p0 = R_d, cp, p0
         self.rho, self.theta, self.p = (np.asarray(a, dtype=np.float64) for a in (rho, theta, p))
         if not (np.all(self.rho > 0) and np.all(self.theta > 0)):
             raise ValueError("reference density and potential temperature must be strictly positive")
+        self.pi = np.array([(pk / p0) ** (R_d / cp) for pk in self.p])
+        self.T = self.theta * self.pi
+        self.N2 = _brunt_vaisala(self.theta, dz, g)
+        self.P = self.rho * self.theta
         rf = np.empty(Nz + 1)
         tf = np.empty(Nz + 1)
         rf[0], tf[0] = self.rho[0], self.theta[0]
@@ -91,6 +118,105 @@ class ReferenceState:
         self.theta_c_pad = _pad_center(self.theta, H)
         self.rho_f_pad = _pad_face(rf, H, Nz)
         self.theta_f_pad = _pad_face(tf, H, Nz)
+        self.P_f = rf * tf
+        self.Pc_pad = self.rho_c_pad * self.theta_c_pad
+        self.Pf_pad = self.rho_f_pad * self.theta_f_pad
+
+    def weights(self, weighting="anelastic"):
+        """(Wc, Wf) interior profiles and their padded twins of a mass
+        weighting: W = ρ̄ (anelastic) or ρ̄θ̄ (pseudo-incompressible);
+        _weight_c/_weight_f (src/Anelastic.jl:1529-1532)."""
+        if weighting == "anelastic":
+            return self.rho, self.rho_f, self.rho_c_pad, self.rho_f_pad
+        if weighting == "pseudo_incompressible":
+            return self.P, self.P_f, self.Pc_pad, self.Pf_pad
+        raise ValueError(f"unknown weighting {weighting!r} (anelastic, pseudo_incompressible)")
+
+    def hydrostatic_residual(self):
+        """(p̄[k] − p̄[k−1])/Δz + g ρ̄_f[k] at the interior faces
+        (hydrostatic_residual, src/Anelastic.jl:627-635)."""
+        return (self.p[1:] - self.p[:-1]) / self.dz + self.g * self.rho_f[1:self.Nz]
+
+    def max_hydrostatic_residual(self):
+        r = self.hydrostatic_residual()
+        return float(np.abs(r).max()) if r.size else 0.0
+
+    def scale_height(self):
+        """reference_scale_height (src/Anelastic.jl:655-660)."""
+        if self.Nz < 2:
+            return math.inf
+        return -((self.Nz - 1) * self.dz) / math.log(self.rho[-1] / self.rho[0])
+
+
+def _density(p, theta, R, cp, p0):
+    """_density (src/Anelastic.jl:312-313): ρ = (p₀/(Rθ)) (p/p₀)^{cᵥ/cₚ}."""
+    return (p0 / (R * theta)) * (p / p0) ** ((cp - R) / cp)
+
+
+def _integrate_hydrostatic(Nz, dz, theta, p_surf, R_d, cp, p0, g):
+    """_integrate_hydrostatic (src/Anelastic.jl:409-445): level 1 by the
+    half-cell balance, then the two-point balance with the log-mean face
+    density, each by fixed-point iteration to 8 eps."""
+    eps = np.finfo(np.float64).eps
+    pb = np.empty(Nz)
+    rb = np.empty(Nz)
+    p = p_surf
+    for _ in range(200):
+        rho = _density(p, theta[0], R_d, cp, p0)
+        pn = p_surf - g * (dz / 2) * rho
+        d = abs(pn - p)
+        p = pn
+        if d <= 8 * eps * abs(p):
+            break
+    pb[0] = p
+    rb[0] = _density(p, theta[0], R_d, cp, p0)
+    for k in range(1, Nz):
+        p = pb[k - 1] - g * dz * rb[k - 1]
+        for _ in range(200):
+            rho = _density(p, theta[k], R_d, cp, p0)
+            pn = pb[k - 1] - g * dz * _logmean(rb[k - 1], rho)
+            if not pn > 0:
+                raise ValueError(f"hydrostatic integration produced non-positive pressure at k={k + 1}; "
+                                 "domain is too deep for the prescribed θ̄ / p_surf")
+            d = abs(pn - p)
+            p = pn
+            if d <= 8 * eps * abs(p):
+                break
+        pb[k] = p
+        rb[k] = _density(p, theta[k], R_d, cp, p0)
+    return pb, rb
+
+
+def isentropic_reference(Nz, dz, H, theta_const=300.0, p_surf=P_REF, R_d=R_DRY, cp=CP_DRY, p0=P_REF, g=G_STD):
+    """isentropic_reference (src/Anelastic.jl:493-504): θ̄ constant, p̄ and ρ̄
+    from the discrete hydrostatic integration."""
+    theta = np.full(Nz, float(theta_const))
+    pb, rb = _integrate_hydrostatic(Nz, dz, theta, float(p_surf), R_d, cp, p0, g)
+    return ReferenceState(Nz, dz, H, theta, pb, rb, R_d, cp, p0, g)
+
+
+def constant_stability_reference(Nz, dz, H, N=0.01, theta_surf=300.0, z0=0.0, p_surf=P_REF, R_d=R_DRY, cp=CP_DRY,
+                                 p0=P_REF, g=G_STD):
+    """constant_stability_reference (src/Anelastic.jl:521-536): θ̄ = θ_s e^{N²ζ/g}."""
+    n2 = N * N
+    zeta = (z0 + (np.arange(Nz) + 0.5) * dz) - z0
+    theta = np.array([theta_surf * math.exp(n2 * z / g) for z in zeta])
+    pb, rb = _integrate_hydrostatic(Nz, dz, theta, float(p_surf), R_d, cp, p0, g)
+    return ReferenceState(Nz, dz, H, theta, pb, rb, R_d, cp, p0, g)
+
+
+def reference_from_profile(Nz, dz, H, theta_profile, p_surf=P_REF, z0=0.0, R_d=R_DRY, cp=CP_DRY, p0=P_REF, g=G_STD):
+    """reference_from_profile (src/Anelastic.jl:551-568): θ̄ from a function
+    of height (sampled at the cell centres) or an Nz-vector, integrated
+    hydrostatically."""
+    if callable(theta_profile):
+        theta = np.array([float(theta_profile(z)) for z in z0 + (np.arange(Nz) + 0.5) * dz])
+    else:
+        theta = np.asarray(theta_profile, dtype=np.float64)
+        if theta.shape != (Nz,):
+            raise ValueError(f"θ_profile has length {theta.size}, expected Nz={Nz}")
+    pb, rb = _integrate_hydrostatic(Nz, dz, theta, float(p_surf), R_d, cp, p0, g)
+    return ReferenceState(Nz, dz, H, theta, pb, rb, R_d, cp, p0, g)
 
 
 def isothermal_reference(Nz, dz, H, z0=0.0, T_surf=288.0, p_surf=P_REF, R_d=R_DRY, cp=CP_DRY, p0=P_REF, g=G_STD):
@@ -110,15 +236,19 @@ class AnelasticSimulation:
     projected with the density-weighted column solver (DirectWeightedPoissonSolver,
     src/Anelastic.jl:987-1050)."""
 
+    weighting = "anelastic"
+
     def __init__(self, grid, ref, nu=0.0, flux_bot=0.0, flux_top=0.0, buoyancy=True,
-                 bcs=("periodic", "periodic", "wall"), theta_bcs=("periodic", "periodic", "noflux"), precision=L.F64):
+                 bcs=("periodic", "periodic", "wall"), theta_bcs=("periodic", "periodic", "noflux"), precision=L.F64,
+                 f_coriolis=0.0, sponge_alpha_max=0.0, sponge_z_start=None, sponge_target=None):
         if grid.zf is not None:
             raise ValueError("the sound-proof core runs on regular grids")
         self.grid, self.ref, self.precision = grid, ref, precision
         self.bcs, self.theta_bcs = bcs, theta_bcs
         N = (grid.Nx, grid.Ny, grid.Nz)
         lat = tuple("periodic" if b == "periodic" else "neumann" for b in bcs[:2])
-        self.solver = ColumnTridiagonalSolver.density_weighted(N, (grid.dx, grid.dy), grid.dz, ref.rho, ref.rho_f,
+        wc, wf, wc_pad, wf_pad = ref.weights(self.weighting)
+        self.solver = ColumnTridiagonalSolver.density_weighted(N, (grid.dx, grid.dy), grid.dz, wc, wf,
                                                                bc_x=lat[0], bc_y=lat[1])
         from .fields import BC_NAMES
         sp = L.MWParams()
@@ -127,8 +257,7 @@ class AnelasticSimulation:
         sp.bc_x, sp.bc_y, sp.bc_z = (BC_NAMES[b] for b in bcs)
         sp.tbc_x, sp.tbc_y, sp.tbc_z = (BC_NAMES[b] for b in theta_bcs)
         sp.precision = precision
-        self._prof = [np.ascontiguousarray(a, dtype=np.float64) for a in (ref.rho_c_pad, ref.rho_f_pad,
-                                                                          ref.theta_f_pad)]
+        self._prof = [np.ascontiguousarray(a, dtype=np.float64) for a in (wc_pad, wf_pad, ref.theta_f_pad)]
         if any(len(a) != grid.Nz + 2 * grid.H for a in self._prof):
             raise ValueError("reference profiles must be padded with the grid's halo")
         h = ctypes.c_void_p()
@@ -138,12 +267,25 @@ class AnelasticSimulation:
         self._h = h.value
         self.u, self.v, self.w, self.theta, self.p = (DeviceArray(grid.storage, precision) for _ in range(5))
         # device copies of the padded weights for the diagnostic divergence (one spare entry)
-        self._wc = DeviceArray.from_numpy(np.append(ref.rho_c_pad, ref.rho_c_pad[-1]))
-        self._wf = DeviceArray.from_numpy(np.append(ref.rho_f_pad, ref.rho_f_pad[-1]))
+        self._wc = DeviceArray.from_numpy(np.append(wc_pad, wc_pad[-1]))
+        self._wf = DeviceArray.from_numpy(np.append(wf_pad, wf_pad[-1]))
         self.t = 0.0
         self._parameters = dict(nu=float(nu), g_accel=float(ref.g), flux_bot=float(flux_bot),
                                 flux_top=float(flux_top), buoyancy=bool(buoyancy), bcs=list(bcs),
-                                theta_bcs=list(theta_bcs))
+                                theta_bcs=list(theta_bcs), f_coriolis=float(f_coriolis),
+                                sponge_alpha_max=float(sponge_alpha_max),
+                                sponge_z_start=None if sponge_z_start is None else float(sponge_z_start))
+        if f_coriolis:
+            L.check(L.init().aeo_mw_stepper_set_coriolis(ctypes.c_void_p(self._h), float(f_coriolis)))
+        z_top = grid.z0 + grid.Lz
+        z_start = z_top if sponge_z_start is None else float(sponge_z_start)
+        if sponge_alpha_max != 0 and z_start < z_top:
+            from .simulation import damping_coordinates, sponge_levels
+            zc, zf = damping_coordinates(grid)
+            self._sponge = [zc, zf, *sponge_levels(sponge_target, grid.Nz)]
+            pdn = lambda a: None if a is None else a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+            L.check(L.init().aeo_mw_stepper_set_sponge(ctypes.c_void_p(self._h), float(sponge_alpha_max), z_start,
+                                                       z_top, *[pdn(a) for a in self._sponge]))
 
     def parameters(self):
         """The run's scalar parameters, compared on restart."""
@@ -169,7 +311,8 @@ class AnelasticSimulation:
         self.t += dt
 
     def mass_flux_divergence(self):
-        """∇·(ρ̄u) over the interior (mass_flux_divergence!, src/Anelastic.jl:1118-1123)."""
+        """∇·(Wu) over the interior, W the core's weight (mass_flux_divergence!,
+        src/Anelastic.jl:1118-1123)."""
         g = self.grid
         for a, loc in ((self.u, 1), (self.v, 2), (self.w, 3)):
             fill_halos(g, a, loc, self.bcs)
@@ -185,3 +328,12 @@ class AnelasticSimulation:
                 L._lib.aeo_mw_stepper_destroy(ctypes.c_void_p(self._h))
         except Exception:
             pass
+
+
+class PseudoIncompressibleSimulation(AnelasticSimulation):
+    """The pseudo-incompressible (W = ρ̄θ̄, Durran) sound-proof core:
+    MassWeightedSimulation{PseudoIncompressibleWeighting}
+    (src/Anelastic.jl:1373, 1523-1532), the same step with the weights
+    Pc_pad, Pf_pad."""
+
+    weighting = "pseudo_incompressible"

@@ -96,8 +96,22 @@ __global__ void __launch_bounds__(256) mw_tendency_kernel(T *__restrict__ us, T 
         const T tface = h * (Fo(-sxy) + Fo(0));
         tw += coef * (tface - tb);
     }
+    if (ph.fcor != 0) {
+        // f-plane Coriolis, _b_coriolis! (src/FiniteVolume.jl:900-911)
+        const T f = T(ph.fcor);
+        tu += f * (T(0.25) * (((Vo(-1) + Vo(0)) + Vo(sx - 1)) + Vo(sx)));
+        tv -= f * (T(0.25) * (((Uo(-sx) + Uo(1 - sx)) + Uo(0)) + Uo(1)));
+    }
     if (k == H) tt += T(ph.fb);
     if (k == H + m.Nz - 1) tt -= T(ph.ft);
+    if (ph.sponge) {
+        // Rayleigh sponge −λ(z)(f − f_target), _b_damping! (src/FiniteVolume.jl:639-647)
+        const T *sp = (const T *)ph.sponge;
+        const int sz = m.Nz + 2 * H;
+        tu -= sp[k] * (Uo(0) - sp[2 * sz + k]);
+        tv -= sp[k] * (Vo(0) - sp[3 * sz + k]);
+        tw -= sp[sz + k] * (Wo(0) - sp[4 * sz + k]);
+    }
 #undef Uo
 #undef Vo
 #undef Wo

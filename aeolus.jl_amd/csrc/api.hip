@@ -1039,27 +1039,20 @@ extern "C" int aeo_stepper_set_coriolis(aeo_stepper *h, double f) {
     return AEO_OK;
 }
 
-// sponge_α_max, sponge_z_start, sponge_target of BoussinesqSimulation
-// (src/Simulations.jl:272-274, 334-352) and add_velocity_sponge!
-// (:375-381): the sin² rate of sponge_profile / damping_rate
-// (src/FiniteVolume.jl:524-566) sampled at the levels' heights, evaluated
-// here once with the reference's arithmetic.
-extern "C" int aeo_stepper_set_sponge(aeo_stepper *h, double alpha_max, double z_start, double z_top,
-                                      const double *zc, const double *zf, const double *target_u,
-                                      const double *target_v, const double *target_w) {
-    NEED_CTX();
-    if (!h) return fail(AEO_ERR_ARG, "null argument");
-    if (h->spg) {
-        HIPCHK(hipStreamSynchronize(g_ctx->stream));
-        hipFree(h->spg);
-        h->spg = nullptr;
-    }
+// the sponge's per-level vectors in the state precision, 5 × (Nz + 2H): the
+// sin² rate of sponge_profile / damping_rate (src/FiniteVolume.jl:524-566)
+// at the centre and z-face heights, evaluated once with the reference's
+// arithmetic, then the targets of u, v, w (*out = nullptr: sponge off)
+static int make_sponge(const aeo_grid *g, int precision, double alpha_max, double z_start, double z_top,
+                       const double *zc, const double *zf, const double *target_u, const double *target_v,
+                       const double *target_w, void **out) {
+    *out = nullptr;
     // has_sponge = α_max ≠ 0 && z_start < z_top; otherwise the sponge is off
     if (alpha_max == 0 || !(z_start < z_top)) return AEO_OK;
     if (!zc || !zf) return fail(AEO_ERR_ARG, "the sponge needs the level heights zc and zf");
     if (!std::isfinite(alpha_max) || !std::isfinite(z_start) || !std::isfinite(z_top))
         return fail(AEO_ERR_ARG, "sponge parameters must be finite");
-    const int Nz = h->g->Nz, H = h->g->H, sz = Nz + 2 * H;
+    const int Nz = g->Nz, H = g->H, sz = Nz + 2 * H;
     const double inv_rise = 1 / (z_top - z_start), half_pi = 0.5 * M_PI;
     auto rate = [&](double xi) {
         if (!(xi >= z_start)) return 0.0;
@@ -1074,15 +1067,33 @@ extern "C" int aeo_stepper_set_sponge(aeo_stepper *h, double alpha_max, double z
         v[3 * sz + H + k] = target_v ? target_v[k] : 0.0;
         v[4 * sz + H + k] = target_w ? target_w[k] : 0.0;
     }
-    if (h->sp.precision == AEO_F64) {
-        HIPCHK(hipMalloc(&h->spg, v.size() * sizeof(double)));
-        HIPCHK(hipMemcpy(h->spg, v.data(), v.size() * sizeof(double), hipMemcpyHostToDevice));
+    void *d = nullptr;
+    if (precision == AEO_F64) {
+        HIPCHK(hipMalloc(&d, v.size() * sizeof(double)));
+        HIPCHK(hipMemcpy(d, v.data(), v.size() * sizeof(double), hipMemcpyHostToDevice));
     } else {
         std::vector<float> f(v.begin(), v.end());
-        HIPCHK(hipMalloc(&h->spg, f.size() * sizeof(float)));
-        HIPCHK(hipMemcpy(h->spg, f.data(), f.size() * sizeof(float), hipMemcpyHostToDevice));
+        HIPCHK(hipMalloc(&d, f.size() * sizeof(float)));
+        HIPCHK(hipMemcpy(d, f.data(), f.size() * sizeof(float), hipMemcpyHostToDevice));
     }
+    *out = d;
     return AEO_OK;
+}
+
+// sponge_α_max, sponge_z_start, sponge_target of BoussinesqSimulation
+// (src/Simulations.jl:272-274, 334-352) and add_velocity_sponge! (:375-381)
+extern "C" int aeo_stepper_set_sponge(aeo_stepper *h, double alpha_max, double z_start, double z_top,
+                                      const double *zc, const double *zf, const double *target_u,
+                                      const double *target_v, const double *target_w) {
+    NEED_CTX();
+    if (!h) return fail(AEO_ERR_ARG, "null argument");
+    if (h->spg) {
+        HIPCHK(hipStreamSynchronize(g_ctx->stream));
+        hipFree(h->spg);
+        h->spg = nullptr;
+    }
+    return make_sponge(h->g, h->sp.precision, alpha_max, z_start, z_top, zc, zf, target_u, target_v, target_w,
+                       &h->spg);
 }
 
 template <typename T>
@@ -1225,6 +1236,8 @@ struct aeo_mw_stepper {
     size_t n, esz;
     void *us, *vs, *ws, *tn;
     void *snap[4] = {};  // the step-start state of step_rk3 (allocated on first use)
+    double fcor = 0;     // f-plane Coriolis parameter
+    void *spg = nullptr; // sponge vectors, as the Boussinesq stepper's
     double *Wc = nullptr, *Wf = nullptr, *thf = nullptr;  // device, Nz + 2H
 };
 
@@ -1280,6 +1293,7 @@ extern "C" int aeo_mw_stepper_destroy(aeo_mw_stepper *h) {
     for (void *b : {h->us, h->vs, h->ws, h->tn}) hipFree(b);
     for (void *b : h->snap)
         if (b) hipFree(b);
+    if (h->spg) hipFree(h->spg);
     for (double *b : {h->Wc, h->Wf, h->thf}) hipFree(b);
     delete h;
     return AEO_OK;
@@ -1299,7 +1313,8 @@ static int mw_step(aeo_mw_stepper *h, T *u, T *v, T *w, T *th, T *p, double dt) 
         launch_halo<T>(st, w, Nx, Ny, Nz, H, LOC_FZ, sp.bc_x, sp.bc_y, sp.bc_z);
         launch_halo<T>(st, th, Nx, Ny, Nz, H, LOC_CC, sp.tbc_x, sp.tbc_y, sp.tbc_z);
     }
-    MWPhys ph{sp.nu, dt, sp.g_accel, sp.flux_bot * (1 / g->dz), sp.flux_top * (1 / g->dz), sp.buoyancy};
+    MWPhys ph{sp.nu, dt, sp.g_accel, sp.flux_bot * (1 / g->dz), sp.flux_top * (1 / g->dz), sp.buoyancy,
+              h->fcor, h->spg};
     {
         Timed t("tendency");
         launch_mw_tendency<T>(st, metrics<T>(g), us, vs, ws, tn, u, v, w, th, h->Wc, h->Wf, h->thf, ph);
@@ -1339,6 +1354,29 @@ extern "C" int aeo_mw_step(aeo_mw_stepper *h, void *u, void *v, void *w, void *t
     return h->sp.precision == AEO_F64
                ? mw_step<double>(h, (double *)u, (double *)v, (double *)w, (double *)theta, (double *)p, dt)
                : mw_step<float>(h, (float *)u, (float *)v, (float *)w, (float *)theta, (float *)p, dt);
+}
+
+// f_coriolis and the sponge of a sound-proof core (_build_mass_weighted,
+// src/Anelastic.jl:1559-1625; step 6 of _mass_weighted_step!, :1849-1852)
+extern "C" int aeo_mw_stepper_set_coriolis(aeo_mw_stepper *h, double f) {
+    NEED_CTX();
+    if (!h) return fail(AEO_ERR_ARG, "null argument");
+    if (!std::isfinite(f)) return fail(AEO_ERR_ARG, "the Coriolis parameter must be finite");
+    h->fcor = f;
+    return AEO_OK;
+}
+extern "C" int aeo_mw_stepper_set_sponge(aeo_mw_stepper *h, double alpha_max, double z_start, double z_top,
+                                         const double *zc, const double *zf, const double *target_u,
+                                         const double *target_v, const double *target_w) {
+    NEED_CTX();
+    if (!h) return fail(AEO_ERR_ARG, "null argument");
+    if (h->spg) {
+        HIPCHK(hipStreamSynchronize(g_ctx->stream));
+        hipFree(h->spg);
+        h->spg = nullptr;
+    }
+    return make_sponge(h->g, h->sp.precision, alpha_max, z_start, z_top, zc, zf, target_u, target_v, target_w,
+                       &h->spg);
 }
 
 // step_rk3! of the sound-proof core (src/Anelastic.jl:1905-1945): three

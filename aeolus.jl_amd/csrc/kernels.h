@@ -56,6 +56,8 @@ void launch_diag(hipStream_t st, double *part, int nblocks, const T *u, const T 
 struct MWPhys {
     double nu, dt, g_accel, fb, ft;  // fb, ft: surface fluxes times 1/Δz
     int buoyancy;
+    double fcor;         // f-plane Coriolis parameter (0: off)
+    const void *sponge;  // as StepPhys::sponge (nullptr: off)
 };
 template <typename T>
 void launch_mw_tendency(hipStream_t st, const Metrics<T> &m, T *us, T *vs, T *ws, T *tn, const T *u, const T *v,

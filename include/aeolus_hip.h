@@ -223,6 +223,14 @@ int aeo_mw_stepper_create(aeo_grid *g, const aeo_mw_params *p, const double *Wc,
                           const double *thetabar_f, aeo_poisson *solver, aeo_mw_stepper **h);
 int aeo_mw_stepper_destroy(aeo_mw_stepper *h);
 int aeo_mw_step(aeo_mw_stepper *h, void *u, void *v, void *w, void *theta, void *p, double dt);
+/* f_coriolis and sponge_α_max, sponge_z_start, sponge_target of a sound-proof
+ * core (_build_mass_weighted, src/Anelastic.jl:1559-1625; step 6 of
+ * _mass_weighted_step!, :1849-1852): the same terms and arguments as
+ * aeo_stepper_set_coriolis / aeo_stepper_set_sponge */
+int aeo_mw_stepper_set_coriolis(aeo_mw_stepper *h, double f);
+int aeo_mw_stepper_set_sponge(aeo_mw_stepper *h, double alpha_max, double z_start, double z_top, const double *zc,
+                              const double *zf, const double *target_u, const double *target_v,
+                              const double *target_w);
 /* step_rk3! of the sound-proof core (src/Anelastic.jl:1905-1945): three
  * aeo_mw_step stages; the whole storage of u, v, w, θ combined as
  * ¾·start + ¼·f after the second and ⅓·start + ⅔·f after the third */

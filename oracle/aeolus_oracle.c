@@ -2048,6 +2048,11 @@ void orc_mw_step(void *h, double *u, double *v, double *w, double *th, double *p
                     double tface = 0.5 * (F(i, j, k - 1) + F(i, j, k));
                     tw += coef * (tface - tb);
                 }
+                if (s->fcor != 0) {
+                    /* 6. Coriolis (:1849), _b_coriolis! (src/FiniteVolume.jl:900-911) */
+                    tu += s->fcor * (0.25 * (((V(i - 1, j, k) + V(i, j, k)) + V(i - 1, j + 1, k)) + V(i, j + 1, k)));
+                    tv -= s->fcor * (0.25 * (((U(i, j - 1, k) + U(i + 1, j - 1, k)) + U(i, j, k)) + U(i + 1, j, k)));
+                }
                 s->tu[c] = tu; s->tv[c] = tv; s->tw[c] = tw; s->tt[c] = tt;
             }
 #undef U
@@ -2062,6 +2067,23 @@ void orc_mw_step(void *h, double *u, double *v, double *w, double *th, double *p
                 s->tt[ID(i, j, kb)] += fb;
                 s->tt[ID(i, j, kt)] -= ft;
             }
+    }
+    /* the sponge (:1852), as the Boussinesq core's */
+    if (s->sp_zc) {
+        for (int k = H + 1; k <= H + g->Nz; k++) {
+            int kl = k - H - 1;
+            double lc = sponge_rate(s, s->sp_zc[kl]), lf = sponge_rate(s, s->sp_zf[kl]);
+            int inc = s->sp_zc[kl] >= s->sp_start, inf = s->sp_zf[kl] >= s->sp_start;
+            for (int j = H + 1; j <= H + g->Ny; j++)
+                for (int i = H + 1; i <= H + g->Nx; i++) {
+                    size_t c = ID(i, j, k);
+                    if (inc) {
+                        s->tu[c] -= lc * (u[c] - s->sp_tu[kl]);
+                        s->tv[c] -= lc * (v[c] - s->sp_tv[kl]);
+                    }
+                    if (inf) s->tw[c] -= lf * (w[c] - s->sp_tw[kl]);
+                }
+        }
     }
     /* 7. predictor */
 #pragma omp parallel for schedule(static)

@@ -89,3 +89,28 @@ def test_anelastic_rk3_is_third_order(orc):
     e2 = np.abs(run(T / 16, 16) - ref_u).max()
     print("rk3 ratio", e1 / e2)
     assert 5.5 < e1 / e2 < 11.0, (e1, e2)
+
+
+def test_reference_builders():
+    # the constructors drive the discrete hydrostatic balance to round-off
+    # (src/Anelastic.jl:616-640); isentropic: N² ≡ 0; constant stability:
+    # N² = N² to round-off (the log form is exact for θ̄ = θ_s e^{N²z/g});
+    # a profile given as a function or as its samples gives the same state
+    from aeolus_hip.anelastic import (constant_stability_reference, isentropic_reference,
+                                      reference_from_profile)
+    Nz, dz, H = 60, 150.0, 2
+    iso = isentropic_reference(Nz, dz, H, theta_const=300.0)
+    assert iso.max_hydrostatic_residual() < 1e-12 * 9.81 * iso.rho.max()
+    assert np.all(iso.N2 == 0.0)
+    cs = constant_stability_reference(Nz, dz, H, N=0.012)
+    assert cs.max_hydrostatic_residual() < 1e-12 * 9.81 * cs.rho.max()
+    assert np.abs(cs.N2 - 0.012 ** 2).max() < 1e-12 * 0.012 ** 2
+    prof = lambda z: 300.0 + 0.004 * z + (2.0 if z > 3000 else 0.0)
+    a = reference_from_profile(Nz, dz, H, prof)
+    b = reference_from_profile(Nz, dz, H, [prof((k + 0.5) * dz) for k in range(Nz)])
+    assert np.array_equal(a.p, b.p) and np.array_equal(a.rho, b.rho)
+    assert a.max_hydrostatic_residual() < 1e-12 * 9.81 * a.rho.max()
+    # the density of the ideal gas through the Exner function: ρ = p/(R T̄)
+    assert np.abs(a.rho - a.p / (287.0 * a.T)).max() < 1e-13 * a.rho.max()
+    # P = ρ̄θ̄ and its face twin
+    assert np.array_equal(cs.P, cs.rho * cs.theta) and np.array_equal(cs.P_f, cs.rho_f * cs.theta_f)

@@ -346,6 +346,75 @@ def test_anelastic_step_parity_fp64(hip, orc, nu, rk3):
     assert np.abs(d).max() < 1e-10 * ref.rho.max() * np.abs(got["u"][I]).max() / g.dx
 
 
+def _mw_state(og, ref, seed):
+    H, N = og.H, (og.Nx, og.Ny, og.Nz)
+    rng = np.random.default_rng(seed)
+    st = og.storage
+    th = np.zeros(st)
+    th[:, :, H:H + N[2]] = ref.theta[None, None, :]
+    I = og.interior
+    th[I] += 0.5 * rng.standard_normal(th[I].shape)
+    u, v, w = (np.zeros(st) for _ in range(3))
+    u[I] += rng.standard_normal(u[I].shape)
+    v[I] += rng.standard_normal(v[I].shape)
+    return [np.asfortranarray(a) for a in (u, v, w, th)]
+
+
+@pytest.mark.parametrize("forcing", [False, True])
+def test_pseudo_incompressible_step_parity_fp64(hip, orc, forcing):
+    # the W = ρ̄θ̄ core on a constant-stability base state, two RK3 steps;
+    # with forcing: f-plane Coriolis and a sponge over the top half
+    from aeolus_hip.anelastic import PseudoIncompressibleSimulation, constant_stability_reference
+    N, H, dz = (16, 8, 20), 1, 100.0
+    extent = (N[0] * 200.0, N[1] * 200.0, N[2] * dz)
+    g, og = hip.Grid(N, extent, halo=H), orc.Grid(N, extent, halo=H)
+    ref = constant_stability_reference(N[2], dz, H, N=0.012)
+    u, v, w, th = _mw_state(og, ref, 22)
+    fk = dict(f_coriolis=1e-3, sponge_alpha_max=0.01, sponge_z_start=0.5 * g.Lz,
+              sponge_target=dict(u=1.5)) if forcing else {}
+    sim = PseudoIncompressibleSimulation(g, ref, nu=1.0, flux_bot=0.05, **fk)
+    sim.set(u, v, w, th)
+    ost = orc.MWStepper(og, ref.Pc_pad, ref.Pf_pad, ref.theta_f_pad, ref.P, ref.P_f, nu=1.0, flux_bot=0.05)
+    if forcing:
+        from aeolus_hip.simulation import damping_coordinates
+        zc, zf = damping_coordinates(g)
+        ost.set_forcing(1e-3, 0.01, 0.5 * g.Lz, g.z0 + g.Lz, zc, zf, (np.full(N[2], 1.5), None, None))
+    p = og.zeros()
+    sim.step(1e-9)
+    ost.step(u, v, w, th, p, 1e-9)
+    for _ in range(2):
+        sim.step_rk3(3.0)
+        ost.step_rk3(u, v, w, th, p, 3.0)
+    got = sim.state()
+    I = og.interior
+    for n, ref_a in (("u", u), ("v", v), ("w", w), ("theta", th), ("p", p)):
+        assert rel_err(got[n][I], ref_a[I]) <= TOL64, n
+    d = sim.mass_flux_divergence()
+    assert np.abs(d).max() < 1e-10 * ref.P.max() * np.abs(got["u"][I]).max() / g.dx
+
+
+def test_anelastic_and_pseudo_incompressible_agree_on_isentropic_state(hip, orc):
+    # with θ̄ constant the two weights differ by a constant factor, so the
+    # two cores give the same trajectory (isentropic_reference's docstring,
+    # src/Anelastic.jl:479-492), up to round-off
+    from aeolus_hip.anelastic import AnelasticSimulation, PseudoIncompressibleSimulation, isentropic_reference
+    N, H, dz = (16, 16, 16), 1, 100.0
+    extent = (N[0] * 200.0, N[1] * 200.0, N[2] * dz)
+    g, og = hip.Grid(N, extent, halo=H), orc.Grid(N, extent, halo=H)
+    ref = isentropic_reference(N[2], dz, H)
+    state = _mw_state(og, ref, 23)
+    out = []
+    for cls in (AnelasticSimulation, PseudoIncompressibleSimulation):
+        sim = cls(g, ref, nu=0.5)
+        sim.set(*state)
+        for _ in range(3):
+            sim.step_rk3(2.0)
+        out.append(sim.state())
+    I = og.interior
+    for n in ("u", "v", "w", "theta"):
+        assert rel_err(out[0][n][I], out[1][n][I]) < 1e-12, n
+
+
 def _terrain_case(hip, orc, N=(32, 24, 16), H=2, dz=50.0):
     from aeolus_hip.terrain import TerrainFollowingGrid
     extent = (N[0] * 100.0, N[1] * 100.0, N[2] * dz)
