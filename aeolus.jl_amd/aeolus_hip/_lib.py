@@ -107,6 +107,7 @@ SIGNATURES = {
     "aeo_mw_step": [_vp, _vp, _vp, _vp, _vp, _vp, _d],
     "aeo_mw_step_rk3": [_vp, _vp, _vp, _vp, _vp, _vp, _d],
     "aeo_mw_stepper_set_coriolis": [_vp, _d],
+    "aeo_mw_stepper_set_sgs": [_vp, _d, _d],
     "aeo_mw_stepper_set_sponge": [_vp, _d, _d, _d, _pd, _pd, _pd, _pd, _pd],
     "aeo_weighted_divergence": [_vp, _i, _vp, _vp, _vp, _vp, _vp, _vp, _i],
     "aeo_cutcell_divergence": [_vp, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp],

@@ -240,7 +240,8 @@ class AnelasticSimulation:
 
     def __init__(self, grid, ref, nu=0.0, flux_bot=0.0, flux_top=0.0, buoyancy=True,
                  bcs=("periodic", "periodic", "wall"), theta_bcs=("periodic", "periodic", "noflux"), precision=L.F64,
-                 f_coriolis=0.0, sponge_alpha_max=0.0, sponge_z_start=None, sponge_target=None):
+                 f_coriolis=0.0, sponge_alpha_max=0.0, sponge_z_start=None, sponge_target=None, C_s=0.0,
+                 Pr_t=1 / 3):
         if grid.zf is not None:
             raise ValueError("the sound-proof core runs on regular grids")
         self.grid, self.ref, self.precision = grid, ref, precision
@@ -274,7 +275,10 @@ class AnelasticSimulation:
                                 flux_top=float(flux_top), buoyancy=bool(buoyancy), bcs=list(bcs),
                                 theta_bcs=list(theta_bcs), f_coriolis=float(f_coriolis),
                                 sponge_alpha_max=float(sponge_alpha_max),
-                                sponge_z_start=None if sponge_z_start is None else float(sponge_z_start))
+                                sponge_z_start=None if sponge_z_start is None else float(sponge_z_start),
+                                C_s=float(C_s), Pr_t=float(Pr_t))
+        if C_s:
+            L.check(L.init().aeo_mw_stepper_set_sgs(ctypes.c_void_p(self._h), float(C_s), float(Pr_t)))
         if f_coriolis:
             L.check(L.init().aeo_mw_stepper_set_coriolis(ctypes.c_void_p(self._h), float(f_coriolis)))
         z_top = grid.z0 + grid.Lz

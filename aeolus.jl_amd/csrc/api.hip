@@ -1238,6 +1238,9 @@ struct aeo_mw_stepper {
     void *snap[4] = {};  // the step-start state of step_rk3 (allocated on first use)
     double fcor = 0;     // f-plane Coriolis parameter
     void *spg = nullptr; // sponge vectors, as the Boussinesq stepper's
+    // the Smagorinsky closure: νt and κt = νt/Pr_t (storage), per-level (C_s Δ)² and 1/Δz
+    void *nut = nullptr, *kap = nullptr, *sgs_lv = nullptr;
+    double Pr_t = 0;
     double *Wc = nullptr, *Wf = nullptr, *thf = nullptr;  // device, Nz + 2H
 };
 
@@ -1294,6 +1297,8 @@ extern "C" int aeo_mw_stepper_destroy(aeo_mw_stepper *h) {
     for (void *b : h->snap)
         if (b) hipFree(b);
     if (h->spg) hipFree(h->spg);
+    for (void *b : {h->nut, h->kap, h->sgs_lv})
+        if (b) hipFree(b);
     for (double *b : {h->Wc, h->Wf, h->thf}) hipFree(b);
     delete h;
     return AEO_OK;
@@ -1315,9 +1320,25 @@ static int mw_step(aeo_mw_stepper *h, T *u, T *v, T *w, T *th, T *p, double dt) 
     }
     MWPhys ph{sp.nu, dt, sp.g_accel, sp.flux_bot * (1 / g->dz), sp.flux_top * (1 / g->dz), sp.buoyancy,
               h->fcor, h->spg};
+    if (h->sgs_lv) {
+        // νt from the step-start state, its halo with the velocity BCs at a cell centre
+        Timed t("sgs");
+        const int n = Nz + 2 * H;
+        launch_smag_nut<T>(st, metrics<T>(g), (T *)h->nut, u, v, w, (const T *)h->sgs_lv, (const T *)h->sgs_lv + n);
+        int bx = sp.bc_x == AEO_PERIODIC ? BC_PERIODIC : BC_NOFLUX;
+        int by = sp.bc_y == AEO_PERIODIC ? BC_PERIODIC : BC_NOFLUX;
+        int bz = sp.bc_z == AEO_PERIODIC ? BC_PERIODIC : BC_NOFLUX;
+        launch_halo<T>(st, (T *)h->nut, Nx, Ny, Nz, H, LOC_CC, bx, by, bz);
+    }
     {
         Timed t("tendency");
         launch_mw_tendency<T>(st, metrics<T>(g), us, vs, ws, tn, u, v, w, th, h->Wc, h->Wf, h->thf, ph);
+    }
+    if (h->sgs_lv) {
+        // the W-weighted eddy terms (flux_weights of a sound-proof core), added to the predictor
+        Timed t("sgs");
+        launch_smag_apply_weighted<T>(st, metrics<T>(g), us, vs, ws, tn, u, v, w, th, (const T *)h->nut, (T *)h->kap,
+                                      h->Wc, h->Wf, h->Pr_t, dt);
     }
     {
         Timed t("halo");
@@ -1377,6 +1398,43 @@ extern "C" int aeo_mw_stepper_set_sponge(aeo_mw_stepper *h, double alpha_max, do
     }
     return make_sponge(h->g, h->sp.precision, alpha_max, z_start, z_top, zc, zf, target_u, target_v, target_w,
                        &h->spg);
+}
+
+// C_s and Pr_t of a sound-proof core (_build_mass_weighted, src/Anelastic.jl:
+// 1559-1625): StaticSmagorinsky(C_s) with the W-weighted SGS fluxes
+// (flux_weights, src/operators/sgs_closures.jl:200-217)
+extern "C" int aeo_mw_stepper_set_sgs(aeo_mw_stepper *h, double C_s, double Pr_t) {
+    NEED_CTX();
+    if (!h) return fail(AEO_ERR_ARG, "null argument");
+    if (!std::isfinite(C_s) || !std::isfinite(Pr_t) || Pr_t == 0)
+        return fail(AEO_ERR_ARG, "C_s must be finite and Pr_t finite and non-zero");
+    HIPCHK(hipStreamSynchronize(g_ctx->stream));
+    if (h->sgs_lv) { hipFree(h->sgs_lv); h->sgs_lv = nullptr; }
+    if (C_s == 0) return AEO_OK;
+    const aeo_grid *g = h->g;
+    const int n = g->Nz + 2 * g->H;
+    std::vector<double> v(2 * (size_t)n, 0.0);
+    const double D = std::pow(g->dx * g->dy * g->dz, 1.0 / 3);  // _filter_width of UniformZ
+    const double cd = C_s * D;
+    for (int k = 0; k < n; k++) {
+        v[k] = cd * cd;
+        v[n + k] = 1 / g->dz;  // _idz_off of UniformZ
+    }
+    for (void **b : {&h->nut, &h->kap})
+        if (!*b) {
+            HIPCHK(hipMalloc(b, h->n * h->esz));
+            HIPCHK(hipMemset(*b, 0, h->n * h->esz));
+        }
+    if (h->sp.precision == AEO_F64) {
+        HIPCHK(hipMalloc(&h->sgs_lv, v.size() * sizeof(double)));
+        HIPCHK(hipMemcpy(h->sgs_lv, v.data(), v.size() * sizeof(double), hipMemcpyHostToDevice));
+    } else {
+        std::vector<float> f(v.begin(), v.end());
+        HIPCHK(hipMalloc(&h->sgs_lv, f.size() * sizeof(float)));
+        HIPCHK(hipMemcpy(h->sgs_lv, f.data(), f.size() * sizeof(float), hipMemcpyHostToDevice));
+    }
+    h->Pr_t = Pr_t;
+    return AEO_OK;
 }
 
 // step_rk3! of the sound-proof core (src/Anelastic.jl:1905-1945): three

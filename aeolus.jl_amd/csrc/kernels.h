@@ -40,6 +40,10 @@ template <typename T>
 void launch_smag_apply(hipStream_t st, const Metrics<T> &m, T *us, T *vs, T *ws, T *so, const T *u, const T *v,
                        const T *w, const T *th, const T *nut, double dt, double bt, double sc);
 template <typename T>
+void launch_smag_apply_weighted(hipStream_t st, const Metrics<T> &m, T *us, T *vs, T *ws, T *tn, const T *u,
+                                const T *v, const T *w, const T *th, const T *nut, T *kap, const double *Wc,
+                                const double *Wf, double Pr_t, double dt);
+template <typename T>
 void launch_div_into(hipStream_t st, double *buf, const T *u, const T *v, const T *w, const Metrics<double> &md);
 template <typename T>
 void launch_correct(hipStream_t st, T *uo, T *vo, T *wo, const T *us, const T *vs, const T *ws, const T *u0,

@@ -231,6 +231,11 @@ int aeo_mw_stepper_set_coriolis(aeo_mw_stepper *h, double f);
 int aeo_mw_stepper_set_sponge(aeo_mw_stepper *h, double alpha_max, double z_start, double z_top, const double *zc,
                               const double *zf, const double *target_u, const double *target_v,
                               const double *target_w);
+/* C_s and Pr_t of a sound-proof core: StaticSmagorinsky(C_s) with the
+ * W-weighted SGS fluxes (1/W)∇·(W νt ∇u_i), (1/W)∇·(W κt ∇θ), κt = νt/Pr_t
+ * (flux_weights, src/operators/sgs_closures.jl:200-217; _vertical_divergence,
+ * src/FiniteVolume.jl:1556-1563). C_s = 0 turns the closure off. */
+int aeo_mw_stepper_set_sgs(aeo_mw_stepper *h, double C_s, double Pr_t);
 /* step_rk3! of the sound-proof core (src/Anelastic.jl:1905-1945): three
  * aeo_mw_step stages; the whole storage of u, v, w, θ combined as
  * ¾·start + ¼·f after the second and ⅓·start + ⅔·f after the third */

@@ -920,6 +920,7 @@ typedef struct {
     double fcor;                                 /* f_coriolis */
     double C_s, sgs_s;                           /* Smagorinsky C_s, 1/Pr_t (C_s = 0: off) */
     double *nut, *sgs_coef, *sgs_ioff;           /* νt (storage), (C_s Δ_k)², off-diagonal 1/Δz (Nz+2H) */
+    double *kap, sgs_prt;                        /* κt = νt/Pr_t (storage), Pr_t */
     double sp_alpha, sp_start, sp_top;           /* sponge_α_max, sponge_z_start, z0 + Lz */
     double *sp_zc, *sp_zf, *sp_tu, *sp_tv, *sp_tw; /* Nz each (NULL: no sponge) */
 } orc_stepper;
@@ -960,6 +961,7 @@ void orc_stepper_set_sgs(void *h, double C_s, double Pr_t, const double *dzc, co
     int n = g->Nz + 2 * g->H;
     s->C_s = C_s;
     s->sgs_s = 1 / Pr_t;
+    s->sgs_prt = Pr_t;
     free(s->sgs_coef); free(s->sgs_ioff);
     s->sgs_coef = (double *)calloc(n, sizeof(double));
     s->sgs_ioff = (double *)calloc(n, sizeof(double));
@@ -1001,7 +1003,7 @@ void orc_stepper_destroy(void *h) {
     free(s->tu); free(s->tv); free(s->tw); free(s->tt); free(s->u0); free(s->v0); free(s->w0);
     free(s->t0); free(s->phi); free(s->buf); free(s->pbuf);
     free(s->sp_zc); free(s->sp_zf); free(s->sp_tu); free(s->sp_tv); free(s->sp_tw);
-    free(s->nut); free(s->sgs_coef); free(s->sgs_ioff);
+    free(s->nut); free(s->sgs_coef); free(s->sgs_ioff); free(s->kap);
     free(s);
 }
 
@@ -1964,6 +1966,82 @@ void orc_cutcell_heightmap(const orc_grid *g, const double *hfull, int nsub, dou
     free(solid);
 }
 
+/* The sound-proof cores' SGS form (flux_weights: W-weighted vertical
+ * fluxes, src/operators/sgs_closures.jl:200-217): νt and its halo as the
+ * Boussinesq core's, κt = νt/Pr_t over the storage (_divide_storage!), then
+ * per cell _b_eddy_u!/_v!/_w! with _vertical_divergence[_w] weighted
+ * (src/FiniteVolume.jl:966-1042, 1556-1563) and _b_eddy_scalar_masked!
+ * without a mask (:1473-1492). Regular grids. */
+static void sgs_mw_prepare(orc_stepper *s, const double *u, const double *v, const double *w) {
+    const orc_step_params *sp = &s->sp;
+    sgs_terms(s, u, v, w, NULL);
+    int bx = sp->bcx == ORC_PERIODIC ? ORC_PERIODIC : ORC_NOFLUX;
+    int by = sp->bcy == ORC_PERIODIC ? ORC_PERIODIC : ORC_NOFLUX;
+    int bz = sp->bcz == ORC_PERIODIC ? ORC_PERIODIC : ORC_NOFLUX;
+    orc_fill_halos(s->nut, &s->g, ORC_CC, bx, by, bz);
+    if (!s->kap) s->kap = (double *)calloc(s->n, sizeof(double));
+    for (size_t c = 0; c < s->n; c++) s->kap[c] = s->nut[c] / s->sgs_prt;
+}
+static void sgs_mw_at(const orc_stepper *s, int i, int j, int k, const double *u, const double *v, const double *w,
+                      const double *th, const double *Wc, const double *Wf, double out[4]) {
+    const orc_grid *g = &s->g;
+    GRID_DIMS(g);
+    const double *n = s->nut, *K = s->kap;
+    double idx = 1 / g->dx, idy = 1 / g->dy, idz = 1 / g->dz;
+    size_t c = ID(i, j, k);
+    double wck = Wc[k - 1], wckm = Wc[k - 2], wfk = Wf[k - 1], wfkp = Wf[k];
+#define NU(a, b, cc) n[ID(a, b, cc)]
+    {
+        double fxE = NU(i, j, k) * (u[ID(i + 1, j, k)] - u[c]) * idx;
+        double fxW = NU(i - 1, j, k) * (u[c] - u[ID(i - 1, j, k)]) * idx;
+        double eN = 0.25 * (((NU(i - 1, j, k) + NU(i, j, k)) + NU(i - 1, j + 1, k)) + NU(i, j + 1, k));
+        double eS = 0.25 * (((NU(i - 1, j - 1, k) + NU(i, j - 1, k)) + NU(i - 1, j, k)) + NU(i, j, k));
+        double fyN = eN * (u[ID(i, j + 1, k)] - u[c]) * idy;
+        double fyS = eS * (u[c] - u[ID(i, j - 1, k)]) * idy;
+        double eT = 0.25 * (((NU(i - 1, j, k) + NU(i, j, k)) + NU(i - 1, j, k + 1)) + NU(i, j, k + 1));
+        double eB = 0.25 * (((NU(i - 1, j, k - 1) + NU(i, j, k - 1)) + NU(i - 1, j, k)) + NU(i, j, k));
+        double fzT = eT * (u[ID(i, j, k + 1)] - u[c]) * idz;
+        double fzB = eB * (u[c] - u[ID(i, j, k - 1)]) * idz;
+        out[0] = ((fxE - fxW) * idx + (fyN - fyS) * idy) + (wfkp * fzT - wfk * fzB) * idz / wck;
+    }
+    {
+        double fyN = NU(i, j, k) * (v[ID(i, j + 1, k)] - v[c]) * idy;
+        double fyS = NU(i, j - 1, k) * (v[c] - v[ID(i, j - 1, k)]) * idy;
+        double eE = 0.25 * (((NU(i, j - 1, k) + NU(i + 1, j - 1, k)) + NU(i, j, k)) + NU(i + 1, j, k));
+        double eW = 0.25 * (((NU(i - 1, j - 1, k) + NU(i, j - 1, k)) + NU(i - 1, j, k)) + NU(i, j, k));
+        double fxE = eE * (v[ID(i + 1, j, k)] - v[c]) * idx;
+        double fxW = eW * (v[c] - v[ID(i - 1, j, k)]) * idx;
+        double eT = 0.25 * (((NU(i, j - 1, k) + NU(i, j, k)) + NU(i, j - 1, k + 1)) + NU(i, j, k + 1));
+        double eB = 0.25 * (((NU(i, j - 1, k - 1) + NU(i, j, k - 1)) + NU(i, j - 1, k)) + NU(i, j, k));
+        double fzT = eT * (v[ID(i, j, k + 1)] - v[c]) * idz;
+        double fzB = eB * (v[c] - v[ID(i, j, k - 1)]) * idz;
+        out[1] = ((fxE - fxW) * idx + (fyN - fyS) * idy) + (wfkp * fzT - wfk * fzB) * idz / wck;
+    }
+    {
+        double fzT = NU(i, j, k) * (w[ID(i, j, k + 1)] - w[c]) * idz;
+        double fzB = NU(i, j, k - 1) * (w[c] - w[ID(i, j, k - 1)]) * idz;
+        double eE = 0.25 * (((NU(i, j, k - 1) + NU(i + 1, j, k - 1)) + NU(i, j, k)) + NU(i + 1, j, k));
+        double eW = 0.25 * (((NU(i - 1, j, k - 1) + NU(i, j, k - 1)) + NU(i - 1, j, k)) + NU(i, j, k));
+        double fxE = eE * (w[ID(i + 1, j, k)] - w[c]) * idx;
+        double fxW = eW * (w[c] - w[ID(i - 1, j, k)]) * idx;
+        double eN = 0.25 * (((NU(i, j, k - 1) + NU(i, j + 1, k - 1)) + NU(i, j, k)) + NU(i, j + 1, k));
+        double eS = 0.25 * (((NU(i, j - 1, k - 1) + NU(i, j, k - 1)) + NU(i, j - 1, k)) + NU(i, j, k));
+        double fyN = eN * (w[ID(i, j + 1, k)] - w[c]) * idy;
+        double fyS = eS * (w[c] - w[ID(i, j - 1, k)]) * idy;
+        out[2] = ((fxE - fxW) * idx + (fyN - fyS) * idy) + (wck * fzT - wckm * fzB) * idz / wfk;
+    }
+#undef NU
+    {
+        double fxE = 0.5 * (K[c] + K[ID(i + 1, j, k)]) * (th[ID(i + 1, j, k)] - th[c]) * idx;
+        double fxW = 0.5 * (K[ID(i - 1, j, k)] + K[c]) * (th[c] - th[ID(i - 1, j, k)]) * idx;
+        double fyN = 0.5 * (K[c] + K[ID(i, j + 1, k)]) * (th[ID(i, j + 1, k)] - th[c]) * idy;
+        double fyS = 0.5 * (K[ID(i, j - 1, k)] + K[c]) * (th[c] - th[ID(i, j - 1, k)]) * idy;
+        double fzT = 0.5 * (K[c] + K[ID(i, j, k + 1)]) * (th[ID(i, j, k + 1)] - th[c]) * idz;
+        double fzB = 0.5 * (K[ID(i, j, k - 1)] + K[c]) * (th[c] - th[ID(i, j, k - 1)]) * idz;
+        out[3] = ((fxE - fxW) * idx + (fyN - fyS) * idy) + (wfkp * fzT - wfk * fzB) * idz / wck;
+    }
+}
+
 /* ---------------------------------------------------------------------------
  * The sound-proof (mass-weighted) core: one projected forward-Euler step,
  * _mass_weighted_step! (src/Anelastic.jl:1791-1876), anelastic weighting
@@ -1985,6 +2063,8 @@ void orc_mw_step(void *h, double *u, double *v, double *w, double *th, double *p
     orc_fill_halos(v, g, ORC_FY, sp->bcx, sp->bcy, sp->bcz);
     orc_fill_halos(w, g, ORC_FZ, sp->bcx, sp->bcy, sp->bcz);
     orc_fill_halos(th, g, ORC_CC, sp->tbcx, sp->tbcy, sp->tbcz);
+    /* 4. the SGS closure's coefficients from the step-start state */
+    if (s->C_s != 0) sgs_mw_prepare(s, u, v, w);
 #define U(i, j, k) u[ID(i, j, k)]
 #define V(i, j, k) v[ID(i, j, k)]
 #define W(i, j, k) w[ID(i, j, k)]
@@ -2040,6 +2120,12 @@ void orc_mw_step(void *h, double *u, double *v, double *w, double *th, double *p
                     tv = tv + sp->nu * lap_at(g, sx, sy, i, j, k, v, ORC_FY);
                     tw = tw + sp->nu * lap_at(g, sx, sy, i, j, k, w, ORC_FZ);
                     tt = tt + sp->nu * lap_at(g, sx, sy, i, j, k, th, ORC_CC);
+                }
+                if (s->C_s != 0) {
+                    /* 4. add_sgs_momentum! / add_sgs_scalar! in the W-weighted form */
+                    double e[4];
+                    sgs_mw_at(s, i, j, k, u, v, w, th, Wc, Wf, e);
+                    tu += e[0]; tv += e[1]; tw += e[2]; tt += e[3];
                 }
                 if (buoyancy) {
                     /* _anelastic_buoyancy (:1352-1360) */

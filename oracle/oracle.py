@@ -398,6 +398,10 @@ class MWStepper:
         lib().orc_mw_step(self.h, ptr(u), ptr(v), ptr(w), ptr(th), ptr(p), dt, *[ptr(a) for a in self.prof],
                           self.g_accel, self.buoyancy)
 
+    def set_sgs(self, C_s, Pr_t=1 / 3):
+        """StaticSmagorinsky(C_s) in the W-weighted form (regular grids)."""
+        lib().orc_stepper_set_sgs(self.h, C_s, Pr_t, None, None)
+
     def set_forcing(self, fcor=0.0, alpha_max=0.0, z_start=0.0, z_top=0.0, zc=None, zf=None, targets=(None,) * 3):
         """f-plane Coriolis and the velocity sponge (zc, zf: the Nz level heights)."""
         keep = [None if a is None else np.ascontiguousarray(a, dtype=np.float64) for a in (zc, zf, *targets)]

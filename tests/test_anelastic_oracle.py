@@ -114,3 +114,33 @@ def test_reference_builders():
     assert np.abs(a.rho - a.p / (287.0 * a.T)).max() < 1e-13 * a.rho.max()
     # P = ρ̄θ̄ and its face twin
     assert np.array_equal(cs.P, cs.rho * cs.theta) and np.array_equal(cs.P_f, cs.rho_f * cs.theta_f)
+
+
+def test_weighted_sgs_conserves_weighted_totals(orc):
+    # the sound-proof cores take the SGS fluxes in the W-weighted flux form
+    # (flux_weights, src/operators/sgs_closures.jl:200-217): with walls in z,
+    # periodic sides and no forcing, the whole step keeps Σ W_c θ and Σ W_c u
+    # (the advection's flux form and the projection keep them too); the
+    # closure is active (it changes the state)
+    from aeolus_hip.anelastic import constant_stability_reference
+    N, H, dz = (12, 10, 16), 1, 100.0
+    g = orc.Grid(N, (N[0] * 200.0, N[1] * 200.0, N[2] * dz), halo=H)
+    ref = constant_stability_reference(N[2], dz, H, N=0.012)
+
+    def run(C_s):
+        st = orc.MWStepper(g, ref.Pc_pad, ref.Pf_pad, ref.theta_f_pad, ref.P, ref.P_f, nu=0.0, buoyancy=False)
+        if C_s:
+            st.set_sgs(C_s)
+        u, v, w, th = _state(g, ref, 5, amp=1.0)
+        p = g.zeros()
+        st.step(u, v, w, th, p, 1e-9)
+        before = [(ref.P[None, None, :] * a[g.interior]).sum() for a in (u, th)]
+        st.step(u, v, w, th, p, 2.0)
+        after = [(ref.P[None, None, :] * a[g.interior]).sum() for a in (u, th)]
+        return before, after, u.copy(), th.copy()
+    b, a, u1, t1 = run(0.25)
+    scale = (ref.P[None, None, :] * np.abs(t1[g.interior])).sum()
+    assert abs(a[1] - b[1]) < 1e-13 * scale
+    assert abs(a[0] - b[0]) < 1e-12 * (ref.P[None, None, :] * np.abs(u1[g.interior])).sum()
+    _, _, u0, t0 = run(0.0)
+    assert np.abs(t1 - t0).max() > 1e-6 and np.abs(u1 - u0).max() > 1e-6

@@ -393,6 +393,33 @@ def test_pseudo_incompressible_step_parity_fp64(hip, orc, forcing):
     assert np.abs(d).max() < 1e-10 * ref.P.max() * np.abs(got["u"][I]).max() / g.dx
 
 
+@pytest.mark.parametrize("weighting", ["anelastic", "pseudo_incompressible"])
+def test_sound_proof_smagorinsky_parity_fp64(hip, orc, weighting):
+    # StaticSmagorinsky(C_s) in the W-weighted flux form of the sound-proof cores
+    from aeolus_hip.anelastic import AnelasticSimulation, PseudoIncompressibleSimulation, constant_stability_reference
+    N, H, dz = (16, 8, 20), 1, 100.0
+    extent = (N[0] * 200.0, N[1] * 200.0, N[2] * dz)
+    g, og = hip.Grid(N, extent, halo=H), orc.Grid(N, extent, halo=H)
+    ref = constant_stability_reference(N[2], dz, H, N=0.012)
+    u, v, w, th = _mw_state(og, ref, 24)
+    cls = AnelasticSimulation if weighting == "anelastic" else PseudoIncompressibleSimulation
+    sim = cls(g, ref, nu=0.0, C_s=0.2, Pr_t=0.4)
+    sim.set(u, v, w, th)
+    wc, wf, wc_pad, wf_pad = ref.weights(weighting)
+    ost = orc.MWStepper(og, wc_pad, wf_pad, ref.theta_f_pad, wc, wf, nu=0.0)
+    ost.set_sgs(0.2, 0.4)
+    p = og.zeros()
+    sim.step(1e-9)
+    ost.step(u, v, w, th, p, 1e-9)
+    for _ in range(2):
+        sim.step_rk3(3.0)
+        ost.step_rk3(u, v, w, th, p, 3.0)
+    got = sim.state()
+    I = og.interior
+    for n, ref_a in (("u", u), ("v", v), ("w", w), ("theta", th), ("p", p)):
+        assert rel_err(got[n][I], ref_a[I]) <= TOL64, n
+
+
 def test_anelastic_and_pseudo_incompressible_agree_on_isentropic_state(hip, orc):
     # with θ̄ constant the two weights differ by a constant factor, so the
     # two cores give the same trajectory (isentropic_reference's docstring,
