@@ -90,6 +90,7 @@ SIGNATURES = {
     "aeo_stepper_destroy": [_vp],
     "aeo_stepper_set_coriolis": [_vp, _d],
     "aeo_stepper_set_sgs": [_vp, _d, _d],
+    "aeo_stepper_cfl": [_vp, _vp, _vp, _vp, _vp, _pd, _pd, _pd],
     "aeo_stepper_set_sponge": [_vp, _d, _d, _d, _pd, _pd, _pd, _pd, _pd],
     "aeo_step_rk3": [_vp, _vp, _vp, _vp, _vp, _vp, _d],
     "aeo_step_euler": [_vp, _vp, _vp, _vp, _vp, _vp, _d],

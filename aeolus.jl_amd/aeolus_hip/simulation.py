@@ -2,6 +2,7 @@
 step_rk3! and step! (src/Simulations.jl:168-288, 474-487, 843-904).
 """
 import ctypes
+import math
 
 import numpy as np
 
@@ -167,3 +168,89 @@ class BoussinesqSimulation:
                 L._lib.aeo_stepper_destroy(ctypes.c_void_p(self._h))
         except Exception:
             pass
+
+
+class CFLConstraints:
+    """CFLConstraints (src/TimeIntegration.jl:127-163): safety factors of the
+    advective, diffusive, buoyancy and Coriolis limits, the dt clamp and the
+    growth bound between consecutive steps."""
+
+    def __init__(self, C_advective=0.5, C_diffusive=0.25, C_buoyancy=0.5, C_coriolis=0.5, dt_min=1e-8,
+                 dt_max=math.inf, growth_max=1.1):
+        if not C_advective > 0:
+            raise ValueError("C_advective must be > 0")
+        if not C_diffusive > 0:
+            raise ValueError("C_diffusive must be > 0")
+        if not C_buoyancy > 0:
+            raise ValueError("C_buoyancy must be > 0")
+        if not C_coriolis > 0:
+            raise ValueError("C_coriolis must be > 0")
+        if not 0 < dt_min <= dt_max:
+            raise ValueError("need 0 < dt_min ≤ dt_max")
+        if not growth_max >= 1:
+            raise ValueError("growth_max must be ≥ 1")
+        self.C_advective, self.C_diffusive, self.C_buoyancy = float(C_advective), float(C_diffusive), float(C_buoyancy)
+        self.C_coriolis, self.dt_min, self.dt_max = float(C_coriolis), float(dt_min), float(dt_max)
+        self.growth_max = float(growth_max)
+
+
+def cfl_diagnostics(sim, c=None, limits=()):
+    """cfl_diagnostics(sim, constraints; limits) (src/TimeIntegration.jl:291-400)
+    with the simulation's own closure and sgs_stress = :laplacian: the limit of
+    each process, the binding one and the maxima it came from. `limits` are
+    (dt, label) pairs of further components."""
+    c = c or CFLConstraints()
+    rate, nut, N = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+    L.check(L.init().aeo_stepper_cfl(ctypes.c_void_p(sim._h), L.dptr(sim.u), L.dptr(sim.v), L.dptr(sim.w),
+                                     L.dptr(sim.theta), ctypes.byref(rate), ctypes.byref(nut), ctypes.byref(N)))
+    return cfl_limits(rate.value, nut.value, N.value, sim.parameters(), sim.grid, c, limits)
+
+
+def cfl_limits(rate, nut_max, N, p, g, c, limits=()):
+    """The limits of cfl_diagnostics from the state maxima (the advective rate,
+    max νt, N), the run parameters p (nu, C_s, Pr_t, f_coriolis) and the grid."""
+    inf = math.inf
+    dt_adv = math.nan if math.isnan(rate) else (c.C_advective / rate if rate > 0 else inf)
+    kappa_max = nut_max / p["Pr_t"] if p["C_s"] else 0.0
+    dz_min = g.dz if g.zf is None else float(np.min(np.diff(np.asarray(g.zf, dtype=np.float64))))
+    d2harm = 1.0 / (1 / g.dx ** 2 + 1 / g.dy ** 2 + 1 / dz_min ** 2)
+    nut_cfl = nut_max
+    nu_total = p["nu"] + nut_cfl
+    dt_diff = c.C_diffusive * d2harm / nu_total if nu_total > 0 else inf
+    k_sgs = max(nut_cfl, kappa_max)
+    dt_sgs = c.C_diffusive * d2harm / k_sgs if k_sgs > 0 else inf
+    dt_buoy = c.C_buoyancy / N if N > 0 else inf
+    f = abs(p["f_coriolis"])
+    dt_cor = c.C_coriolis / f if f > 0 else inf
+    dt_lim, limiter = dt_adv, "advective"
+    for d, name in ((dt_diff, "diffusive"), (dt_buoy, "buoyancy"), (dt_cor, "coriolis")):
+        if d < dt_lim:
+            dt_lim, limiter = d, name
+    dt_limits, lim_label = inf, "none"
+    for lim in limits:
+        if lim is None:
+            continue
+        d_l, label = float(lim[0]), lim[-1]
+        if d_l < dt_limits:
+            dt_limits, lim_label = d_l, label
+        if d_l < dt_lim:
+            dt_lim, limiter = d_l, label
+    if dt_sgs < dt_lim:
+        dt_lim, limiter = dt_sgs, "sgs"
+    return dict(dt_adv=dt_adv, dt_diff=dt_diff, dt_buoy=dt_buoy, dt_cor=dt_cor, dt_sgs=dt_sgs, dt_limits=dt_limits,
+                limits_limiter=lim_label, dt_limiting=dt_lim, limiter=limiter, advective_rate=rate, N=N,
+                nut_max=nut_max, kappa_max=kappa_max, nut_cfl=nut_cfl, nu_total=nu_total)
+
+
+def adaptive_dt(sim, c=None, dt_prev=None, diagnostics=None, limits=()):
+    """adaptive_dt (src/TimeIntegration.jl:436-466): the binding limit, then the
+    growth bound against dt_prev, then the clamp to [dt_min, dt_max]; a NaN
+    limit collapses to dt_min."""
+    c = c or CFLConstraints()
+    d = diagnostics or cfl_diagnostics(sim, c, limits)
+    dt = d["dt_limiting"]
+    if math.isnan(dt):
+        return c.dt_min
+    if dt_prev is not None and dt_prev > 0 and c.growth_max * dt_prev < dt:
+        dt = c.growth_max * dt_prev
+    return min(max(dt, c.dt_min), c.dt_max)

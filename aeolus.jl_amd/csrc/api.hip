@@ -1513,6 +1513,73 @@ static int run_diag(aeo_grid *g, int precision, const void *u, const void *v, co
     return AEO_OK;
 }
 
+// The state maxima of cfl_diagnostics (src/TimeIntegration.jl:291-400) on a
+// Boussinesq stepper: the velocity halos filled (as the reference does), the
+// advective rate max Σ|u_i|/Δx_i; with the Smagorinsky closure, νt from the
+// state (compute_sgs!) and its interior maximum (0 without); N from the plane
+// means of θ (_brunt_vaisala, :209-232: the largest Boussinesq
+// g/θ₀·Δθ̄/Δz_f between consecutive levels, √ of its positive part; 0 without
+// buoyancy).
+extern "C" int aeo_stepper_cfl(aeo_stepper *h, void *u, void *v, void *w, const void *theta, double *rate,
+                               double *nut_max, double *N) {
+    NEED_CTX();
+    if (!h || !u || !v || !w || !theta || !rate || !nut_max || !N) return fail(AEO_ERR_ARG, "null argument");
+    aeo_grid *g = h->g;
+    const aeo_step_params &sp = h->sp;
+    hipStream_t st = g_ctx->stream;
+    const int Nx = g->Nx, Ny = g->Ny, Nz = g->Nz, H = g->H;
+    const bool f64 = sp.precision == AEO_F64;
+    if (f64) {
+        launch_halo<double>(st, (double *)u, Nx, Ny, Nz, H, LOC_FX, sp.bc_x, sp.bc_y, sp.bc_z);
+        launch_halo<double>(st, (double *)v, Nx, Ny, Nz, H, LOC_FY, sp.bc_x, sp.bc_y, sp.bc_z);
+        launch_halo<double>(st, (double *)w, Nx, Ny, Nz, H, LOC_FZ, sp.bc_x, sp.bc_y, sp.bc_z);
+    } else {
+        launch_halo<float>(st, (float *)u, Nx, Ny, Nz, H, LOC_FX, sp.bc_x, sp.bc_y, sp.bc_z);
+        launch_halo<float>(st, (float *)v, Nx, Ny, Nz, H, LOC_FY, sp.bc_x, sp.bc_y, sp.bc_z);
+        launch_halo<float>(st, (float *)w, Nx, Ny, Nz, H, LOC_FZ, sp.bc_x, sp.bc_y, sp.bc_z);
+    }
+    int rc = run_diag(g, sp.precision, u, v, w, 1, rate);
+    if (rc) return rc;
+    *nut_max = 0;
+    if (h->sgs_lv) {
+        const int n = Nz + 2 * H;
+        if (f64)
+            launch_smag_nut<double>(st, g->m64, (double *)h->nut, (const double *)u, (const double *)v,
+                                    (const double *)w, (const double *)h->sgs_lv, (const double *)h->sgs_lv + n);
+        else
+            launch_smag_nut<float>(st, g->m32, (float *)h->nut, (const float *)u, (const float *)v, (const float *)w,
+                                   (const float *)h->sgs_lv, (const float *)h->sgs_lv + n);
+        rc = run_diag(g, sp.precision, h->nut, h->nut, h->nut, 3, nut_max);
+        if (rc) return rc;
+    }
+    *N = 0;
+    if (sp.theta0 != 0) {
+        double *d = nullptr;
+        HIPCHK(hipMalloc(&d, sizeof(double) * Nz));
+        if (f64) launch_plane_sum<double>(st, d, (const double *)theta, g->m64);
+        else launch_plane_sum<float>(st, d, (const float *)theta, g->m64);
+        std::vector<double> sums(Nz);
+        HIPCHK(hipMemcpyAsync(sums.data(), d, sizeof(double) * Nz, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        hipFree(d);
+        const double np = (double)Nx * Ny;
+        double n2max = 0, prev = 0;
+        for (int k = 0; k < Nz; k++) {
+            const double mean = sums[k] / np;
+            if (k > 0) {
+                // buoyancy_frequency_squared, Boussinesq form (src/TurbulenceStability.jl:131-132),
+                // the plane means Δz_f apart
+                const double dzf = g->dzf[H + k];
+                const double n2 = (sp.g_accel / sp.theta0) * (mean - prev) * (1 / dzf);
+                n2max = n2 > n2max ? n2 : n2max;
+            }
+            prev = mean;
+        }
+        *N = std::sqrt(n2max > 0 ? n2max : 0.0);
+    }
+    return AEO_OK;
+}
+
 extern "C" int aeo_max_divergence(aeo_grid *g, int precision, const void *u, const void *v, const void *w, double *out) {
     return run_diag(g, precision, u, v, w, 0, out);
 }

@@ -53,6 +53,8 @@ template <typename T>
 void launch_shell(hipStream_t st, T *out, const T *x, const T *snap, int Nx, int Ny, int Nz, int H, double ca,
                   double cb);
 template <typename T>
+void launch_plane_sum(hipStream_t st, double *out, const T *f, const Metrics<double> &md);
+template <typename T>
 void launch_diag(hipStream_t st, double *part, int nblocks, const T *u, const T *v, const T *w,
                  const Metrics<double> &md, int what);
 

@@ -577,7 +577,8 @@ template void launch_shell<float>(hipStream_t, float *, const float *, const flo
 // ---------------------------------------------------------------------------
 // Diagnostics over the interior cells, one partial per block (grid-stride
 // over the flattened interior, a tree in LDS): what = 0 max |∇·u|
-// (max_divergence, src/Simulations.jl:1045-1050); 1 max Σ|u_i|/Δx_i
+// (max_divergence, src/Simulations.jl:1045-1050); 3 max of the cell field u;
+// 1 max Σ|u_i|/Δx_i
 // (_max_advective_rate, src/TimeIntegration.jl:164-182); 2 Σ of the kinetic
 // energy density (u² + v²)·Δz_c + w²·Δz_f (kinetic_energy,
 // src/Simulations.jl:992-1015; uniform grids Δz_c = Δz_f = Δz).
@@ -606,6 +607,8 @@ __global__ void __launch_bounds__(256) diag_kernel(double *__restrict__ part, co
             const double av = fmax(fabs((double)v[c]), fabs((double)v[c + sx]));
             const double aw = fmax(fabs((double)w[c]), fabs((double)w[c + (size_t)sx * sy]));
             val = (au * md.idx + av * md.idy) + aw * md.m_idz(K);
+        } else if (what == 3) {
+            val = (double)u[c];  // the maximum of a cell field (the SGS coefficients' maxima)
         } else {
             const double uu = (double)u[c], vv = (double)v[c], ww = (double)w[c];
             if (md.stretched) val = (uu * uu + vv * vv) * (1.0 / md.idzc[K]) + ww * ww * (1.0 / md.idzf[K]);
@@ -626,6 +629,34 @@ __global__ void __launch_bounds__(256) diag_kernel(double *__restrict__ part, co
     }
     if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
 }
+
+// per-level sums of a cell field over the interior plane (one block per
+// level; a fixed-order tree): the plane means of cfl_diagnostics' N
+template <typename T>
+__global__ void __launch_bounds__(256) plane_sum_kernel(double *__restrict__ out, const T *__restrict__ f,
+                                                        Metrics<double> md) {
+    __shared__ double sh[256];
+    const int K = blockIdx.x + md.H;
+    double acc = 0.0;
+    const int np = md.Nx * md.Ny;
+    for (int e = threadIdx.x; e < np; e += blockDim.x) {
+        const int j = e / md.Nx, i = e - j * md.Nx;
+        acc += (double)f[lin(i + md.H, j + md.H, K, md.sx, md.sy)];
+    }
+    sh[threadIdx.x] = acc;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[blockIdx.x] = sh[0];
+}
+template <typename T>
+void launch_plane_sum(hipStream_t st, double *out, const T *f, const Metrics<double> &md) {
+    hipLaunchKernelGGL(plane_sum_kernel<T>, dim3(md.Nz), dim3(256), 0, st, out, f, md);
+}
+template void launch_plane_sum<double>(hipStream_t, double *, const double *, const Metrics<double> &);
+template void launch_plane_sum<float>(hipStream_t, double *, const float *, const Metrics<double> &);
 
 template <typename T>
 void launch_diag(hipStream_t st, double *part, int nblocks, const T *u, const T *v, const T *w,

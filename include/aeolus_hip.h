@@ -199,6 +199,15 @@ int aeo_stepper_set_sponge(aeo_stepper *h, double alpha_max, double z_start, dou
  * (1/Pr_t)∂_j(νt ∂_j θ) on θ (:1280-1300), every substage, after ν∇².
  * C_s = 0 turns the closure off. */
 int aeo_stepper_set_sgs(aeo_stepper *h, double C_s, double Pr_t);
+/* The state maxima of cfl_diagnostics (src/TimeIntegration.jl:291-400):
+ * fills the halos of u, v, w (the velocity BCs), then rate = max Σ|u_i|/Δx_i
+ * (_max_advective_rate), nut_max = max νt of the step's Smagorinsky closure
+ * computed from this state (0 without a closure), N = √max(0, max_k
+ * g/θ₀·(θ̄_k − θ̄_{k−1})/Δz_f[k]) from the plane means θ̄_k of theta
+ * (_brunt_vaisala, :209-232; 0 without buoyancy). The time-step limits
+ * follow on the host (the Python mirror's cfl_diagnostics / adaptive_dt). */
+int aeo_stepper_cfl(aeo_stepper *h, void *u, void *v, void *w, const void *theta, double *rate, double *nut_max,
+                    double *N);
 /* step!(sim, dt) (src/Simulations.jl:474-487): one projected forward-Euler step */
 int aeo_step_euler(aeo_stepper *h, void *u, void *v, void *w, void *theta, void *p, double dt);
 

@@ -4,6 +4,8 @@ FP64 results must agree with the oracle to 1e-12 relative (max |gpu - oracle|
 / max |oracle|); integer/copy work (halo fills) bit for bit. FP32 results
 state their error against the FP64 oracle.
 """
+import ctypes
+
 import numpy as np
 import pytest
 
@@ -235,6 +237,53 @@ def test_rk3_step_smagorinsky_parity_fp64(hip, orc, kind):
     got, ref, og = _run_step(hip, orc, kind, "weno_z5", steps=2, C_s=0.17)
     for n in ("u", "v", "w", "theta", "p"):
         assert rel_err(got[n], ref[n]) <= TOL64, n
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_cfl_diagnostics_parity(hip, orc, prec):
+    # the state maxima of cfl_diagnostics: the advective rate and max νt against
+    # the oracle, N against the plane means of θ in numpy (stretched DCBL grid,
+    # a stable stratification so N > 0); the limits and adaptive_dt from them
+    from aeolus_hip import workloads as W
+    from aeolus_hip.simulation import CFLConstraints, adaptive_dt, cfl_diagnostics, cfl_limits
+    Nxy, Nz = 32, 24
+    zf, Lz = W.dcbl_faces(Nz)
+    ext = (20.0 * Nxy, 20.0 * Nxy, Lz)
+    g = hip.Grid((Nxy, Nxy, Nz), ext, halo=3, zf=zf)
+    og = orc.Grid((Nxy, Nxy, Nz), ext, halo=3, zf=zf)
+    u, v, w, th = W.dcbl_state(g)
+    rng = np.random.default_rng(17)
+    I = og.interior
+    for a in (u, v, w):
+        a[I] += 0.7 * rng.uniform(-1, 1, size=a[I].shape)
+    zc = 0.5 * (zf[:-1] + zf[1:])
+    th[I] += 0.004 * zc[None, None, :]
+    u, v, w, th = (np.asfortranarray(a) for a in (u, v, w, th))
+    phys = dict(W.DCBL_PHYS, nu=0.5)
+    sim = hip.BoussinesqSimulation(g, scheme="weno_z5", precision=prec, C_s=0.17, f_coriolis=1e-4, **phys)
+    sim.set(u, v, w, th)
+    d = cfl_diagnostics(sim)
+    bcs = phys["bcs"]
+    for a, loc in ((u, orc.FX), (v, orc.FY), (w, orc.FZ)):
+        orc.fill_halos(og, a, loc, bcs)
+    tol = 1e-13 if prec == 0 else 1e-5
+    rate = orc.lib().orc_max_adv_rate(orc.ptr(u), orc.ptr(v), orc.ptr(w), ctypes.byref(og.c))
+    assert abs(d["advective_rate"] - rate) <= tol * rate
+    ost = orc.Stepper(og, scheme="weno_z5", **phys)
+    ost.set_sgs(0.17)
+    nut = ost.sgs_nut(u, v, w)[I].max()
+    assert nut > 0 and abs(d["nut_max"] - nut) <= tol * nut
+    means = th[I].mean(axis=(0, 1))
+    dzf = g.interior_dz()[1]
+    n2 = (9.81 / 300.0) * np.diff(means) / dzf[1:Nz]
+    N = np.sqrt(max(n2.max(), 0.0))
+    assert N > 0 and abs(d["N"] - N) <= (1e-12 if prec == 0 else 1e-4) * N
+    ref = cfl_limits(rate, nut, N, sim.parameters(), g, CFLConstraints())
+    assert d["limiter"] == ref["limiter"]
+    assert d["dt_limiting"] == pytest.approx(ref["dt_limiting"], rel=1e-4)
+    c = CFLConstraints(dt_max=1e9)
+    dt = adaptive_dt(sim, c, dt_prev=1e-3)
+    assert dt == pytest.approx(min(1.1e-3, ref["dt_limiting"]), rel=1e-4)
 
 
 def test_euler_step_parity_fp64(hip, orc):
