@@ -13,7 +13,10 @@ from .fields import DeviceArray, Grid, faces_geometric, faces_uniform, fill_halo
 from . import learned, workloads
 from .pressure import (ColumnTridiagonalSolver, FFTPoissonSolver, MGPreconditionedCG, MultigridPoissonSolver, solve,
                        solve_interior, solve_source_buffer, solver_for, solver_stats, source_buffer, stretched_operator)
-from .simulation import BoussinesqSimulation, damping_coordinates, sponge_levels
+from .simulation import (BoussinesqSimulation, CFLConstraints, adaptive_dt, cfl_diagnostics, damping_coordinates,
+                         sponge_levels)
+from .timestepping import (NumericalFailure, StateSnapshot, SubsteppingSchedule, is_numerical_failure,
+                           restore_state, save_state, state_snapshot, step_adaptive, step_guard, substeps_for)
 from .state_io import checkpoint, read_checkpoint, restart
 
 __all__ = [
@@ -21,5 +24,7 @@ __all__ = [
     "DeviceArray", "Grid", "faces_geometric", "faces_uniform", "fill_halos", "ColumnTridiagonalSolver",
     "FFTPoissonSolver", "MGPreconditionedCG", "MultigridPoissonSolver", "stretched_operator", "solve", "solve_interior", "solve_source_buffer", "solver_for", "solver_stats",
     "source_buffer", "BoussinesqSimulation", "damping_coordinates", "sponge_levels", "checkpoint",
-    "read_checkpoint", "restart",
+    "read_checkpoint", "restart", "CFLConstraints", "adaptive_dt", "cfl_diagnostics", "NumericalFailure",
+    "StateSnapshot", "SubsteppingSchedule", "is_numerical_failure", "restore_state", "save_state", "state_snapshot",
+    "step_adaptive", "step_guard", "substeps_for",
 ]

@@ -91,6 +91,9 @@ SIGNATURES = {
     "aeo_stepper_set_coriolis": [_vp, _d],
     "aeo_stepper_set_sgs": [_vp, _d, _d],
     "aeo_stepper_cfl": [_vp, _vp, _vp, _vp, _vp, _pd, _pd, _pd],
+    "aeo_stepper_guard": [_vp, _vp, _vp, _vp, _vp, _vp, ctypes.POINTER(ctypes.c_int), _pd, _pd],
+    "aeo_mw_stepper_guard": [_vp, _vp, _vp, _vp, _vp, _vp, ctypes.POINTER(ctypes.c_int), _pd, _pd],
+    "aeo_mw_stepper_cfl": [_vp, _vp, _vp, _vp, _vp, _pd, _pd, _pd],
     "aeo_stepper_set_sponge": [_vp, _d, _d, _d, _pd, _pd, _pd, _pd, _pd],
     "aeo_step_rk3": [_vp, _vp, _vp, _vp, _vp, _vp, _d],
     "aeo_step_euler": [_vp, _vp, _vp, _vp, _vp, _vp, _d],

@@ -245,8 +245,12 @@ def cfl_limits(rate, nut_max, N, p, g, c, limits=()):
 def adaptive_dt(sim, c=None, dt_prev=None, diagnostics=None, limits=()):
     """adaptive_dt (src/TimeIntegration.jl:436-466): the binding limit, then the
     growth bound against dt_prev, then the clamp to [dt_min, dt_max]; a NaN
-    limit collapses to dt_min."""
+    limit collapses to dt_min. On a sound-proof core, that core's own method
+    (src/Anelastic.jl:2240-2300)."""
     c = c or CFLConstraints()
+    if diagnostics is None and getattr(sim, "weighting", None) is not None:
+        from .timestepping import mw_adaptive_dt  # the sound-proof cores' method
+        return mw_adaptive_dt(sim, c, dt_prev, limits)
     d = diagnostics or cfl_diagnostics(sim, c, limits)
     dt = d["dt_limiting"]
     if math.isnan(dt):

@@ -282,9 +282,12 @@ __global__ void copy_kernel(uint4 *__restrict__ dst, const uint4 *__restrict__ s
 extern "C" int aeo_memcpy_d2d(void *dst, const void *src, size_t bytes) {
     NEED_CTX();
     if (!dst || !src) return fail(AEO_ERR_ARG, "null argument");
-    if (bytes % 16 || ((uintptr_t)dst | (uintptr_t)src) % 16)
-        return fail(AEO_ERR_ARG, "aeo_memcpy_d2d: 16-byte aligned pointers and sizes");
     Timed t("copy");
+    if (bytes % 16 || ((uintptr_t)dst | (uintptr_t)src) % 16) {
+        // sizes or pointers off the 16-byte grid: the runtime's copy, same stream
+        HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, g_ctx->stream));
+        return AEO_OK;
+    }
     size_t n = bytes / 16;
     hipLaunchKernelGGL(copy_kernel, dim3(4096), dim3(256), 0, g_ctx->stream, (uint4 *)dst, (const uint4 *)src, n);
     HIPCHK(hipGetLastError());
@@ -1571,11 +1574,148 @@ extern "C" int aeo_stepper_cfl(aeo_stepper *h, void *u, void *v, void *w, const 
                 // the plane means Δz_f apart
                 const double dzf = g->dzf[H + k];
                 const double n2 = (sp.g_accel / sp.theta0) * (mean - prev) * (1 / dzf);
-                n2max = n2 > n2max ? n2 : n2max;
+                n2max = (n2 > n2max || n2 != n2) ? n2 : n2max;  // max propagates NaN
             }
             prev = mean;
         }
-        *N = std::sqrt(n2max > 0 ? n2max : 0.0);
+        *N = n2max != n2max ? n2max : std::sqrt(n2max > 0 ? n2max : 0.0);
+    }
+    return AEO_OK;
+}
+
+// the guard sweep and its reduction over the block partials (NaN-propagating)
+static int run_guard(aeo_grid *g, int precision, const void *u, const void *v, const void *w, const void *th,
+                     const void *p, const double *Wc, const double *Wf, int *finite, double *div, double *speed) {
+    constexpr int NB = 512;
+    double *d = nullptr;
+    HIPCHK(hipMalloc(&d, sizeof(double) * 3 * NB));
+    if (precision == AEO_F64)
+        launch_guard<double>(g_ctx->stream, d, NB, (const double *)u, (const double *)v, (const double *)w,
+                             (const double *)th, (const double *)p, Wc, Wf, g->m64);
+    else
+        launch_guard<float>(g_ctx->stream, d, NB, (const float *)u, (const float *)v, (const float *)w,
+                            (const float *)th, (const float *)p, Wc, Wf, g->m64);
+    std::vector<double> hp(3 * NB);
+    const hipError_t e1 = hipMemcpyAsync(hp.data(), d, sizeof(double) * 3 * NB, hipMemcpyDeviceToHost, g_ctx->stream);
+    const hipError_t e2 = e1 == hipSuccess ? hipStreamSynchronize(g_ctx->stream) : e1;
+    hipFree(d);
+    HIPCHK(e2);
+    double r[3] = {0, 0, 0};
+    for (int b = 0; b < NB; b++)
+        for (int q = 0; q < 3; q++) {
+            const double x = hp[3 * b + q];
+            r[q] = (x > r[q] || x != x) ? x : r[q];
+        }
+    *finite = r[0] == 0;
+    *div = r[1];
+    *speed = r[2];
+    return AEO_OK;
+}
+
+// step_guard(sim) (src/TimeIntegration.jl:673-710): the velocity halos filled
+// with the stepper's BCs, then finite (every interior u, v, w, θ, p value),
+// max |∇·u| and max(|u|, |v|, |w|) in one sweep
+extern "C" int aeo_stepper_guard(aeo_stepper *h, void *u, void *v, void *w, const void *theta, const void *p,
+                                 int *finite, double *divergence, double *speed) {
+    NEED_CTX();
+    if (!h || !u || !v || !w || !theta || !p || !finite || !divergence || !speed)
+        return fail(AEO_ERR_ARG, "null argument");
+    aeo_grid *g = h->g;
+    const aeo_step_params &sp = h->sp;
+    const int Nx = g->Nx, Ny = g->Ny, Nz = g->Nz, H = g->H;
+    hipStream_t st = g_ctx->stream;
+    if (sp.precision == AEO_F64) {
+        launch_halo<double>(st, (double *)u, Nx, Ny, Nz, H, LOC_FX, sp.bc_x, sp.bc_y, sp.bc_z);
+        launch_halo<double>(st, (double *)v, Nx, Ny, Nz, H, LOC_FY, sp.bc_x, sp.bc_y, sp.bc_z);
+        launch_halo<double>(st, (double *)w, Nx, Ny, Nz, H, LOC_FZ, sp.bc_x, sp.bc_y, sp.bc_z);
+    } else {
+        launch_halo<float>(st, (float *)u, Nx, Ny, Nz, H, LOC_FX, sp.bc_x, sp.bc_y, sp.bc_z);
+        launch_halo<float>(st, (float *)v, Nx, Ny, Nz, H, LOC_FY, sp.bc_x, sp.bc_y, sp.bc_z);
+        launch_halo<float>(st, (float *)w, Nx, Ny, Nz, H, LOC_FZ, sp.bc_x, sp.bc_y, sp.bc_z);
+    }
+    return run_guard(g, sp.precision, u, v, w, theta, p, nullptr, nullptr, finite, divergence, speed);
+}
+
+// step_guard(sim::MassWeightedSimulation) (src/Anelastic.jl:2333-2346): the
+// divergence is the core's constraint residual max |(1/W_c)∇·(W u)|
+// (constraint_residual, :2035-2044), W = ρ̄ (anelastic) or ρ̄θ̄ (pseudo-incompressible)
+extern "C" int aeo_mw_stepper_guard(aeo_mw_stepper *h, void *u, void *v, void *w, const void *theta, const void *p,
+                                    int *finite, double *divergence, double *speed) {
+    NEED_CTX();
+    if (!h || !u || !v || !w || !theta || !p || !finite || !divergence || !speed)
+        return fail(AEO_ERR_ARG, "null argument");
+    aeo_grid *g = h->g;
+    const aeo_mw_params &sp = h->sp;
+    const int Nx = g->Nx, Ny = g->Ny, Nz = g->Nz, H = g->H;
+    hipStream_t st = g_ctx->stream;
+    if (sp.precision == AEO_F64) {
+        launch_halo<double>(st, (double *)u, Nx, Ny, Nz, H, LOC_FX, sp.bc_x, sp.bc_y, sp.bc_z);
+        launch_halo<double>(st, (double *)v, Nx, Ny, Nz, H, LOC_FY, sp.bc_x, sp.bc_y, sp.bc_z);
+        launch_halo<double>(st, (double *)w, Nx, Ny, Nz, H, LOC_FZ, sp.bc_x, sp.bc_y, sp.bc_z);
+    } else {
+        launch_halo<float>(st, (float *)u, Nx, Ny, Nz, H, LOC_FX, sp.bc_x, sp.bc_y, sp.bc_z);
+        launch_halo<float>(st, (float *)v, Nx, Ny, Nz, H, LOC_FY, sp.bc_x, sp.bc_y, sp.bc_z);
+        launch_halo<float>(st, (float *)w, Nx, Ny, Nz, H, LOC_FZ, sp.bc_x, sp.bc_y, sp.bc_z);
+    }
+    return run_guard(g, sp.precision, u, v, w, theta, p, h->Wc, h->Wf, finite, divergence, speed);
+}
+
+// The state maxima of adaptive_dt on a sound-proof core (src/Anelastic.jl:
+// 2240-2290): the velocity halos filled, the advective rate; with the
+// Smagorinsky closure νt recomputed from the state and its interior maximum
+// (0 without); N from the plane means of θ with the local mean θ as the
+// reference, (g/θ̄_k)(θ̄_k − θ̄_{k−1})/Δz, θ̄_k = (θ̄_{k−1} + θ̄_k)/2 (0 without buoyancy).
+extern "C" int aeo_mw_stepper_cfl(aeo_mw_stepper *h, void *u, void *v, void *w, const void *theta, double *rate,
+                                  double *nut_max, double *N) {
+    NEED_CTX();
+    if (!h || !u || !v || !w || !theta || !rate || !nut_max || !N) return fail(AEO_ERR_ARG, "null argument");
+    aeo_grid *g = h->g;
+    const aeo_mw_params &sp = h->sp;
+    hipStream_t st = g_ctx->stream;
+    const int Nx = g->Nx, Ny = g->Ny, Nz = g->Nz, H = g->H;
+    const bool f64 = sp.precision == AEO_F64;
+    if (f64) {
+        launch_halo<double>(st, (double *)u, Nx, Ny, Nz, H, LOC_FX, sp.bc_x, sp.bc_y, sp.bc_z);
+        launch_halo<double>(st, (double *)v, Nx, Ny, Nz, H, LOC_FY, sp.bc_x, sp.bc_y, sp.bc_z);
+        launch_halo<double>(st, (double *)w, Nx, Ny, Nz, H, LOC_FZ, sp.bc_x, sp.bc_y, sp.bc_z);
+    } else {
+        launch_halo<float>(st, (float *)u, Nx, Ny, Nz, H, LOC_FX, sp.bc_x, sp.bc_y, sp.bc_z);
+        launch_halo<float>(st, (float *)v, Nx, Ny, Nz, H, LOC_FY, sp.bc_x, sp.bc_y, sp.bc_z);
+        launch_halo<float>(st, (float *)w, Nx, Ny, Nz, H, LOC_FZ, sp.bc_x, sp.bc_y, sp.bc_z);
+    }
+    int rc = run_diag(g, sp.precision, u, v, w, 1, rate);
+    if (rc) return rc;
+    *nut_max = 0;
+    if (h->sgs_lv) {
+        const int n = Nz + 2 * H;
+        if (f64)
+            launch_smag_nut<double>(st, g->m64, (double *)h->nut, (const double *)u, (const double *)v,
+                                    (const double *)w, (const double *)h->sgs_lv, (const double *)h->sgs_lv + n);
+        else
+            launch_smag_nut<float>(st, g->m32, (float *)h->nut, (const float *)u, (const float *)v, (const float *)w,
+                                   (const float *)h->sgs_lv, (const float *)h->sgs_lv + n);
+        rc = run_diag(g, sp.precision, h->nut, h->nut, h->nut, 3, nut_max);
+        if (rc) return rc;
+    }
+    *N = 0;
+    if (sp.buoyancy) {
+        double *d = nullptr;
+        HIPCHK(hipMalloc(&d, sizeof(double) * Nz));
+        if (f64) launch_plane_sum<double>(st, d, (const double *)theta, g->m64);
+        else launch_plane_sum<float>(st, d, (const float *)theta, g->m64);
+        std::vector<double> sums(Nz);
+        const hipError_t e1 = hipMemcpyAsync(sums.data(), d, sizeof(double) * Nz, hipMemcpyDeviceToHost, st);
+        const hipError_t e2 = e1 == hipSuccess ? hipStreamSynchronize(st) : e1;
+        hipFree(d);
+        HIPCHK(e2);
+        const double np = (double)Nx * Ny;
+        double n2max = 0;
+        for (int k = 1; k < Nz; k++) {
+            const double lo = sums[k - 1] / np, hi = sums[k] / np;
+            const double n2 = (sp.g_accel / ((lo + hi) / 2)) * (hi - lo) * (1 / g->dz);
+            n2max = (n2 > n2max || n2 != n2) ? n2 : n2max;
+        }
+        *N = std::sqrt(n2max);  // NaN stays NaN
     }
     return AEO_OK;
 }

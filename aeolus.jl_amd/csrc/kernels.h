@@ -55,6 +55,9 @@ void launch_shell(hipStream_t st, T *out, const T *x, const T *snap, int Nx, int
 template <typename T>
 void launch_plane_sum(hipStream_t st, double *out, const T *f, const Metrics<double> &md);
 template <typename T>
+void launch_guard(hipStream_t st, double *part, int nblocks, const T *u, const T *v, const T *w, const T *th,
+                  const T *p, const double *Wc, const double *Wf, const Metrics<double> &md);
+template <typename T>
 void launch_diag(hipStream_t st, double *part, int nblocks, const T *u, const T *v, const T *w,
                  const Metrics<double> &md, int what);
 

@@ -582,6 +582,9 @@ template void launch_shell<float>(hipStream_t, float *, const float *, const flo
 // (_max_advective_rate, src/TimeIntegration.jl:164-182); 2 Σ of the kinetic
 // energy density (u² + v²)·Δz_c + w²·Δz_f (kinetic_energy,
 // src/Simulations.jl:992-1015; uniform grids Δz_c = Δz_f = Δz).
+// max(a, b) that propagates NaN, as Julia's max does (fmax drops it)
+__device__ inline double nanmax(double a, double b) { return (b > a || b != b) ? b : a; }
+
 template <typename T>
 __global__ void __launch_bounds__(256) diag_kernel(double *__restrict__ part, const T *__restrict__ u,
                                                    const T *__restrict__ v, const T *__restrict__ w,
@@ -603,9 +606,9 @@ __global__ void __launch_bounds__(256) diag_kernel(double *__restrict__ part, co
             const double dw = (double)w[c + (size_t)sx * sy] - (double)w[c];
             val = fabs((du * md.idx + dv * md.idy) + dw * md.m_idz(K));
         } else if (what == 1) {
-            const double au = fmax(fabs((double)u[c]), fabs((double)u[c + 1]));
-            const double av = fmax(fabs((double)v[c]), fabs((double)v[c + sx]));
-            const double aw = fmax(fabs((double)w[c]), fabs((double)w[c + (size_t)sx * sy]));
+            const double au = nanmax(fabs((double)u[c]), fabs((double)u[c + 1]));
+            const double av = nanmax(fabs((double)v[c]), fabs((double)v[c + sx]));
+            const double aw = nanmax(fabs((double)w[c]), fabs((double)w[c + (size_t)sx * sy]));
             val = (au * md.idx + av * md.idy) + aw * md.m_idz(K);
         } else if (what == 3) {
             val = (double)u[c];  // the maximum of a cell field (the SGS coefficients' maxima)
@@ -657,6 +660,69 @@ void launch_plane_sum(hipStream_t st, double *out, const T *f, const Metrics<dou
 }
 template void launch_plane_sum<double>(hipStream_t, double *, const double *, const Metrics<double> &);
 template void launch_plane_sum<float>(hipStream_t, double *, const float *, const Metrics<double> &);
+
+// The post-step checks of step_adaptive! in one sweep over the interior
+// (step_guard, src/TimeIntegration.jl:673-710; the sound-proof cores'
+// step_guard, src/Anelastic.jl:2333-2346). Per block three partials:
+// part[3b] = 1 where some u, v, w, θ or p value is not finite (0 otherwise),
+// part[3b+1] = max |∇·u| (Wc == nullptr: the stencil of max_divergence) or
+// max |(1/W_c)∇·(W u)| (the constraint residual of a sound-proof core, the
+// stencil of weighted_divergence!), part[3b+2] = max(|u|, |v|, |w|). The
+// maxima propagate NaN, as the reference's range_max does.
+template <typename T>
+__global__ void __launch_bounds__(256) guard_kernel(double *__restrict__ part, const T *__restrict__ u,
+                                                    const T *__restrict__ v, const T *__restrict__ w,
+                                                    const T *__restrict__ th, const T *__restrict__ p,
+                                                    const double *__restrict__ Wc, const double *__restrict__ Wf,
+                                                    Metrics<double> md) {
+    __shared__ double sh[3][256];
+    const int H = md.H, sx = md.sx, sy = md.sy;
+    const size_t sxy = (size_t)sx * sy;
+    const long n = (long)md.Nx * md.Ny * md.Nz;
+    double bad = 0.0, dmax = 0.0, smax = 0.0;
+    for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x) {
+        const int i = (int)(e % md.Nx);
+        const long r = e / md.Nx;
+        const int j = (int)(r % md.Ny), k = (int)(r / md.Ny);
+        const int K = k + H;
+        const size_t c = lin(i + H, j + H, K, sx, sy);
+        const double uc = (double)u[c], vc = (double)v[c], wc = (double)w[c];
+        const bool fin = isfinite(uc) && isfinite(vc) && isfinite(wc) && isfinite((double)th[c]) &&
+                         isfinite((double)p[c]);
+        bad = fin ? bad : 1.0;
+        const double du = (double)u[c + 1] - uc, dv = (double)v[c + sx] - vc;
+        double d;
+        if (Wc) {
+            const double wk = Wc[K];
+            d = (1 / wk) * (wk * (du * md.idx + dv * md.idy) + (Wf[K + 1] * (double)w[c + sxy] - Wf[K] * wc) * md.idz);
+        } else {
+            d = (du * md.idx + dv * md.idy) + ((double)w[c + sxy] - wc) * md.m_idz(K);
+        }
+        dmax = nanmax(dmax, fabs(d));
+        smax = nanmax(smax, nanmax(nanmax(fabs(uc), fabs(vc)), fabs(wc)));
+    }
+    sh[0][threadIdx.x] = bad;
+    sh[1][threadIdx.x] = dmax;
+    sh[2][threadIdx.x] = smax;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s)
+            for (int q = 0; q < 3; q++) sh[q][threadIdx.x] = nanmax(sh[q][threadIdx.x], sh[q][threadIdx.x + s]);
+        __syncthreads();
+    }
+    if (threadIdx.x < 3) part[3 * blockIdx.x + threadIdx.x] = sh[threadIdx.x][0];
+}
+template <typename T>
+void launch_guard(hipStream_t st, double *part, int nblocks, const T *u, const T *v, const T *w, const T *th,
+                  const T *p, const double *Wc, const double *Wf, const Metrics<double> &md) {
+    hipLaunchKernelGGL(guard_kernel<T>, dim3(nblocks), dim3(256), 0, st, part, u, v, w, th, p, Wc, Wf, md);
+}
+template void launch_guard<double>(hipStream_t, double *, int, const double *, const double *, const double *,
+                                   const double *, const double *, const double *, const double *,
+                                   const Metrics<double> &);
+template void launch_guard<float>(hipStream_t, double *, int, const float *, const float *, const float *,
+                                  const float *, const float *, const double *, const double *,
+                                  const Metrics<double> &);
 
 template <typename T>
 void launch_diag(hipStream_t st, double *part, int nblocks, const T *u, const T *v, const T *w,

@@ -68,8 +68,9 @@ int aeo_free(void *dptr);
 int aeo_memcpy_h2d(void *dst, const void *src, size_t bytes);
 int aeo_memcpy_d2h(void *dst, const void *src, size_t bytes);
 int aeo_memset(void *dptr, int value, size_t bytes);
-/* device-to-device copy by a plain streaming copy kernel (16-byte aligned);
- * also the reference kernel for HBM-traffic counter calibration */
+/* device-to-device copy on the library's stream: a plain streaming copy kernel
+ * when pointers and size are 16-byte aligned (also the kernel that calibrates
+ * the HBM-traffic counters), the runtime's copy otherwise */
 int aeo_memcpy_d2d(void *dst, const void *src, size_t bytes);
 
 /* ---- grids ---------------------------------------------------------------
@@ -253,6 +254,27 @@ int aeo_mw_step_rk3(aeo_mw_stepper *h, void *u, void *v, void *w, void *theta, v
  * out's interior; Wc, Wf device arrays (Nz + 2H + 1 entries) */
 int aeo_weighted_divergence(aeo_grid *g, int precision, void *out, const void *u, const void *v, const void *w,
                             const double *Wc, const double *Wf, int normalize);
+
+/* ---- the adaptive step's checks ------------------------------------------- */
+/* step_guard(sim) (src/TimeIntegration.jl:673-710): fills the velocity halos
+ * with the stepper's BCs, then in one sweep over the interior: *finite = 1
+ * when every u, v, w, θ and p value is finite, *divergence = max |∇·u| (the
+ * bits of aeo_max_divergence), *speed = max(|u|, |v|, |w|); both maxima
+ * propagate NaN */
+int aeo_stepper_guard(aeo_stepper *h, void *u, void *v, void *w, const void *theta, const void *p, int *finite,
+                      double *divergence, double *speed);
+/* step_guard(sim::MassWeightedSimulation) (src/Anelastic.jl:2333-2346): the
+ * same, with *divergence the core's constraint residual max |(1/W_c)∇·(W u)|
+ * (constraint_residual, :2035-2044) */
+int aeo_mw_stepper_guard(aeo_mw_stepper *h, void *u, void *v, void *w, const void *theta, const void *p,
+                         int *finite, double *divergence, double *speed);
+/* the state maxima of adaptive_dt(sim::MassWeightedSimulation)
+ * (src/Anelastic.jl:2240-2290): the advective rate, max νt of the closure
+ * recomputed from the state (0 without one) and N from the plane means of θ
+ * with their local mean as θ_ref (0 without buoyancy); the velocity halos are
+ * filled */
+int aeo_mw_stepper_cfl(aeo_mw_stepper *h, void *u, void *v, void *w, const void *theta, double *rate,
+                       double *nut_max, double *N);
 
 /* ---- diagnostics (halos of the inputs filled) ------------------------------ */
 /* max_divergence (src/Simulations.jl:1045-1050) */

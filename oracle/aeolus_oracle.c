@@ -487,6 +487,8 @@ double orc_max_divergence(const double *u, const double *v, const double *w, con
 }
 
 /* _adv_rate_at, src/TimeIntegration.jl:172-182 */
+static double nan_max(double a, double b) { return (b > a || b != b) ? b : a; }
+
 double orc_max_adv_rate(const double *u, const double *v, const double *w, const orc_grid *g) {
     GRID_DIMS(g);
     int H = g->H;
@@ -494,11 +496,12 @@ double orc_max_adv_rate(const double *u, const double *v, const double *w, const
     for (int k = H + 1; k <= H + g->Nz; k++)
         for (int j = H + 1; j <= H + g->Ny; j++)
             for (int i = H + 1; i <= H + g->Nx; i++) {
-                double au = fmax(fabs(u[ID(i, j, k)]), fabs(u[ID(i + 1, j, k)]));
-                double av = fmax(fabs(v[ID(i, j, k)]), fabs(v[ID(i, j + 1, k)]));
-                double aw = fmax(fabs(w[ID(i, j, k)]), fabs(w[ID(i, j, k + 1)]));
+                /* Julia's max propagates NaN (fmax would drop it) */
+                double au = nan_max(fabs(u[ID(i, j, k)]), fabs(u[ID(i + 1, j, k)]));
+                double av = nan_max(fabs(v[ID(i, j, k)]), fabs(v[ID(i, j + 1, k)]));
+                double aw = nan_max(fabs(w[ID(i, j, k)]), fabs(w[ID(i, j, k + 1)]));
                 double r = (au * (1.0 / g->dx) + av * (1.0 / g->dy)) + aw * m_idz(g, k);
-                if (r > m) m = r;
+                m = nan_max(m, r);
             }
     return m;
 }

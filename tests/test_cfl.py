@@ -67,3 +67,39 @@ def test_adaptive_dt_growth_clamp_and_nan():
     assert adaptive_dt(None, c, diagnostics=d(1e-6)) == 1e-3
     assert adaptive_dt(None, c, diagnostics=d(math.nan)) == 1e-3
     assert adaptive_dt(None, c, diagnostics=d(math.inf)) == 2.0
+
+
+def _rate(orc, og, u, v, w):
+    import ctypes
+    return orc.lib().orc_max_adv_rate(orc.ptr(u), orc.ptr(v), orc.ptr(w), ctypes.byref(og.c))
+
+
+@pytest.mark.parametrize("stretched", [False, True])
+def test_oracle_advective_rate(orc, stretched):
+    # _max_advective_rate (src/TimeIntegration.jl:164-182): max over cells of
+    # Σ max(|u_face_lo|, |u_face_hi|)/Δx_i, the thinnest cell binding on a stretched grid
+    import numpy as np
+    zf = np.array([0.0, 1.0, 3.0, 6.0, 10.0, 15.0, 21.0]) if stretched else None
+    og = orc.Grid((6, 5, 6), (3.0, 10.0, 21.0), halo=2, zf=zf)
+    H = og.H
+    dz = np.diff(zf) if stretched else np.full(6, 3.5)
+    u, v, w = (og.zeros() for _ in range(3))
+    u[...], v[...], w[...] = 2.0, -3.0, 0.5
+    want = 2.0 / 0.5 + 3.0 / 2.0 + 0.5 / dz.min()          # uniform flow: closed form
+    assert _rate(orc, og, u, v, w) == pytest.approx(want, rel=1e-15)
+    rng = np.random.default_rng(4)
+    for a in (u, v, w):
+        a[...] = rng.standard_normal(a.shape)
+    I = slice(H, H + 6), slice(H, H + 5), slice(H, H + 6)
+    au = np.maximum(np.abs(u[H:H + 6, I[1], I[2]]), np.abs(u[H + 1:H + 7, I[1], I[2]]))
+    av = np.maximum(np.abs(v[I[0], H:H + 5, I[2]]), np.abs(v[I[0], H + 1:H + 6, I[2]]))
+    aw = np.maximum(np.abs(w[I[0], I[1], H:H + 6]), np.abs(w[I[0], I[1], H + 1:H + 7]))
+    brute = (au / 0.5 + av / 2.0 + aw / dz[None, None, :]).max()
+    assert _rate(orc, og, u, v, w) == pytest.approx(brute, rel=1e-14)
+    # a NaN anywhere on the faces the interior reads makes the rate NaN
+    # (the dt pick then collapses to dt_min), also on the last face of a row
+    for a, idx in ((u, (H + 6, H + 1, H + 1)), (v, (H + 2, H, H + 3)), (w, (H, H + 4, H + 6))):
+        keep = a[idx]
+        a[idx] = np.nan
+        assert math.isnan(_rate(orc, og, u, v, w))
+        a[idx] = keep
