@@ -91,6 +91,9 @@ SIGNATURES = {
     "aeo_stepper_set_coriolis": [_vp, _d],
     "aeo_stepper_set_sgs": [_vp, _d, _d],
     "aeo_stepper_cfl": [_vp, _vp, _vp, _vp, _vp, _pd, _pd, _pd],
+    "aeo_flux_limiter_workspace_create": [_vp, _i, ctypes.POINTER(_vp)],
+    "aeo_flux_limiter_workspace_destroy": [_vp],
+    "aeo_advect_positive": [_vp, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _d, _vp, ctypes.POINTER(_i), _i],
     "aeo_stepper_guard": [_vp, _vp, _vp, _vp, _vp, _vp, ctypes.POINTER(ctypes.c_int), _pd, _pd],
     "aeo_mw_stepper_guard": [_vp, _vp, _vp, _vp, _vp, _vp, ctypes.POINTER(ctypes.c_int), _pd, _pd],
     "aeo_mw_stepper_cfl": [_vp, _vp, _vp, _vp, _vp, _pd, _pd, _pd],

@@ -73,3 +73,47 @@ def momentum_advect_weno_z(grid, tu, tv, tw, u, v, w, order=5, p=2):
     if order not in (5, 7):
         raise ValueError(f"WENO-Z order must be 5 or 7 (got {order})")
     return AdvectionScheme("weno_z5" if order == 5 else "weno_z7", grid, p).momentum(tu, tv, tw, u, v, w)
+
+
+class FluxLimiterWorkspace:
+    """FluxLimiterWorkspace(mesh) (src/AdvectionSchemes.jl:800-822): the
+    reconstructed and upwind face fluxes and the limiter ratio of
+    advect_positive, device arrays of one grid and precision, allocated once."""
+
+    def __init__(self, grid, precision=L.F64):
+        import ctypes
+        self.grid, self.precision = grid, precision
+        h = ctypes.c_void_p()
+        L.check(L.init().aeo_flux_limiter_workspace_create(grid.handle, precision, ctypes.byref(h)))
+        self._h = h.value
+
+    def __del__(self):
+        try:
+            if getattr(self, "_h", None) and L._lib is not None and L._initialised:
+                L._lib.aeo_flux_limiter_workspace_destroy(self._h)
+                self._h = None
+        except Exception:
+            pass
+
+
+def advect_positive(grid, tend, u, v, w, phi, dt, scheme="weno_z5", p=2, workspace=None, bcs=None, accumulate=True):
+    """advect_positive!(tend, u, v, w, ϕ, dt; scheme, p, workspace, bcs,
+    accumulate) (src/AdvectionSchemes.jl:1135-1160): −∂(uⱼϕ)/∂xⱼ with the face
+    fluxes limited so that ϕ + dt·tend stays ≥ 0 for ϕ ≥ 0 under the
+    first-order CFL condition. scheme: centered, weno5 (WENO-JS5 here),
+    weno_js5, weno_z5, weno_z7 (centered only on a stretched grid). bcs: the
+    (x, y, z) conditions of ϕ for the limiter's halo, or None for a one-layer
+    periodic wrap. accumulate=False writes the interior of tend instead of
+    adding to it."""
+    import ctypes
+    from .fields import BC_NAMES
+    if scheme not in ("centered", "weno5", "weno_js5", "weno_z5", "weno_z7"):
+        raise ValueError(f"unknown reconstruction {scheme!r}; expected one of centered, weno5, weno_js5, weno_z5, "
+                         "weno_z7")
+    if workspace is not None and (workspace.grid is not grid or workspace.precision != tend.precision):
+        raise ValueError("the workspace belongs to another grid or precision")
+    bc = None if bcs is None else (ctypes.c_int * 3)(*[BC_NAMES[b] for b in bcs])
+    L.check(L.init().aeo_advect_positive(grid.handle, tend.precision, SCHEMES[scheme][0], int(p), L.dptr(tend),
+                                         L.dptr(u), L.dptr(v), L.dptr(w), L.dptr(phi), float(dt),
+                                         None if workspace is None else workspace._h, bc, int(bool(accumulate))))
+    return tend

@@ -360,6 +360,94 @@ extern "C" int aeo_advect_scalar(aeo_grid *g, int precision, int scheme, int p, 
     return AEO_OK;
 }
 
+// FluxLimiterWorkspace(mesh) (src/AdvectionSchemes.jl:800-822): the face
+// fluxes F (x, y, z), the upwind fluxes L (x, y, z) and the ratio R, seven
+// storage-sized arrays of the grid at one precision
+struct aeo_flux_workspace {
+    const aeo_grid *g;
+    int precision;
+    void *a[7];
+};
+
+extern "C" int aeo_flux_limiter_workspace_create(aeo_grid *g, int precision, aeo_flux_workspace **out) {
+    NEED_CTX();
+    if (!g || !out) return fail(AEO_ERR_ARG, "null argument");
+    if (precision != AEO_F64 && precision != AEO_F32) return fail(AEO_ERR_ARG, "bad precision %d", precision);
+    auto *ws = new aeo_flux_workspace{g, precision, {}};
+    const size_t bytes = (size_t)(g->Nx + 2 * g->H) * (g->Ny + 2 * g->H) * (g->Nz + 2 * g->H) *
+                         (precision == AEO_F64 ? 8 : 4);
+    for (int q = 0; q < 7; q++) {
+        if (hipMalloc(&ws->a[q], bytes) != hipSuccess || hipMemset(ws->a[q], 0, bytes) != hipSuccess) {
+            for (int r = 0; r <= q; r++)
+                if (ws->a[r]) hipFree(ws->a[r]);
+            delete ws;
+            return fail(AEO_ERR_HIP, "flux limiter workspace: out of device memory");
+        }
+    }
+    *out = ws;
+    return AEO_OK;
+}
+
+extern "C" int aeo_flux_limiter_workspace_destroy(aeo_flux_workspace *ws) {
+    if (!ws) return AEO_OK;
+    if (g_ctx) hipStreamSynchronize(g_ctx->stream);
+    for (int q = 0; q < 7; q++)
+        if (ws->a[q]) hipFree(ws->a[q]);
+    delete ws;
+    return AEO_OK;
+}
+
+// advect_positive!(tend, u, v, w, ϕ, dt; scheme, p, workspace, bcs, accumulate)
+// (src/AdvectionSchemes.jl:1135-1160)
+extern "C" int aeo_advect_positive(aeo_grid *g, int precision, int scheme, int p, void *tend, const void *u,
+                                   const void *v, const void *w, const void *phi, double dt, aeo_flux_workspace *ws,
+                                   const int *bcs, int accumulate) {
+    NEED_CTX();
+    if (!g || !tend || !u || !v || !w || !phi) return fail(AEO_ERR_ARG, "null argument");
+    if (precision != AEO_F64 && precision != AEO_F32) return fail(AEO_ERR_ARG, "bad precision %d", precision);
+    // _reconstruction (:1692-1702): :weno5 names WENO-JS5 on this path; no :upwind
+    if (scheme == AEO_WENO5) scheme = AEO_WENO_JS5;
+    if (scheme != AEO_CENTERED && scheme != AEO_WENO_JS5 && scheme != AEO_WENO_Z5 && scheme != AEO_WENO_Z7)
+        return fail(AEO_ERR_ARG, "advect_positive!: unknown reconstruction %d; expected centered, weno5, "
+                                 "weno_js5, weno_z5 or weno_z7", scheme);
+    int rc = check_scheme(g, scheme, p);
+    if (rc) return rc;
+    // _check_stretched_recon (:839-843)
+    if (g->stretched && scheme != AEO_CENTERED)
+        return fail(AEO_ERR_ARG, "advect_positive! on a StretchedGrid takes scheme = :centered only (the stencil "
+                                 "assumes uniform spacing)");
+    if (bcs)
+        for (int q = 0; q < 3; q++)
+            if (bcs[q] < AEO_PERIODIC || bcs[q] > AEO_WALL) return fail(AEO_ERR_ARG, "bad boundary condition");
+    if (ws && (ws->g != g || ws->precision != precision))
+        return fail(AEO_ERR_ARG, "the flux limiter workspace belongs to another grid or precision");
+    aeo_flux_workspace *own = nullptr;
+    if (!ws) {  // the reference allocates a workspace per call when none is passed
+        rc = aeo_flux_limiter_workspace_create(g, precision, &own);
+        if (rc) return rc;
+        ws = own;
+    }
+    Timed t("advect_positive");
+    if (precision == AEO_F64) {
+        double *a[7];
+        for (int q = 0; q < 7; q++) a[q] = (double *)ws->a[q];
+        rc = launch_advect_positive<double>(g_ctx->stream, g->m64, scheme, p, (double *)tend, (const double *)u,
+                                            (const double *)v, (const double *)w, (const double *)phi, dt, a, bcs,
+                                            accumulate);
+    } else {
+        float *a[7];
+        for (int q = 0; q < 7; q++) a[q] = (float *)ws->a[q];
+        rc = launch_advect_positive<float>(g_ctx->stream, g->m32, scheme, p, (float *)tend, (const float *)u,
+                                           (const float *)v, (const float *)w, (const float *)phi, (float)dt, a, bcs,
+                                           accumulate);
+    }
+    const hipError_t e = hipGetLastError();
+    if (own) aeo_flux_limiter_workspace_destroy(own);
+    if (rc) return fail(AEO_ERR_ARG, "advect_positive launch failed");
+    HIPCHK(e);
+    return AEO_OK;
+}
+
 extern "C" int aeo_advect_momentum(aeo_grid *g, int precision, int scheme, int p, void *tu, void *tv, void *tw,
                                    const void *u, const void *v, const void *w) {
     NEED_CTX();

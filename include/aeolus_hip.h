@@ -95,6 +95,23 @@ int aeo_fill_halos(aeo_grid *g, int precision, void *field, int loc, int bc_x, i
  * inputs must be filled and H >= required_halo(scheme) (3; 4 for WENO-Z7). */
 int aeo_advect_scalar(aeo_grid *g, int precision, int scheme, int p, void *tend, const void *u, const void *v,
                       const void *w, const void *phi);
+/* FluxLimiterWorkspace(mesh) (src/AdvectionSchemes.jl:800-822): scratch of
+ * aeo_advect_positive for one grid and precision (seven storage arrays) */
+typedef struct aeo_flux_workspace aeo_flux_workspace;
+int aeo_flux_limiter_workspace_create(aeo_grid *g, int precision, aeo_flux_workspace **ws);
+int aeo_flux_limiter_workspace_destroy(aeo_flux_workspace *ws);
+/* advect_positive!(tend, u, v, w, ϕ, dt; scheme, p, workspace, bcs, accumulate)
+ * (src/AdvectionSchemes.jl:1135-1160): adds (accumulate = 1) or writes
+ * (accumulate = 0) −∇·(u ϕ) over tend's interior with the face fluxes limited
+ * so that ϕ + dt·tend ≥ 0 for ϕ ≥ 0 (halos included) under the first-order
+ * CFL condition. scheme: AEO_CENTERED, AEO_WENO5 or AEO_WENO_JS5 (both
+ * WENO-JS5 on this path), AEO_WENO_Z5, AEO_WENO_Z7; a stretched grid takes
+ * AEO_CENTERED only. bcs: three AEO_* of ϕ for the limiter's halo, or NULL
+ * for the reference's one-layer periodic wrap. ws may be NULL (a workspace
+ * is then allocated and freed inside the call). */
+int aeo_advect_positive(aeo_grid *g, int precision, int scheme, int p, void *tend, const void *u, const void *v,
+                        const void *w, const void *phi, double dt, aeo_flux_workspace *ws, const int *bcs,
+                        int accumulate);
 /* momentum_advect_weno_z!(tu, tv, tw, u, v, w) (src/AdvectionSchemes.jl:1195-1270)
  * and the momentum_advection keyword of step! (src/Simulations.jl:418-425):
  * ADDS −∂(uⱼuᵢ)/∂xⱼ to the face-centred tendencies. */

@@ -299,6 +299,137 @@ void orc_advect_scalar(double *tend, const double *u, const double *v, const dou
 }
 
 /* ------------------------------------------------------------------------
+ * Positivity-preserving advection: advect_positive!,
+ * src/AdvectionSchemes.jl:858-1160 (Zalesak / Zhang-Shu flux limiting)
+ * ------------------------------------------------------------------------ */
+/* Julia's max(x, 0) and min(1, x): NaN propagates; max(-0.0, 0.0) is 0.0 */
+static inline double jl_max0(double x) { return x != x ? x : (x > 0.0 ? x : 0.0); }
+static inline double jl_min1(double x) { return x != x ? x : (x < 1.0 ? x : 1.0); }
+
+/* the limited flux of a face: lo + c·a, a = F − lo, c the donor's ratio
+ * (_b_limit_x!/_y!/_z!, :1056-1081): the cell below the face when a ≥ 0 */
+static inline double pos_limited(double F, double lo, double r_below, double r_above) {
+    double a = F - lo;
+    double c = a >= 0.0 ? r_below : r_above;
+    return lo + c * a;
+}
+
+/* advect_positive!(tend, u, v, w, ϕ, dt; scheme, p, bcs, accumulate)
+ * (:1135-1160). Phases, each over the reference's index ranges:
+ *   1. the reconstructed face fluxes F = vel·face(ϕ) (_b_rflux_*, :858-889)
+ *      and the first-order upwind ones L (_b_uflux_*, :891-909) on the
+ *      interior faces plus the closing face of each axis;
+ *   2. the ratio of every interior cell (_b_limit_ratio!, :1026-1053):
+ *      P⁻ = dt Σ antidiffusive outflow, Q = max(ϕ − dt ∇·L, 0),
+ *      R = P⁻ > 0 ? min(1, Q/P⁻) : 1;
+ *   3. R's halo: fill_halos!(R, bcs) (use_bcs), else a one-layer periodic
+ *      wrap over the interior lines (_wrap_one_layer!, :944-958);
+ *   4. each face flux limited with its donor's ratio (:1056-1081);
+ *   5. tend −= ∇·F (accumulate) or tend = −∇·F (_b_flux_div!, :912-928).
+ * scheme: ORC_CENTERED, ORC_WENO_JS5, ORC_WENO_Z5, ORC_WENO_Z7 (the
+ * reference's :centered, :weno5/:weno_js5, :weno_z5, :weno_z7). ratio, when
+ * non-null, receives R (storage-sized) for inspection. */
+void orc_advect_positive(double *tend, const double *u, const double *v, const double *w, const double *phi,
+                         const orc_grid *g, int scheme, int p, double dt, int use_bcs, int bcx, int bcy, int bcz,
+                         int accumulate, double *ratio) {
+    GRID_DIMS(g);
+    const int H = g->H, Nx = g->Nx, Ny = g->Ny, Nz = g->Nz;
+    const size_t n = (size_t)sx * sy * sz;
+    double *F[3], *L[3];
+    for (int a = 0; a < 3; a++) {
+        F[a] = calloc(n, sizeof(double));
+        L[a] = calloc(n, sizeof(double));
+    }
+    double *R = calloc(n, sizeof(double));
+    const ptrdiff_t stx = 1, sty = sx, stz = (ptrdiff_t)sx * sy;
+    /* 1. fluxes; face index f is the west/south/bottom face of cell f */
+    for (int k = H + 1; k <= H + Nz + 1; k++)
+        for (int j = H + 1; j <= H + Ny + 1; j++)
+            for (int i = H + 1; i <= H + Nx + 1; i++) {
+                if (j <= H + Ny && k <= H + Nz) {
+                    double vel = u[ID(i, j, k)];
+                    size_t b = ID(1, j, k) - stx;
+                    F[0][ID(i, j, k)] = vel * face_value(scheme, p, phi, b, stx, i - 1, vel);
+                    L[0][ID(i, j, k)] = vel * (vel >= 0 ? phi[ID(i - 1, j, k)] : phi[ID(i, j, k)]);
+                }
+                if (i <= H + Nx && k <= H + Nz) {
+                    double vel = v[ID(i, j, k)];
+                    size_t b = ID(i, 1, k) - sty;
+                    F[1][ID(i, j, k)] = vel * face_value(scheme, p, phi, b, sty, j - 1, vel);
+                    L[1][ID(i, j, k)] = vel * (vel >= 0 ? phi[ID(i, j - 1, k)] : phi[ID(i, j, k)]);
+                }
+                if (i <= H + Nx && j <= H + Ny) {
+                    double vel = w[ID(i, j, k)];
+                    size_t b = ID(i, j, 1) - stz;
+                    F[2][ID(i, j, k)] = vel * face_value(scheme, p, phi, b, stz, k - 1, vel);
+                    L[2][ID(i, j, k)] = vel * (vel >= 0 ? phi[ID(i, j, k - 1)] : phi[ID(i, j, k)]);
+                }
+            }
+    const double idx = 1.0 / g->dx, idy = 1.0 / g->dy;
+    /* 2. ratios */
+    for (int k = H + 1; k <= H + Nz; k++)
+        for (int j = H + 1; j <= H + Ny; j++)
+            for (int i = H + 1; i <= H + Nx; i++) {
+                const double idz = m_idz(g, k);
+                const size_t c = ID(i, j, k), ce = ID(i + 1, j, k), cn = ID(i, j + 1, k), ct = ID(i, j, k + 1);
+                double lxe = L[0][ce], lxw = L[0][c], lyn = L[1][cn], lys = L[1][c], lzt = L[2][ct], lzb = L[2][c];
+                double axe = F[0][ce] - lxe, axw = F[0][c] - lxw;
+                double ayn = F[1][cn] - lyn, ays = F[1][c] - lys;
+                double azt = F[2][ct] - lzt, azb = F[2][c] - lzb;
+                double Pm = dt * (((jl_max0(axe) + jl_max0(-axw)) * idx + (jl_max0(ayn) + jl_max0(-ays)) * idy) +
+                                  (jl_max0(azt) + jl_max0(-azb)) * idz);
+                double div_lo = ((lxe - lxw) * idx + (lyn - lys) * idy) + (lzt - lzb) * idz;
+                double Q = jl_max0(phi[c] - dt * div_lo);
+                R[c] = Pm > 0.0 ? jl_min1(Q / Pm) : 1.0;
+            }
+    /* 3. R's halo */
+    if (use_bcs) {
+        orc_fill_halos(R, g, ORC_CC, bcx, bcy, bcz);
+    } else {
+        for (int k = H + 1; k <= H + Nz; k++)
+            for (int j = H + 1; j <= H + Ny; j++) {
+                R[ID(H, j, k)] = R[ID(H + Nx, j, k)];
+                R[ID(H + Nx + 1, j, k)] = R[ID(H + 1, j, k)];
+            }
+        for (int k = H + 1; k <= H + Nz; k++)
+            for (int i = H + 1; i <= H + Nx; i++) {
+                R[ID(i, H, k)] = R[ID(i, H + Ny, k)];
+                R[ID(i, H + Ny + 1, k)] = R[ID(i, H + 1, k)];
+            }
+        for (int j = H + 1; j <= H + Ny; j++)
+            for (int i = H + 1; i <= H + Nx; i++) {
+                R[ID(i, j, H)] = R[ID(i, j, H + Nz)];
+                R[ID(i, j, H + Nz + 1)] = R[ID(i, j, H + 1)];
+            }
+    }
+    /* 4. limited fluxes, on the same face ranges as phase 1 */
+    for (int k = H + 1; k <= H + Nz + 1; k++)
+        for (int j = H + 1; j <= H + Ny + 1; j++)
+            for (int i = H + 1; i <= H + Nx + 1; i++) {
+                const size_t c = ID(i, j, k);
+                if (j <= H + Ny && k <= H + Nz) F[0][c] = pos_limited(F[0][c], L[0][c], R[ID(i - 1, j, k)], R[c]);
+                if (i <= H + Nx && k <= H + Nz) F[1][c] = pos_limited(F[1][c], L[1][c], R[ID(i, j - 1, k)], R[c]);
+                if (i <= H + Nx && j <= H + Ny) F[2][c] = pos_limited(F[2][c], L[2][c], R[ID(i, j, k - 1)], R[c]);
+            }
+    /* 5. the flux divergence */
+    for (int k = H + 1; k <= H + Nz; k++)
+        for (int j = H + 1; j <= H + Ny; j++)
+            for (int i = H + 1; i <= H + Nx; i++) {
+                const double idz = m_idz(g, k);
+                const size_t c = ID(i, j, k);
+                double d = ((F[0][ID(i + 1, j, k)] - F[0][c]) * idx + (F[1][ID(i, j + 1, k)] - F[1][c]) * idy) +
+                           (F[2][ID(i, j, k + 1)] - F[2][c]) * idz;
+                tend[c] = accumulate ? tend[c] - d : -d;
+            }
+    if (ratio) memcpy(ratio, R, n * sizeof(double));
+    for (int a = 0; a < 3; a++) {
+        free(F[a]);
+        free(L[a]);
+    }
+    free(R);
+}
+
+/* ------------------------------------------------------------------------
  * Momentum advection
  * ------------------------------------------------------------------------ */
 /* _mom_u_at/_mom_v_at/_mom_w_at, src/FiniteVolume.jl:730-797 */

@@ -56,6 +56,12 @@ void orc_fill_halos(double *a, const orc_grid *g, int loc, int bcx, int bcy, int
 
 void orc_advect_scalar(double *tend, const double *u, const double *v, const double *w,
                        const double *phi, const orc_grid *g, int scheme, int p);
+/* advect_positive! (src/AdvectionSchemes.jl:1135-1160): limited fluxes so a
+ * forward-Euler update with the same dt keeps ϕ ≥ 0; use_bcs = 0 wraps the
+ * limiter's halo periodically by one layer (the reference's bcs = nothing) */
+void orc_advect_positive(double *tend, const double *u, const double *v, const double *w, const double *phi,
+                         const orc_grid *g, int scheme, int p, double dt, int use_bcs, int bcx, int bcy, int bcz,
+                         int accumulate, double *ratio);
 void orc_advect_momentum(double *tu, double *tv, double *tw, const double *u, const double *v,
                          const double *w, const orc_grid *g, int scheme, int p);
 void orc_laplacian(double *out, const double *f, const orc_grid *g, int loc);

@@ -67,6 +67,7 @@ def lib():
             "orc_weno5_classic_left": (d, [d] * 5),
             "orc_fill_halos": (None, [pd, pg, i, i, i, i]),
             "orc_advect_scalar": (None, [pd, pd, pd, pd, pd, pg, i, i]),
+            "orc_advect_positive": (None, [pd, pd, pd, pd, pd, pg, i, i, d, i, i, i, i, i, pd]),
             "orc_advect_momentum": (None, [pd, pd, pd, pd, pd, pd, pg, i, i]),
             "orc_laplacian": (None, [pd, pd, pg, i]),
             "orc_divergence_into": (None, [pd, pd, pd, pd, pg]),
@@ -200,6 +201,17 @@ def fill_halos(g, a, loc, bcs):
 def advect_scalar(g, tend, u, v, w, phi, scheme, p=2):
     lib().orc_advect_scalar(ptr(tend), ptr(u), ptr(v), ptr(w), ptr(phi), ctypes.byref(g.c), SCHEMES[scheme], p)
     return tend
+
+
+def advect_positive(g, tend, u, v, w, phi, dt, scheme="weno_z5", p=2, bcs=None, accumulate=True, ratio=False):
+    """advect_positive! (src/AdvectionSchemes.jl:1135-1160); :weno5 is WENO-JS5
+    there (_reconstruction, :1692-1702). Returns tend (and the limiter ratio)."""
+    sid = SCHEMES["weno_js5" if scheme == "weno5" else scheme]
+    bc = [BCS[b] for b in bcs] if bcs is not None else [0, 0, 0]
+    R = g.zeros() if ratio else None
+    lib().orc_advect_positive(ptr(tend), ptr(u), ptr(v), ptr(w), ptr(phi), ctypes.byref(g.c), sid, p, float(dt),
+                              int(bcs is not None), *bc, int(accumulate), None if R is None else ptr(R))
+    return (tend, R) if ratio else tend
 
 
 def advect_momentum(g, tu, tv, tw, u, v, w, scheme, p=2):
