@@ -176,6 +176,51 @@ def poisson_legs(reps=3, weights=None, log=print):
     return out
 
 
+def tracer_leg(reps=20, warmup=3):
+    """advect_positive (WENO-Z5, FP64) of a sharp positive tracer on the
+    256x256x128 grid (regular, periodic; the WENO stencils need uniform
+    spacing) in a seeded random flow, with a reusable workspace: ms per call
+    (the library's event timer on its stream), the compulsory HBM bytes
+    (read u, v, w, ϕ; write tend: 5 arrays) and the rate they imply."""
+    import ctypes
+    import aeolus_hip as hip
+    from aeolus_hip import _lib as L
+    hip.init(0)
+    g = hip.Grid((NX, NY, NZ), (float(NX), float(NY), float(NZ)), halo=3)
+    rng = np.random.default_rng(11)
+    bcs = ("periodic",) * 3
+    dev = []
+    for loc in (1, 2, 3):
+        d = hip.DeviceArray.from_numpy(np.asfortranarray(rng.standard_normal(g.storage)))
+        hip.fill_halos(g, d, loc, bcs)
+        dev.append(d)
+    phi = hip.DeviceArray.from_numpy(np.asfortranarray(rng.uniform(0, 1, g.storage) ** 4))
+    hip.fill_halos(g, phi, 0, bcs)
+    tend = hip.DeviceArray(g.storage)
+    ws = hip.FluxLimiterWorkspace(g)
+    dt = 0.05
+    for _ in range(warmup):
+        hip.advect_positive(g, tend, *dev, phi, dt, workspace=ws, accumulate=False)
+    hip.sync()
+    lib = L.init()
+    L.check(lib.aeo_profile_enable(1))
+    L.check(lib.aeo_profile_reset())
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        hip.advect_positive(g, tend, *dev, phi, dt, workspace=ws, accumulate=False)
+    hip.sync()
+    wall = (time.perf_counter() - t0) / reps
+    c, ms = ctypes.c_int(), ctypes.c_double()
+    L.check(lib.aeo_profile_get(b"advect_positive", ctypes.byref(c), ctypes.byref(ms)))
+    L.check(lib.aeo_profile_enable(0))
+    ev_ms = ms.value / max(c.value, 1)
+    alg = NX * NY * NZ * 8 * 5
+    return {"ms_per_call": ev_ms, "wall_ms_per_call": wall * 1e3, "compulsory_bytes": alg,
+            "compulsory_GBs": alg / (ev_ms / 1e3) / 1e9,
+            "workload": "advect_positive WENO-Z5 FP64, 256x256x128 regular periodic grid, seeded random flow, "
+                        "tracer uniform(0,1)^4, reusable workspace, accumulate=false"}
+
+
 def read_valu_insts():
     """VALU wave-instructions per tendency launch from the committed
     rocprofv3 --pmc SQ summary (SQ_INSTS_VALU per dispatch)."""
@@ -249,10 +294,15 @@ def main():
     ap.add_argument("--no-poisson", action="store_true", help="skip the cut-cell Poisson leg")
     ap.add_argument("--weights", default=os.path.join(ROOT, "aeolus.jl_amd", "aeolus_hip", "weights",
                                                       "unet_cutcell_dx2.npy"))
+    ap.add_argument("--tracer-pass", action="store_true",
+                    help="only the advect_positive leg (--steps calls), for rocprofv3 kernel traces")
     ap.add_argument("--profile-pass", action="store_true",
                     help="a short FP64 run for rocprofv3 counter passes: the steps, then one 1 GiB copy kernel "
                          "(the calibration of FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
+    if args.tracer_pass:
+        print(json.dumps({"tracer_pass": True, **tracer_leg(args.steps, args.warmup)}), flush=True)
+        return
     if args.profile_pass:
         profile_pass(args.steps, args.warmup)
         return
@@ -265,6 +315,7 @@ def main():
     t64, prof64, div64 = run_gpu(args.steps, args.warmup, 0)
     t32, prof32, div32 = run_gpu(args.steps, args.warmup, 1)
     pms = poisson_ms()
+    tracer = tracer_leg()
     legs = None if args.no_poisson else poisson_legs(weights=args.weights, log=lambda m: print(m, file=sys.stderr))
     cells = NX * NY * NZ
 
@@ -314,6 +365,7 @@ def main():
                         "velocity; to ||r|| <= 1e-8 ||b||",
             "legs": legs,
         },
+        "tracer_positive": tracer,
         "max_divergence_after_step": div64,
         "ms_per_step_by_kernel_f64": breakdown,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
