@@ -8,7 +8,8 @@ HIP kernels; a missing library raises AeolusError.
 """
 from . import _lib
 from ._lib import AeolusError, init, sync
-from .advection import (FluxLimiterWorkspace, advect_positive, advect_weno_z, advection_scheme,
+from .advection import (FluxLimiterWorkspace, advect_positive, advect_teno, advect_variance_conserving,
+                        advect_weno_z, advection_scheme,
                         momentum_advect_weno_z, required_halo)
 from .fields import DeviceArray, Grid, faces_geometric, faces_uniform, fill_halos
 from . import learned, workloads
@@ -28,4 +29,5 @@ __all__ = [
     "read_checkpoint", "restart", "CFLConstraints", "adaptive_dt", "cfl_diagnostics", "NumericalFailure",
     "StateSnapshot", "SubsteppingSchedule", "is_numerical_failure", "restore_state", "save_state", "state_snapshot",
     "step_adaptive", "step_guard", "substeps_for", "FluxLimiterWorkspace", "advect_positive",
+    "advect_teno", "advect_variance_conserving",
 ]

@@ -16,6 +16,7 @@ F64, F32 = 0, 1
 CELL, FACE_X, FACE_Y, FACE_Z = 0, 1, 2, 3
 PERIODIC, NOFLUX, WALL = 0, 1, 2
 CENTERED, UPWIND, WENO5, WENO_JS5, WENO_Z5, WENO_Z7 = 0, 1, 2, 3, 4, 5
+TENO5, TENO8, VARIANCE_CONSERVING = 6, 7, 8
 CONVERGED, MAXIT, BREAKDOWN, ZERO_RHS, DIRECT = 0, 1, 2, 3, 4
 STATUS_NAMES = {CONVERGED: "converged", MAXIT: "maxit", BREAKDOWN: "breakdown", ZERO_RHS: "zero_rhs",
                 DIRECT: "direct"}
@@ -93,7 +94,8 @@ SIGNATURES = {
     "aeo_stepper_cfl": [_vp, _vp, _vp, _vp, _vp, _pd, _pd, _pd],
     "aeo_flux_limiter_workspace_create": [_vp, _i, ctypes.POINTER(_vp)],
     "aeo_flux_limiter_workspace_destroy": [_vp],
-    "aeo_advect_positive": [_vp, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _d, _vp, ctypes.POINTER(_i), _i],
+    "aeo_advect_positive": [_vp, _i, _i, _i, _d, _vp, _vp, _vp, _vp, _vp, _d, _vp, ctypes.POINTER(_i), _i],
+    "aeo_advect_teno": [_vp, _i, _i, _d, _i, _vp, _vp, _vp, _vp, _vp],
     "aeo_stepper_guard": [_vp, _vp, _vp, _vp, _vp, _vp, ctypes.POINTER(ctypes.c_int), _pd, _pd],
     "aeo_mw_stepper_guard": [_vp, _vp, _vp, _vp, _vp, _vp, ctypes.POINTER(ctypes.c_int), _pd, _pd],
     "aeo_mw_stepper_cfl": [_vp, _vp, _vp, _vp, _vp, _pd, _pd, _pd],

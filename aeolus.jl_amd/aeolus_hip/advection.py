@@ -15,6 +15,9 @@ SCHEMES = {
     "weno_js5": (L.WENO_JS5, 3),
     "weno_z5": (L.WENO_Z5, 3),
     "weno_z7": (L.WENO_Z7, 4),
+    "teno5": (L.TENO5, 3),
+    "teno8": (L.TENO8, 5),
+    "variance_conserving": (L.VARIANCE_CONSERVING, 1),
 }
 
 
@@ -28,17 +31,25 @@ def scheme_id(name):
 class AdvectionScheme:
     """Callable wrapper of one scheme (src/AdvectionSchemes.jl:1688-1700)."""
 
-    def __init__(self, name, grid, p=2):
+    def __init__(self, name, grid, p=2, C_T=1e-5):
         if name not in SCHEMES:
             raise ValueError(f"unknown advection scheme {name!r}; expected one of {', '.join(SCHEMES)}")
         if p not in (1, 2):
             raise ValueError(f"WENO-Z τ-exponent p must be 1 or 2 (got {p})")
+        if not C_T > 0:
+            raise ValueError(f"TENO cut-off C_T must be positive (got {C_T})")
         self.name = name
         self.grid = grid
         self.halo = SCHEMES[name][1]
         self.p = p
+        self.C_T = float(C_T)
 
     def __call__(self, tend, u, v, w, phi):
+        if self.name in ("teno5", "teno8"):
+            L.check(L.init().aeo_advect_teno(self.grid.handle, tend.precision, 5 if self.name == "teno5" else 8,
+                                             self.C_T, 6, L.dptr(tend), L.dptr(u), L.dptr(v), L.dptr(w),
+                                             L.dptr(phi)))
+            return tend
         L.check(L.init().aeo_advect_scalar(self.grid.handle, tend.precision, SCHEMES[self.name][0], self.p,
                                            L.dptr(tend), L.dptr(u), L.dptr(v), L.dptr(w), L.dptr(phi)))
         return tend
@@ -52,9 +63,21 @@ class AdvectionScheme:
         return f"AdvectionScheme(:{self.name}, halo ≥ {self.halo})"
 
 
-def advection_scheme(name, grid, p=2):
-    """advection_scheme(name; p) (src/AdvectionSchemes.jl:1746-1775)."""
-    return AdvectionScheme(name, grid, p)
+def advection_scheme(name, grid, p=2, C_T=1e-5):
+    """advection_scheme(name; C_T, p) (src/AdvectionSchemes.jl:1746-1775)."""
+    return AdvectionScheme(name, grid, p, C_T)
+
+
+def advect_teno(grid, tend, u, v, w, phi, order=5, C_T=1e-5, q=6):
+    """advect_teno!(tend, u, v, w, ϕ; order, C_T, q) (src/AdvectionSchemes.jl:750-756)."""
+    L.check(L.init().aeo_advect_teno(grid.handle, tend.precision, int(order), float(C_T), int(q), L.dptr(tend),
+                                     L.dptr(u), L.dptr(v), L.dptr(w), L.dptr(phi)))
+    return tend
+
+
+def advect_variance_conserving(grid, tend, u, v, w, phi):
+    """advect_variance_conserving!(tend, u, v, w, ϕ) (src/AdvectionSchemes.jl:1428-1446)."""
+    return AdvectionScheme("variance_conserving", grid)(tend, u, v, w, phi)
 
 
 def required_halo(s):
@@ -96,24 +119,27 @@ class FluxLimiterWorkspace:
             pass
 
 
-def advect_positive(grid, tend, u, v, w, phi, dt, scheme="weno_z5", p=2, workspace=None, bcs=None, accumulate=True):
+def advect_positive(grid, tend, u, v, w, phi, dt, scheme="weno_z5", C_T=1e-5, p=2, workspace=None, bcs=None,
+                    accumulate=True):
     """advect_positive!(tend, u, v, w, ϕ, dt; scheme, p, workspace, bcs,
     accumulate) (src/AdvectionSchemes.jl:1135-1160): −∂(uⱼϕ)/∂xⱼ with the face
     fluxes limited so that ϕ + dt·tend stays ≥ 0 for ϕ ≥ 0 under the
     first-order CFL condition. scheme: centered, weno5 (WENO-JS5 here),
-    weno_js5, weno_z5, weno_z7 (centered only on a stretched grid). bcs: the
+    weno_js5, weno_z5, weno_z7, teno5, teno8 with cut-off C_T (centered only
+    on a stretched grid). bcs: the
     (x, y, z) conditions of ϕ for the limiter's halo, or None for a one-layer
     periodic wrap. accumulate=False writes the interior of tend instead of
     adding to it."""
     import ctypes
     from .fields import BC_NAMES
-    if scheme not in ("centered", "weno5", "weno_js5", "weno_z5", "weno_z7"):
+    if scheme not in ("centered", "weno5", "weno_js5", "weno_z5", "weno_z7", "teno5", "teno8"):
         raise ValueError(f"unknown reconstruction {scheme!r}; expected one of centered, weno5, weno_js5, weno_z5, "
-                         "weno_z7")
+                         "weno_z7, teno5, teno8")
     if workspace is not None and (workspace.grid is not grid or workspace.precision != tend.precision):
         raise ValueError("the workspace belongs to another grid or precision")
     bc = None if bcs is None else (ctypes.c_int * 3)(*[BC_NAMES[b] for b in bcs])
-    L.check(L.init().aeo_advect_positive(grid.handle, tend.precision, SCHEMES[scheme][0], int(p), L.dptr(tend),
+    L.check(L.init().aeo_advect_positive(grid.handle, tend.precision, SCHEMES[scheme][0], int(p), float(C_T),
+                                         L.dptr(tend),
                                          L.dptr(u), L.dptr(v), L.dptr(w), L.dptr(phi), float(dt),
                                          None if workspace is None else workspace._h, bc, int(bool(accumulate))))
     return tend

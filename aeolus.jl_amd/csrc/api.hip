@@ -119,6 +119,7 @@ static void fill_metrics(Metrics<T> &m, const aeo_grid *g, T *const *dv) {
     m.idx2 = T(1 / (g->dx * g->dx)); m.idy2 = T(1 / (g->dy * g->dy));
     m.stretched = g->stretched;
     m.idz = T(1 / g->dz); m.idz2 = T(1 / (g->dz * g->dz));
+    m.teno_ct = T(1e-5);  // TENO(N; C_T = 1e-5), src/AdvectionSchemes.jl:535
     m.idzc = dv[0]; m.idzf = dv[1]; m.wlo = dv[2]; m.whi = dv[3]; m.alo = dv[4]; m.ahi = dv[5];
 }
 
@@ -325,8 +326,20 @@ static int required_halo(int scheme) {
     case AEO_CENTERED: case AEO_UPWIND: return 1;
     case AEO_WENO5: case AEO_WENO_JS5: case AEO_WENO_Z5: return 3;
     case AEO_WENO_Z7: return 4;
+    case AEO_TENO5: return 3;
+    case AEO_TENO8: return 5;  // required_halo(::TENO{8}), src/AdvectionSchemes.jl:569
+    case AEO_VARIANCE_CONSERVING: return 1;
     }
     return -1;
+}
+
+// the schemes of the momentum operator and of the step: the reconstructions
+// the fused kernels are built for (TENO and the skew form are scalar operators)
+static int check_step_scheme(int scheme) {
+    if (scheme == AEO_TENO5 || scheme == AEO_TENO8 || scheme == AEO_VARIANCE_CONSERVING)
+        return fail(AEO_ERR_UNSUPPORTED, "scheme %d is a scalar advection operator here; momentum and the step take "
+                                         "centered, upwind, weno5, weno_js5, weno_z5 or weno_z7", scheme);
+    return AEO_OK;
 }
 
 static int check_scheme(const aeo_grid *g, int scheme, int p) {
@@ -341,23 +354,54 @@ static int check_scheme(const aeo_grid *g, int scheme, int p) {
     return AEO_OK;
 }
 
-extern "C" int aeo_advect_scalar(aeo_grid *g, int precision, int scheme, int p, void *tend, const void *u,
-                                 const void *v, const void *w, const void *phi) {
+static int advect_scalar_ct(aeo_grid *g, int precision, int scheme, int p, double C_T, void *tend, const void *u,
+                            const void *v, const void *w, const void *phi) {
     NEED_CTX();
     if (!g || !tend || !u || !v || !w || !phi) return fail(AEO_ERR_ARG, "null argument");
     int rc = check_scheme(g, scheme, p);
     if (rc) return rc;
     Timed t("advect");
-    if (precision == AEO_F64)
-        rc = launch_advect<double>(g_ctx->stream, g->m64, 0, scheme, p, (const double *)u, (const double *)v,
+    if (scheme == AEO_VARIANCE_CONSERVING) {
+        // _advect_variance_conserving! has a RegularGrid method only (:1440-1446)
+        if (g->stretched) return fail(AEO_ERR_UNSUPPORTED, "advect_variance_conserving!: regular grids");
+        if (precision == AEO_F64)
+            launch_varcons<double>(g_ctx->stream, g->m64, (double *)tend, (const double *)u, (const double *)v,
+                                   (const double *)w, (const double *)phi);
+        else if (precision == AEO_F32)
+            launch_varcons<float>(g_ctx->stream, g->m32, (float *)tend, (const float *)u, (const float *)v,
+                                  (const float *)w, (const float *)phi);
+        else return fail(AEO_ERR_ARG, "bad precision %d", precision);
+        HIPCHK(hipGetLastError());
+        return AEO_OK;
+    }
+    if (precision == AEO_F64) {
+        Metrics<double> m = g->m64;
+        m.teno_ct = C_T;
+        rc = launch_advect<double>(g_ctx->stream, m, 0, scheme, p, (const double *)u, (const double *)v,
                                    (const double *)w, (const double *)phi, nullptr, nullptr, nullptr, (double *)tend);
-    else if (precision == AEO_F32)
-        rc = launch_advect<float>(g_ctx->stream, g->m32, 0, scheme, p, (const float *)u, (const float *)v,
+    } else if (precision == AEO_F32) {
+        Metrics<float> m = g->m32;
+        m.teno_ct = (float)C_T;
+        rc = launch_advect<float>(g_ctx->stream, m, 0, scheme, p, (const float *)u, (const float *)v,
                                   (const float *)w, (const float *)phi, nullptr, nullptr, nullptr, (float *)tend);
-    else return fail(AEO_ERR_ARG, "bad precision %d", precision);
+    } else return fail(AEO_ERR_ARG, "bad precision %d", precision);
     if (rc) return fail(AEO_ERR_ARG, "advection launch failed");
     HIPCHK(hipGetLastError());
     return AEO_OK;
+}
+
+extern "C" int aeo_advect_scalar(aeo_grid *g, int precision, int scheme, int p, void *tend, const void *u,
+                                 const void *v, const void *w, const void *phi) {
+    return advect_scalar_ct(g, precision, scheme, p, 1e-5, tend, u, v, w, phi);
+}
+
+// advect_teno!(tend, u, v, w, ϕ; order, C_T, q) (src/AdvectionSchemes.jl:750-756)
+extern "C" int aeo_advect_teno(aeo_grid *g, int precision, int order, double C_T, int q, void *tend, const void *u,
+                               const void *v, const void *w, const void *phi) {
+    if (order != 5 && order != 8) return fail(AEO_ERR_ARG, "TENO order must be 5 or 8 (got %d)", order);
+    if (!(C_T > 0)) return fail(AEO_ERR_ARG, "TENO cut-off C_T must be positive (got %g)", C_T);
+    if (q != 6) return fail(AEO_ERR_UNSUPPORTED, "this build's TENO uses the exponent q = 6 (got %d)", q);
+    return advect_scalar_ct(g, precision, order == 5 ? AEO_TENO5 : AEO_TENO8, 2, C_T, tend, u, v, w, phi);
 }
 
 // FluxLimiterWorkspace(mesh) (src/AdvectionSchemes.jl:800-822): the face
@@ -399,17 +443,20 @@ extern "C" int aeo_flux_limiter_workspace_destroy(aeo_flux_workspace *ws) {
 
 // advect_positive!(tend, u, v, w, ϕ, dt; scheme, p, workspace, bcs, accumulate)
 // (src/AdvectionSchemes.jl:1135-1160)
-extern "C" int aeo_advect_positive(aeo_grid *g, int precision, int scheme, int p, void *tend, const void *u,
-                                   const void *v, const void *w, const void *phi, double dt, aeo_flux_workspace *ws,
-                                   const int *bcs, int accumulate) {
+extern "C" int aeo_advect_positive(aeo_grid *g, int precision, int scheme, int p, double C_T, void *tend,
+                                   const void *u, const void *v, const void *w, const void *phi, double dt,
+                                   aeo_flux_workspace *ws, const int *bcs, int accumulate) {
     NEED_CTX();
     if (!g || !tend || !u || !v || !w || !phi) return fail(AEO_ERR_ARG, "null argument");
     if (precision != AEO_F64 && precision != AEO_F32) return fail(AEO_ERR_ARG, "bad precision %d", precision);
     // _reconstruction (:1692-1702): :weno5 names WENO-JS5 on this path; no :upwind
     if (scheme == AEO_WENO5) scheme = AEO_WENO_JS5;
-    if (scheme != AEO_CENTERED && scheme != AEO_WENO_JS5 && scheme != AEO_WENO_Z5 && scheme != AEO_WENO_Z7)
+    if (scheme != AEO_CENTERED && scheme != AEO_WENO_JS5 && scheme != AEO_WENO_Z5 && scheme != AEO_WENO_Z7 &&
+        scheme != AEO_TENO5 && scheme != AEO_TENO8)
         return fail(AEO_ERR_ARG, "advect_positive!: unknown reconstruction %d; expected centered, weno5, "
-                                 "weno_js5, weno_z5 or weno_z7", scheme);
+                                 "weno_js5, weno_z5, weno_z7, teno5 or teno8", scheme);
+    if ((scheme == AEO_TENO5 || scheme == AEO_TENO8) && !(C_T > 0))
+        return fail(AEO_ERR_ARG, "TENO cut-off C_T must be positive (got %g)", C_T);
     int rc = check_scheme(g, scheme, p);
     if (rc) return rc;
     // _check_stretched_recon (:839-843)
@@ -428,16 +475,20 @@ extern "C" int aeo_advect_positive(aeo_grid *g, int precision, int scheme, int p
         ws = own;
     }
     Timed t("advect_positive");
+    Metrics<double> m64 = g->m64;
+    Metrics<float> m32 = g->m32;
+    m64.teno_ct = C_T;
+    m32.teno_ct = (float)C_T;
     if (precision == AEO_F64) {
         double *a[7];
         for (int q = 0; q < 7; q++) a[q] = (double *)ws->a[q];
-        rc = launch_advect_positive<double>(g_ctx->stream, g->m64, scheme, p, (double *)tend, (const double *)u,
+        rc = launch_advect_positive<double>(g_ctx->stream, m64, scheme, p, (double *)tend, (const double *)u,
                                             (const double *)v, (const double *)w, (const double *)phi, dt, a, bcs,
                                             accumulate);
     } else {
         float *a[7];
         for (int q = 0; q < 7; q++) a[q] = (float *)ws->a[q];
-        rc = launch_advect_positive<float>(g_ctx->stream, g->m32, scheme, p, (float *)tend, (const float *)u,
+        rc = launch_advect_positive<float>(g_ctx->stream, m32, scheme, p, (float *)tend, (const float *)u,
                                            (const float *)v, (const float *)w, (const float *)phi, (float)dt, a, bcs,
                                            accumulate);
     }
@@ -452,7 +503,9 @@ extern "C" int aeo_advect_momentum(aeo_grid *g, int precision, int scheme, int p
                                    const void *u, const void *v, const void *w) {
     NEED_CTX();
     if (!g || !tu || !tv || !tw || !u || !v || !w) return fail(AEO_ERR_ARG, "null argument");
-    int rc = check_scheme(g, scheme, p);
+    int rc = check_step_scheme(scheme);
+    if (rc) return rc;
+    rc = check_scheme(g, scheme, p);
     if (rc) return rc;
     Timed t("advect");
     if (precision == AEO_F64)
@@ -1042,6 +1095,7 @@ extern "C" int aeo_stepper_create(aeo_grid *g, const aeo_step_params *sp, aeo_po
     if (rc) return rc;
     rc = check_scheme(g, sp->scal_scheme, sp->weno_p);
     if (rc) return rc;
+    if ((rc = check_step_scheme(sp->mom_scheme)) || (rc = check_step_scheme(sp->scal_scheme))) return rc;
     if (sp->mom_scheme != sp->scal_scheme)
         return fail(AEO_ERR_UNSUPPORTED, "this build advects momentum and θ with one scheme");
     if (solver->Nx != g->Nx || solver->Ny != g->Ny || solver->Nz != g->Nz)

@@ -21,7 +21,10 @@ enum Scheme {
     SCH_WENO5 = 2,     // FiniteVolume._weno5_left (classic WENO-5, :weno5)
     SCH_WENO_JS5 = 3,  // weno_js5_left
     SCH_WENO_Z5 = 4,   // _weno_z5_left
-    SCH_WENO_Z7 = 5    // _weno_z7_left
+    SCH_WENO_Z7 = 5,   // _weno_z7_left
+    SCH_TENO5 = 6,     // _teno5_left (q = 6)
+    SCH_TENO8 = 7,     // _teno8_left (q = 6)
+    SCH_VARCONS = 8    // advect_variance_conserving! (not a reconstruction)
 };
 
 template <typename T> struct Eps;
@@ -36,6 +39,7 @@ struct Metrics {
     T idx, idy, idx2, idy2;
     int stretched;
     T idz, idz2;  // UniformZ
+    T teno_ct;    // TENO cut-off C_T of the reconstructions that use one
     const T *idzc, *idzf, *wlo, *whi, *alo, *ahi;  // LevelZ (device)
     __device__ __forceinline__ T m_idz(int k) const { return stretched ? idzc[k] : idz; }
     __device__ __forceinline__ T m_idzf(int k) const { return stretched ? idzf[k] : idz; }
@@ -129,7 +133,23 @@ template <> __device__ __forceinline__ float qdiv<float>(float a, float b) { ret
 
 template <int P, typename T> __device__ __forceinline__ T ipow(T x) {
     if constexpr (P == 2) return x * x;
-    else return x;
+    else if constexpr (P == 6) {  // _ipow(x, Val(6)): x³·x³
+        const T x3 = (x * x) * x;
+        return x3 * x3;
+    } else return x;
+}
+
+template <typename T> __device__ __forceinline__ T beta5(T a1, T a2, T a3, T a4) {
+    return ((((a1 * a1 + T(1.0 / 2.0) * (a1 * a3)) + T(13.0 / 3.0) * (a2 * a2)) + T(21.0 / 5.0) * (a2 * a4)) +
+            T(3129.0 / 80.0) * (a3 * a3)) +
+           T(87617.0 / 140.0) * (a4 * a4);
+}
+// _b5_m4, src/AdvectionSchemes.jl:143-147 ({i-4, …, i}); x/48 = (x/3)/16, x/16 exact
+template <typename T> __device__ __forceinline__ T b5_m4(T v1, T v2, T v3, T v4, T v5) {
+    return beta5<T>(div3<T>((((T(9) * v1 - T(50) * v2) + T(120) * v3) - T(174) * v4) + T(95) * v5) * T(1.0 / 16.0),
+                    ((((T(7) * v1 - T(36) * v2) + T(74) * v3) - T(68) * v4) + T(23) * v5) / T(16),
+                    div12<T>((((T(3) * v1 - T(14) * v2) + T(24) * v3) - T(18) * v4) + T(5) * v5),
+                    div24<T>((((v1 - T(4) * v2) + T(6) * v3) - T(4) * v4) + v5));
 }
 
 // _weno_z5_left, src/AdvectionSchemes.jl:240-255
@@ -162,6 +182,55 @@ __device__ __forceinline__ T weno_z7(T v1, T v2, T v3, T v4, T v5, T v6, T v7) {
     T a2 = T(18.0 / 35.0) * (T(1) + ipow<P>(qdiv<T>(tau, b2 + eps)));
     T a3 = T(4.0 / 35.0) * (T(1) + ipow<P>(qdiv<T>(tau, b3 + eps)));
     return qdiv<T>(((a0 * q0 + a1 * q1) + a2 * q2) + a3 * q3, ((a0 + a1) + a2) + a3);
+}
+// _teno5_left, src/AdvectionSchemes.jl:405-427 (q = 6, cut-off ct). The
+// cut-off g_k/Σg < C_T is a discrete choice of stencils: it is taken with an
+// IEEE division so that the same candidates survive as in the reference.
+template <typename T>
+__device__ __forceinline__ T teno5(T v1, T v2, T v3, T v4, T v5, T ct) {
+    T q0 = div6<T>((T(2) * v1 - T(7) * v2) + T(11) * v3);
+    T q1 = div6<T>((-v2 + T(5) * v3) + T(2) * v4);
+    T q2 = div6<T>((T(2) * v3 + T(5) * v4) - v5);
+    T b0 = b3_m2<T>(v1, v2, v3), b1 = b3_m1<T>(v2, v3, v4), b2 = b3_p0<T>(v3, v4, v5);
+    const T eps = Eps<T>::v * Eps<T>::v;
+    T tau = fabs(b0 - b2);
+    T g0 = ipow<6>(T(1) + qdiv<T>(tau, b0 + eps));
+    T g1 = ipow<6>(T(1) + qdiv<T>(tau, b1 + eps));
+    T g2 = ipow<6>(T(1) + qdiv<T>(tau, b2 + eps));
+    T gs = (g0 + g1) + g2;
+    T d0 = g0 / gs < ct ? T(0) : T(1.0 / 10.0);
+    T d1 = g1 / gs < ct ? T(0) : T(6.0 / 10.0);
+    T d2 = g2 / gs < ct ? T(0) : T(3.0 / 10.0);
+    return qdiv<T>((d0 * q0 + d1 * q1) + d2 * q2, (d0 + d1) + d2);
+}
+// _teno8_left, src/AdvectionSchemes.jl:456-495 (q = 6, cut-off ct)
+template <typename T>
+__device__ __forceinline__ T teno8(T v1, T v2, T v3, T v4, T v5, T v6, T v7, T v8, T ct) {
+    T q0 = div6<T>((T(2) * v3 - T(7) * v4) + T(11) * v5);
+    T q1 = div6<T>((-v4 + T(5) * v5) + T(2) * v6);
+    T q2 = div6<T>((T(2) * v5 + T(5) * v6) - v7);
+    T q3 = div12<T>(((T(-3) * v2 + T(13) * v3) - T(23) * v4) + T(25) * v5);
+    T q4 = div12<T>(((T(3) * v5 + T(13) * v6) - T(5) * v7) + v8);
+    T q5 = ((((T(12) * v1 - T(63) * v2) + T(137) * v3) - T(163) * v4) + T(137) * v5) / T(60);
+    T b0 = b3_m2<T>(v3, v4, v5), b1 = b3_m1<T>(v4, v5, v6), b2 = b3_p0<T>(v5, v6, v7);
+    T b3 = b4_m3<T>(v2, v3, v4, v5), b4 = b4_p0<T>(v5, v6, v7, v8), b5 = b5_m4<T>(v1, v2, v3, v4, v5);
+    const T eps = Eps<T>::v * Eps<T>::v;
+    T tau = fabs(b5 - div6<T>((b0 + T(4) * b1) + b2));
+    T g0 = ipow<6>(T(1) + qdiv<T>(tau, b0 + eps));
+    T g1 = ipow<6>(T(1) + qdiv<T>(tau, b1 + eps));
+    T g2 = ipow<6>(T(1) + qdiv<T>(tau, b2 + eps));
+    T g3 = ipow<6>(T(1) + qdiv<T>(tau, b3 + eps));
+    T g4 = ipow<6>(T(1) + qdiv<T>(tau, b4 + eps));
+    T g5 = ipow<6>(T(1) + qdiv<T>(tau, b5 + eps));
+    T gs = ((((g0 + g1) + g2) + g3) + g4) + g5;
+    T d0 = g0 / gs < ct ? T(0) : T(3.0 / 28.0);
+    T d1 = g1 / gs < ct ? T(0) : T(15.0 / 28.0);
+    T d2 = g2 / gs < ct ? T(0) : T(3.0 / 14.0);
+    T d3 = g3 / gs < ct ? T(0) : T(3.0 / 56.0);
+    T d4 = g4 / gs < ct ? T(0) : T(1.0 / 14.0);
+    T d5 = g5 / gs < ct ? T(0) : T(1.0 / 56.0);
+    return qdiv<T>(((((d0 * q0 + d1 * q1) + d2 * q2) + d3 * q3) + d4 * q4) + d5 * q5,
+                   ((((d0 + d1) + d2) + d3) + d4) + d5);
 }
 // weno_js5_left, src/AdvectionSchemes.jl:319-342 (ε = 1e-6)
 template <typename T>
@@ -200,6 +269,7 @@ template <int S> struct SchemeInfo { static constexpr int R = 3; };
 template <> struct SchemeInfo<SCH_CENTERED> { static constexpr int R = 1; };
 template <> struct SchemeInfo<SCH_UPWIND1> { static constexpr int R = 1; };
 template <> struct SchemeInfo<SCH_WENO_Z7> { static constexpr int R = 4; };
+template <> struct SchemeInfo<SCH_TENO8> { static constexpr int R = 5; };
 
 // A load at a byte offset from a field's base. Offsets are 32-bit (every
 // array is below 2 GiB): with the base in SGPRs the load takes one VGPR of
@@ -217,7 +287,7 @@ template <typename T> __device__ __forceinline__ void sto(T *__restrict__ p, uin
 // left-biased stencil for vel >= 0, else the mirrored right-biased one, into
 // one left reconstruction (_upwind_value, src/AdvectionSchemes.jl:659-666).
 template <int S, int P, typename T>
-__device__ __forceinline__ T face_value(const T *__restrict__ d, uint32_t o0, uint32_t st, T vel) {
+__device__ __forceinline__ T face_value(const T *__restrict__ d, uint32_t o0, uint32_t st, T vel, T ct) {
     if constexpr (S == SCH_CENTERED) {
         return (ldo(d, o0) + ldo(d, o0 + st)) / T(2);
     } else if constexpr (S == SCH_UPWIND1) {
@@ -232,16 +302,25 @@ __device__ __forceinline__ T face_value(const T *__restrict__ d, uint32_t o0, ui
             const T a0 = ldo(d, o0), a1 = ldo(d, o0 + st), a2 = ldo(d, o0 + 2 * st), a3 = ldo(d, o0 + 3 * st),
                     a4 = ldo(d, o0 + 4 * st), a5 = ldo(d, o0 + 5 * st);
             t[0] = up ? a0 : a5; t[1] = up ? a1 : a4; t[2] = up ? a2 : a3; t[3] = up ? a3 : a2; t[4] = up ? a4 : a1;
-        } else {
+        } else if constexpr (R == 4) {
             const T a0 = ldo(d, o0), a1 = ldo(d, o0 + st), a2 = ldo(d, o0 + 2 * st), a3 = ldo(d, o0 + 3 * st),
                     a4 = ldo(d, o0 + 4 * st), a5 = ldo(d, o0 + 5 * st), a6 = ldo(d, o0 + 6 * st),
                     a7 = ldo(d, o0 + 7 * st);
             t[0] = up ? a0 : a7; t[1] = up ? a1 : a6; t[2] = up ? a2 : a5; t[3] = up ? a3 : a4;
             t[4] = up ? a4 : a3; t[5] = up ? a5 : a2; t[6] = up ? a6 : a1;
+        } else {
+            // TENO8: the left stencil is cells c-4..c+3, the mirrored right one c+5..c-2
+            T a[10];
+#pragma unroll
+            for (int m = 0; m < 10; m++) a[m] = ldo(d, o0 + (uint32_t)m * st);
+#pragma unroll
+            for (int m = 0; m < 8; m++) t[m] = up ? a[m] : a[9 - m];
         }
         if constexpr (S == SCH_WENO5) return weno5_classic<T>(t[0], t[1], t[2], t[3], t[4]);
         else if constexpr (S == SCH_WENO_JS5) return weno_js5<T>(t[0], t[1], t[2], t[3], t[4]);
         else if constexpr (S == SCH_WENO_Z5) return weno_z5<P, T>(t[0], t[1], t[2], t[3], t[4]);
+        else if constexpr (S == SCH_TENO5) return teno5<T>(t[0], t[1], t[2], t[3], t[4], ct);
+        else if constexpr (S == SCH_TENO8) return teno8<T>(t[0], t[1], t[2], t[3], t[4], t[5], t[6], t[7], ct);
         else return weno_z7<P, T>(t[0], t[1], t[2], t[3], t[4], t[5], t[6]);
     }
 }

@@ -55,6 +55,8 @@ void launch_shell(hipStream_t st, T *out, const T *x, const T *snap, int Nx, int
 template <typename T>
 void launch_plane_sum(hipStream_t st, double *out, const T *f, const Metrics<double> &md);
 template <typename T>
+void launch_varcons(hipStream_t st, const Metrics<T> &m, T *tend, const T *u, const T *v, const T *w, const T *phi);
+template <typename T>
 int launch_advect_positive(hipStream_t st, const Metrics<T> &m, int scheme, int p, T *tend, const T *u, const T *v,
                            const T *w, const T *phi, T dt, T *const ws[7], const int *bcs, int accumulate);
 template <typename T>

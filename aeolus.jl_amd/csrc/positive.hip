@@ -45,17 +45,17 @@ __global__ void __launch_bounds__(256) pos_flux_kernel(T *__restrict__ Fx, T *__
     const uint32_t c = lin(i + m.H, j + m.H, k + m.H, m.sx, m.sy);  // face c: west/south/bottom face of cell c
     if (j < m.Ny && k < m.Nz) {
         const T vel = u[c];
-        Fx[c] = vel * face_value<S, P, T>(phi, (c - R) * (uint32_t)sizeof(T), (uint32_t)sizeof(T), vel);
+        Fx[c] = vel * face_value<S, P, T>(phi, (c - R) * (uint32_t)sizeof(T), (uint32_t)sizeof(T), vel, m.teno_ct);
         Lx[c] = vel * (vel >= T(0) ? phi[c - 1] : phi[c]);
     }
     if (i < m.Nx && k < m.Nz) {
         const T vel = v[c];
-        Fy[c] = vel * face_value<S, P, T>(phi, (c - R * sx) * (uint32_t)sizeof(T), sx * (uint32_t)sizeof(T), vel);
+        Fy[c] = vel * face_value<S, P, T>(phi, (c - R * sx) * (uint32_t)sizeof(T), sx * (uint32_t)sizeof(T), vel, m.teno_ct);
         Ly[c] = vel * (vel >= T(0) ? phi[c - sx] : phi[c]);
     }
     if (i < m.Nx && j < m.Ny) {
         const T vel = w[c];
-        Fz[c] = vel * face_value<S, P, T>(phi, (c - R * sxy) * (uint32_t)sizeof(T), sxy * (uint32_t)sizeof(T), vel);
+        Fz[c] = vel * face_value<S, P, T>(phi, (c - R * sxy) * (uint32_t)sizeof(T), sxy * (uint32_t)sizeof(T), vel, m.teno_ct);
         Lz[c] = vel * (vel >= T(0) ? phi[c - sxy] : phi[c]);
     }
 }
@@ -134,6 +134,8 @@ int launch_advect_positive(hipStream_t st, const Metrics<T> &m, int scheme, int 
         if (p == 1) launch_flux<SCH_WENO_Z7, 1, T>(st, ws, u, v, w, phi, m);
         else launch_flux<SCH_WENO_Z7, 2, T>(st, ws, u, v, w, phi, m);
         break;
+    case SCH_TENO5: launch_flux<SCH_TENO5, 0, T>(st, ws, u, v, w, phi, m); break;
+    case SCH_TENO8: launch_flux<SCH_TENO8, 0, T>(st, ws, u, v, w, phi, m); break;
     default: return -1;
     }
     const dim3 blk(64, 4), grd((m.Nx + 63) / 64, (m.Ny + 3) / 4, m.Nz);

@@ -153,7 +153,7 @@ __device__ __forceinline__ T flux_minus(const Fields<T> &f, const Metrics<T> &m,
         else vel = (lo + hi) / T(2);
     }
     constexpr int R = SchemeInfo<S>::R;
-    return vel * face_value<S, P, T>(field_ptr<F>(f), cb - (uint32_t)R * bd, bd, vel);
+    return vel * face_value<S, P, T>(field_ptr<F>(f), cb - (uint32_t)R * bd, bd, vel, m.teno_ct);
 }
 
 // _lap_at, src/FiniteVolume.jl:271-283 (f at byte offset cb, level k)
@@ -400,9 +400,47 @@ static int launch_march_mode(hipStream_t st, const MarchArgs<T> &a, int scheme, 
         if (p == 1) launch_march_sp<MODE, SCH_WENO_Z7, 1, T>(st, a);
         else launch_march_sp<MODE, SCH_WENO_Z7, 2, T>(st, a);
         return 0;
+    case SCH_TENO5:
+        if constexpr (MODE == MODE_SCALAR) {
+            launch_march_sp<MODE, SCH_TENO5, 0, T>(st, a);
+            return 0;
+        }
+        return -1;
+    case SCH_TENO8:
+        if constexpr (MODE == MODE_SCALAR) {
+            launch_march_sp<MODE, SCH_TENO8, 0, T>(st, a);
+            return 0;
+        }
+        return -1;
     }
     return -1;
 }
+
+// advect_variance_conserving! (src/AdvectionSchemes.jl:1428-1446), regular
+// grids: tend −= (u_e ϕ_{i+1} − u_w ϕ_{i−1})/(2Δx) + (y) + (z), the skew form
+// ½[∇·(uϕ) + u·∇ϕ] that conserves Σϕ² for any velocity. 1/(2Δ) = (1/Δ)/2 exactly.
+template <typename T>
+__global__ void __launch_bounds__(256) varcons_kernel(T *__restrict__ tend, const T *__restrict__ u,
+                                                      const T *__restrict__ v, const T *__restrict__ w,
+                                                      const T *__restrict__ phi, Metrics<T> m) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y * blockDim.y + threadIdx.y, k = blockIdx.z;
+    if (i >= m.Nx || j >= m.Ny) return;
+    const uint32_t sx = m.sx, sxy = (uint32_t)m.sx * m.sy;
+    const uint32_t c = lin(i + m.H, j + m.H, k + m.H, m.sx, m.sy);
+    const T h2x = m.idx * T(0.5), h2y = m.idy * T(0.5), h2z = m.idz * T(0.5);
+    const T d = ((u[c + 1] * phi[c + 1] - u[c] * phi[c - 1]) * h2x + (v[c + sx] * phi[c + sx] - v[c] * phi[c - sx]) * h2y) +
+                (w[c + sxy] * phi[c + sxy] - w[c] * phi[c - sxy]) * h2z;
+    tend[c] = tend[c] - d;
+}
+template <typename T>
+void launch_varcons(hipStream_t st, const Metrics<T> &m, T *tend, const T *u, const T *v, const T *w, const T *phi) {
+    const dim3 blk(64, 4), grd((m.Nx + 63) / 64, (m.Ny + 3) / 4, m.Nz);
+    hipLaunchKernelGGL(varcons_kernel<T>, grd, blk, 0, st, tend, u, v, w, phi, m);
+}
+template void launch_varcons<double>(hipStream_t, const Metrics<double> &, double *, const double *, const double *,
+                                     const double *, const double *);
+template void launch_varcons<float>(hipStream_t, const Metrics<float> &, float *, const float *, const float *,
+                                    const float *, const float *);
 
 template <typename T>
 int launch_advect(hipStream_t st, const Metrics<T> &m, int mode, int scheme, int p, const T *u, const T *v,

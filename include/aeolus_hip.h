@@ -49,7 +49,10 @@ enum {
     AEO_WENO5 = 2,    /* :weno5     (FiniteVolume advect_weno5!)              */
     AEO_WENO_JS5 = 3, /* :weno_js5                                            */
     AEO_WENO_Z5 = 4,  /* :weno_z5                                             */
-    AEO_WENO_Z7 = 5   /* :weno_z7                                             */
+    AEO_WENO_Z7 = 5,  /* :weno_z7                                             */
+    AEO_TENO5 = 6,    /* :teno5     (TENO(5; C_T = 1e-5), q = 6)              */
+    AEO_TENO8 = 7,    /* :teno8     (TENO(8; C_T = 1e-5), q = 6; halo 5)      */
+    AEO_VARIANCE_CONSERVING = 8 /* :variance_conserving (regular grids)      */
 };
 /* pressure-solver status, pcg! (src/Backends.jl:1482-1487) */
 enum { AEO_CONVERGED = 0, AEO_MAXIT = 1, AEO_BREAKDOWN = 2, AEO_ZERO_RHS = 3, AEO_DIRECT = 4 };
@@ -92,9 +95,16 @@ int aeo_fill_halos(aeo_grid *g, int precision, void *field, int loc, int bc_x, i
 /* ---- advection operators: the AdvectionScheme callable
  * scheme(tend, u, v, w, ϕ) (src/AdvectionSchemes.jl:1712-1718), which ADDS
  * −∂(uⱼϕ)/∂xⱼ to tend; `p` is the WENO-Z τ exponent (1 or 2). Halos of the
- * inputs must be filled and H >= required_halo(scheme) (3; 4 for WENO-Z7). */
+ * inputs must be filled and H >= required_halo(scheme) (1 centred, upwind and
+ * variance-conserving; 3 WENO-5 and TENO5; 4 WENO-Z7; 5 TENO8). The TENO
+ * schemes use C_T = 1e-5 here (advection_scheme(:teno5), :1765-1767);
+ * AEO_VARIANCE_CONSERVING is advect_variance_conserving! (:1428-1446). */
 int aeo_advect_scalar(aeo_grid *g, int precision, int scheme, int p, void *tend, const void *u, const void *v,
                       const void *w, const void *phi);
+/* advect_teno!(tend, u, v, w, ϕ; order, C_T, q) (src/AdvectionSchemes.jl:750-756):
+ * order 5 or 8, cut-off C_T > 0; q = 6 (the reference's default) only */
+int aeo_advect_teno(aeo_grid *g, int precision, int order, double C_T, int q, void *tend, const void *u,
+                    const void *v, const void *w, const void *phi);
 /* FluxLimiterWorkspace(mesh) (src/AdvectionSchemes.jl:800-822): scratch of
  * aeo_advect_positive for one grid and precision (seven storage arrays) */
 typedef struct aeo_flux_workspace aeo_flux_workspace;
@@ -105,13 +115,14 @@ int aeo_flux_limiter_workspace_destroy(aeo_flux_workspace *ws);
  * (accumulate = 0) −∇·(u ϕ) over tend's interior with the face fluxes limited
  * so that ϕ + dt·tend ≥ 0 for ϕ ≥ 0 (halos included) under the first-order
  * CFL condition. scheme: AEO_CENTERED, AEO_WENO5 or AEO_WENO_JS5 (both
- * WENO-JS5 on this path), AEO_WENO_Z5, AEO_WENO_Z7; a stretched grid takes
- * AEO_CENTERED only. bcs: three AEO_* of ϕ for the limiter's halo, or NULL
+ * WENO-JS5 on this path), AEO_WENO_Z5, AEO_WENO_Z7, AEO_TENO5, AEO_TENO8
+ * (cut-off C_T, ignored by the others); a stretched grid takes AEO_CENTERED
+ * only. bcs: three AEO_* of ϕ for the limiter's halo, or NULL
  * for the reference's one-layer periodic wrap. ws may be NULL (a workspace
  * is then allocated and freed inside the call). */
-int aeo_advect_positive(aeo_grid *g, int precision, int scheme, int p, void *tend, const void *u, const void *v,
-                        const void *w, const void *phi, double dt, aeo_flux_workspace *ws, const int *bcs,
-                        int accumulate);
+int aeo_advect_positive(aeo_grid *g, int precision, int scheme, int p, double C_T, void *tend, const void *u,
+                        const void *v, const void *w, const void *phi, double dt, aeo_flux_workspace *ws,
+                        const int *bcs, int accumulate);
 /* momentum_advect_weno_z!(tu, tv, tw, u, v, w) (src/AdvectionSchemes.jl:1195-1270)
  * and the momentum_advection keyword of step! (src/Simulations.jl:418-425):
  * ADDS −∂(uⱼuᵢ)/∂xⱼ to the face-centred tendencies. */

@@ -85,8 +85,70 @@ static inline double b4_m1(double v1, double v2, double v3, double v4) {
     return beta4((((-7 * v1 - 15 * v2) + 27 * v3) - 5 * v4) / 24, ((v1 - 2 * v2) + v3) / 2,
                  (((-v1 + 3 * v2) - 3 * v3) + v4) / 6);
 }
-/* _ipow, src/AdvectionSchemes.jl:173-180 (p = 1, 2) */
-static inline double ipow(double x, int p) { return p == 2 ? x * x : x; }
+/* _ipow, src/AdvectionSchemes.jl:173-180 (p = 1, 2; 6 as x³·x³) */
+static inline double ipow(double x, int p) {
+    if (p == 6) {
+        double x3 = (x * x) * x;
+        return x3 * x3;
+    }
+    return p == 2 ? x * x : x;
+}
+/* _beta5 and _b5_m4, src/AdvectionSchemes.jl:111-114, 143-147 */
+static inline double beta5(double a1, double a2, double a3, double a4) {
+    return ((((a1 * a1 + (1.0 / 2.0) * (a1 * a3)) + (13.0 / 3.0) * (a2 * a2)) + (21.0 / 5.0) * (a2 * a4)) +
+            (3129.0 / 80.0) * (a3 * a3)) +
+           (87617.0 / 140.0) * (a4 * a4);
+}
+static inline double b5_m4(double v1, double v2, double v3, double v4, double v5) {
+    return beta5(((((9 * v1 - 50 * v2) + 120 * v3) - 174 * v4) + 95 * v5) / 48,
+                 ((((7 * v1 - 36 * v2) + 74 * v3) - 68 * v4) + 23 * v5) / 16,
+                 ((((3 * v1 - 14 * v2) + 24 * v3) - 18 * v4) + 5 * v5) / 12,
+                 ((((v1 - 4 * v2) + 6 * v3) - 4 * v4) + v5) / 24);
+}
+
+static double teno_ct = 1e-5;
+void orc_set_teno_cutoff(double C_T) { teno_ct = C_T; }
+
+/* _teno5_left, src/AdvectionSchemes.jl:405-427 (q = 6) */
+double orc_teno5_left(double v1, double v2, double v3, double v4, double v5, double c) {
+    double q0 = ((2 * v1 - 7 * v2) + 11 * v3) / 6;
+    double q1 = ((-v2 + 5 * v3) + 2 * v4) / 6;
+    double q2 = ((2 * v3 + 5 * v4) - v5) / 6;
+    double b0 = b3_m2(v1, v2, v3), b1 = b3_m1(v2, v3, v4), b2 = b3_p0(v3, v4, v5);
+    double eps = DBL_EPSILON * DBL_EPSILON;
+    double tau = fabs(b0 - b2);
+    double g0 = ipow(1.0 + tau / (b0 + eps), 6);
+    double g1 = ipow(1.0 + tau / (b1 + eps), 6);
+    double g2 = ipow(1.0 + tau / (b2 + eps), 6);
+    double gs = (g0 + g1) + g2;
+    double d0 = g0 / gs < c ? 0.0 : 1.0 / 10;
+    double d1 = g1 / gs < c ? 0.0 : 6.0 / 10;
+    double d2 = g2 / gs < c ? 0.0 : 3.0 / 10;
+    return ((d0 * q0 + d1 * q1) + d2 * q2) / ((d0 + d1) + d2);
+}
+
+/* _teno8_left, src/AdvectionSchemes.jl:456-495 (q = 6) */
+double orc_teno8_left(double v1, double v2, double v3, double v4, double v5, double v6, double v7, double v8,
+                      double c) {
+    double q0 = ((2 * v3 - 7 * v4) + 11 * v5) / 6;
+    double q1 = ((-v4 + 5 * v5) + 2 * v6) / 6;
+    double q2 = ((2 * v5 + 5 * v6) - v7) / 6;
+    double q3 = (((-3 * v2 + 13 * v3) - 23 * v4) + 25 * v5) / 12;
+    double q4 = (((3 * v5 + 13 * v6) - 5 * v7) + v8) / 12;
+    double q5 = ((((12 * v1 - 63 * v2) + 137 * v3) - 163 * v4) + 137 * v5) / 60;
+    double b0 = b3_m2(v3, v4, v5), b1 = b3_m1(v4, v5, v6), b2 = b3_p0(v5, v6, v7);
+    double b3 = b4_m3(v2, v3, v4, v5), b4 = b4_p0(v5, v6, v7, v8), b5 = b5_m4(v1, v2, v3, v4, v5);
+    double eps = DBL_EPSILON * DBL_EPSILON;
+    double tau = fabs(b5 - ((b0 + 4 * b1) + b2) / 6);
+    double g[6] = {b0, b1, b2, b3, b4, b5};
+    for (int m = 0; m < 6; m++) g[m] = ipow(1.0 + tau / (g[m] + eps), 6);
+    double gs = ((((g[0] + g[1]) + g[2]) + g[3]) + g[4]) + g[5];
+    const double dl[6] = {3.0 / 28, 15.0 / 28, 3.0 / 14, 3.0 / 56, 1.0 / 14, 1.0 / 56};
+    double d[6];
+    for (int m = 0; m < 6; m++) d[m] = g[m] / gs < c ? 0.0 : dl[m];
+    return (((((d[0] * q0 + d[1] * q1) + d[2] * q2) + d[3] * q3) + d[4] * q4) + d[5] * q5) /
+           (((((d[0] + d[1]) + d[2]) + d[3]) + d[4]) + d[5]);
+}
 
 /* _weno_z5_left, src/AdvectionSchemes.jl:240-255; _zeps = eps(T)^2 (:152) */
 double orc_weno_z5_left(double v1, double v2, double v3, double v4, double v5, int p) {
@@ -167,6 +229,8 @@ static inline double recon_left(int scheme, int p, const double *t) {
     case ORC_WENO_JS5: return orc_weno_js5_left(t[0], t[1], t[2], t[3], t[4], 1e-6);
     case ORC_WENO_Z5: return orc_weno_z5_left(t[0], t[1], t[2], t[3], t[4], p);
     case ORC_WENO_Z7: return orc_weno_z7_left(t[0], t[1], t[2], t[3], t[4], t[5], t[6], p);
+    case ORC_TENO5: return orc_teno5_left(t[0], t[1], t[2], t[3], t[4], teno_ct);
+    case ORC_TENO8: return orc_teno8_left(t[0], t[1], t[2], t[3], t[4], t[5], t[6], t[7], teno_ct);
     }
     return NAN;
 }
@@ -177,6 +241,7 @@ static void stencil_geom(int scheme, int *lo_l, int *lo_r, int *n) {
     case ORC_CENTERED: *lo_l = 0; *lo_r = 0; *n = 2; break;
     case ORC_UPWIND1: *lo_l = 0; *lo_r = 1; *n = 1; break;
     case ORC_WENO_Z7: *lo_l = -3; *lo_r = -2; *n = 7; break;
+    case ORC_TENO8: *lo_l = -4; *lo_r = -2; *n = 8; break;
     default: *lo_l = -2; *lo_r = -1; *n = 5; break;
     }
 }
@@ -285,6 +350,22 @@ static inline double scalar_adv_at(const orc_grid *g, int sx, int sy, int scheme
                                    const double *u, const double *v, const double *w, const double *d) {
     if (scheme == ORC_CENTERED) return adv_centered_at(g, sx, sy, i, j, k, u, v, w, d);
     return adv_recon_at(g, sx, sy, scheme, p, i, j, k, u, v, w, d);
+}
+
+/* _b_varcons!, src/AdvectionSchemes.jl:1433-1438: 1/(2Δ) from the grid spacing */
+void orc_advect_variance_conserving(double *tend, const double *u, const double *v, const double *w,
+                                    const double *phi, const orc_grid *g) {
+    GRID_DIMS(g);
+    int H = g->H;
+    double h2x = 1 / (2 * g->dx), h2y = 1 / (2 * g->dy), h2z = 1 / (2 * g->dz);
+    for (int k = H + 1; k <= H + g->Nz; k++)
+        for (int j = H + 1; j <= H + g->Ny; j++)
+            for (int i = H + 1; i <= H + g->Nx; i++) {
+                double d = ((u[ID(i + 1, j, k)] * phi[ID(i + 1, j, k)] - u[ID(i, j, k)] * phi[ID(i - 1, j, k)]) * h2x +
+                            (v[ID(i, j + 1, k)] * phi[ID(i, j + 1, k)] - v[ID(i, j, k)] * phi[ID(i, j - 1, k)]) * h2y) +
+                           (w[ID(i, j, k + 1)] * phi[ID(i, j, k + 1)] - w[ID(i, j, k)] * phi[ID(i, j, k - 1)]) * h2z;
+                tend[ID(i, j, k)] -= d;
+            }
 }
 
 void orc_advect_scalar(double *tend, const double *u, const double *v, const double *w, const double *phi,

@@ -43,7 +43,9 @@ enum {
     ORC_WENO5 = 2,      /* FiniteVolume._weno5_left (advect_weno5!)          */
     ORC_WENO_JS5 = 3,   /* weno_js5_left                                     */
     ORC_WENO_Z5 = 4,    /* _weno_z5_left                                     */
-    ORC_WENO_Z7 = 5     /* _weno_z7_left                                     */
+    ORC_WENO_Z7 = 5,    /* _weno_z7_left                                     */
+    ORC_TENO5 = 6,      /* _teno5_left (q = 6, cut-off orc_set_teno_cutoff)  */
+    ORC_TENO8 = 7       /* _teno8_left (q = 6)                               */
 };
 
 double orc_weno_z5_left(double v1, double v2, double v3, double v4, double v5, int p);
@@ -56,6 +58,14 @@ void orc_fill_halos(double *a, const orc_grid *g, int loc, int bcx, int bcy, int
 
 void orc_advect_scalar(double *tend, const double *u, const double *v, const double *w,
                        const double *phi, const orc_grid *g, int scheme, int p);
+/* TENO's cut-off C_T for the following calls (default 1e-5, TENO(N; C_T)) */
+void orc_set_teno_cutoff(double C_T);
+double orc_teno5_left(double v1, double v2, double v3, double v4, double v5, double C_T);
+double orc_teno8_left(double v1, double v2, double v3, double v4, double v5, double v6, double v7, double v8,
+                      double C_T);
+/* advect_variance_conserving! (src/AdvectionSchemes.jl:1428-1446), regular grids */
+void orc_advect_variance_conserving(double *tend, const double *u, const double *v, const double *w,
+                                    const double *phi, const orc_grid *g);
 /* advect_positive! (src/AdvectionSchemes.jl:1135-1160): limited fluxes so a
  * forward-Euler update with the same dt keeps ϕ ≥ 0; use_bcs = 0 wraps the
  * limiter's halo periodically by one layer (the reference's bcs = nothing) */

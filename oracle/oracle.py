@@ -19,9 +19,9 @@ LIB = os.path.join(HERE, "liboracle.so")
 
 CC, FX, FY, FZ = 0, 1, 2, 3
 PERIODIC, NOFLUX, WALL = 0, 1, 2
-CENTERED, UPWIND1, WENO5, WENO_JS5, WENO_Z5, WENO_Z7 = 0, 1, 2, 3, 4, 5
+CENTERED, UPWIND1, WENO5, WENO_JS5, WENO_Z5, WENO_Z7, TENO5, TENO8 = 0, 1, 2, 3, 4, 5, 6, 7
 SCHEMES = {"centered": CENTERED, "upwind": UPWIND1, "weno5": WENO5, "weno_js5": WENO_JS5, "weno_z5": WENO_Z5,
-           "weno_z7": WENO_Z7}
+           "weno_z7": WENO_Z7, "teno5": TENO5, "teno8": TENO8}
 BCS = {"periodic": PERIODIC, "noflux": NOFLUX, "neumann": NOFLUX, "wall": WALL}
 
 
@@ -68,6 +68,10 @@ def lib():
             "orc_fill_halos": (None, [pd, pg, i, i, i, i]),
             "orc_advect_scalar": (None, [pd, pd, pd, pd, pd, pg, i, i]),
             "orc_advect_positive": (None, [pd, pd, pd, pd, pd, pg, i, i, d, i, i, i, i, i, pd]),
+            "orc_set_teno_cutoff": (None, [d]),
+            "orc_teno5_left": (d, [d, d, d, d, d, d]),
+            "orc_teno8_left": (d, [d, d, d, d, d, d, d, d, d]),
+            "orc_advect_variance_conserving": (None, [pd, pd, pd, pd, pd, pg]),
             "orc_advect_momentum": (None, [pd, pd, pd, pd, pd, pd, pg, i, i]),
             "orc_laplacian": (None, [pd, pd, pg, i]),
             "orc_divergence_into": (None, [pd, pd, pd, pd, pg]),
@@ -200,6 +204,16 @@ def fill_halos(g, a, loc, bcs):
 
 def advect_scalar(g, tend, u, v, w, phi, scheme, p=2):
     lib().orc_advect_scalar(ptr(tend), ptr(u), ptr(v), ptr(w), ptr(phi), ctypes.byref(g.c), SCHEMES[scheme], p)
+    return tend
+
+
+def teno_cutoff(C_T=1e-5):
+    """TENO's C_T for the oracle's following TENO evaluations."""
+    lib().orc_set_teno_cutoff(float(C_T))
+
+
+def advect_variance_conserving(g, tend, u, v, w, phi):
+    lib().orc_advect_variance_conserving(ptr(tend), ptr(u), ptr(v), ptr(w), ptr(phi), ctypes.byref(g.c))
     return tend
 
 

@@ -74,6 +74,78 @@ def test_advect_scalar_parity(hip, orc, scheme, stretched):
 
 
 @pytest.mark.parametrize("stretched", [False, True])
+@pytest.mark.parametrize("scheme", ["teno5", "teno8", "variance_conserving"])
+def test_advect_scalar_teno_skew_parity(hip, orc, scheme, stretched):
+    # the rest of advection_scheme's names (src/AdvectionSchemes.jl:1765-1770)
+    H = 5 if scheme == "teno8" else 3
+    g, og = grids(hip, orc, (70, 21, 37), (3.0, 1.0, 2.0), H, stretched)
+    bcs = ("periodic", "periodic", "wall" if stretched else "periodic")
+    u, v, w, phi = _fields(orc, og, 300, bcs)
+    t0 = rand_storage(og.storage, 98)
+    dt_ = up(hip, t0)
+    args = (up(hip, u), up(hip, v), up(hip, w), up(hip, phi))
+    if scheme == "variance_conserving" and stretched:
+        with pytest.raises(hip.AeolusError, match="regular grids"):
+            hip.advection_scheme(scheme, g)(dt_, *args)
+        return
+    t = t0.copy(order="F")
+    if scheme == "variance_conserving":
+        orc.advect_variance_conserving(og, t, u, v, w, phi)
+    else:
+        orc.advect_scalar(og, t, u, v, w, phi, scheme)
+    hip.advection_scheme(scheme, g)(dt_, *args)
+    I = og.interior
+    assert rel_err(dt_.download()[I], t[I]) <= TOL64
+
+
+def test_teno_cutoff_selector_and_refusals(hip, orc):
+    g, og = grids(hip, orc, (40, 24, 20), (2.0, 1.0, 1.0), 3)
+    bcs = ("periodic",) * 3
+    u, v, w, phi = _fields(orc, og, 400, bcs)
+    args = [up(hip, a) for a in (u, v, w, phi)]
+    I = og.interior
+    # advect_teno!(…; C_T = 1e-3) against the oracle with the same cut-off
+    orc.teno_cutoff(1e-3)
+    try:
+        ref = orc.advect_scalar(og, og.zeros(), u, v, w, phi, "teno5")
+    finally:
+        orc.teno_cutoff(1e-5)
+    a = up(hip, og.zeros())
+    hip.advect_teno(g, a, *args, order=5, C_T=1e-3)
+    assert rel_err(a.download()[I], ref[I]) <= TOL64
+    # the selector's :teno5 is advect_teno!(…; order = 5), bit for bit (test/advection.jl:171-186)
+    b, c = up(hip, og.zeros()), up(hip, og.zeros())
+    hip.advection_scheme("teno5", g)(b, *args)
+    hip.advect_teno(g, c, *args, order=5)
+    assert np.array_equal(b.download(), c.download())
+    assert not np.array_equal(a.download(), c.download())  # the cut-off matters on rough data
+    with pytest.raises(hip.AeolusError, match="halo"):
+        hip.advect_teno(g, c, *args, order=8)  # halo 3 < 5
+    with pytest.raises(hip.AeolusError):
+        hip.advect_teno(g, c, *args, order=5, q=4)
+    with pytest.raises(hip.AeolusError):
+        hip.advection_scheme("teno5", g).momentum(b, c, a, *args[:3])
+    with pytest.raises(hip.AeolusError):
+        hip.BoussinesqSimulation(g, scheme="teno5")
+
+
+@pytest.mark.parametrize("scheme", ["teno5", "teno8"])
+def test_advect_positive_teno_parity(hip, orc, scheme):
+    H = 5 if scheme == "teno8" else 3
+    g, og = grids(hip, orc, (32, 20, 16), (32.0, 20.0, 16.0), H)
+    bcs = ("periodic",) * 3
+    u, v, w, _ = _fields(orc, og, 500, bcs)
+    phi = og.zeros()
+    phi[...] = np.random.default_rng(5).uniform(0, 1, phi.shape) ** 4
+    orc.fill_halos(og, phi, orc.CC, bcs)
+    ref = orc.advect_positive(og, og.zeros(), u, v, w, phi, 0.1, scheme=scheme, accumulate=False)
+    t = up(hip, og.zeros())
+    hip.advect_positive(g, t, *(up(hip, a) for a in (u, v, w, phi)), 0.1, scheme=scheme, accumulate=False)
+    I = og.interior
+    assert rel_err(t.download()[I], ref[I]) <= TOL64
+
+
+@pytest.mark.parametrize("stretched", [False, True])
 @pytest.mark.parametrize("scheme", SCHEMES)
 def test_advect_momentum_parity(hip, orc, scheme, stretched):
     H = 4 if scheme == "weno_z7" else 3
@@ -90,7 +162,7 @@ def test_advect_momentum_parity(hip, orc, scheme, stretched):
         assert rel_err(d.download()[I], ref[I]) <= TOL64
 
 
-@pytest.mark.parametrize("scheme", ["weno_z5", "weno5"])
+@pytest.mark.parametrize("scheme", ["weno_z5", "weno5", "teno5"])
 def test_advect_fp32_error_stated(hip, orc, scheme):
     # FP32 tendencies against the FP64 oracle on smooth data: stated bound 1e-5
     g, og = grids(hip, orc, (64, 32, 16), (2 * np.pi, np.pi, np.pi / 2))
