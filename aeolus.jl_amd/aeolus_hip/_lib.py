@@ -62,6 +62,7 @@ SIGNATURES = {
     "aeo_shutdown": [],
     "aeo_sync": [],
     "aeo_last_error": [],
+    "aeo_build_info": [],
     "aeo_malloc": [_pvp, _sz],
     "aeo_free": [_vp],
     "aeo_memcpy_h2d": [_vp, _vp, _sz],
@@ -92,6 +93,8 @@ SIGNATURES = {
     "aeo_stepper_set_coriolis": [_vp, _d],
     "aeo_stepper_set_sgs": [_vp, _d, _d],
     "aeo_stepper_cfl": [_vp, _vp, _vp, _vp, _vp, _pd, _pd, _pd],
+    "aeo_device_info": [_i, ctypes.c_char_p, ctypes.c_size_t, ctypes.c_char_p, ctypes.c_size_t,
+                        ctypes.POINTER(_i), ctypes.POINTER(ctypes.c_size_t)],
     "aeo_flux_limiter_workspace_create": [_vp, _i, ctypes.POINTER(_vp)],
     "aeo_flux_limiter_workspace_destroy": [_vp],
     "aeo_advect_positive": [_vp, _i, _i, _i, _d, _vp, _vp, _vp, _vp, _vp, _d, _vp, ctypes.POINTER(_i), _i],
@@ -155,7 +158,7 @@ def load():
             for name, argt in SIGNATURES.items():
                 f = getattr(lib, name)
                 f.argtypes = argt
-                f.restype = ctypes.c_char_p if name == "aeo_last_error" else ctypes.c_int
+                f.restype = ctypes.c_char_p if name in ("aeo_last_error", "aeo_build_info") else ctypes.c_int
             _lib = lib
     return _lib
 

@@ -22,6 +22,8 @@ import warnings
 
 import numpy as np
 
+from .provenance import provenance
+
 CHECKPOINT_FORMAT = "Aeolus.checkpoint"
 CHECKPOINT_VERSION = 1
 _FIELDS = ("u", "v", "w", "p", "theta")
@@ -58,7 +60,7 @@ def checkpoint_payload(sim, extra=None):
     meta = dict(format=CHECKPOINT_FORMAT, version=CHECKPOINT_VERSION, created=time.time(),
                 sim_type=type(sim).__name__, mesh=_mesh_record(sim.grid), t=float(getattr(sim, "t", 0.0)),
                 parameters=_jsonable(sim.parameters()), reference=_jsonable(checkpoint_reference(sim)),
-                precision=int(sim.precision), extra=_jsonable(extra or {}))
+                precision=int(sim.precision), extra=_jsonable(extra or {}), provenance=provenance())
     return meta, fields
 
 

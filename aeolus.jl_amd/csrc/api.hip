@@ -213,6 +213,31 @@ extern "C" int aeo_grid_metrics(const aeo_grid *g, double *idzc, double *idzf) {
 // ---------------------------------------------------------------------------
 extern "C" const char *aeo_last_error(void) { return g_err; }
 
+// What built the library: compiler, HIP version, target (the build half of
+// provenance(), src/io/run_folder.jl:215-260)
+extern "C" const char *aeo_build_info(void) {
+#define AEO_STR2(x) #x
+#define AEO_STR(x) AEO_STR2(x)
+    return "libaeolus_hip; compiler: " __clang_version__ "; HIP " AEO_STR(HIP_VERSION_MAJOR) "." AEO_STR(
+        HIP_VERSION_MINOR) "." AEO_STR(HIP_VERSION_PATCH) "; target gfx950; built " __DATE__ " " __TIME__;
+#undef AEO_STR
+#undef AEO_STR2
+}
+
+// The device a run used: name, gfx architecture, compute units, HBM bytes
+extern "C" int aeo_device_info(int device, char *name, size_t name_len, char *arch, size_t arch_len, int *cus,
+                               size_t *mem_bytes) {
+    if (!name || !arch || !cus || !mem_bytes || name_len == 0 || arch_len == 0)
+        return fail(AEO_ERR_ARG, "null argument");
+    hipDeviceProp_t pr;
+    HIPCHK(hipGetDeviceProperties(&pr, device));
+    snprintf(name, name_len, "%s", pr.name);
+    snprintf(arch, arch_len, "%s", pr.gcnArchName);
+    *cus = pr.multiProcessorCount;
+    *mem_bytes = pr.totalGlobalMem;
+    return AEO_OK;
+}
+
 extern "C" int aeo_init(int device) {
     if (g_ctx) {
         if (g_ctx->device == device) return AEO_OK;

@@ -312,6 +312,7 @@ def main():
         sys.exit(2)
     rank = int(os.environ.get("RANK", "0"))
 
+    from aeolus_hip.provenance import provenance
     t64, prof64, div64 = run_gpu(args.steps, args.warmup, 0)
     t32, prof32, div32 = run_gpu(args.steps, args.warmup, 1)
     pms = poisson_ms()
@@ -374,6 +375,7 @@ def main():
                      "avg_launch_ms": avg_launch_s * 1e3, "algorithmic_bytes_per_launch": alg,
                      "valu": valu},
         "cpu_baseline": cpu,
+        "provenance": provenance(device=0),
     }
     print(json.dumps(out), flush=True)
 

@@ -58,6 +58,12 @@ enum {
 enum { AEO_CONVERGED = 0, AEO_MAXIT = 1, AEO_BREAKDOWN = 2, AEO_ZERO_RHS = 3, AEO_DIRECT = 4 };
 
 /* ---- context ------------------------------------------------------------ */
+/* provenance of a run (provenance(), src/io/run_folder.jl:215-260): the
+ * compiler, HIP version and target the library was built with (a static
+ * string), and the device's name, gfx architecture, compute units and HBM */
+const char *aeo_build_info(void);
+int aeo_device_info(int device, char *name, size_t name_len, char *arch, size_t arch_len, int *cus,
+                    size_t *mem_bytes);
 /* No reference counterpart: selects the device and creates the stream. */
 int aeo_init(int device);
 int aeo_shutdown(void);

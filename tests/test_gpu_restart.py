@@ -58,3 +58,19 @@ def test_restart_continues_bit_for_bit(hip, tmp_path, core):
     for n in ("u", "v", "w", "theta", "p"):
         assert np.array_equal(got[n], cont[n]), n
     assert other.t == sim.t
+
+
+def test_provenance_names_the_device_and_library(hip, tmp_path):
+    from aeolus_hip.provenance import device_record, provenance
+    d = device_record(0)
+    assert d["name"] and d["arch"].startswith("gfx9") and d["compute_units"] > 0 and d["memory_bytes"] > 0
+    print(f"device: {d}")
+    rec = provenance(device=0)
+    assert rec["device"] == d and len(rec["library"]["sha256"]) == 64
+    # a checkpoint carries the provenance of the run that wrote it
+    g = hip.Grid((8, 8, 8), (1.0, 1.0, 1.0), halo=3)
+    sim = hip.BoussinesqSimulation(g)
+    path = str(tmp_path / "c.npz")
+    hip.checkpoint(sim, path)
+    meta, _ = hip.read_checkpoint(path)
+    assert meta["provenance"]["library"]["sha256"] == rec["library"]["sha256"]
