@@ -1293,10 +1293,12 @@ static int substage(aeo_stepper *h, int stage, T *u, T *v, T *w, T *th, T *p, do
       *wo = (stage == 0 || stage == 3) ? w : wa;
     {
         Timed t("halo");
-        launch_halo<T>(st, ui, Nx, Ny, Nz, H, LOC_FX, sp.bc_x, sp.bc_y, sp.bc_z);
-        launch_halo<T>(st, vi, Nx, Ny, Nz, H, LOC_FY, sp.bc_x, sp.bc_y, sp.bc_z);
-        launch_halo<T>(st, wi, Nx, Ny, Nz, H, LOC_FZ, sp.bc_x, sp.bc_y, sp.bc_z);
-        launch_halo<T>(st, ti, Nx, Ny, Nz, H, LOC_CC, sp.tbc_x, sp.tbc_y, sp.tbc_z);
+        HaloSet<T> hs{};
+        hs.add(ui, LOC_FX, sp.bc_x, sp.bc_y, sp.bc_z);
+        hs.add(vi, LOC_FY, sp.bc_x, sp.bc_y, sp.bc_z);
+        hs.add(wi, LOC_FZ, sp.bc_x, sp.bc_y, sp.bc_z);
+        hs.add(ti, LOC_CC, sp.tbc_x, sp.tbc_y, sp.tbc_z);
+        launch_halo_multi<T>(st, hs, Nx, Ny, Nz, H);
     }
     StepPhys ph{sp.nu,       dt, sp.theta0, sp.g_accel, sp.flux_bot, sp.flux_top, g->idzc[H], g->idzc[H + Nz - 1],
                 h->fcor, h->spg};
@@ -1328,9 +1330,11 @@ static int substage(aeo_stepper *h, int stage, T *u, T *v, T *w, T *th, T *p, do
     }
     {
         Timed t("halo");
-        launch_halo<T>(st, us, Nx, Ny, Nz, H, LOC_FX, sp.bc_x, sp.bc_y, sp.bc_z);
-        launch_halo<T>(st, vs, Nx, Ny, Nz, H, LOC_FY, sp.bc_x, sp.bc_y, sp.bc_z);
-        launch_halo<T>(st, ws, Nx, Ny, Nz, H, LOC_FZ, sp.bc_x, sp.bc_y, sp.bc_z);
+        HaloSet<T> hs{};
+        hs.add(us, LOC_FX, sp.bc_x, sp.bc_y, sp.bc_z);
+        hs.add(vs, LOC_FY, sp.bc_x, sp.bc_y, sp.bc_z);
+        hs.add(ws, LOC_FZ, sp.bc_x, sp.bc_y, sp.bc_z);
+        launch_halo_multi<T>(st, hs, Nx, Ny, Nz, H);
     }
     {
         Timed t("divergence");
@@ -1512,9 +1516,11 @@ static int mw_step(aeo_mw_stepper *h, T *u, T *v, T *w, T *th, T *p, double dt) 
     }
     {
         Timed t("halo");
-        launch_halo<T>(st, us, Nx, Ny, Nz, H, LOC_FX, sp.bc_x, sp.bc_y, sp.bc_z);
-        launch_halo<T>(st, vs, Nx, Ny, Nz, H, LOC_FY, sp.bc_x, sp.bc_y, sp.bc_z);
-        launch_halo<T>(st, ws, Nx, Ny, Nz, H, LOC_FZ, sp.bc_x, sp.bc_y, sp.bc_z);
+        HaloSet<T> hs{};
+        hs.add(us, LOC_FX, sp.bc_x, sp.bc_y, sp.bc_z);
+        hs.add(vs, LOC_FY, sp.bc_x, sp.bc_y, sp.bc_z);
+        hs.add(ws, LOC_FZ, sp.bc_x, sp.bc_y, sp.bc_z);
+        launch_halo_multi<T>(st, hs, Nx, Ny, Nz, H);
     }
     {
         Timed t("divergence");

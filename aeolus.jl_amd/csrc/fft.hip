@@ -334,6 +334,59 @@ __global__ void c2c_cols_kernel(double2 *__restrict__ d, FFTLen f, long es, long
 // reference's constructor (:648-660) is evaluated on the fly, the same
 // operations in the same order, with c'_k kept in `cp` for the back
 // substitution. The (0, 0) column is the mean mode (_mean_mode!, :780-805).
+// wave64 scans in a fixed order (Hillis-Steele over shuffles): the same
+// association for every run
+__device__ __forceinline__ double wave_incl_scan(double x) {
+    const int l = threadIdx.x & 63;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const double y = __shfl_up(x, o, 64);
+        if (l >= o) x += y;
+    }
+    return x;
+}
+
+// The mean-mode column of the column solve with one wave (blockDim = 64, all
+// lanes present), in chunks of 64 levels, lane l on level kb + l:
+//   mb   = Σ_k h_k m_k / Σ m
+//   F_k  = Σ_{j<k} (h_j − mb) m_j,  φ_k = Σ_{1≤j≤k} F_j icoup_j   (φ_0 = 0)
+//   φ_k ← φ_k − Σ φ g / Σ g
+// The prefix sums are wave scans with the chunk carries added in level order.
+// φ goes through cp (this lane's own levels) between the passes.
+__device__ void mean_mode_wave(double2 *__restrict__ hat, double *__restrict__ cp, size_t plane, int Nz,
+                               const double *__restrict__ measure, const double *__restrict__ icoup,
+                               const double *__restrict__ gauge, double measure_sum, double gauge_sum) {
+    const int l = threadIdx.x & 63;
+    double mb = 0.0;
+    for (int kb = 0; kb < Nz; kb += 64) {
+        const int k = kb + l;
+        const double t = k < Nz ? hat[plane * k].x * measure[k] : 0.0;
+        mb += __shfl(wave_incl_scan(t), 63, 64);
+    }
+    mb /= measure_sum;
+    double Fc = 0.0, Pc = 0.0, mphi = 0.0;
+    for (int kb = 0; kb < Nz; kb += 64) {
+        const int k = kb + l;
+        const bool in = k < Nz;
+        const double a = in ? (hat[plane * k].x - mb) * measure[k] : 0.0;
+        const double ai = wave_incl_scan(a);
+        double ex = __shfl_up(ai, 1, 64);
+        if (l == 0) ex = 0.0;
+        const double F = Fc + ex;  // Σ_{j<k} (h_j − mb) m_j
+        const double b = (in && k >= 1) ? F * icoup[k] : 0.0;
+        const double ph = Pc + wave_incl_scan(b);
+        if (in) cp[plane * k] = ph;
+        mphi += __shfl(wave_incl_scan(in ? ph * gauge[k] : 0.0), 63, 64);
+        Fc += __shfl(ai, 63, 64);
+        Pc = __shfl(ph, 63, 64);
+    }
+    const double sh = mphi / gauge_sum;
+    for (int kb = 0; kb < Nz; kb += 64) {
+        const int k = kb + l;
+        if (k < Nz) hat[plane * k] = make_double2(cp[plane * k] - sh, 0.0);
+    }
+}
+
 __global__ void __launch_bounds__(64) thomas_kernel(double2 *__restrict__ hat, double *__restrict__ cp, int nkx, int Ny, int Nz, int LD,
                               const double *__restrict__ lx, const double *__restrict__ ly,
                               const double *__restrict__ lower, const double *__restrict__ upper,
@@ -342,62 +395,16 @@ __global__ void __launch_bounds__(64) thomas_kernel(double2 *__restrict__ hat, d
                               double gauge_sum) {
     const int kx = blockIdx.x * blockDim.x + threadIdx.x;
     const int ky = blockIdx.y;
-    if (kx >= nkx) return;
     const size_t plane = (size_t)LD * Ny;
     const size_t c0 = (size_t)kx + (size_t)LD * ky;
-    if (kx == 0 && ky == 0) {
-        // the mean mode: remove the m-weighted mean, integrate the flux up,
-        // gauge. One thread walks the column three times; the loads go in
-        // batches of MB levels (one memory round trip per batch, not per
-        // level); every sum keeps the level order.
-        constexpr int MB = 16;
-        double mb = 0.0;
-        for (int kb = 0; kb < Nz; kb += MB) {
-            double h[MB];
-#pragma unroll
-            for (int q = 0; q < MB; q++)
-                if (kb + q < Nz) h[q] = hat[c0 + plane * (kb + q)].x;
-#pragma unroll
-            for (int q = 0; q < MB; q++)
-                if (kb + q < Nz) mb += h[q] * measure[kb + q];
-        }
-        mb /= measure_sum;
-        // φ_0 = 0, φ_k = φ_{k−1} + icoup_k Σ_{j<k} (b_j − mb) m_j; mphi = Σ φ_k g_k
-        double F = 0.0, ph = 0.0, mphi = 0.0, prev = 0.0;
-        for (int kb = 0; kb < Nz; kb += MB) {
-            double h[MB], o[MB];
-#pragma unroll
-            for (int q = 0; q < MB; q++)
-                if (kb + q < Nz) h[q] = hat[c0 + plane * (kb + q)].x;
-#pragma unroll
-            for (int q = 0; q < MB; q++) {
-                const int k = kb + q;
-                if (k < Nz) {
-                    if (k > 0) {
-                        F += prev * measure[k - 1];
-                        ph += F * icoup[k];
-                    }
-                    prev = h[q] - mb;
-                    o[q] = ph;
-                    mphi += ph * gauge[k];
-                }
-            }
-#pragma unroll
-            for (int q = 0; q < MB; q++)
-                if (kb + q < Nz) cp[c0 + plane * (kb + q)] = o[q];
-        }
-        const double sh = mphi / gauge_sum;
-        for (int kb = 0; kb < Nz; kb += MB) {
-            double o[MB];
-#pragma unroll
-            for (int q = 0; q < MB; q++)
-                if (kb + q < Nz) o[q] = cp[c0 + plane * (kb + q)];
-#pragma unroll
-            for (int q = 0; q < MB; q++)
-                if (kb + q < Nz) hat[c0 + plane * (kb + q)] = make_double2(o[q] - sh, 0.0);
-        }
-        return;
+    if (blockIdx.x == 0 && ky == 0) {
+        // the mean mode (kx = ky = 0), by the whole wave of this block before
+        // its lanes 1..63 solve their own columns: remove the m-weighted mean,
+        // integrate the flux up, gauge (_mean_mode!, src/PressureSolvers.jl:780-805)
+        mean_mode_wave(hat, cp, plane, Nz, measure, icoup, gauge, measure_sum, gauge_sum);
+        if (kx == 0) return;
     }
+    if (kx >= nkx) return;
     const double lam = lx[kx] + ly[ky];
     // The sweeps take TB levels at a time and are software-pipelined: the
     // loads of the next batch are issued before the current one is worked

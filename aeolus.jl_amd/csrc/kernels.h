@@ -26,6 +26,22 @@ inline void stage_coeffs(int stage, const StepPhys *, T *a, T *b, int *combine) 
 
 template <typename T>
 void launch_halo(hipStream_t st, T *d, int Nx, int Ny, int Nz, int H, int loc, int bcx, int bcy, int bcz);
+// up to four fields of one grid, each with its own BCs; wall_w[f] = the field
+// is w (FaceZ) under a z wall (the wall plane zeroed, an antisymmetric halo)
+template <typename T> struct HaloSet {
+    T *d[4];
+    int bc[4][3];
+    int wall_w[4];
+    int n;
+    void add(T *p, int loc, int bx, int by, int bz) {
+        d[n] = p;
+        bc[n][0] = bx; bc[n][1] = by; bc[n][2] = bz;
+        wall_w[n] = (loc == LOC_FZ && bz == BC_WALL) ? 1 : 0;
+        n++;
+    }
+};
+template <typename T>
+void launch_halo_multi(hipStream_t st, const HaloSet<T> &hs, int Nx, int Ny, int Nz, int H);
 template <typename T>
 int launch_advect(hipStream_t st, const Metrics<T> &m, int mode, int scheme, int p, const T *u, const T *v,
                   const T *w, const T *s, T *tu, T *tv, T *tw, T *ts);

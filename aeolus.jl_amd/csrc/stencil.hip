@@ -27,8 +27,8 @@ __device__ __forceinline__ int map_axis(int idx, int H, int N, int bc) {
 }
 
 template <typename T>
-__global__ void halo_gather_kernel(T *__restrict__ d, int Nx, int Ny, int Nz, int H, int bcx, int bcy, int bcz,
-                                   int wall_w) {
+__device__ __forceinline__ void halo_gather_body(T *__restrict__ d, int Nx, int Ny, int Nz, int H, int bcx, int bcy,
+                                                 int bcz, int wall_w) {
     const int sx = Nx + 2 * H, sy = Ny + 2 * H, sz = Nz + 2 * H;
     // enumeration: z-slabs (2H planes), then y-slabs, then x-slabs, then the
     // w wall plane k = H (interior) when wall_w
@@ -86,6 +86,20 @@ __global__ void halo_gather_kernel(T *__restrict__ d, int Nx, int Ny, int Nz, in
 }
 
 template <typename T>
+__global__ void halo_gather_kernel(T *__restrict__ d, int Nx, int Ny, int Nz, int H, int bcx, int bcy, int bcz,
+                                   int wall_w) {
+    halo_gather_body(d, Nx, Ny, Nz, H, bcx, bcy, bcz, wall_w);
+}
+
+// several fields of one grid in one launch (blockIdx.y = field); each gather
+// reads interior values only, so the fields and their points are independent
+template <typename T>
+__global__ void halo_multi_kernel(HaloSet<T> hs, int Nx, int Ny, int Nz, int H) {
+    const int f = blockIdx.y;
+    halo_gather_body(hs.d[f], Nx, Ny, Nz, H, hs.bc[f][0], hs.bc[f][1], hs.bc[f][2], hs.wall_w[f]);
+}
+
+template <typename T>
 void launch_halo(hipStream_t st, T *d, int Nx, int Ny, int Nz, int H, int loc, int bcx, int bcy, int bcz) {
     int wall_w = (loc == LOC_FZ && bcz == BC_WALL) ? 1 : 0;
     const int sx = Nx + 2 * H, sy = Ny + 2 * H;
@@ -96,6 +110,17 @@ void launch_halo(hipStream_t st, T *d, int Nx, int Ny, int Nz, int H, int loc, i
 }
 template void launch_halo<double>(hipStream_t, double *, int, int, int, int, int, int, int, int);
 template void launch_halo<float>(hipStream_t, float *, int, int, int, int, int, int, int, int);
+
+template <typename T>
+void launch_halo_multi(hipStream_t st, const HaloSet<T> &hs, int Nx, int Ny, int Nz, int H) {
+    const int sx = Nx + 2 * H, sy = Ny + 2 * H;
+    const long total = (long)2 * H * sx * sy + (long)Nz * 2 * H * sx + (long)Nz * Ny * 2 * H + (long)sx * sy;
+    int blocks = (int)((total + 255) / 256);
+    if (blocks > 65535) blocks = 65535;
+    hipLaunchKernelGGL(halo_multi_kernel<T>, dim3(blocks, hs.n), dim3(256), 0, st, hs, Nx, Ny, Nz, H);
+}
+template void launch_halo_multi<double>(hipStream_t, const HaloSet<double> &, int, int, int, int);
+template void launch_halo_multi<float>(hipStream_t, const HaloSet<float> &, int, int, int, int);
 
 // ---------------------------------------------------------------------------
 // Face fluxes vel * face_value of each transported field, per direction, at
