@@ -777,17 +777,12 @@ static void cols(SpectralPlan *p, hipStream_t st, int axis, int sign, int mode) 
     int nouter = axis == 1 ? p->Nz : p->Ny;
     int ncols = p->mode == 1 ? p->nkx : p->Nx / 2 + 1;
     EigArgs ea{p->kx2, p->ky2, p->kz2, p->nrm, ncols};
-    if (L <= 256) {
-        constexpr int TB = 16;
-        size_t lds = sizeof(double2) * (size_t)L * TB;
-        hipLaunchKernelGGL(c2c_cols_kernel<TB>, dim3(p->LD / TB, nouter), dim3(256), lds, st, p->hat, f, es, os,
-                           ncols, sign, mode, ea);
-    } else {
-        constexpr int TB = 8;
-        size_t lds = sizeof(double2) * (size_t)L * TB;
-        hipLaunchKernelGGL(c2c_cols_kernel<TB>, dim3(p->LD / TB, nouter), dim3(256), lds, st, p->hat, f, es, os,
-                           ncols, sign, mode, ea);
-    }
+    // 8 columns per block (128-byte row segments; 32 KB of LDS at L = 256, so
+    // five blocks per CU) over the valid columns only
+    constexpr int TB = 8;
+    size_t lds = sizeof(double2) * (size_t)L * TB;
+    hipLaunchKernelGGL(c2c_cols_kernel<TB>, dim3((ncols + TB - 1) / TB, nouter), dim3(256), lds, st, p->hat, f, es,
+                       os, ncols, sign, mode, ea);
 }
 
 int spectral_fft_solve(SpectralPlan *p, hipStream_t st, double *phi, const double *src) {
@@ -819,17 +814,10 @@ static void ytrans(SpectralPlan *p, hipStream_t st, int dir) {
     int L = p->Ny;
     const FFTLen f = fftlen(p, L);
     long es = (long)p->LD, os = (long)p->LD * p->Ny;
-    if (L <= 256) {
-        constexpr int TB = 16;
-        size_t lds = sizeof(double2) * (size_t)L * TB;
-        hipLaunchKernelGGL(dct_cols_kernel<TB>, dim3(p->LD / TB, p->Nz), dim3(256), lds, st, p->hat, f, es, os,
-                           p->nkx, dir);
-    } else {
-        constexpr int TB = 8;
-        size_t lds = sizeof(double2) * (size_t)L * TB;
-        hipLaunchKernelGGL(dct_cols_kernel<TB>, dim3(p->LD / TB, p->Nz), dim3(256), lds, st, p->hat, f, es, os,
-                           p->nkx, dir);
-    }
+    constexpr int TB = 8;
+    size_t lds = sizeof(double2) * (size_t)L * TB;
+    hipLaunchKernelGGL(dct_cols_kernel<TB>, dim3((p->nkx + TB - 1) / TB, p->Nz), dim3(256), lds, st, p->hat, f, es,
+                       os, p->nkx, dir);
 }
 
 int spectral_column_solve(SpectralPlan *p, hipStream_t st, double *phi, const double *src) {
