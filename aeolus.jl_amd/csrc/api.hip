@@ -1336,12 +1336,29 @@ static int substage(aeo_stepper *h, int stage, T *u, T *v, T *w, T *th, T *p, do
         hs.add(ws, LOC_FZ, sp.bc_x, sp.bc_y, sp.bc_z);
         launch_halo_multi<T>(st, hs, Nx, Ny, Nz, H);
     }
-    {
-        Timed t("divergence");
-        launch_div_into<T>(st, h->solver->src, us, vs, ws, g->m64);
+    int rc = AEO_OK;
+    bool fused = false;
+    if (h->solver->kind == 1) {
+        // the column solver forms ∇·u* inside its forward row transform (no
+        // source buffer round trip; the same values as the divergence kernel).
+        // It returns -1, having launched nothing, where x is not periodic.
+        Timed t("poisson");
+        fused = spectral_column_solve_div<T>(h->solver->sp, st, h->solver->phi, us, vs, ws, g->m64) == 0;
+        if (fused) {
+            HIPCHK(hipGetLastError());
+            h->solver->iters = 0;
+            h->solver->resid = NAN;
+            h->solver->status = AEO_DIRECT;
+        }
     }
-    int rc = poisson_run(h->solver, h->solver->phi);
-    if (rc) return rc;
+    if (!fused) {
+        {
+            Timed t("divergence");
+            launch_div_into<T>(st, h->solver->src, us, vs, ws, g->m64);
+        }
+        rc = poisson_run(h->solver, h->solver->phi);
+        if (rc) return rc;
+    }
     double ca, cb;
     int combine;
     stage_coeffs<double>(stage, &ph, &ca, &cb, &combine);

@@ -212,8 +212,36 @@ __device__ void lds_fft_dif(double2 *s, const FFTLen &f, int nb, int ld, int sig
 // ---- r2c along x: rows of N = 2L reals -> N/2+1 complex ---------------------
 // z[m] = x[2m] + i x[2m+1]; Z = DFT_L(z); X[k] = E[k] + W^k O[k] with
 // E = (Z[k] + conj Z[L-k]) / 2, O = (Z[k] - conj Z[L-k]) / 2i, W = e^{-2 pi i/N}.
-__global__ void r2c_rows_kernel(double2 *__restrict__ hat, const double *__restrict__ src, FFTLen f, int rpb,
-                                int nrows, int LD, const double2 *__restrict__ twN) {
+// the rows of the solver's source buffer
+struct SrcRows {
+    const double *src;
+    __device__ __forceinline__ double2 operator()(int row, int m, int N) const {
+        return reinterpret_cast<const double2 *>(src + (size_t)row * N)[m];
+    }
+};
+// the rows of ∇·u* computed where they are read: cells 2m and 2m+1 of row
+// (j, k), the expression of div_into_kernel (_div_at, src/FiniteVolume.jl:312-315)
+template <typename T> struct DivRows {
+    const T *u, *v, *w;
+    Metrics<double> md;
+    __device__ __forceinline__ double at(size_t c, int K) const {
+        const size_t sx = md.sx, sxy = (size_t)md.sx * md.sy;
+        const double du = (double)u[c + 1] - (double)u[c];
+        const double dv = (double)v[c + sx] - (double)v[c];
+        const double dw = (double)w[c + sxy] - (double)w[c];
+        return (du * md.idx + dv * md.idy) + dw * md.m_idz(K);
+    }
+    __device__ __forceinline__ double2 operator()(int row, int m, int) const {
+        const int j = row % md.Ny, k = row / md.Ny;
+        const int K = k + md.H;
+        const size_t c = (size_t)(2 * m + md.H) + (size_t)md.sx * ((size_t)(j + md.H) + (size_t)md.sy * K);
+        return make_double2(at(c, K), at(c + 1, K));
+    }
+};
+
+template <class Rows>
+__global__ void r2c_rows_kernel(double2 *__restrict__ hat, Rows rows, FFTLen f, int rpb, int nrows, int LD,
+                                const double2 *__restrict__ twN) {
     extern __shared__ double2 sm[];
     const int row0 = blockIdx.x * rpb;
     const int L = f.L, N = 2 * L;
@@ -221,7 +249,7 @@ __global__ void r2c_rows_kernel(double2 *__restrict__ hat, const double *__restr
         int r = t / L, m = t - r * L;
         int row = row0 + r;
         double2 z = make_double2(0, 0);
-        if (row < nrows) z = reinterpret_cast<const double2 *>(src + (size_t)row * N)[m];
+        if (row < nrows) z = rows(row, m, N);
         sm[f.perm[m] * rpb + r] = z;
     }
     __syncthreads();
@@ -750,15 +778,16 @@ static void row_shape(int L, int *tpr, int *rpb, int *bt) {
     *bt = *rpb * *tpr;
 }
 
-static void xfwd(SpectralPlan *p, hipStream_t st, const double *src) {
+template <class Rows> static void xfwd_rows(SpectralPlan *p, hipStream_t st, const Rows &rows) {
     int L = p->Nx / 2;
     int tpr, rpb, bt;
     row_shape(L, &tpr, &rpb, &bt);
     int nrows = p->Ny * p->Nz;
     size_t lds = sizeof(double2) * (size_t)rpb * L;
-    hipLaunchKernelGGL(r2c_rows_kernel, dim3((nrows + rpb - 1) / rpb), dim3(bt), lds, st, p->hat, src, fftlen(p, L),
-                       rpb, nrows, p->LD, table(p, 2 * L));
+    hipLaunchKernelGGL(r2c_rows_kernel<Rows>, dim3((nrows + rpb - 1) / rpb), dim3(bt), lds, st, p->hat, rows,
+                       fftlen(p, L), rpb, nrows, p->LD, table(p, 2 * L));
 }
+static void xfwd(SpectralPlan *p, hipStream_t st, const double *src) { xfwd_rows(p, st, SrcRows{src}); }
 static void xinv(SpectralPlan *p, hipStream_t st, double *dst, double scale) {
     int L = p->Nx / 2;
     int tpr, rpb, bt;
@@ -820,10 +849,32 @@ static void ytrans(SpectralPlan *p, hipStream_t st, int dir) {
                        os, p->nkx, dir);
 }
 
+static void column_tail(SpectralPlan *p, hipStream_t st, double *phi);
+
 int spectral_column_solve(SpectralPlan *p, hipStream_t st, double *phi, const double *src) {
     if (p->mode != 1) return -1;
     if (p->neu_x) xdct(p, st, src, nullptr, +1, 1.0);
     else xfwd(p, st, src);
+    column_tail(p, st, phi);
+    return 0;
+}
+
+// the column solve of ∇·u* without the source buffer: the divergence is formed
+// in the forward row transform (periodic x only; -1 otherwise)
+template <typename T>
+int spectral_column_solve_div(SpectralPlan *p, hipStream_t st, double *phi, const T *u, const T *v, const T *w,
+                              const Metrics<double> &md) {
+    if (p->mode != 1 || p->neu_x) return -1;
+    xfwd_rows(p, st, DivRows<T>{u, v, w, md});
+    column_tail(p, st, phi);
+    return 0;
+}
+template int spectral_column_solve_div<double>(SpectralPlan *, hipStream_t, double *, const double *, const double *,
+                                               const double *, const Metrics<double> &);
+template int spectral_column_solve_div<float>(SpectralPlan *, hipStream_t, double *, const float *, const float *,
+                                              const float *, const Metrics<double> &);
+
+static void column_tail(SpectralPlan *p, hipStream_t st, double *phi) {
     ytrans(p, st, +1);
     int nkx = p->nkx;
     hipLaunchKernelGGL(thomas_kernel, dim3((nkx + 63) / 64, p->Ny), dim3(64), 0, st, p->hat, p->cp, nkx, p->Ny, p->Nz,
@@ -832,7 +883,6 @@ int spectral_column_solve(SpectralPlan *p, hipStream_t st, double *phi, const do
     ytrans(p, st, -1);
     if (p->neu_x) xdct(p, st, nullptr, phi, -1, p->norm);
     else xinv(p, st, phi, p->norm);
-    return 0;
 }
 
 }  // namespace aeo

@@ -129,6 +129,9 @@ int spectral_column_setup(SpectralPlan *p, double dx, double dy, const double *l
                           const double *hweight, const double *measure, const double *inv_coupling,
                           const double *gauge, int neu_x, int neu_y);
 int spectral_column_solve(SpectralPlan *p, hipStream_t st, double *phi, const double *src);
+template <typename T>
+int spectral_column_solve_div(SpectralPlan *p, hipStream_t st, double *phi, const T *u, const T *v, const T *w,
+                              const Metrics<double> &md);
 
 // ---- multigrid and MG-preconditioned CG (mg.hip) ----
 struct MGPlan;
