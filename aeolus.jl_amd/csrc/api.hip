@@ -1373,17 +1373,21 @@ static int substage(aeo_stepper *h, int stage, T *u, T *v, T *w, T *th, T *p, do
     if (stage == 3) {
         // the halo shell of the combined state (_shell_map!, src/Simulations.jl:741, 693)
         Timed t("shell");
-        launch_shell<T>(st, u, us, u, Nx, Ny, Nz, H, ca, cb);
-        launch_shell<T>(st, v, vs, v, Nx, Ny, Nz, H, ca, cb);
-        launch_shell<T>(st, w, ws, w, Nx, Ny, Nz, H, ca, cb);
-        launch_shell<T>(st, th, ti, th, Nx, Ny, Nz, H, ca, cb);
+        ShellSet<T> ss{};
+        ss.add(u, us, u);
+        ss.add(v, vs, v);
+        ss.add(w, ws, w);
+        ss.add(th, ti, th);
+        launch_shell<T>(st, ss, Nx, Ny, Nz, H, ca, cb);
     }
     if (stage == 0) {
         // step!: u's halo holds u*'s fill; θ advanced in place over the interior
         Timed t("shell");
-        launch_shell<T>(st, u, us, u, Nx, Ny, Nz, H, 0.0, 1.0);
-        launch_shell<T>(st, v, vs, v, Nx, Ny, Nz, H, 0.0, 1.0);
-        launch_shell<T>(st, w, ws, w, Nx, Ny, Nz, H, 0.0, 1.0);
+        ShellSet<T> ss{};
+        ss.add(u, us, u);
+        ss.add(v, vs, v);
+        ss.add(w, ws, w);
+        launch_shell<T>(st, ss, Nx, Ny, Nz, H, 0.0, 1.0);
         dim3 grid((Nx + 255) / 256, Ny, Nz);
         hipLaunchKernelGGL(copy_interior_kernel<T>, grid, dim3(256), 0, st, th, ta, Nx, Ny, Nz, H);
     }
@@ -1394,8 +1398,10 @@ static int substage(aeo_stepper *h, int stage, T *u, T *v, T *w, T *th, T *p, do
 template <typename T>
 static int run_step(aeo_stepper *h, void *u, void *v, void *w, void *th, void *p, double dt, bool rk3) {
     if (!rk3) return substage<T>(h, 0, (T *)u, (T *)v, (T *)w, (T *)th, (T *)p, dt);
+    // p = φ/Δt of the third stage is the step's pressure; stages 1 and 2 would
+    // write values that nothing reads before the third overwrites them
     for (int s = 1; s <= 3; s++) {
-        int rc = substage<T>(h, s, (T *)u, (T *)v, (T *)w, (T *)th, (T *)p, dt);
+        int rc = substage<T>(h, s, (T *)u, (T *)v, (T *)w, (T *)th, s == 3 ? (T *)p : nullptr, dt);
         if (rc) return rc;
     }
     return AEO_OK;
@@ -1551,9 +1557,11 @@ static int mw_step(aeo_mw_stepper *h, T *u, T *v, T *w, T *th, T *p, double dt) 
         int by = sp.bc_y == AEO_PERIODIC ? BC_PERIODIC : BC_NOFLUX;
         int bz = sp.bc_z == AEO_PERIODIC ? BC_PERIODIC : BC_NOFLUX;
         launch_correct<T>(st, u, v, w, us, vs, ws, u, v, w, h->solver->phi, p, g->m64, bx, by, bz, 0.0, 1.0, 0, dt);
-        launch_shell<T>(st, u, us, u, Nx, Ny, Nz, H, 0.0, 1.0);
-        launch_shell<T>(st, v, vs, v, Nx, Ny, Nz, H, 0.0, 1.0);
-        launch_shell<T>(st, w, ws, w, Nx, Ny, Nz, H, 0.0, 1.0);
+        ShellSet<T> ss{};
+        ss.add(u, us, u);
+        ss.add(v, vs, v);
+        ss.add(w, ws, w);
+        launch_shell<T>(st, ss, Nx, Ny, Nz, H, 0.0, 1.0);
         dim3 grid((Nx + 255) / 256, Ny, Nz);
         hipLaunchKernelGGL(copy_interior_kernel<T>, grid, dim3(256), 0, st, th, tn, Nx, Ny, Nz, H);
     }

@@ -65,9 +65,19 @@ template <typename T>
 void launch_correct(hipStream_t st, T *uo, T *vo, T *wo, const T *us, const T *vs, const T *ws, const T *u0,
                     const T *v0, const T *w0, const double *phi, T *p, const Metrics<double> &md, int bcx, int bcy,
                     int bcz, double ca, double cb, int combine, double dt);
+// up to four (out, x, snap) triples whose halo shells are combined in one launch
+template <typename T> struct ShellSet {
+    T *out[4];
+    const T *x[4];
+    const T *snap[4];
+    int n;
+    void add(T *o, const T *xx, const T *sn) {
+        out[n] = o; x[n] = xx; snap[n] = sn;
+        n++;
+    }
+};
 template <typename T>
-void launch_shell(hipStream_t st, T *out, const T *x, const T *snap, int Nx, int Ny, int Nz, int H, double ca,
-                  double cb);
+void launch_shell(hipStream_t st, const ShellSet<T> &ss, int Nx, int Ny, int Nz, int H, double ca, double cb);
 template <typename T>
 void launch_plane_sum(hipStream_t st, double *out, const T *f, const Metrics<double> &md);
 template <typename T>

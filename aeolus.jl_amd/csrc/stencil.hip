@@ -600,37 +600,58 @@ template void launch_correct<float>(hipStream_t, float *, float *, float *, cons
 // The RK3 combination of the halo shell, out = a·snap + b·x at every storage
 // point outside the interior (_shell_map!, src/Simulations.jl:1249-1286).
 // ---------------------------------------------------------------------------
+// The halo shell of up to four fields (blockIdx.y = field) combined as the
+// interior is, out = ca·snap + cb·x (ca = 0, cb = 1: a copy), visiting the
+// shell's points only: the two z slabs, then the y and the x slabs of the
+// interior levels (the enumeration of halo_gather_body).
 template <typename T>
-__global__ void shell_kernel(T *__restrict__ out, const T *__restrict__ x, const T *__restrict__ snap, int Nx, int Ny,
-                             int Nz, int H, double ca, double cb) {
-    const int sx = Nx + 2 * H, sy = Ny + 2 * H, sz = Nz + 2 * H;
-    const long total = (long)sx * sy * sz;
+__global__ void shell_kernel(ShellSet<T> ss, int Nx, int Ny, int Nz, int H, double ca, double cb) {
+    const int f = blockIdx.y;
+    T *__restrict__ out = ss.out[f];
+    const T *__restrict__ x = ss.x[f];
+    const T *__restrict__ snap = ss.snap[f];
+    const int sx = Nx + 2 * H, sy = Ny + 2 * H;
+    const long nz_slab = (long)2 * H * sx * sy;
+    const long ny_slab = (long)Nz * 2 * H * sx;
+    const long nx_slab = (long)Nz * Ny * 2 * H;
+    const long total = nz_slab + ny_slab + nx_slab;
+    const bool copy = ca == 0.0 && cb == 1.0;
     for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < total; t += (long)gridDim.x * blockDim.x) {
-        int i = (int)(t % sx);
-        long q = t / sx;
-        int j = (int)(q % sy), k = (int)(q / sy);
-        bool in = i >= H && i < H + Nx && j >= H && j < H + Ny && k >= H && k < H + Nz;
-        if (in) {
-            // skip the interior rows quickly
-            continue;
+        int i, j, k;
+        if (t < nz_slab) {
+            long q = t;
+            i = (int)(q % sx); q /= sx;
+            j = (int)(q % sy); q /= sy;
+            const int l = (int)q;
+            k = l < H ? l : H + Nz + (l - H);
+        } else if (t < nz_slab + ny_slab) {
+            long q = t - nz_slab;
+            i = (int)(q % sx); q /= sx;
+            const int l = (int)(q % (2 * H)); q /= (2 * H);
+            j = l < H ? l : H + Ny + (l - H);
+            k = H + (int)q;
+        } else {
+            long q = t - nz_slab - ny_slab;
+            const int l = (int)(q % (2 * H)); q /= (2 * H);
+            i = l < H ? l : H + Nx + (l - H);
+            j = H + (int)(q % Ny);
+            k = H + (int)(q / Ny);
         }
-        // ca = 0, cb = 1 is a copy (step!'s halo of u*), written as one
-        out[t] = (ca == 0.0 && cb == 1.0) ? x[t] : (T)(ca * (double)snap[t] + cb * (double)x[t]);
+        const size_t c = lin(i, j, k, sx, sy);
+        out[c] = copy ? x[c] : (T)(ca * (double)snap[c] + cb * (double)x[c]);
     }
 }
 
 template <typename T>
-void launch_shell(hipStream_t st, T *out, const T *x, const T *snap, int Nx, int Ny, int Nz, int H, double ca,
-                  double cb) {
-    long total = (long)(Nx + 2 * H) * (Ny + 2 * H) * (Nz + 2 * H);
+void launch_shell(hipStream_t st, const ShellSet<T> &ss, int Nx, int Ny, int Nz, int H, double ca, double cb) {
+    const int sx = Nx + 2 * H, sy = Ny + 2 * H;
+    const long total = (long)2 * H * sx * sy + (long)Nz * 2 * H * sx + (long)Nz * Ny * 2 * H;
     int blocks = (int)((total + 255) / 256);
-    if (blocks > 8192) blocks = 8192;
-    hipLaunchKernelGGL(shell_kernel<T>, dim3(blocks), dim3(256), 0, st, out, x, snap, Nx, Ny, Nz, H, ca, cb);
+    if (blocks > 65535) blocks = 65535;
+    hipLaunchKernelGGL(shell_kernel<T>, dim3(blocks, ss.n), dim3(256), 0, st, ss, Nx, Ny, Nz, H, ca, cb);
 }
-template void launch_shell<double>(hipStream_t, double *, const double *, const double *, int, int, int, int, double,
-                                   double);
-template void launch_shell<float>(hipStream_t, float *, const float *, const float *, int, int, int, int, double,
-                                  double);
+template void launch_shell<double>(hipStream_t, const ShellSet<double> &, int, int, int, int, double, double);
+template void launch_shell<float>(hipStream_t, const ShellSet<float> &, int, int, int, int, double, double);
 
 // ---------------------------------------------------------------------------
 // Maxima (exact in any order): max |∇·u| (max_divergence, src/Simulations.jl:
