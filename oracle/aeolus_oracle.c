@@ -13,6 +13,7 @@
 #include "aeolus_oracle.h"
 
 #include <float.h>
+#include <omp.h>
 #include <math.h>
 #include <stdint.h>
 #include <stdlib.h>
@@ -105,6 +106,11 @@ static inline double b5_m4(double v1, double v2, double v3, double v4, double v5
                  ((((3 * v1 - 14 * v2) + 24 * v3) - 18 * v4) + 5 * v5) / 12,
                  ((((v1 - 4 * v2) + 6 * v3) - 4 * v4) + v5) / 24);
 }
+
+/* the OpenMP team size of the following calls (the CPU baseline's 1-thread
+ * and all-core timings) and its current value */
+void orc_set_threads(int n) { omp_set_num_threads(n > 0 ? n : 1); }
+int orc_get_threads(void) { return omp_get_max_threads(); }
 
 static double teno_ct = 1e-5;
 void orc_set_teno_cutoff(double C_T) { teno_ct = C_T; }

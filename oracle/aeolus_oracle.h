@@ -58,6 +58,9 @@ void orc_fill_halos(double *a, const orc_grid *g, int loc, int bcx, int bcy, int
 
 void orc_advect_scalar(double *tend, const double *u, const double *v, const double *w,
                        const double *phi, const orc_grid *g, int scheme, int p);
+/* OpenMP threads of the following calls, and the current team size */
+void orc_set_threads(int n);
+int orc_get_threads(void);
 /* TENO's cut-off C_T for the following calls (default 1e-5, TENO(N; C_T)) */
 void orc_set_teno_cutoff(double C_T);
 double orc_teno5_left(double v1, double v2, double v3, double v4, double v5, double C_T);

@@ -69,6 +69,8 @@ def lib():
             "orc_advect_scalar": (None, [pd, pd, pd, pd, pd, pg, i, i]),
             "orc_advect_positive": (None, [pd, pd, pd, pd, pd, pg, i, i, d, i, i, i, i, i, pd]),
             "orc_set_teno_cutoff": (None, [d]),
+            "orc_set_threads": (None, [i]),
+            "orc_get_threads": (i, []),
             "orc_teno5_left": (d, [d, d, d, d, d, d]),
             "orc_teno8_left": (d, [d, d, d, d, d, d, d, d, d]),
             "orc_advect_variance_conserving": (None, [pd, pd, pd, pd, pd, pg]),
