@@ -438,6 +438,7 @@ struct MGPlan {
     // CG arrays and reduction scratch
     double *x = nullptr, *r = nullptr, *z = nullptr, *p = nullptr, *Ap = nullptr;
     double *part = nullptr, *scal = nullptr;  // scal: device scalars
+    double *part2 = nullptr;                  // the block sums of a long partial list (≤ 2048)
     int npart = 0;
 };
 
@@ -470,12 +471,25 @@ static dim3 grid3(int nx, int ny, int nz) { return dim3((nx + RB - 1) / RB, ny, 
 static int nblocks3(int nx, int ny, int nz) { return ((nx + RB - 1) / RB) * ny * nz; }
 static int flat_blocks(int n) { int b = (n + RB - 1) / RB; return b < 2048 ? b : 2048; }
 
+// out = scale · Σ part[0..n): one block for a short list; a long one (the
+// per-row partials of a fine level) is first folded into ≤ 2048 block sums,
+// so no single block walks hundreds of thousands of values. Fixed order.
+static void finalize_to(MGPlan *P, hipStream_t st, double *out, const double *part, int n, double scale) {
+    if (n <= 4096) {
+        hipLaunchKernelGGL(finalize_kernel, dim3(1), dim3(RB), 0, st, out, part, n, scale);
+        return;
+    }
+    const int nb = flat_blocks(n);
+    hipLaunchKernelGGL(sum_kernel<double>, dim3(nb), dim3(RB), 0, st, part, n, P->part2);
+    hipLaunchKernelGGL(finalize_kernel, dim3(1), dim3(RB), 0, st, out, P->part2, nb, scale);
+}
+
 // the mean of the interior of `a` (level l) into scal[slot], via partials
 template <typename T>
 static void mean_to(MGPlan *P, hipStream_t st, const T *a, int n, int slot) {
     int nb = flat_blocks(n);
     hipLaunchKernelGGL(sum_kernel<T>, dim3(nb), dim3(RB), 0, st, a, n, P->part);
-    hipLaunchKernelGGL(finalize_kernel, dim3(1), dim3(RB), 0, st, P->scal + slot, P->part, nb, 1.0 / n);
+    finalize_to(P, st, P->scal + slot, P->part, nb, 1.0 / n);
 }
 
 template <typename T>
@@ -529,11 +543,11 @@ static void vcycle(MGPlan *P, hipStream_t st, int l) {
     LevelD<T> L = devlevel<T>(P, l), C = devlevel<T>(P, l + 1);
     int nb = nblocks3(h.nx, h.ny, h.nz);
     hipLaunchKernelGGL(residual_kernel<T>, grid3(h.nx, h.ny, h.nz), dim3(RB), 0, st, L, P->part);
-    hipLaunchKernelGGL(finalize_kernel, dim3(1), dim3(RB), 0, st, P->scal + 5, P->part, nb, 1.0 / n);
+    finalize_to(P, st, P->scal + 5, P->part, nb, 1.0 / n);
     int nbc = nblocks3(c.nx, c.ny, c.nz);
     hipLaunchKernelGGL(restrict_kernel<T>, grid3(c.nx, c.ny, c.nz), dim3(RB), 0, st, C, L, (const double *)(P->scal + 5),
                        P->part);
-    hipLaunchKernelGGL(finalize_kernel, dim3(1), dim3(RB), 0, st, P->scal + 6, P->part, nbc, 1.0 / nc);
+    finalize_to(P, st, P->scal + 6, P->part, nbc, 1.0 / nc);
     hipLaunchKernelGGL(shift_kernel<T>, dim3(flat_blocks(nc)), dim3(RB), 0, st, (T *)c.b, nc, P->scal + 6);
     hipLaunchKernelGGL(zero_kernel<T>, dim3(flat_blocks(nc)), dim3(RB), 0, st, (T *)c.phi, nc);
     for (int rec = 0; rec < (P->wcycle ? 2 : 1); rec++) vcycle<T>(P, st, l + 1);
@@ -541,7 +555,7 @@ static void vcycle(MGPlan *P, hipStream_t st, int l) {
     L = devlevel<T>(P, l);
     C = devlevel<T>(P, l + 1);
     hipLaunchKernelGGL(prolong_kernel<T>, grid3(h.nx, h.ny, h.nz), dim3(RB), 0, st, L, C, P->part);
-    hipLaunchKernelGGL(finalize_kernel, dim3(1), dim3(RB), 0, st, P->scal + 4, P->part, nb, 1.0 / n);
+    finalize_to(P, st, P->scal + 4, P->part, nb, 1.0 / n);
     hipLaunchKernelGGL(shift_kernel<T>, dim3(flat_blocks(n)), dim3(RB), 0, st, (T *)h.phi, n, P->scal + 4);
     smooth<T>(P, st, l, P->nu2);
     project_mean<T>(P, st, l);
@@ -654,6 +668,7 @@ MGPlan *mg_create(int nx, int ny, int nz, double dx, double dy, double dz, int b
     P->npart = nblocks3(nx, ny, nz);
     if (P->npart < 3 * 2048) P->npart = 3 * 2048;
     if (hipMalloc(&P->part, sizeof(double) * P->npart) != hipSuccess) return fail("mg: allocation failed");
+    if (hipMalloc(&P->part2, sizeof(double) * 2048) != hipSuccess) return fail("mg: allocation failed");
     if (hipMalloc(&P->scal, sizeof(double) * 16) != hipSuccess) return fail("mg: allocation failed");
     hipMemset(P->scal, 0, sizeof(double) * 16);
     if (rhs_scale) {
@@ -672,7 +687,7 @@ void mg_destroy(MGPlan *P) {
     }
     hipFree(P->cx64); hipFree(P->cy64); hipFree(P->cz64); hipFree(P->rhs_scale);
     hipFree(P->x); hipFree(P->r); hipFree(P->z); hipFree(P->p); hipFree(P->Ap);
-    hipFree(P->part); hipFree(P->scal);
+    hipFree(P->part); hipFree(P->part2); hipFree(P->scal);
     delete P;
 }
 
@@ -752,7 +767,7 @@ int mg_pcg_solve(MGPlan *P, hipStream_t st, double *xout, const double *bin, dou
     while (status != 0 && iters < maxit) {
         hipLaunchKernelGGL(cg_apply_kernel, grid3(h.nx, h.ny, h.nz), dim3(RB), 0, st, F, (const double *)P->p, P->Ap,
                            P->part);
-        hipLaunchKernelGGL(finalize_kernel, dim3(1), dim3(RB), 0, st, P->scal + 11, P->part, nb3, 1.0);
+        finalize_to(P, st, P->scal + 11, P->part, nb3, 1.0);
         double pAp = read_scal(st, P->scal + 11);
         if (!(fabs(pAp) > DBL_EPSILON * fabs(rz))) { status = 2; break; }
         iters++;
@@ -801,7 +816,7 @@ static void fmg_start(MGPlan *P, hipStream_t st) {
         const int nbc = nblocks3(c.nx, c.ny, c.nz);
         hipLaunchKernelGGL(restrict_kernel<double>, grid3(c.nx, c.ny, c.nz), dim3(RB), 0, st, C, L,
                            (const double *)nullptr, P->part);
-        hipLaunchKernelGGL(finalize_kernel, dim3(1), dim3(RB), 0, st, P->scal + 6, P->part, nbc, 1.0 / nc);
+        finalize_to(P, st, P->scal + 6, P->part, nbc, 1.0 / nc);
         hipLaunchKernelGGL(shift_kernel<double>, dim3(flat_blocks(nc)), dim3(RB), 0, st, (double *)c.b, nc,
                            P->scal + 6);
     }
@@ -836,7 +851,7 @@ int mg_cycles_solve(MGPlan *P, hipStream_t st, double *xout, const double *bin, 
                        P->scal + 7);
     hipLaunchKernelGGL(zero_kernel<double>, dim3(fb), dim3(RB), 0, st, (double *)h.phi, n);
     hipLaunchKernelGGL(sumsq_kernel<double>, dim3(fb), dim3(RB), 0, st, (const double *)h.b, n, P->part);
-    hipLaunchKernelGGL(finalize_kernel, dim3(1), dim3(RB), 0, st, P->scal, P->part, fb, 1.0);
+    finalize_to(P, st, P->scal, P->part, fb, 1.0);
     const double bn = sqrt(read_scal(st, P->scal));
     if (bn == 0) {
         hipLaunchKernelGGL(fill_kernel, dim3(fb), dim3(RB), 0, st, xout, n, 0.0);
@@ -853,10 +868,10 @@ int mg_cycles_solve(MGPlan *P, hipStream_t st, double *xout, const double *bin, 
         // _residual_norm!: r, its mean removed, ‖r‖
         LevelD<double> L = devlevel<double>(P, 0);
         hipLaunchKernelGGL(residual_kernel<double>, grid3(h.nx, h.ny, h.nz), dim3(RB), 0, st, L, P->part);
-        hipLaunchKernelGGL(finalize_kernel, dim3(1), dim3(RB), 0, st, P->scal + 5, P->part, nb3, 1.0 / n);
+        finalize_to(P, st, P->scal + 5, P->part, nb3, 1.0 / n);
         hipLaunchKernelGGL(shift_kernel<double>, dim3(fb), dim3(RB), 0, st, (double *)h.r, n, P->scal + 5);
         hipLaunchKernelGGL(sumsq_kernel<double>, dim3(fb), dim3(RB), 0, st, (const double *)h.r, n, P->part);
-        hipLaunchKernelGGL(finalize_kernel, dim3(1), dim3(RB), 0, st, P->scal, P->part, fb, 1.0);
+        finalize_to(P, st, P->scal, P->part, fb, 1.0);
         rel = sqrt(read_scal(st, P->scal)) / bn;
         ncyc = cyc;
         if (rel < tol) break;

@@ -221,6 +221,72 @@ def tracer_leg(reps=20, warmup=3):
                         "tracer uniform(0,1)^4, reusable workspace, accumulate=false"}
 
 
+TARGET_N = (512, 512, 256)
+
+
+def target_leg(steps=2, log=print):
+    """A full SSP-RK3 step at 512x512x256 projected by MG-PCG to ||r|| <= 1e-8
+    ||b||, in the plain FP64 form (FP64 WENO-Z5 tendencies, FP64 red-black
+    V-cycle) and in the FP32 form (FP32 tendencies, FP32 Jacobi V-cycle inside
+    the FP64 CG). Convective boundary layer on a regular 20 m grid (periodic
+    x-y, walls in z): theta = 300 K to 1 km then +3 K/km, seeded +-0.1 K noise in
+    the lowest 10 levels, seeded +-0.5 m/s horizontal velocity, surface flux
+    0.1 K m/s. Each form: one forward-Euler step that projects the random start
+    (RK3 keeps a third of its start's divergence, so it must start solenoidal),
+    one warm-up RK3 step, then `steps` timed steps; the FP32 forms' fields are
+    compared with the FP64 form's after the same steps."""
+    import aeolus_hip as hip
+    from aeolus_hip import workloads as W
+    hip.init(0)
+    N, dx = TARGET_N, 20.0
+    g = hip.Grid(N, (N[0] * dx, N[1] * dx, N[2] * dx), halo=3)
+    H, I = g.H, g.interior
+    zc = (np.arange(N[2]) + 0.5) * dx
+    rng = np.random.default_rng(W.DCBL_SEED)
+    th = np.zeros(g.storage, order="F")
+    th[:, :, H:H + N[2]] = np.where(zc <= 1000.0, 300.0, 300.0 + 0.003 * (zc - 1000.0))[None, None, :]
+    th[H:H + N[0], H:H + N[1], H:H + 10] += rng.uniform(-0.1, 0.1, (N[0], N[1], 10))
+    u, v, w = (np.zeros(g.storage, order="F") for _ in range(3))
+    u[I] = 0.5 * rng.uniform(-1, 1, N)
+    v[I] = 0.5 * rng.uniform(-1, 1, N)
+    out, fields = {}, {}
+    forms = (("fp64_mgpcg", 0, dict(smoother="rbgs", vcycle="f64")),
+             ("fp32_mgpcg", 1, dict(smoother="jacobi", omega=0.8, vcycle="f32")),
+             ("fp32_mgpcg_rbgs", 1, dict(smoother="rbgs", vcycle="f32")))
+    for name, prec, kw in forms:
+        solver = hip.MGPreconditionedCG(N, (dx,) * 3, bcs=("periodic", "periodic", "neumann"), tol=1e-8, maxit=200,
+                                        nu1=2, nu2=2, **kw)
+        sim = hip.BoussinesqSimulation(g, scheme="weno_z5", precision=prec, solver=solver, **W.DCBL_PHYS)
+        sim.set(u, v, w, th)
+        sim.step(DT)
+        sim.step_rk3(DT)
+        hip.sync()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            sim.step_rk3(DT)
+        hip.sync()
+        ms = (time.perf_counter() - t0) / steps * 1e3
+        its, res = hip.solver_stats(solver)
+        out[name] = {"ms_per_step": ms, "cell_updates_per_s": N[0] * N[1] * N[2] / (ms / 1e3),
+                     "last_solve_iterations": its, "last_solve_residual": res, "status": solver.last_status,
+                     "max_divergence_after_step": sim.max_divergence()}
+        fields[name] = {n: a[I].astype(np.float64) for n, a in sim.state().items() if n != "p"}
+        log(f"target {name}: {out[name]}")
+        del sim, solver
+    a = fields["fp64_mgpcg"]
+    for name in ("fp32_mgpcg", "fp32_mgpcg_rbgs"):
+        b = fields[name]
+        err = {n: float(np.abs(b[n] - a[n]).max() / np.abs(a[n]).max()) for n in ("u", "v", "w")}
+        err["theta"] = float(np.abs(b["theta"] - a["theta"]).max() / np.abs(a["theta"]).max())
+        err["theta_minus_300"] = float(np.abs(b["theta"] - a["theta"]).max() / np.abs(a["theta"] - 300.0).max())
+        out[name]["rel_error_vs_fp64"] = err
+        out[name]["speedup_vs_fp64"] = out["fp64_mgpcg"]["ms_per_step"] / out[name]["ms_per_step"]
+    out["workload"] = ("convective boundary layer 512x512x256, regular 20 m grid, WENO-Z5, SSP-RK3, MG-PCG "
+                       "projection to 1e-8 (FP64 form: FP64 tendencies + FP64 RBGS V-cycle; FP32 forms: FP32 "
+                       "tendencies + FP32 Jacobi or RBGS V-cycle in FP64 CG)")
+    return out
+
+
 def read_valu_insts():
     """VALU wave-instructions per tendency launch from the committed
     rocprofv3 --pmc SQ summary (SQ_INSTS_VALU per dispatch)."""
@@ -301,12 +367,22 @@ def main():
     ap.add_argument("--no-poisson", action="store_true", help="skip the cut-cell Poisson leg")
     ap.add_argument("--weights", default=os.path.join(ROOT, "aeolus.jl_amd", "aeolus_hip", "weights",
                                                       "unet_cutcell_dx2.npy"))
+    ap.add_argument("--no-target", action="store_true", help="skip the 512x512x256 MG-PCG step leg")
+    ap.add_argument("--poisson-pass", action="store_true", help="only the cut-cell Poisson legs, for kernel traces")
+    ap.add_argument("--target-pass", action="store_true", help="only the 512x512x256 MG-PCG step leg")
     ap.add_argument("--tracer-pass", action="store_true",
                     help="only the advect_positive leg (--steps calls), for rocprofv3 kernel traces")
     ap.add_argument("--profile-pass", action="store_true",
                     help="a short FP64 run for rocprofv3 counter passes: the steps, then one 1 GiB copy kernel "
                          "(the calibration of FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
+    if args.poisson_pass:
+        legs = poisson_legs(weights=args.weights, log=lambda m: print(m, file=sys.stderr))
+        print(json.dumps({"poisson_pass": True, "legs": legs}), flush=True)
+        return
+    if args.target_pass:
+        print(json.dumps({"target_pass": True, **target_leg(log=lambda m: print(m, file=sys.stderr))}), flush=True)
+        return
     if args.tracer_pass:
         print(json.dumps({"tracer_pass": True, **tracer_leg(args.steps, args.warmup)}), flush=True)
         return
@@ -324,6 +400,7 @@ def main():
     t32, prof32, div32 = run_gpu(args.steps, args.warmup, 1)
     pms = poisson_ms()
     tracer = tracer_leg()
+    target = None if args.no_target else target_leg(log=lambda m: print(m, file=sys.stderr))
     legs = None if args.no_poisson else poisson_legs(weights=args.weights, log=lambda m: print(m, file=sys.stderr))
     cells = NX * NY * NZ
 
@@ -376,6 +453,7 @@ def main():
             "legs": legs,
         },
         "tracer_positive": tracer,
+        "target_step_512x512x256": target,
         "max_divergence_after_step": div64,
         "ms_per_step_by_kernel_f64": breakdown,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
