@@ -261,12 +261,14 @@ __global__ void pool_kernel(const bf16 *__restrict__ in, int ics, int ioff, bf16
     out[e] = (bf16)(s * 0.125f);
 }
 // out[v][ooff + c] = in[parent(v)][c] (fine grid g)
+// 8 channels (16 bytes) per thread: C, ocs and ooff are multiples of 8
 __global__ void up_kernel(const bf16 *__restrict__ in, bf16 *__restrict__ out, int ocs, int ooff, int C, Vol g) {
-    const int nf = g.nx * g.ny * g.nz;
+    const int nf = g.nx * g.ny * g.nz, c8 = C / 8;
     const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= (size_t)nf * C) return;
-    const int v = (int)(e / C), c = (int)(e - (size_t)v * C);
-    out[(size_t)v * ocs + ooff + c] = in[(size_t)parent_of(v, g) * C + c];
+    if (e >= (size_t)nf * c8) return;
+    const int v = (int)(e / c8), q = (int)(e - (size_t)v * c8);
+    *reinterpret_cast<bf16x8 *>(out + (size_t)v * ocs + ooff + 8 * q) =
+        *reinterpret_cast<const bf16x8 *>(in + (size_t)parent_of(v, g) * C + 8 * q);
 }
 // out[v][c] = y[v][yoff + c] > 0 ? gin[v][goff + c] : 0
 __global__ void relu_mask_kernel(const bf16 *gin, int gcs, int goff, const bf16 *__restrict__ y, int ycs,
@@ -687,10 +689,11 @@ int unet_forward(UNet *N, hipStream_t st, const double *src, const double *mask,
     hipLaunchKernelGGL(pool_kernel, dim3(nblk((size_t)N->n2 * 32)), dim3(UB), 0, st, N->cat1, 96, 64, N->p1, 32, N->g1);
     rc |= conv_run(st, 32, 64, N->p1, L[4].wf, W + L[4].boff, N->a4, 64, 0, N->g2, 1);
     rc |= conv_run(st, 64, 64, N->a4, L[5].wf, W + L[5].boff, N->a5, 64, 0, N->g2, 1);
-    hipLaunchKernelGGL(up_kernel, dim3(nblk((size_t)N->n1 * 64)), dim3(UB), 0, st, N->a5, N->cat1, 96, 0, 64, N->g1);
+    static_assert(64 % 8 == 0 && 96 % 8 == 0 && 32 % 8 == 0 && 48 % 8 == 0, "up_kernel moves 8 channels");
+    hipLaunchKernelGGL(up_kernel, dim3(nblk((size_t)N->n1 * 64 / 8)), dim3(UB), 0, st, N->a5, N->cat1, 96, 0, 64, N->g1);
     rc |= conv_run(st, 96, 32, N->cat1, L[6].wf, W + L[6].boff, N->a6, 32, 0, N->g1, 1);
     rc |= conv_run(st, 32, 32, N->a6, L[7].wf, W + L[7].boff, N->a7, 32, 0, N->g1, 1);
-    hipLaunchKernelGGL(up_kernel, dim3(nblk((size_t)n0 * 32)), dim3(UB), 0, st, N->a7, N->cat0, 48, 0, 32, N->g0);
+    hipLaunchKernelGGL(up_kernel, dim3(nblk((size_t)n0 * 32 / 8)), dim3(UB), 0, st, N->a7, N->cat0, 48, 0, 32, N->g0);
     rc |= conv_run(st, 48, 16, N->cat0, L[8].wf, W + L[8].boff, N->a8, 16, 0, N->g0, 1);
     rc |= conv_run(st, 16, 16, N->a8, L[9].wf, W + L[9].boff, N->a9, 16, 0, N->g0, 1);
     hipLaunchKernelGGL(head_kernel, dim3(nblk(n0)), dim3(UB), 0, st, N->a9, W + N->headoff, N->scal, out, N->y, n0);
@@ -832,11 +835,21 @@ __global__ void fcg_sums_kernel(const double *__restrict__ a, const double *__re
     }
     if (threadIdx.x < 4) part[blockIdx.x * 4 + threadIdx.x] = sh[threadIdx.x][0];
 }
-__global__ void fcg_finalize_kernel(const double *__restrict__ part, int nb, double *__restrict__ out) {
-    if (threadIdx.x >= 4) return;
+// the four sums over nb block partials: 64 threads per component, each over
+// every 64th block, then a tree in LDS (a fixed order)
+__global__ void __launch_bounds__(256) fcg_finalize_kernel(const double *__restrict__ part, int nb,
+                                                           double *__restrict__ out) {
+    __shared__ double sh[4][64];
+    const int q = threadIdx.x & 3, t = threadIdx.x >> 2;
     double s = 0;
-    for (int b = 0; b < nb; b++) s += part[b * 4 + threadIdx.x];
-    out[threadIdx.x] = s;
+    for (int b = t; b < nb; b += 64) s += part[b * 4 + q];
+    sh[q][t] = s;
+    __syncthreads();
+    for (int w = 32; w > 0; w >>= 1) {
+        if (t < w) sh[q][t] += sh[q][t + w];
+        __syncthreads();
+    }
+    if (threadIdx.x < 4) out[threadIdx.x] = sh[threadIdx.x][0];
 }
 // y = a·x + b·y − m   (x may be null: y = b·y − m)
 __global__ void fcg_axpby_kernel(double *__restrict__ y, const double *__restrict__ x, double a, double b, double m,
@@ -888,7 +901,7 @@ static void fcg_sums(FCGState *F, hipStream_t st, const double *a, const double 
                      double out[4]) {
     const int nb = 1024;
     hipLaunchKernelGGL(fcg_sums_kernel, dim3(nb), dim3(UB), 0, st, a, b, c, d, F->n, F->part);
-    hipLaunchKernelGGL(fcg_finalize_kernel, dim3(1), dim3(64), 0, st, F->part, nb, F->sc);
+    hipLaunchKernelGGL(fcg_finalize_kernel, dim3(1), dim3(256), 0, st, F->part, nb, F->sc);
     hipMemcpyAsync(out, F->sc, sizeof(double) * 4, hipMemcpyDeviceToHost, st);
     hipStreamSynchronize(st);
 }
