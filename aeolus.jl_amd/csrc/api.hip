@@ -42,8 +42,27 @@ struct Context {
     };
     std::map<std::string, Rec> prof;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> pool;
+    double *red = nullptr;  // device scratch of the reductions read back by the host
+    size_t red_n = 0;
 };
 Context *g_ctx = nullptr;
+
+// At least n doubles of the context's reduction scratch (grown, never shrunk).
+// Every user launches into it and reads the result back (synchronising the
+// stream) before it returns, so consecutive calls on the stream may share it.
+static double *red_scratch(size_t n) {
+    if (g_ctx->red_n < n) {
+        if (g_ctx->red) {
+            hipStreamSynchronize(g_ctx->stream);
+            hipFree(g_ctx->red);
+        }
+        g_ctx->red = nullptr;
+        g_ctx->red_n = 0;
+        if (hipMalloc(&g_ctx->red, sizeof(double) * n) != hipSuccess) return nullptr;
+        g_ctx->red_n = n;
+    }
+    return g_ctx->red;
+}
 
 // timing of one launch site: events recorded on the library stream
 struct Timed {
@@ -264,6 +283,7 @@ extern "C" int aeo_shutdown(void) {
         hipEventDestroy(p.second);
     }
     hipStreamDestroy(g_ctx->stream);
+    if (g_ctx->red) hipFree(g_ctx->red);
     delete g_ctx;
     g_ctx = nullptr;
     return AEO_OK;
@@ -1695,8 +1715,8 @@ static int run_diag(aeo_grid *g, int precision, const void *u, const void *v, co
     if (!g || !u || !v || !w || !out) return fail(AEO_ERR_ARG, "null argument");
     if (precision != AEO_F64 && precision != AEO_F32) return fail(AEO_ERR_ARG, "bad precision %d", precision);
     constexpr int NB = 512;
-    double *d = nullptr;
-    HIPCHK(hipMalloc(&d, sizeof(double) * NB));
+    double *d = red_scratch(NB);
+    if (!d) return fail(AEO_ERR_HIP, "reduction scratch: out of device memory");
     if (precision == AEO_F64)
         launch_diag<double>(g_ctx->stream, d, NB, (const double *)u, (const double *)v, (const double *)w, g->m64, what);
     else
@@ -1704,7 +1724,6 @@ static int run_diag(aeo_grid *g, int precision, const void *u, const void *v, co
     double h[NB];
     HIPCHK(hipMemcpyAsync(h, d, sizeof h, hipMemcpyDeviceToHost, g_ctx->stream));
     HIPCHK(hipStreamSynchronize(g_ctx->stream));
-    HIPCHK(hipFree(d));
     double r = 0.0;
     for (int b = 0; b < NB; b++) {
         if (what == 2) r += h[b];
@@ -1755,14 +1774,13 @@ extern "C" int aeo_stepper_cfl(aeo_stepper *h, void *u, void *v, void *w, const 
     }
     *N = 0;
     if (sp.theta0 != 0) {
-        double *d = nullptr;
-        HIPCHK(hipMalloc(&d, sizeof(double) * Nz));
+        double *d = red_scratch(Nz);
+        if (!d) return fail(AEO_ERR_HIP, "reduction scratch: out of device memory");
         if (f64) launch_plane_sum<double>(st, d, (const double *)theta, g->m64);
         else launch_plane_sum<float>(st, d, (const float *)theta, g->m64);
         std::vector<double> sums(Nz);
         HIPCHK(hipMemcpyAsync(sums.data(), d, sizeof(double) * Nz, hipMemcpyDeviceToHost, st));
         HIPCHK(hipStreamSynchronize(st));
-        hipFree(d);
         const double np = (double)Nx * Ny;
         double n2max = 0, prev = 0;
         for (int k = 0; k < Nz; k++) {
@@ -1785,8 +1803,8 @@ extern "C" int aeo_stepper_cfl(aeo_stepper *h, void *u, void *v, void *w, const 
 static int run_guard(aeo_grid *g, int precision, const void *u, const void *v, const void *w, const void *th,
                      const void *p, const double *Wc, const double *Wf, int *finite, double *div, double *speed) {
     constexpr int NB = 512;
-    double *d = nullptr;
-    HIPCHK(hipMalloc(&d, sizeof(double) * 3 * NB));
+    double *d = red_scratch(3 * NB);
+    if (!d) return fail(AEO_ERR_HIP, "reduction scratch: out of device memory");
     if (precision == AEO_F64)
         launch_guard<double>(g_ctx->stream, d, NB, (const double *)u, (const double *)v, (const double *)w,
                              (const double *)th, (const double *)p, Wc, Wf, g->m64);
@@ -1794,10 +1812,8 @@ static int run_guard(aeo_grid *g, int precision, const void *u, const void *v, c
         launch_guard<float>(g_ctx->stream, d, NB, (const float *)u, (const float *)v, (const float *)w,
                             (const float *)th, (const float *)p, Wc, Wf, g->m64);
     std::vector<double> hp(3 * NB);
-    const hipError_t e1 = hipMemcpyAsync(hp.data(), d, sizeof(double) * 3 * NB, hipMemcpyDeviceToHost, g_ctx->stream);
-    const hipError_t e2 = e1 == hipSuccess ? hipStreamSynchronize(g_ctx->stream) : e1;
-    hipFree(d);
-    HIPCHK(e2);
+    HIPCHK(hipMemcpyAsync(hp.data(), d, sizeof(double) * 3 * NB, hipMemcpyDeviceToHost, g_ctx->stream));
+    HIPCHK(hipStreamSynchronize(g_ctx->stream));
     double r[3] = {0, 0, 0};
     for (int b = 0; b < NB; b++)
         for (int q = 0; q < 3; q++) {
@@ -1897,15 +1913,13 @@ extern "C" int aeo_mw_stepper_cfl(aeo_mw_stepper *h, void *u, void *v, void *w, 
     }
     *N = 0;
     if (sp.buoyancy) {
-        double *d = nullptr;
-        HIPCHK(hipMalloc(&d, sizeof(double) * Nz));
+        double *d = red_scratch(Nz);
+        if (!d) return fail(AEO_ERR_HIP, "reduction scratch: out of device memory");
         if (f64) launch_plane_sum<double>(st, d, (const double *)theta, g->m64);
         else launch_plane_sum<float>(st, d, (const float *)theta, g->m64);
         std::vector<double> sums(Nz);
-        const hipError_t e1 = hipMemcpyAsync(sums.data(), d, sizeof(double) * Nz, hipMemcpyDeviceToHost, st);
-        const hipError_t e2 = e1 == hipSuccess ? hipStreamSynchronize(st) : e1;
-        hipFree(d);
-        HIPCHK(e2);
+        HIPCHK(hipMemcpyAsync(sums.data(), d, sizeof(double) * Nz, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
         const double np = (double)Nx * Ny;
         double n2max = 0;
         for (int k = 1; k < Nz; k++) {
