@@ -43,19 +43,20 @@ __global__ void __launch_bounds__(256) pos_flux_kernel(T *__restrict__ Fx, T *__
     constexpr int R = SchemeInfo<S>::R;
     const uint32_t sx = m.sx, sxy = (uint32_t)m.sx * m.sy;
     const uint32_t c = lin(i + m.H, j + m.H, k + m.H, m.sx, m.sy);  // face c: west/south/bottom face of cell c
+    constexpr uint32_t E = sizeof(T);  // byte offsets for face_value
     if (j < m.Ny && k < m.Nz) {
         const T vel = u[c];
-        Fx[c] = vel * face_value<S, P, T>(phi, (c - R) * (uint32_t)sizeof(T), (uint32_t)sizeof(T), vel, m.teno_ct);
+        Fx[c] = vel * face_value<S, P, T>(phi, (c - R) * E, E, vel, m.teno_ct);
         Lx[c] = vel * (vel >= T(0) ? phi[c - 1] : phi[c]);
     }
     if (i < m.Nx && k < m.Nz) {
         const T vel = v[c];
-        Fy[c] = vel * face_value<S, P, T>(phi, (c - R * sx) * (uint32_t)sizeof(T), sx * (uint32_t)sizeof(T), vel, m.teno_ct);
+        Fy[c] = vel * face_value<S, P, T>(phi, (c - R * sx) * E, sx * E, vel, m.teno_ct);
         Ly[c] = vel * (vel >= T(0) ? phi[c - sx] : phi[c]);
     }
     if (i < m.Nx && j < m.Ny) {
         const T vel = w[c];
-        Fz[c] = vel * face_value<S, P, T>(phi, (c - R * sxy) * (uint32_t)sizeof(T), sxy * (uint32_t)sizeof(T), vel, m.teno_ct);
+        Fz[c] = vel * face_value<S, P, T>(phi, (c - R * sxy) * E, sxy * E, vel, m.teno_ct);
         Lz[c] = vel * (vel >= T(0) ? phi[c - sxy] : phi[c]);
     }
 }
