@@ -4,7 +4,9 @@
 of csrc/ with hipcc for gfx950 and links aeolus_hip/libaeolus_hip.so, which
 the ctypes layer loads. FP64 parity needs the reference's arithmetic:
 no FMA contraction (-ffp-contract=off) and correctly rounded division and
-square root (hipcc's default). Packed FP32 instructions are disabled.
+square root (hipcc's default); the reconstructions and the tendency kernel
+opt in to contraction with a pragma in their sources (at most an ulp per
+fused pair). Packed FP32 instructions are disabled.
 """
 import concurrent.futures
 import os

@@ -65,8 +65,14 @@ __device__ __forceinline__ uint32_t lin(int i, int j, int k, int sx, int sy) {
 }
 
 // ---------------------------------------------------------------------------
-// Reconstructions, in the reference's evaluation order (no FMA contraction:
-// the library is built with -ffp-contract=off).
+// Reconstructions, in the reference's evaluation order. The library is built
+// with -ffp-contract=off; the reconstructions (and the tendency kernel, which
+// turns the same pragma on for its file) let the compiler fuse a multiply and
+// an add into one FMA: each fused pair rounds once instead of twice (at most
+// an ulp from the unfused value; the FP64 parity bound is 1e-12), and the
+// FP64 WENO-Z kernel issues 15 % fewer FP64 instructions. Sums of squares
+// stay non-negative: fma(a, a, x) with x ≥ 0 is ≥ 0.
+#pragma clang fp contract(fast)
 // ---------------------------------------------------------------------------
 // RN(x/3) without a division: q = RN(x·r), r = RN(1/3); the remainder
 // x − 3q is exact in one FMA and RN(q + (x − 3q)·r) is RN(x/3). x/3 is never
@@ -324,5 +330,9 @@ __device__ __forceinline__ T face_value(const T *__restrict__ d, uint32_t o0, ui
         else return weno_z7<P, T>(t[0], t[1], t[2], t[3], t[4], t[5], t[6]);
     }
 }
+
+// contraction ends with the reconstructions: the sources that include this
+// header keep -ffp-contract=off unless they opt in themselves
+#pragma clang fp contract(off)
 
 }  // namespace aeo

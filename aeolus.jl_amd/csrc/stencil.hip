@@ -10,6 +10,10 @@
 //   reductions       max |div|, max advective rate
 #include "kernels.h"
 
+// multiply-add pairs may fuse into FMAs here (the reconstructions in common.h
+// do the same); every other source keeps -ffp-contract=off
+#pragma clang fp contract(fast)
+
 namespace aeo {
 
 // ---------------------------------------------------------------------------
