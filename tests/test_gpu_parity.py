@@ -98,6 +98,17 @@ def test_advect_scalar_teno_skew_parity(hip, orc, scheme, stretched):
     assert rel_err(dt_.download()[I], t[I]) <= TOL64
 
 
+def test_advection_refuses_a_short_halo(hip):
+    # test/runtests.jl:482-506: the five-point reconstructions on a grid whose
+    # halo is below their stencil's reach are an error, not a wrong answer
+    g = hip.Grid((16, 8, 8), (1.0, 1.0, 1.0), halo=2)
+    d = [up(hip, np.zeros(g.storage, order="F")) for _ in range(5)]
+    for scheme in ("weno5", "weno_js5", "weno_z5", "teno5", "weno_z7"):
+        with pytest.raises(hip.AeolusError, match="halo"):
+            hip.advection_scheme(scheme, g)(*d)
+    hip.advection_scheme("centered", g)(*d)  # a halo of 1 is enough for the centred form
+
+
 def test_teno_cutoff_selector_and_refusals(hip, orc):
     g, og = grids(hip, orc, (40, 24, 20), (2.0, 1.0, 1.0), 3)
     bcs = ("periodic",) * 3
