@@ -198,11 +198,11 @@ extern "C" int aeo_grid_create(int Nx, int Ny, int Nz, int H, double dx, double 
     for (int q = 0; q < 6; q++) {
         size_t len = vs[q]->size();
         HIPCHK(hipMalloc(&g->d64[q], sizeof(double) * len));
-        HIPCHK(hipMemcpy(g->d64[q], vs[q]->data(), sizeof(double) * len, hipMemcpyHostToDevice));
+        HIPCHK(upload_dev(g->d64[q], vs[q]->data(), sizeof(double) * len));
         std::vector<float> f(len);
         for (size_t c = 0; c < len; c++) f[c] = (float)(*vs[q])[c];
         HIPCHK(hipMalloc(&g->d32[q], sizeof(float) * len));
-        HIPCHK(hipMemcpy(g->d32[q], f.data(), sizeof(float) * len, hipMemcpyHostToDevice));
+        HIPCHK(upload_dev(g->d32[q], f.data(), sizeof(float) * len));
     }
     fill_metrics<double>(g->m64, g.get(), g->d64);
     fill_metrics<float>(g->m32, g.get(), g->d32);
@@ -466,7 +466,7 @@ extern "C" int aeo_flux_limiter_workspace_create(aeo_grid *g, int precision, aeo
     const size_t bytes = (size_t)(g->Nx + 2 * g->H) * (g->Ny + 2 * g->H) * (g->Nz + 2 * g->H) *
                          (precision == AEO_F64 ? 8 : 4);
     for (int q = 0; q < 7; q++) {
-        if (hipMalloc(&ws->a[q], bytes) != hipSuccess || hipMemset(ws->a[q], 0, bytes) != hipSuccess) {
+        if (hipMalloc(&ws->a[q], bytes) != hipSuccess || zero_dev(ws->a[q], bytes) != hipSuccess) {
             for (int r = 0; r <= q; r++)
                 if (ws->a[r]) hipFree(ws->a[r]);
             delete ws;
@@ -962,7 +962,7 @@ extern "C" int aeo_terrain_create(aeo_grid *g, const double *h_plane, double lid
                            (size_t)sz};
     for (int q = 0; q < 7; q++) {
         HIPCHK(hipMalloc(&t->d[q], sizeof(double) * len[q]));
-        HIPCHK(hipMemcpy(t->d[q], src[q], sizeof(double) * len[q], hipMemcpyHostToDevice));
+        HIPCHK(upload_dev(t->d[q], src[q], sizeof(double) * len[q]));
     }
     t->v = TFView{t->d[0], t->d[1], t->d[2], t->d[3], t->d[4], t->d[5], t->d[6], lid, g->dx, g->dy,
                   g->Nx, g->Ny, g->Nz, g->H, sx, sy,
@@ -1205,15 +1205,15 @@ extern "C" int aeo_stepper_set_sgs(aeo_stepper *h, double C_s, double Pr_t) {
     }
     if (!h->nut) {
         HIPCHK(hipMalloc(&h->nut, h->n * h->esz));
-        HIPCHK(hipMemset(h->nut, 0, h->n * h->esz));
+        HIPCHK(zero_dev(h->nut, h->n * h->esz));
     }
     if (h->sp.precision == AEO_F64) {
         HIPCHK(hipMalloc(&h->sgs_lv, v.size() * sizeof(double)));
-        HIPCHK(hipMemcpy(h->sgs_lv, v.data(), v.size() * sizeof(double), hipMemcpyHostToDevice));
+        HIPCHK(upload_dev(h->sgs_lv, v.data(), v.size() * sizeof(double)));
     } else {
         std::vector<float> f(v.begin(), v.end());
         HIPCHK(hipMalloc(&h->sgs_lv, f.size() * sizeof(float)));
-        HIPCHK(hipMemcpy(h->sgs_lv, f.data(), f.size() * sizeof(float), hipMemcpyHostToDevice));
+        HIPCHK(upload_dev(h->sgs_lv, f.data(), f.size() * sizeof(float)));
     }
     h->sgs_s = 1 / Pr_t;
     return AEO_OK;
@@ -1260,11 +1260,11 @@ static int make_sponge(const aeo_grid *g, int precision, double alpha_max, doubl
     void *d = nullptr;
     if (precision == AEO_F64) {
         HIPCHK(hipMalloc(&d, v.size() * sizeof(double)));
-        HIPCHK(hipMemcpy(d, v.data(), v.size() * sizeof(double), hipMemcpyHostToDevice));
+        HIPCHK(upload_dev(d, v.data(), v.size() * sizeof(double)));
     } else {
         std::vector<float> f(v.begin(), v.end());
         HIPCHK(hipMalloc(&d, f.size() * sizeof(float)));
-        HIPCHK(hipMemcpy(d, f.data(), f.size() * sizeof(float), hipMemcpyHostToDevice));
+        HIPCHK(upload_dev(d, f.data(), f.size() * sizeof(float)));
     }
     *out = d;
     return AEO_OK;
@@ -1500,9 +1500,9 @@ extern "C" int aeo_mw_stepper_create(aeo_grid *g, const aeo_mw_params *sp, const
     for (double **d : {&h->Wc, &h->Wf, &h->thf}) HIPCHK(hipMalloc(d, sizeof(double) * (nz + 1)));
     std::vector<double> wc(Wc, Wc + nz), wf(Wf, Wf + nz), tf(thetabar_f, thetabar_f + nz);
     wc.push_back(wc.back()); wf.push_back(wf.back()); tf.push_back(tf.back());
-    HIPCHK(hipMemcpy(h->Wc, wc.data(), sizeof(double) * (nz + 1), hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(h->Wf, wf.data(), sizeof(double) * (nz + 1), hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(h->thf, tf.data(), sizeof(double) * (nz + 1), hipMemcpyHostToDevice));
+    HIPCHK(upload_dev(h->Wc, wc.data(), sizeof(double) * (nz + 1)));
+    HIPCHK(upload_dev(h->Wf, wf.data(), sizeof(double) * (nz + 1)));
+    HIPCHK(upload_dev(h->thf, tf.data(), sizeof(double) * (nz + 1)));
     *out = h.release();
     return AEO_OK;
 }
@@ -1644,15 +1644,15 @@ extern "C" int aeo_mw_stepper_set_sgs(aeo_mw_stepper *h, double C_s, double Pr_t
     for (void **b : {&h->nut, &h->kap})
         if (!*b) {
             HIPCHK(hipMalloc(b, h->n * h->esz));
-            HIPCHK(hipMemset(*b, 0, h->n * h->esz));
+            HIPCHK(zero_dev(*b, h->n * h->esz));
         }
     if (h->sp.precision == AEO_F64) {
         HIPCHK(hipMalloc(&h->sgs_lv, v.size() * sizeof(double)));
-        HIPCHK(hipMemcpy(h->sgs_lv, v.data(), v.size() * sizeof(double), hipMemcpyHostToDevice));
+        HIPCHK(upload_dev(h->sgs_lv, v.data(), v.size() * sizeof(double)));
     } else {
         std::vector<float> f(v.begin(), v.end());
         HIPCHK(hipMalloc(&h->sgs_lv, f.size() * sizeof(float)));
-        HIPCHK(hipMemcpy(h->sgs_lv, f.data(), f.size() * sizeof(float), hipMemcpyHostToDevice));
+        HIPCHK(upload_dev(h->sgs_lv, f.data(), f.size() * sizeof(float)));
     }
     h->Pr_t = Pr_t;
     return AEO_OK;

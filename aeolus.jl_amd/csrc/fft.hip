@@ -622,7 +622,7 @@ static hipError_t upload_table(double2 **dst, int n) {
     for (int m = 0; m < n; m++) h[m] = make_double2(cos(2.0 * M_PI * m / n), -sin(2.0 * M_PI * m / n));
     hipError_t e = hipMalloc(dst, sizeof(double2) * n);
     if (e != hipSuccess) return e;
-    return hipMemcpy(*dst, h.data(), sizeof(double2) * n, hipMemcpyHostToDevice);
+    return upload_dev(*dst, h.data(), sizeof(double2) * n);
 }
 
 // the table of length L (created at plan set-up, never during a solve)
@@ -658,7 +658,7 @@ static hipError_t ensure_table(SpectralPlan *p, int L) {
     int *dp = nullptr;
     if ((e = hipMalloc(&dp, sizeof(int) * L)) != hipSuccess) return e;
     p->perms.emplace_back(L, dp);
-    return hipMemcpy(dp, h.data(), sizeof(int) * L, hipMemcpyHostToDevice);
+    return upload_dev(dp, h.data(), sizeof(int) * L);
 }
 static FFTLen fftlen(const SpectralPlan *p, int L) {
     FFTLen f = fft_factor(L);
@@ -701,7 +701,7 @@ void spectral_destroy(SpectralPlan *p) {
 
 static int upload(double **dst, const std::vector<double> &h) {
     if (hipMalloc(dst, sizeof(double) * h.size()) != hipSuccess) return -1;
-    return hipMemcpy(*dst, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice) == hipSuccess ? 0 : -1;
+    return upload_dev(*dst, h.data(), sizeof(double) * h.size()) == hipSuccess ? 0 : -1;
 }
 
 // FFTPoissonSolver constructor, src/PressureSolvers.jl:121-138
@@ -735,7 +735,7 @@ int spectral_column_setup(SpectralPlan *p, double dx, double dy, const double *l
         size_t nh = (size_t)p->LD * Ny * Nz;
         if (hipMalloc(&p->hat, sizeof(double2) * nh) != hipSuccess) return -1;
         if (hipMalloc(&p->cp, sizeof(double) * nh) != hipSuccess) return -1;
-        if (hipMemset(p->hat, 0, sizeof(double2) * nh) != hipSuccess) return -1;
+        if (zero_dev(p->hat, sizeof(double2) * nh) != hipSuccess) return -1;
     }
     std::vector<double> lx(p->nkx), ly(Ny);
     // _r2r_eigen (src/PressureSolvers.jl:618-624): R2HC index q <-> frequency

@@ -4,6 +4,20 @@
 
 namespace aeo {
 
+// Set-up transfers. The library's stream is created hipStreamNonBlocking, so
+// it does not wait for work queued on the null stream, where hipMemset and
+// hipMemcpy run, and hipMemset may return before the memory is cleared.
+// Memory cleared or uploaded at set-up is therefore waited for before the
+// call returns: a kernel later launched on the library's stream sees it.
+inline hipError_t zero_dev(void *p, size_t bytes) {
+    const hipError_t e = hipMemset(p, 0, bytes);
+    return e == hipSuccess ? hipDeviceSynchronize() : e;
+}
+inline hipError_t upload_dev(void *dst, const void *src, size_t bytes) {
+    const hipError_t e = hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice);
+    return e == hipSuccess ? hipDeviceSynchronize() : e;
+}
+
 // Physics of one substage of the step (src/Simulations.jl:168-288 fields).
 struct StepPhys {
     double nu, dt, theta0, g_accel, flux_bot, flux_top;

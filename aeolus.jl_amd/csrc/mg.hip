@@ -607,7 +607,7 @@ static hipError_t upload_as(void **dst, const std::vector<double> &h) {
     std::vector<T> t(h.begin(), h.end());
     hipError_t e = hipMalloc(dst, sizeof(T) * t.size());
     if (e != hipSuccess) return e;
-    return hipMemcpy(*dst, t.data(), sizeof(T) * t.size(), hipMemcpyHostToDevice);
+    return upload_dev(*dst, t.data(), sizeof(T) * t.size());
 }
 
 MGPlan *mg_create(int nx, int ny, int nz, double dx, double dy, double dz, int bcx, int bcy, int bcz, const double *cx,
@@ -637,7 +637,7 @@ MGPlan *mg_create(int nx, int ny, int nz, double dx, double dy, double dz, int b
         void **arrs[4] = {&h.phi, &h.b, &h.r, &h.tmp};
         for (auto a : arrs) {
             if (hipMalloc(a, esz * n) != hipSuccess) return fail("mg: allocation failed");
-            hipMemset(*a, 0, esz * n);
+            zero_dev(*a, esz * n);
         }
         if (P->coeff) {
             hipError_t e1 = prec == 0 ? upload_as<double>(&h.cx, hx) : upload_as<float>(&h.cx, hx);
@@ -663,14 +663,14 @@ MGPlan *mg_create(int nx, int ny, int nz, double dx, double dy, double dz, int b
     double **cg[5] = {&P->x, &P->r, &P->z, &P->p, &P->Ap};
     for (auto a : cg) {
         if (hipMalloc(a, 8 * n0) != hipSuccess) return fail("mg: allocation failed");
-        hipMemset(*a, 0, 8 * n0);
+        zero_dev(*a, 8 * n0);
     }
     P->npart = nblocks3(nx, ny, nz);
     if (P->npart < 3 * 2048) P->npart = 3 * 2048;
     if (hipMalloc(&P->part, sizeof(double) * P->npart) != hipSuccess) return fail("mg: allocation failed");
     if (hipMalloc(&P->part2, sizeof(double) * 2048) != hipSuccess) return fail("mg: allocation failed");
     if (hipMalloc(&P->scal, sizeof(double) * 16) != hipSuccess) return fail("mg: allocation failed");
-    hipMemset(P->scal, 0, sizeof(double) * 16);
+    zero_dev(P->scal, sizeof(double) * 16);
     if (rhs_scale) {
         std::vector<double> rs(rhs_scale, rhs_scale + nz);
         if (upload_as<double>((void **)&P->rhs_scale, rs)) return fail("mg: allocation failed");

@@ -233,7 +233,7 @@ struct MarchArgs {
 };
 
 constexpr int MW = 64;   // lanes per wave (gfx950)
-constexpr int KCM = 16;  // levels per chunk
+constexpr int KCM = 8;   // levels per chunk
 
 template <int MODE> struct ModeFields { static constexpr int lo = F_U, hi = F_S; };
 template <> struct ModeFields<MODE_SCALAR> { static constexpr int lo = F_S, hi = F_S; };

@@ -617,7 +617,7 @@ UNet *unet_create(int nx, int ny, int nz, int inc, uint64_t seed, char *err, siz
     bool ok = true;
     auto A = [&](void **p, size_t bytes) {
         if (hipMalloc(p, bytes) != hipSuccess) ok = false;
-        else (void)hipMemset(*p, 0, bytes);
+        else (void)zero_dev(*p, bytes);
     };
     const size_t b2 = sizeof(bf16);
     A((void **)&N->W, off * 4); A((void **)&N->G, off * 4); A((void **)&N->M, off * 4); A((void **)&N->V, off * 4);
@@ -649,7 +649,7 @@ UNet *unet_create(int nx, int ny, int nz, int inc, uint64_t seed, char *err, siz
     }
     std::vector<float> w;
     init_params(N, w, seed);
-    if (hipMemcpy(N->W, w.data(), off * 4, hipMemcpyHostToDevice) != hipSuccess || refresh_layouts(N, 0) != 0 ||
+    if (upload_dev(N->W, w.data(), off * 4) != hipSuccess || refresh_layouts(N, 0) != 0 ||
         hipDeviceSynchronize() != hipSuccess) {
         snprintf(err, errlen, "U-Net initialisation failed");
         unet_destroy(N);
@@ -664,9 +664,9 @@ int unet_get_params(UNet *N, float *host) {
     return hipMemcpy(host, N->W, N->nparam * 4, hipMemcpyDeviceToHost) == hipSuccess ? 0 : -2;
 }
 int unet_set_params(UNet *N, hipStream_t st, const float *host) {
-    if (hipMemcpy(N->W, host, N->nparam * 4, hipMemcpyHostToDevice) != hipSuccess) return -2;
-    (void)hipMemset(N->M, 0, N->nparam * 4);
-    (void)hipMemset(N->V, 0, N->nparam * 4);
+    if (upload_dev(N->W, host, N->nparam * 4) != hipSuccess) return -2;
+    (void)zero_dev(N->M, N->nparam * 4);
+    (void)zero_dev(N->V, N->nparam * 4);
     N->step = 0;
     return refresh_layouts(N, st);
 }
