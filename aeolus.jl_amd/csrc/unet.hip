@@ -127,6 +127,164 @@ __global__ void __launch_bounds__(256) conv3_kernel(const bf16 *__restrict__ in,
         }
 }
 
+// the epilogue of the tiled convolutions: D[4·group + i][r] of tile t is the
+// voxel (x0 + 16·t + 4·group + i, y, z), output channel 16·u + r
+template <int TM, int TN>
+__device__ __forceinline__ void tile_store(const f32x4 (&acc)[TM][TN], const float *__restrict__ bias,
+                                           bf16 *__restrict__ out, int ocs, int ooff, Vol g, int relu, int x0, int y,
+                                           int z, int r, int grp) {
+    if (y >= g.ny) return;
+#pragma unroll
+    for (int t = 0; t < TM; t++)
+#pragma unroll
+        for (int u = 0; u < TN; u++) {
+            const int n = 16 * u + r;
+            const float bn = bias ? bias[n] : 0.f;
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const int x = x0 + 16 * t + 4 * grp + i;
+                if (x < g.nx) {
+                    float v = acc[t][u][i] + bn;
+                    if (relu) v = fmaxf(v, 0.f);
+                    out[(size_t)(x + g.nx * (y + (size_t)g.ny * z)) * ocs + ooff + n] = (bf16)v;
+                }
+            }
+        }
+}
+
+// ---------------------------------------------------------------------------
+// The same convolution with the input tile staged in LDS (CIN <= 32). A block
+// is 32 (x) × 4 (y) output voxels of one z-level: wave w does row y0 + w, its
+// lanes r + 16·t (t < 2) the voxels x0 + 16·t + r. The block first copies the
+// 34 × 6 × 3 input voxels its 27 taps read (x and y wrapped, zero planes
+// outside z) into LDS, one 16-byte chunk of 8 channels per load, so each input
+// voxel is read from memory about 4.8 times per output voxel instead of 27.
+// The K loop, the MFMA lane layout and the epilogue are those of
+// conv3_kernel; only the A fragments come from LDS.
+// ---------------------------------------------------------------------------
+template <int CIN, int COUT>
+__global__ void __launch_bounds__(256) conv3_tile_kernel(const bf16 *__restrict__ in, const bf16 *__restrict__ wb,
+                                                         const float *__restrict__ bias, bf16 *__restrict__ out,
+                                                         int ocs, int ooff, Vol g, int relu) {
+    constexpr int TX = 32, TY = 4, SX = TX + 2, SY = TY + 2, C8 = CIN / 8;
+    constexpr int TN = COUT / 16, TM = TX / 16;
+    constexpr int NCH = (27 * CIN + 31) / 32;
+    static_assert(CIN % 8 == 0 && COUT % 16 == 0, "channel counts");
+    __shared__ bf16x8 tile[3][SY][SX][C8];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int x0 = blockIdx.x * TX, y0 = blockIdx.y * TY, z = blockIdx.z;
+    // the input tile: positions x0 − 1 … x0 + TX (wrapped); positions past
+    // nx (or ny) feed no valid output and are zero, which keeps every read
+    // inside the volume when the axis is shorter than the tile
+    for (int e = tid; e < 3 * SY * SX * C8; e += 256) {
+        const int c8 = e % C8;
+        int rest = e / C8;
+        const int sx = rest % SX;
+        rest /= SX;
+        const int sy = rest % SY, pz = rest / SY;
+        const int xr = x0 + sx - 1, yr = y0 + sy - 1, zz = z + pz - 1;
+        bf16x8 val = zero8();
+        if (zz >= 0 && zz < g.nz && xr <= g.nx && yr <= g.ny) {
+            const int xx = wrapi(xr, g.nx), yy = wrapi(yr, g.ny);
+            val = *reinterpret_cast<const bf16x8 *>(in + ((size_t)(xx + g.nx * (yy + g.ny * zz)) * CIN + c8 * 8));
+        }
+        tile[pz][sy][sx][c8] = val;
+    }
+    __syncthreads();
+    const int r = lane & 15, grp = lane >> 4;
+    f32x4 acc[TM][TN];
+#pragma unroll
+    for (int t = 0; t < TM; t++)
+#pragma unroll
+        for (int u = 0; u < TN; u++) acc[t][u] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll 2
+    for (int ch = 0; ch < NCH; ch++) {
+        const int k0 = ch * 32 + grp * 8;
+        const int tap = k0 / CIN, c8 = (k0 - tap * CIN) >> 3;
+        bf16x8 b[TN];
+#pragma unroll
+        for (int u = 0; u < TN; u++)
+            b[u] = *reinterpret_cast<const bf16x8 *>(wb + ((size_t)(k0 >> 3) * COUT + 16 * u + r) * 8);
+        // the zero-padded last chunk (tap 27) indexes tap 26 and reads zeros
+        const int tq = tap < 27 ? tap : 26;
+        const int dx = tq % 3 - 1, dy = (tq / 3) % 3 - 1, dz = tq / 9 - 1;
+        bf16x8 a[TM];
+#pragma unroll
+        for (int t = 0; t < TM; t++) a[t] = tap < 27 ? tile[dz + 1][wave + 1 + dy][16 * t + r + 1 + dx][c8] : zero8();
+#pragma unroll
+        for (int t = 0; t < TM; t++)
+#pragma unroll
+            for (int u = 0; u < TN; u++) acc[t][u] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[t], b[u], acc[t][u], 0, 0, 0);
+    }
+    tile_store<TM, TN>(acc, bias, out, ocs, ooff, g, relu, x0, y0 + wave, z, r, grp);
+}
+
+// The tiled convolution with the tile staged one slice of SW channels at a
+// time, for the channel counts whose whole tile would leave room for fewer
+// than three blocks per CU (48, 64, 96). With SW = 32 (39 KB of LDS) each tap
+// is one K-chunk (k = tap·CIN + 32·slice + 8·group); with SW = 16 (20 KB) a
+// K-chunk is two taps (groups 0-1 the even tap, 2-3 the odd one), and the
+// missing 28th tap of the last chunk contributes zeros.
+template <int CIN, int COUT, int SW>
+__global__ void __launch_bounds__(256) conv3_slice_kernel(const bf16 *__restrict__ in, const bf16 *__restrict__ wb,
+                                                          const float *__restrict__ bias, bf16 *__restrict__ out,
+                                                          int ocs, int ooff, Vol g, int relu) {
+    constexpr int TX = 32, TY = 4, SX = TX + 2, SY = TY + 2;
+    constexpr int TN = COUT / 16, TM = TX / 16, NS = CIN / SW, C8 = SW / 8, NC = SW == 32 ? 27 : 14;
+    static_assert((SW == 16 || SW == 32) && CIN % SW == 0 && COUT % 16 == 0, "channel counts");
+    __shared__ bf16x8 tile[3][SY][SX][C8];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int x0 = blockIdx.x * TX, y0 = blockIdx.y * TY, z = blockIdx.z;
+    const int r = lane & 15, grp = lane >> 4;
+    f32x4 acc[TM][TN];
+#pragma unroll
+    for (int t = 0; t < TM; t++)
+#pragma unroll
+        for (int u = 0; u < TN; u++) acc[t][u] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int sl = 0; sl < NS; sl++) {
+        if (sl) __syncthreads();  // every wave has read the previous slice
+        for (int e = tid; e < 3 * SY * SX * C8; e += 256) {
+            const int c8 = e % C8;
+            int rest = e / C8;
+            const int sx = rest % SX;
+            rest /= SX;
+            const int sy = rest % SY, pz = rest / SY;
+            const int xr = x0 + sx - 1, yr = y0 + sy - 1, zz = z + pz - 1;
+            bf16x8 val = zero8();
+            if (zz >= 0 && zz < g.nz && xr <= g.nx && yr <= g.ny) {
+                const int xx = wrapi(xr, g.nx), yy = wrapi(yr, g.ny);
+                val = *reinterpret_cast<const bf16x8 *>(in + ((size_t)(xx + g.nx * (yy + g.ny * zz)) * CIN + SW * sl +
+                                                              c8 * 8));
+            }
+            tile[pz][sy][sx][c8] = val;
+        }
+        __syncthreads();
+#pragma unroll 3
+        for (int j = 0; j < NC; j++) {
+            const int tap = SW == 32 ? j : 2 * j + (grp >> 1);
+            const int c8 = SW == 32 ? grp : (grp & 1);
+            const bool live = tap < 27;
+            const int tq = live ? tap : 26;
+            const int k0 = tq * CIN + SW * sl + c8 * 8;
+            bf16x8 b[TN];
+#pragma unroll
+            for (int u = 0; u < TN; u++)
+                b[u] = live ? *reinterpret_cast<const bf16x8 *>(wb + ((size_t)(k0 >> 3) * COUT + 16 * u + r) * 8)
+                            : zero8();
+            const int dx = tq % 3 - 1, dy = (tq / 3) % 3 - 1, dz = tq / 9 - 1;
+            bf16x8 a[TM];
+#pragma unroll
+            for (int t = 0; t < TM; t++) a[t] = live ? tile[dz + 1][wave + 1 + dy][16 * t + r + 1 + dx][c8] : zero8();
+#pragma unroll
+            for (int t = 0; t < TM; t++)
+#pragma unroll
+                for (int u = 0; u < TN; u++)
+                    acc[t][u] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[t], b[u], acc[t][u], 0, 0, 0);
+        }
+    }
+    tile_store<TM, TN>(acc, bias, out, ocs, ooff, g, relu, x0, y0 + wave, z, r, grp);
+}
+
 // ---------------------------------------------------------------------------
 // Weight gradient of a 3³ convolution: part[blk, tap, grp][c][n] =
 // Σ_{v in the block's range, v ≡ grp} in[nb(v, tap)][c] · dy[v][n], FP32.
@@ -482,9 +640,24 @@ static int nblk(size_t n, int b = UB) { return (int)((n + b - 1) / b); }
 template <int CIN, int COUT>
 static void conv_launch(hipStream_t st, const bf16 *in, const bf16 *wb, const float *bias, bf16 *out, int ocs,
                         int ooff, Vol g, int relu) {
-    const int nv = g.nx * g.ny * g.nz;
-    hipLaunchKernelGGL((conv3_kernel<CIN, COUT>), dim3((nv + 255) / 256), dim3(256), 0, st, in, wb, bias, out, ocs, ooff,
-                       g, relu);
+    if constexpr (CIN == 48) {
+        const dim3 grid((g.nx + 31) / 32, (g.ny + 3) / 4, g.nz);
+        hipLaunchKernelGGL((conv3_slice_kernel<CIN, COUT, 16>), grid, dim3(256), 0, st, in, wb, bias, out, ocs, ooff,
+                           g, relu);
+    } else if constexpr (CIN <= 32) {
+        // the whole LDS tile (3 · 6 · 34 · CIN · 2 bytes: 39 KB at 32 channels)
+        const dim3 grid((g.nx + 31) / 32, (g.ny + 3) / 4, g.nz);
+        hipLaunchKernelGGL((conv3_tile_kernel<CIN, COUT>), grid, dim3(256), 0, st, in, wb, bias, out, ocs, ooff, g,
+                           relu);
+    } else if constexpr (CIN % 32 == 0) {
+        const dim3 grid((g.nx + 31) / 32, (g.ny + 3) / 4, g.nz);
+        hipLaunchKernelGGL((conv3_slice_kernel<CIN, COUT, 32>), grid, dim3(256), 0, st, in, wb, bias, out, ocs, ooff,
+                           g, relu);
+    } else {
+        const int nv = g.nx * g.ny * g.nz;
+        hipLaunchKernelGGL((conv3_kernel<CIN, COUT>), dim3((nv + 255) / 256), dim3(256), 0, st, in, wb, bias, out, ocs,
+                           ooff, g, relu);
+    }
 }
 
 // dispatch on (cin, cout) to the instantiated convolutions
