@@ -408,25 +408,43 @@ __device__ __forceinline__ int child_of(int p, int q, Vol g) {
     return (2 * x + (q & 1)) + g.nx * ((2 * y + ((q >> 1) & 1)) + g.ny * (2 * z + (q >> 2)));
 }
 
-// out[p][c] = mean of the 8 children in[child][ioff + c] (fine grid g)
-__global__ void pool_kernel(const bf16 *__restrict__ in, int ics, int ioff, bf16 *__restrict__ out, int C, Vol g) {
-    const int nc = (g.nx >> 1) * (g.ny >> 1) * (g.nz >> 1);
-    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= (size_t)nc * C) return;
-    const int p = (int)(e / C), c = (int)(e - (size_t)p * C);
-    float s = 0.f;
-    for (int q = 0; q < 8; q++) s += (float)in[(size_t)child_of(p, q, g) * ics + ioff + c];
-    out[e] = (bf16)(s * 0.125f);
+// out[p][c] = mean of the 8 children in[child][ioff + c] (fine grid g).
+// Grid (⌈(nx/2)·C/8 / 256⌉, ny/2, nz/2): a thread does 8 channels (16-byte
+// loads and store) of one coarse voxel; C/8 is a power of two (lc8 = log2).
+__global__ void pool_kernel(const bf16 *__restrict__ in, int ics, int ioff, bf16 *__restrict__ out, int C, int lc8,
+                            Vol g) {
+    const int cx = g.nx >> 1, cy = g.ny >> 1;
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (cx << lc8)) return;
+    const int x = e >> lc8, q = e & ((1 << lc8) - 1);
+    const int y = blockIdx.y, z = blockIdx.z;
+    float s[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+        const size_t v = (size_t)(2 * x + (k & 1)) +
+                         (size_t)g.nx * ((2 * y + ((k >> 1) & 1)) + (size_t)g.ny * (2 * z + (k >> 2)));
+        const bf16x8 a = *reinterpret_cast<const bf16x8 *>(in + v * ics + ioff + 8 * q);
+#pragma unroll
+        for (int i = 0; i < 8; i++) s[i] += (float)a[i];
+    }
+    bf16x8 o;
+#pragma unroll
+    for (int i = 0; i < 8; i++) o[i] = (bf16)(s[i] * 0.125f);
+    const size_t p = (size_t)x + (size_t)cx * (y + (size_t)cy * z);
+    *reinterpret_cast<bf16x8 *>(out + p * C + 8 * q) = o;
 }
-// out[v][ooff + c] = in[parent(v)][c] (fine grid g)
-// 8 channels (16 bytes) per thread: C, ocs and ooff are multiples of 8
-__global__ void up_kernel(const bf16 *__restrict__ in, bf16 *__restrict__ out, int ocs, int ooff, int C, Vol g) {
-    const int nf = g.nx * g.ny * g.nz, c8 = C / 8;
-    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= (size_t)nf * c8) return;
-    const int v = (int)(e / c8), q = (int)(e - (size_t)v * c8);
-    *reinterpret_cast<bf16x8 *>(out + (size_t)v * ocs + ooff + 8 * q) =
-        *reinterpret_cast<const bf16x8 *>(in + (size_t)parent_of(v, g) * C + 8 * q);
+// out[v][ooff + c] = in[parent(v)][c] (fine grid g). Grid (⌈nx·C/8 / 256⌉,
+// ny, nz): a thread does 8 channels (16 bytes); C/8 is a power of two
+// (lc8 = log2), C, ocs and ooff are multiples of 8
+__global__ void up_kernel(const bf16 *__restrict__ in, bf16 *__restrict__ out, int ocs, int ooff, int C, int lc8,
+                          Vol g) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (g.nx << lc8)) return;
+    const int x = e >> lc8, q = e & ((1 << lc8) - 1);
+    const int y = blockIdx.y, z = blockIdx.z;
+    const size_t v = (size_t)x + (size_t)g.nx * (y + (size_t)g.ny * z);
+    const size_t p = (size_t)(x >> 1) + (size_t)(g.nx >> 1) * ((y >> 1) + (size_t)(g.ny >> 1) * (z >> 1));
+    *reinterpret_cast<bf16x8 *>(out + v * ocs + ooff + 8 * q) = *reinterpret_cast<const bf16x8 *>(in + p * C + 8 * q);
 }
 // out[v][c] = y[v][yoff + c] > 0 ? gin[v][goff + c] : 0
 __global__ void relu_mask_kernel(const bf16 *gin, int gcs, int goff, const bf16 *__restrict__ y, int ycs,
@@ -856,17 +874,21 @@ int unet_forward(UNet *N, hipStream_t st, const double *src, const double *mask,
     int rc = 0;
     rc |= conv_run(st, 8, 16, N->x0, L[0].wf, W + L[0].boff, N->a0, 16, 0, N->g0, 1);
     rc |= conv_run(st, 16, 16, N->a0, L[1].wf, W + L[1].boff, N->cat0, 48, 32, N->g0, 1);  // skip0
-    hipLaunchKernelGGL(pool_kernel, dim3(nblk((size_t)N->n1 * 16)), dim3(UB), 0, st, N->cat0, 48, 32, N->p0, 16, N->g0);
+    hipLaunchKernelGGL(pool_kernel, dim3(nblk((size_t)(N->g0.nx / 2) * 2), N->g0.ny / 2, N->g0.nz / 2), dim3(UB), 0, st,
+                       N->cat0, 48, 32, N->p0, 16, 1, N->g0);
     rc |= conv_run(st, 16, 32, N->p0, L[2].wf, W + L[2].boff, N->a2, 32, 0, N->g1, 1);
     rc |= conv_run(st, 32, 32, N->a2, L[3].wf, W + L[3].boff, N->cat1, 96, 64, N->g1, 1);  // skip1
-    hipLaunchKernelGGL(pool_kernel, dim3(nblk((size_t)N->n2 * 32)), dim3(UB), 0, st, N->cat1, 96, 64, N->p1, 32, N->g1);
+    hipLaunchKernelGGL(pool_kernel, dim3(nblk((size_t)(N->g1.nx / 2) * 4), N->g1.ny / 2, N->g1.nz / 2), dim3(UB), 0, st,
+                       N->cat1, 96, 64, N->p1, 32, 2, N->g1);
     rc |= conv_run(st, 32, 64, N->p1, L[4].wf, W + L[4].boff, N->a4, 64, 0, N->g2, 1);
     rc |= conv_run(st, 64, 64, N->a4, L[5].wf, W + L[5].boff, N->a5, 64, 0, N->g2, 1);
     static_assert(64 % 8 == 0 && 96 % 8 == 0 && 32 % 8 == 0 && 48 % 8 == 0, "up_kernel moves 8 channels");
-    hipLaunchKernelGGL(up_kernel, dim3(nblk((size_t)N->n1 * 64 / 8)), dim3(UB), 0, st, N->a5, N->cat1, 96, 0, 64, N->g1);
+    hipLaunchKernelGGL(up_kernel, dim3(nblk((size_t)N->g1.nx * 8), N->g1.ny, N->g1.nz), dim3(UB), 0, st, N->a5, N->cat1,
+                       96, 0, 64, 3, N->g1);
     rc |= conv_run(st, 96, 32, N->cat1, L[6].wf, W + L[6].boff, N->a6, 32, 0, N->g1, 1);
     rc |= conv_run(st, 32, 32, N->a6, L[7].wf, W + L[7].boff, N->a7, 32, 0, N->g1, 1);
-    hipLaunchKernelGGL(up_kernel, dim3(nblk((size_t)n0 * 32 / 8)), dim3(UB), 0, st, N->a7, N->cat0, 48, 0, 32, N->g0);
+    hipLaunchKernelGGL(up_kernel, dim3(nblk((size_t)N->g0.nx * 4), N->g0.ny, N->g0.nz), dim3(UB), 0, st, N->a7, N->cat0,
+                       48, 0, 32, 2, N->g0);
     rc |= conv_run(st, 48, 16, N->cat0, L[8].wf, W + L[8].boff, N->a8, 16, 0, N->g0, 1);
     rc |= conv_run(st, 16, 16, N->a8, L[9].wf, W + L[9].boff, N->a9, 16, 0, N->g0, 1);
     hipLaunchKernelGGL(head_kernel, dim3(nblk(n0)), dim3(UB), 0, st, N->a9, W + N->headoff, N->scal, out, N->y, n0);
