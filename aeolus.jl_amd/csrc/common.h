@@ -161,16 +161,19 @@ template <typename T> __device__ __forceinline__ T b5_m4(T v1, T v2, T v3, T v4,
 // _weno_z5_left, src/AdvectionSchemes.jl:240-255
 template <int P, typename T>
 __device__ __forceinline__ T weno_z5(T v1, T v2, T v3, T v4, T v5) {
-    T q0 = div6<T>((T(2) * v1 - T(7) * v2) + T(11) * v3);
-    T q1 = div6<T>((-v2 + T(5) * v3) + T(2) * v4);
-    T q2 = div6<T>((T(2) * v3 + T(5) * v4) - v5);
+    // the candidate polynomials times 6: the common 1/6 is taken once, in the
+    // normalisation (Σ ω_k q_k = Σ a_k (6 q_k) / (6 Σ a_k)), instead of three
+    // exact divisions by 6 (12 FP64 instructions per face fewer)
+    T q0 = (T(2) * v1 - T(7) * v2) + T(11) * v3;
+    T q1 = (-v2 + T(5) * v3) + T(2) * v4;
+    T q2 = (T(2) * v3 + T(5) * v4) - v5;
     T b0 = b3_m2<T>(v1, v2, v3), b1 = b3_m1<T>(v2, v3, v4), b2 = b3_p0<T>(v3, v4, v5);
     const T eps = Eps<T>::v * Eps<T>::v;
     T tau = fabs(b0 - b2);
     T a0 = T(1.0 / 10.0) * (T(1) + ipow<P>(qdiv<T>(tau, b0 + eps)));
     T a1 = T(6.0 / 10.0) * (T(1) + ipow<P>(qdiv<T>(tau, b1 + eps)));
     T a2 = T(3.0 / 10.0) * (T(1) + ipow<P>(qdiv<T>(tau, b2 + eps)));
-    return qdiv<T>((a0 * q0 + a1 * q1) + a2 * q2, (a0 + a1) + a2);
+    return qdiv<T>((a0 * q0 + a1 * q1) + a2 * q2, T(6) * ((a0 + a1) + a2));
 }
 // _weno_z7_left, src/AdvectionSchemes.jl:284-303
 template <int P, typename T>
