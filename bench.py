@@ -414,7 +414,8 @@ def main():
     if valu_insts:
         floor_s = valu_insts * VALU_CYCLES_PER_INST / (N_SIMD * CLOCK_HZ)
         valu = {"insts_per_launch": valu_insts, "issue_floor_ms": floor_s * 1e3, "frac": floor_s / avg_launch_s,
-                "note": "FP64 VALU issue: the kernel's actual bound (SQ_INSTS_VALU from profiles/)"}
+                "note": "FP64 VALU issue floor beside the HBM one (SQ_INSTS_VALU from profiles/); the kernel "
+                        "is bound by its L2-miss traffic and load latency, not by VALU issue"}
 
     cpu = None
     if rank == 0 and not args.no_cpu:
