@@ -45,7 +45,7 @@ def test_unet_params_roundtrip(hip):
     assert np.array_equal(net.params(), q)
 
 
-@pytest.mark.parametrize("shape,inc", [((16, 16, 8), 2), ((12, 8, 16), 1)])
+@pytest.mark.parametrize("shape,inc", [((16, 16, 8), 2), ((12, 8, 16), 1), ((40, 24, 16), 2)])
 def test_unet_forward_matches_oracle(hip, orc, shape, inc):
     from aeolus_hip.learned import UNet
     net = UNet(shape, in_channels=inc, seed=5)
