@@ -209,6 +209,28 @@ def test_mgpcg_zero_rhs(hip):
     assert s.last_status == "zero_rhs" and it == 0 and np.all(x == 0)
 
 
+def test_mgpcg_rhs_scale_independent(hip):
+    """The breakdown test is relative to r·z (test/multigrid.jl:68-83, issue
+    #132): an absolute |p·Ap| < eps stopped CG early for a small RHS. The
+    reference's case, 8³ over 250 × 250 × 100 m, periodic x and y, Neumann
+    z: the solutions for the RHS and 1e-6 × the RHS agree after rescaling
+    (rtol 1e-8) and take the same iteration count."""
+    N = (8, 8, 8)
+    i, j, k = np.meshgrid(np.arange(1, 9), np.arange(1, 9), np.arange(1, 9), indexing="ij")
+    b0 = np.sin(0.7 * i) * np.cos(1.3 * j) + 0.1 * np.sin(0.9 * k) * i / 8
+    sols, its = [], []
+    for scale in (1.0, 1e-6):
+        s = hip.MGPreconditionedCG(N, (250.0 / 8, 250.0 / 8, 100.0 / 8), bcs=("periodic", "periodic", "neumann"),
+                                   tol=1e-10, vcycle="f64")
+        x = hip.solve_interior(s, scale * b0)
+        it, _ = hip.solver_stats(s)
+        assert s.last_status == "converged"
+        sols.append(x / scale)
+        its.append(it)
+    assert its[0] == its[1]
+    assert np.linalg.norm(sols[1] - sols[0]) <= 1e-8 * np.linalg.norm(sols[0])
+
+
 @pytest.mark.parametrize("coeff", ["uniform", "cutcell"])
 def test_single_vcycle_parity(hip, orc, coeff):
     # one V-cycle from zero (the preconditioner itself): the red-black colour
