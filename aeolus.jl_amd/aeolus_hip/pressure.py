@@ -153,6 +153,26 @@ class MGPreconditionedCG(PressureSolver):
         super().__init__(h.value, (nx, ny, nz))
 
     @classmethod
+    def density_weighted(cls, size, dxy, dz, w_c, w_f, bcs=("periodic", "periodic", "neumann"), **kw):
+        """The iterative solver of ∇·(w(z)∇φ) = b, the operator of
+        DensityWeightedPoissonSolver (_neg_wlap_at, src/Anelastic.jl:734-935):
+        horizontal faces weighted by w_c of their level, vertical faces by w_f.
+        The reference preconditions its CG with point Jacobi; this one uses the
+        multigrid V-cycle on the same face-coefficient operator, so the
+        solution agrees to the tolerance while the iteration counts differ."""
+        nx, ny, nz = size
+        wc = np.asarray(w_c, dtype=np.float64)
+        wf = np.asarray(w_f, dtype=np.float64)
+        if len(wc) != nz or len(wf) != nz + 1:
+            raise ValueError("w_c must have length Nz and w_f length Nz+1")
+        if not (np.all(wc > 0) and np.all(wf > 0)):
+            raise ValueError("Poisson weights must be strictly positive")
+        cx = np.broadcast_to(wc[None, None, :], (nx + 1, ny, nz))
+        cy = np.broadcast_to(wc[None, None, :], (nx, ny + 1, nz))
+        cz = np.broadcast_to(wf[None, None, :], (nx, ny, nz + 1))
+        return cls(size, (float(dxy[0]), float(dxy[1]), float(dz)), bcs=bcs, coeff=(cx, cy, cz), **kw)
+
+    @classmethod
     def stretched(cls, grid, **kw):
         """MGPreconditionedCG(g::StretchedGrid) (src/Multigrid.jl:949-953) through
         the row-scaled uniform operator of _stretched_operator (:641-658)."""
