@@ -36,33 +36,33 @@ __device__ __forceinline__ void halo_gather_body(T *__restrict__ d, int Nx, int 
     const int sx = Nx + 2 * H, sy = Ny + 2 * H, sz = Nz + 2 * H;
     // enumeration: z-slabs (2H planes), then y-slabs, then x-slabs, then the
     // w wall plane k = H (interior) when wall_w
-    const long nz_slab = (long)2 * H * sx * sy;
-    const long ny_slab = (long)Nz * 2 * H * sx;
-    const long nx_slab = (long)Nz * Ny * 2 * H;
-    const long nwall = wall_w ? (long)sx * sy : 0;
-    const long total = nz_slab + ny_slab + nx_slab + nwall;
-    for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < total; t += (long)gridDim.x * blockDim.x) {
+    const uint32_t nz_slab = (uint32_t)2 * H * sx * sy;
+    const uint32_t ny_slab = (uint32_t)Nz * 2 * H * sx;
+    const uint32_t nx_slab = (uint32_t)Nz * Ny * 2 * H;
+    const uint32_t nwall = wall_w ? (uint32_t)sx * sy : 0;
+    const uint32_t total = nz_slab + ny_slab + nx_slab + nwall;
+    for (uint32_t t = blockIdx.x * (uint32_t)blockDim.x + threadIdx.x; t < total; t += (uint32_t)gridDim.x * blockDim.x) {
         int i, j, k;
         if (t < nz_slab) {
-            long q = t;
+            uint32_t q = t;
             i = (int)(q % sx); q /= sx;
             j = (int)(q % sy); q /= sy;
             int l = (int)q;
             k = l < H ? l : H + Nz + (l - H);
         } else if (t < nz_slab + ny_slab) {
-            long q = t - nz_slab;
+            uint32_t q = t - nz_slab;
             i = (int)(q % sx); q /= sx;
             int l = (int)(q % (2 * H)); q /= (2 * H);
             j = l < H ? l : H + Ny + (l - H);
             k = H + (int)q;
         } else if (t < nz_slab + ny_slab + nx_slab) {
-            long q = t - nz_slab - ny_slab;
+            uint32_t q = t - nz_slab - ny_slab;
             int l = (int)(q % (2 * H)); q /= (2 * H);
             i = l < H ? l : H + Nx + (l - H);
             j = H + (int)(q % Ny);
             k = H + (int)(q / Ny);
         } else {
-            long q = t - nz_slab - ny_slab - nx_slab;
+            uint32_t q = t - nz_slab - ny_slab - nx_slab;
             i = (int)(q % sx);
             j = (int)(q / sx);
             k = H;
@@ -615,27 +615,27 @@ __global__ void shell_kernel(ShellSet<T> ss, int Nx, int Ny, int Nz, int H, doub
     const T *__restrict__ x = ss.x[f];
     const T *__restrict__ snap = ss.snap[f];
     const int sx = Nx + 2 * H, sy = Ny + 2 * H;
-    const long nz_slab = (long)2 * H * sx * sy;
-    const long ny_slab = (long)Nz * 2 * H * sx;
-    const long nx_slab = (long)Nz * Ny * 2 * H;
-    const long total = nz_slab + ny_slab + nx_slab;
+    const uint32_t nz_slab = (uint32_t)2 * H * sx * sy;
+    const uint32_t ny_slab = (uint32_t)Nz * 2 * H * sx;
+    const uint32_t nx_slab = (uint32_t)Nz * Ny * 2 * H;
+    const uint32_t total = nz_slab + ny_slab + nx_slab;
     const bool copy = ca == 0.0 && cb == 1.0;
-    for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < total; t += (long)gridDim.x * blockDim.x) {
+    for (uint32_t t = blockIdx.x * (uint32_t)blockDim.x + threadIdx.x; t < total; t += (uint32_t)gridDim.x * blockDim.x) {
         int i, j, k;
         if (t < nz_slab) {
-            long q = t;
+            uint32_t q = t;
             i = (int)(q % sx); q /= sx;
             j = (int)(q % sy); q /= sy;
             const int l = (int)q;
             k = l < H ? l : H + Nz + (l - H);
         } else if (t < nz_slab + ny_slab) {
-            long q = t - nz_slab;
+            uint32_t q = t - nz_slab;
             i = (int)(q % sx); q /= sx;
             const int l = (int)(q % (2 * H)); q /= (2 * H);
             j = l < H ? l : H + Ny + (l - H);
             k = H + (int)q;
         } else {
-            long q = t - nz_slab - ny_slab;
+            uint32_t q = t - nz_slab - ny_slab;
             const int l = (int)(q % (2 * H)); q /= (2 * H);
             i = l < H ? l : H + Nx + (l - H);
             j = H + (int)(q % Ny);
