@@ -679,12 +679,13 @@ __global__ void __launch_bounds__(256) diag_kernel(double *__restrict__ part, co
                                                    Metrics<double> md, int what) {
     __shared__ double sh[256];
     const int H = md.H, sx = md.sx, sy = md.sy;
-    const long n = (long)md.Nx * md.Ny * md.Nz;
+    // 32-bit index arithmetic: every interior is below 2^32 cells
+    const uint32_t n = (uint32_t)md.Nx * md.Ny * md.Nz;
     double acc = 0.0;
-    for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x) {
-        const int i = (int)(e % md.Nx);
-        const long r = e / md.Nx;
-        const int j = (int)(r % md.Ny), k = (int)(r / md.Ny);
+    for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x) {
+        const int i = (int)(e % (uint32_t)md.Nx);
+        const uint32_t r = e / (uint32_t)md.Nx;
+        const int j = (int)(r % (uint32_t)md.Ny), k = (int)(r / (uint32_t)md.Ny);
         const int K = k + H;
         const size_t c = lin(i + H, j + H, K, sx, sy);
         double val;
@@ -766,12 +767,12 @@ __global__ void __launch_bounds__(256) guard_kernel(double *__restrict__ part, c
     __shared__ double sh[3][256];
     const int H = md.H, sx = md.sx, sy = md.sy;
     const size_t sxy = (size_t)sx * sy;
-    const long n = (long)md.Nx * md.Ny * md.Nz;
+    const uint32_t n = (uint32_t)md.Nx * md.Ny * md.Nz;
     double bad = 0.0, dmax = 0.0, smax = 0.0;
-    for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x) {
-        const int i = (int)(e % md.Nx);
-        const long r = e / md.Nx;
-        const int j = (int)(r % md.Ny), k = (int)(r / md.Ny);
+    for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x) {
+        const int i = (int)(e % (uint32_t)md.Nx);
+        const uint32_t r = e / (uint32_t)md.Nx;
+        const int j = (int)(r % (uint32_t)md.Ny), k = (int)(r / (uint32_t)md.Ny);
         const int K = k + H;
         const size_t c = lin(i + H, j + H, K, sx, sy);
         const double uc = (double)u[c], vc = (double)v[c], wc = (double)w[c];
